@@ -1,0 +1,148 @@
+"""Benchmark: p = 3 VR + RHS cell-updates/s of the SSP-RK3 hot path on one GPU.
+
+Workload (default): the four-quadrant 2D Riemann problem on [0,1]^2,
+1024 x 1024 quads cut along seeded diagonals (2,097,152 triangles), interior
+nodes jittered by U(-0.2h, 0.2h); p = 3 VR, HLLC, SSP-RK3 at CFL 0.5 with a
+global time step, 5 Jacobi sweeps per stage, FP64.  One "step" is one SSP-RK3
+step: EvaluateDt once, then three stages of (5 sweeps + EvaluateRHS + stage
+update), all on the device with the state resident in HBM.
+
+value = cells x 3 stages x steps / wall time of the timed steps.
+
+Printed: one JSON line with the metric, the roofline of the dominant kernel
+(the Jacobi sweep: algorithmic bytes per launch / mean launch time from HIP
+events recorded by the library on its stream) and a CPU baseline (the FP64
+oracle, single thread, one SSP-RK3 step on the same mesh).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024, help="quads per direction")
+    ap.add_argument("--sweeps", type=int, default=5)
+    ap.add_argument("--cfl", type=float, default=0.5)
+    ap.add_argument("--threads", type=int, default=0, help="host threads for the setup")
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on one step (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    from dnds_amd import _capi as A
+    from dnds_amd import cases
+    from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
+
+    t0 = time.time()
+    log(f"building mesh + VR setup, {args.n}x{args.n} quads")
+    case = cases.riemann_2d(args.n, nthreads=args.threads, build_limiter_data=False)
+    nc = case.n_cell
+    log(f"setup done: {nc} cells, {case.mesh.n_face} faces, {time.time() - t0:.1f}s")
+    ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType=case.riemann, farFieldStaticValue=case.far)
+    log(f"device context: {ev.device.device_bytes() / 2**30:.2f} GiB")
+    u = ev.BuildMean().upload(case.u0)
+    urec0 = np.zeros((nc, 36))
+    r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl)
+    t = 0.0
+    for _ in range(args.warmup):
+        t += ode.Step(u, r, rn, t)
+    ev.device.sync()
+    log(f"warmup {args.warmup} steps done")
+    lib = A.load()
+    lib.dndsmi_profile_enable(ev.device.ctx, 1)
+    ev.device.sync()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        t += ode.Step(u, r, rn, t)
+    ev.device.sync()
+    te = time.perf_counter()
+    elapsed = te - ts
+    log(f"timed {args.steps} steps: {elapsed:.3f}s, t = {t:.5f}")
+    prof = {}
+    import ctypes as C
+    for k, name in enumerate(A.KERNEL_NAMES):
+        cnt, tot = C.c_int64(), C.c_double()
+        lib.dndsmi_profile_query(ev.device.ctx, k, C.byref(cnt), C.byref(tot))
+        if cnt.value:
+            b = int(lib.dndsmi_kernel_bytes(ev.device.ctx, k))
+            avg_ms = tot.value / cnt.value
+            prof[name] = dict(launches=cnt.value, avg_ms=avg_ms, bytes=b, gbs=b / (avg_ms * 1e-3) / 1e9)
+    lib.dndsmi_profile_enable(ev.device.ctx, 0)
+    for name, p in prof.items():
+        log(f"{name}: {p['launches']} launches, {p['avg_ms']:.4f} ms avg, {p['gbs']:.0f} GB/s algorithmic")
+    uf = u.download()
+    if not np.isfinite(uf).all():
+        raise RuntimeError("non-finite state after the timed steps")
+
+    stages = 3 * args.steps
+    value = nc * stages / elapsed
+    jac = prof.get("k_jacobi", {})
+    roof = None
+    if jac:
+        roof = {"bound": "hbm", "achieved": jac["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": jac["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": "k_jacobi",
+                "bytes_per_launch": jac["bytes"], "avg_ms": jac["avg_ms"]}
+
+    cpu = None
+    if args.cpu_baseline:
+        import oracle
+        log("CPU baseline: oracle (1 thread), one SSP-RK3 step on the same mesh")
+        orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(["volume", "face_area", "face_center",
+                                                                  "face_norm_center", "face_norm", "face_weight",
+                                                                  "face_d0", "ainv", "ainv_b", "ainv_bvec"]),
+                            riemann_solver=2 if case.riemann == "HLLC" else 1, far_field=case.far)
+        c0 = time.perf_counter()
+        orc.ssprk3_steps(case.u0, urec0, 1, args.sweeps, args.cfl)
+        c1 = time.perf_counter()
+        cpu = {"value": nc * 3 / (c1 - c0), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps + RHS) on the same {nc}-cell mesh, "
+                         f"{c1 - c0:.1f}s"}
+        log(f"CPU baseline {cpu['value']:.3e} cell-updates/s ({c1 - c0:.1f}s)")
+
+    out = {
+        "metric": "p=3 VR+RHS cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mesh, four-quadrant Riemann initial state)",
+        "config": {"workload": f"2D Riemann, {args.n}x{args.n} quads split to {nc} triangles, jitter 0.2h, "
+                               f"p=3 VR, HLLC, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, "
+                               "no limiter",
+                   "cells": nc, "sweeps_per_stage": args.sweeps, "limiter": "none", "precision": "fp64"},
+        "kernels": prof,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+"""Benchmark and test cases: meshes, settings and initial conditions.
+
+* `isentropic_vortex` -- the reference's vortex initializer
+  (specialBuiltinInitializer 2, DNDS_FV_EulerSolver.hpp:536-582): cell means
+  of the exact vortex (strength 5, centre (5, 5), free stream (1, 1, 1, 1))
+  from the cell quadrature (TRI_4 / QUAD_9), on the periodic [0,10]^2.
+* `riemann_2d` -- the four-quadrant Riemann problem of the reference's
+  2D Riemann configuration (data/euler_config_2DRiemann.json: far-field
+  state plus three box initializers, applied in order at cell barycentres;
+  Special_2DRiemannFar boundaries, split at x = y = 0.8).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _capi as A
+from .host import RIEMANN_VR_SETTING, CompactFacedMeshSerialRW, VRFiniteVolume2D, vr_setting
+
+GAMMA = 1.4
+
+TRI4 = np.array([[1. / 3., 1. / 3., -27. / 96.], [0.6, 0.2, 25. / 96.], [0.2, 0.6, 25. / 96.],
+                 [0.2, 0.2, 25. / 96.]])
+_g = 0.7745966692414833
+_gw = np.array([5. / 9., 8. / 9., 5. / 9.])
+_gx = np.array([-_g, 0.0, _g])
+QUAD9 = np.array([[_gx[i], _gx[j], _gw[i] * _gw[j]] for j in range(3) for i in range(3)])
+
+
+def cell_means(mesh: CompactFacedMeshSerialRW, volume: np.ndarray, fstate) -> np.ndarray:
+    """(1/V) * sum_g fstate(x_g) detJ_g w_g with TRI_4 / QUAD_9 (as the reference's initializer)."""
+    m = mesh.arrays()
+    cp, c2n, xy = m["cell_ptr"], m["cell2node"], m["node_xy"]
+    nv = np.diff(cp)
+    out = np.zeros((m["n_cell"], 4))
+    for n, rule in ((3, TRI4), (4, QUAD9)):
+        idx = np.nonzero(nv == n)[0]
+        if idx.size == 0:
+            continue
+        nodes = c2n[cp[idx][:, None] + np.arange(n)[None, :]]
+        X, Y = xy[nodes, 0], xy[nodes, 1]
+        acc = np.zeros((idx.size, 4))
+        for xi, eta, w in rule:
+            if n == 3:
+                N = np.array([1 - xi - eta, xi, eta])
+                Nx = np.array([-1.0, 1.0, 0.0])
+                Ny = np.array([-1.0, 0.0, 1.0])
+            else:
+                N = np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta), (1 + xi) * (1 + eta),
+                              (1 - xi) * (1 + eta)]) * 0.25
+                Nx = np.array([-(1 - eta), (1 - eta), (1 + eta), -(1 + eta)]) * 0.25
+                Ny = np.array([-(1 - xi), -(1 + xi), (1 + xi), (1 - xi)]) * 0.25
+            px, py = X @ N, Y @ N
+            det = (X @ Nx) * (Y @ Ny) - (X @ Ny) * (Y @ Nx)
+            acc += fstate(px, py) * (det * w)[:, None]
+        out[idx] = acc / volume[idx][:, None]
+    return out
+
+
+def vortex_state(px, py, gamma=GAMMA, chi=5.0, rm=8.0):
+    """Exact isentropic vortex at t = 0 (DNDS_FV_EulerSolver.hpp:556-573)."""
+    r = np.sqrt((px - 5) ** 2 + (py - 5) ** 2)
+    with np.errstate(over="ignore"):
+        damp = 1 - 1.0 / np.exp(np.maximum(rm ** 2 - r ** 2, 0.0))
+    dT = -(gamma - 1) / (8 * gamma * np.pi ** 2) * chi ** 2 * np.exp(1 - r ** 2) * damp
+    dux = chi / 2 / np.pi * np.exp((1 - r ** 2) / 2) * -(py - 5) * damp
+    duy = chi / 2 / np.pi * np.exp((1 - r ** 2) / 2) * +(px - 5) * damp
+    T = dT + 1
+    ux, uy = dux + 1, duy + 1
+    rho = T ** (1 / (gamma - 1))
+    p = T * rho
+    E = 0.5 * (ux ** 2 + uy ** 2) * rho + p / (gamma - 1)
+    return np.stack([rho, rho * ux, rho * uy, E], axis=1)
+
+
+def vortex_exact(px, py, t, gamma=GAMMA, chi=5.0):
+    """Vortex convected by the free stream (1, 1) on the periodic [0,10]^2
+    (nearest periodic image; the far field is undisturbed to < 1e-10)."""
+    cx = np.mod(5 + t, 10.0)
+    cy = np.mod(5 + t, 10.0)
+    dx = np.mod(px - cx + 5, 10.0) - 5
+    dy = np.mod(py - cy + 5, 10.0) - 5
+    return vortex_state(dx + 5, dy + 5, gamma, chi, rm=1e6)
+
+
+# conservative states (rho, rho u, rho v, E) of the 2D Riemann boxes
+RIEMANN_FAR = (1.5, 0.0, 0.0, 3.75)
+RIEMANN_BOXES = [  # (x0, x1, y0, y1, state)
+    (-1e100, 0.8, -1e100, 1e100, (0.532258064516129, 0.641927378778465, 0.0, 1.137096774193549)),
+    (-1e100, 1e100, -1e100, 0.8, (0.532258064516129, 0.0, 0.641927378778465, 1.137096774193549)),
+    (-1e100, 0.8, -1e100, 0.8, (0.137992831541219, 0.166425616720343, 0.166425616720343, 0.273297491039427)),
+]
+
+
+def riemann_state(bary: np.ndarray) -> np.ndarray:
+    """Box initializers in order (DNDS_FV_EulerSolver.hpp:484-497)."""
+    u = np.tile(np.array(RIEMANN_FAR), (bary.shape[0], 1))
+    for x0, x1, y0, y1, st in RIEMANN_BOXES:
+        sel = (bary[:, 0] > x0) & (bary[:, 0] < x1) & (bary[:, 1] > y0) & (bary[:, 1] < y1)
+        u[sel] = np.array(st)
+    return u
+
+
+class Case:
+    """A mesh, its VR setup, the Euler settings and the initial state."""
+
+    def __init__(self, name, mesh, vfv, riemann, far, u0, steps, sweeps, cfl, limiter):
+        self.name = name
+        self.mesh = mesh
+        self.vfv = vfv
+        self.riemann = riemann
+        self.far = far
+        self.u0 = u0
+        self.steps = steps
+        self.sweeps = sweeps
+        self.cfl = cfl
+        self.limiter = limiter
+
+    @property
+    def n_cell(self):
+        return self.mesh.n_cell
+
+
+def isentropic_vortex(n=40, seed=1, nthreads=0, build_limiter_data=False) -> Case:
+    """Periodic [0,10]^2, n x n quads cut along seeded diagonals (2 n^2 cells);
+    Roe, no limiter, CFL 0.5, 5 sweeps per stage, 100 steps."""
+    mesh = CompactFacedMeshSerialRW.generate(A.MESH_TRI_SPLIT, n, n, 0.0, 10.0, 0.0, 10.0, periodic=(True, True),
+                                             jitter=0.0, seed=seed)
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**RIEMANN_VR_SETTING, build_limiter_data=int(build_limiter_data)))
+    vfv.Initialization(nthreads)
+    vol = vfv.arrays(["volume"])["volume"]
+    u0 = cell_means(mesh, vol, vortex_state)
+    return Case("isentropic_vortex", mesh, vfv, "Roe", (1.0, 1.0, 1.0, 2.5 + 1.0), u0, 100, 5, 0.5, False)
+
+
+def riemann_2d(n=1024, seed=2, jitter=0.2, nthreads=0, build_limiter_data=True) -> Case:
+    """[0,1]^2, n x n quads cut along seeded diagonals (2 n^2 cells), interior
+    nodes jittered by U(-0.2h, 0.2h); HLLC, SSP-RK3, 200 steps."""
+    bc = A.BC_SPECIAL_2DRIEMANNFAR
+    mesh = CompactFacedMeshSerialRW.generate(A.MESH_TRI_SPLIT, n, n, 0.0, 1.0, 0.0, 1.0, jitter=jitter, seed=seed,
+                                             bc=(bc, bc, bc, bc))
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**RIEMANN_VR_SETTING, build_limiter_data=int(build_limiter_data)))
+    vfv.Initialization(nthreads)
+    bary = vfv.arrays(["cell_bary"])["cell_bary"].reshape(-1, 2)
+    u0 = riemann_state(bary)
+    return Case("riemann_2d", mesh, vfv, "HLLC", RIEMANN_FAR, u0, 200, 1, 0.5, True)

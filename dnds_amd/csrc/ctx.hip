@@ -1,0 +1,1145 @@
+// Device context, arrays and the FP64 kernels of the hot path (gfx950).
+//
+// The context holds the mesh and the VR setup in device memory, repacked from
+// the reference layouts into structure-of-arrays streams so that a wavefront
+// of 64 cells (or faces) reads each matrix element as one contiguous 512-byte
+// segment.  All kernels are launched on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+
+using namespace dndsmi_dev;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
+inline unsigned grid_for(int64_t n, int block = kBlock) { return unsigned((n + block - 1) / block); }
+
+} // namespace
+
+struct dndsmi_ctx {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    std::string err;
+    int64_t nc = 0, nf = 0, ldc = 0, ldf = 0;
+    int nfmax = 3;
+    GasParams gas{};
+    double relax = 1.0;
+    int64_t bytes = 0;
+    std::vector<void *> allocs;
+
+    // cell side
+    int32_t *nbr = nullptr;    // [nfmax][ldc] neighbour cell, -1 = boundary slot, -2 = absent slot
+    int32_t *sbc = nullptr;    // [nfmax][ldc] boundary kind of the slot's face
+    int32_t *sface = nullptr;  // [nfmax][ldc] face index of the slot
+    int32_t *gat = nullptr;    // [nfmax][ldc] (face << 1 | side) sorted by face, -1 = none
+    double *M = nullptr;       // [nfmax*81][ldc]
+    double *bv = nullptr;      // [nfmax*9][ldc]
+    double *vol = nullptr;     // [ldc]
+    int32_t *wall_idx = nullptr; // [nfmax][ldc] index into the wall-face arrays, -1 = none
+    int64_t n_wall = 0;
+    double *wall_ainv = nullptr; // [81][n_wall]
+    double *wall_d0 = nullptr;   // [27][n_wall]
+    double *wall_nrm = nullptr;  // [6][n_wall]
+    double *wall_w0 = nullptr;   // [n_wall]
+    int32_t *wall_cell = nullptr; // [n_wall]
+    int32_t *wall_bc = nullptr;   // [n_wall]
+    double *wall_corr = nullptr;  // [36][n_wall]
+    int64_t n_wall_faces = 0;     // real count (n_wall >= 1 for allocation)
+    // face side
+    int32_t *f2c = nullptr;    // [2][ldf]
+    int32_t *fbc = nullptr;    // [ldf]
+    double *fd0 = nullptr;     // [54][ldf]
+    double *fn = nullptr;      // [6][ldf] normals at the LINE_3 points
+    double *fnc = nullptr;     // [2][ldf] normal at the face centre
+    double *farea = nullptr;   // [ldf]
+    double *fctr = nullptr;    // [2][ldf]
+    int32_t *bfaces = nullptr; // boundary face indices
+    int64_t n_bface = 0;
+    // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
+    double *lambda_face = nullptr;
+    double *dlambda_face = nullptr;
+    double *fflux = nullptr;   // [12][ldf] F, FLfix, FRfix integrated over the face
+    double *dt_part = nullptr; // [grid] partial minima
+    double *dt_min = nullptr;  // [1]
+    int32_t *errw = nullptr;   // [1] error bits
+    double *xlast = nullptr;   // [ldc][4] SSP-RK3 xLast
+    double *dt_tmp = nullptr;  // [ldc]
+    // per-kernel HIP-event timing (dndsmi_profile_*)
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> prof_recs; // (kernel id, index of the start event)
+    int prof_open = -1;
+    size_t prof_open_ev = 0;
+
+    template <class T>
+    int alloc(T *&p, int64_t n) {
+        size_t b = sizeof(T) * size_t(std::max<int64_t>(n, 1));
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, b);
+        if (e != hipSuccess) {
+            err = std::string("hipMalloc: ") + hipGetErrorString(e);
+            return DNDSMI_ERR_OUT_OF_MEMORY;
+        }
+        allocs.push_back(q);
+        bytes += int64_t(b);
+        p = static_cast<T *>(q);
+        return DNDSMI_OK;
+    }
+    int hip(hipError_t e, const char *what) {
+        if (e != hipSuccess) {
+            err = std::string(what) + ": " + hipGetErrorString(e);
+            return DNDSMI_ERR_HIP;
+        }
+        return DNDSMI_OK;
+    }
+    template <class T>
+    int upload(T *dst, const std::vector<T> &src) {
+        int rc = hip(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, stream),
+                     "hipMemcpyAsync");
+        if (rc)
+            return rc;
+        return hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    int launch_check(const char *what) { return hip(hipGetLastError(), what); }
+};
+
+namespace {
+
+hipEvent_t prof_event(dndsmi_ctx *c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess)
+            return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// brackets a launch with events when profiling is on
+void prof_begin(dndsmi_ctx *c, int id) {
+    if (!c->prof_on)
+        return;
+    c->prof_open = id;
+    c->prof_open_ev = c->ev_used;
+    hipEvent_t e = prof_event(c);
+    if (e)
+        (void)hipEventRecord(e, c->stream);
+}
+
+void prof_end(dndsmi_ctx *c) {
+    if (!c->prof_on || c->prof_open < 0)
+        return;
+    hipEvent_t e = prof_event(c);
+    if (e) {
+        (void)hipEventRecord(e, c->stream);
+        c->prof_recs.push_back({c->prof_open, c->prof_open_ev});
+    }
+    c->prof_open = -1;
+}
+
+} // namespace
+
+struct dndsmi_array {
+    dndsmi_ctx *ctx = nullptr;
+    int32_t kind = 0;
+    double *d = nullptr;
+    int64_t elems = 0; // doubles allocated
+};
+
+#define DNDSMI_TRY(expr)                                                                                               \
+    do {                                                                                                               \
+        int rc_ = (expr);                                                                                              \
+        if (rc_ != DNDSMI_OK)                                                                                          \
+            return rc_;                                                                                                \
+    } while (0)
+
+// ========================================================================
+// kernels
+// ========================================================================
+
+// Boundary correction of a Wall_Euler / Wall_NoSlip face to its cell's sweep
+// (DNDS_FV_VR.hpp:1577-1657): corr = w A^-1 sum_g w_g |n_g| D0^T W0^2 (uBV - u_i).
+// Computed before the sweep from the current iterate; the sweep adds it in
+// the face's slot, which keeps the reference's accumulation order.
+__global__ __launch_bounds__(kBlock) void k_wall_corr(int32_t nw, int64_t ldc, const int32_t *__restrict__ wcell,
+                                                      const int32_t *__restrict__ wbc,
+                                                      const double *__restrict__ wall_ainv,
+                                                      const double *__restrict__ wall_d0,
+                                                      const double *__restrict__ wall_nrm,
+                                                      const double *__restrict__ wall_w0, int64_t ldw,
+                                                      const double *__restrict__ u, const double *__restrict__ x,
+                                                      double relax, GasParams gp, double t,
+                                                      double *__restrict__ corr, int32_t *errw) {
+    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw)
+        return;
+    const int32_t i = wcell[w];
+    const int32_t bc = wbc[w];
+    double xi[36], ui[4];
+    for (int k = 0; k < 36; k++)
+        xi[k] = x[k * ldc + i];
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+    double BCc[36];
+    for (int k = 0; k < 36; k++)
+        BCc[k] = 0.0;
+    const double w0 = wall_w0[w];
+    for (int g = 0; g < 3; g++) {
+        const double gw = g == 1 ? kLine3W1 : kLine3W0;
+        const double nx = wall_nrm[(2 * g) * ldw + w], ny = wall_nrm[(2 * g + 1) * ldw + w];
+        double uBL[4];
+        for (int v = 0; v < 4; v++) {
+            double acc = 0;
+            for (int c = 0; c < 9; c++)
+                acc += wall_d0[(g * 9 + c) * ldw + w] * xi[c + 9 * v];
+            uBL[v] = acc;
+        }
+        for (int v = 0; v < 4; v++)
+            uBL[v] += ui[v];
+        double n0, n1;
+        stable_normalized(nx, ny, n0, n1);
+        double uBV[4];
+        if (bc == BC_WALL_EULER) {
+            for (int v = 0; v < 4; v++)
+                uBV[v] = uBL[v];
+            const double dd = n0 * uBV[1] + n1 * uBV[2];
+            uBV[1] = uBL[1] - n0 * dd;
+            uBV[2] = uBL[2] - n1 * dd;
+        } else if (!boundary_value(gp, uBL, n0, n1, 0.0, 0.0, t, bc, uBV)) {
+            atomicOr(errw, ERRBIT_NONPOSITIVE);
+            for (int v = 0; v < 4; v++)
+                uBV[v] = uBL[v];
+        }
+        const double nl = sqrt(nx * nx + ny * ny);
+        for (int r = 0; r < 9; r++)
+            for (int v = 0; v < 4; v++) {
+                double inc = wall_d0[(g * 9 + r) * ldw + w] * w0 * w0 * (uBV[v] - ui[v]);
+                inc *= nl;
+                BCc[r + 9 * v] += inc * gw;
+            }
+    }
+    for (int r = 0; r < 9; r++)
+        for (int v = 0; v < 4; v++) {
+            double acc = 0;
+            for (int c = 0; c < 9; c++)
+                acc += wall_ainv[(r * 9 + c) * ldw + w] * BCc[c + 9 * v];
+            corr[(r + 9 * v) * ldw + w] = relax * acc;
+        }
+}
+
+// One block-Jacobi VR sweep, SOR off (ReconstructionJacobiStep,
+// DNDS_FV_VR.hpp:1495-1682): new_i = (1-w) x_i + w sum_f [M_if x_j + b_if (u_j - u_i)^T]
+// in cell2face order, with the reference's boundary variants: far field
+// couples the cell to itself (M_if x_i), a Wall face adds b_if (0 - u_i)^T,
+// Wall_Euler / Wall_NoSlip faces add the correction of k_wall_corr.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, const int32_t *__restrict__ nbr,
+                                                   const int32_t *__restrict__ sbc,
+                                                   const int32_t *__restrict__ wall_idx,
+                                                   const double *__restrict__ M, const double *__restrict__ bv,
+                                                   const double *__restrict__ u, const double *__restrict__ x,
+                                                   double *__restrict__ xn, double relax,
+                                                   const double *__restrict__ corr, int64_t ldw, int32_t *errw) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    double nw[36];
+#pragma unroll
+    for (int k = 0; k < 36; k++)
+        nw[k] = (1 - relax) * x[k * ldc + i];
+    double ui[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+#pragma unroll 1
+    for (int s = 0; s < NF; s++) {
+        int32_t j = nbr[s * ldc + i];
+        if (j == -2)
+            continue;
+        const double *bs = bv + int64_t(s) * 9 * ldc + i;
+        double bmul = 1.0;
+        if (j < 0) {
+            const int32_t bc = sbc[s * ldc + i];
+            if (bc == BC_WALL) {
+#pragma unroll
+                for (int r = 0; r < 9; r++) {
+                    const double b = bs[r * ldc];
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        nw[r + 9 * v] += relax * ((0.0 - ui[v]) * b);
+                }
+                continue;
+            }
+            if (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) {
+                const int64_t w = wall_idx[s * ldc + i];
+#pragma unroll
+                for (int k = 0; k < 36; k++)
+                    nw[k] += corr[k * ldw + w];
+                continue;
+            }
+            if (!is_far(bc)) {
+                atomicOr(errw, ERRBIT_BC);
+                continue;
+            }
+            j = i; // far field: M_if x_i, no mean-value term
+            bmul = 0.0;
+        }
+        const double *Ms = M + int64_t(s) * 81 * ldc + i;
+        double du[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            du[v] = bmul * (u[4 * int64_t(j) + v] - ui[v]);
+        double xj[36];
+#pragma unroll
+        for (int k = 0; k < 36; k++)
+            xj[k] = x[k * ldc + j];
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+            double tt[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+                const double m = Ms[(r * 9 + c) * ldc];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    tt[v] += m * xj[c + 9 * v];
+            }
+            const double b = bmul * bs[r * ldc];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 36; k++)
+        xn[k * ldc + i] = nw[k];
+}
+
+// EvaluateDt face loop (DNDS_FV_EulerEvaluator.cpp:25-137): lambdaFace and
+// deltaLambdaFace per face (muGas = 0).
+__global__ __launch_bounds__(kBlock) void k_dt_face(int32_t nf, int64_t ldf, const int32_t *__restrict__ f2c,
+                                                    const double *__restrict__ fnc, const double *__restrict__ u,
+                                                    double gamma, double *__restrict__ lf, double *__restrict__ dlf,
+                                                    int32_t *errw) {
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    const int32_t L = f2c[f], R = f2c[ldf + f];
+    double n0, n1;
+    normalized(fnc[f], fnc[ldf + f], n0, n1);
+    double UL[4];
+    for (int k = 0; k < 4; k++)
+        UL[k] = u[4 * int64_t(L) + k];
+    double uMean[4] = {UL[0], UL[1], UL[2], UL[3]};
+    double pL, asqrL, HL, pR, asqrR, HR;
+    const double vL0 = UL[1] / UL[0], vL1 = UL[2] / UL[0];
+    double vR0 = vL0, vR1 = vL1;
+    ideal_gas_thermal(UL[3], UL[0], vL0 * vL0 + vL1 * vL1, gamma, pL, asqrL, HL);
+    pR = pL, HR = HL, asqrR = asqrL;
+    if (R >= 0) {
+        double UR[4];
+        for (int k = 0; k < 4; k++)
+            UR[k] = u[4 * int64_t(R) + k];
+        for (int k = 0; k < 4; k++)
+            uMean[k] = (uMean[k] + UR[k]) * 0.5;
+        vR0 = UR[1] / UR[0], vR1 = UR[2] / UR[0];
+        ideal_gas_thermal(UR[3], UR[0], vR0 * vR0 + vR1 * vR1, gamma, pR, asqrR, HR);
+    }
+    if (!(uMean[0] > 0))
+        atomicOr(errw, ERRBIT_NONPOSITIVE);
+    const double veloNMean = 0.5 * ((vL0 + vR0) * n0 + (vL1 + vR1) * n1);
+    const double pMean = (pL + pR) * 0.5;
+    const double aMean = sqrt(gamma * pMean / uMean[0]);
+    lf[f] = fabs(veloNMean) + aMean;
+    dlf[f] = fabs((vR0 - vL0) * n0 + (vR1 - vL1) * n1) + sqrt(fabs(asqrR - asqrL)) * 0.7071;
+}
+
+// EvaluateDt cell loop (DNDS_FV_EulerEvaluator.cpp:132-150): lambdaCell from
+// the faces in ascending order, dt_i and the block minimum.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_dt_cell(int32_t nc, int64_t ldc, const int32_t *__restrict__ gat,
+                                                    const double *__restrict__ lf, const double *__restrict__ farea,
+                                                    const double *__restrict__ vol, double cfl, double max_dt,
+                                                    double *__restrict__ dt, double *__restrict__ part) {
+    __shared__ double red[kBlock];
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 3e200;
+    if (i < nc) {
+        double lam = 0.0;
+#pragma unroll
+        for (int s = 0; s < NF; s++) {
+            const int32_t g = gat[s * ldc + i];
+            if (g < 0)
+                continue;
+            const int32_t f = g >> 1;
+            lam += lf[f] * farea[f];
+        }
+        d = fmin(cfl * vol[i] / (lam + 1e-100), max_dt);
+        dt[i] = d;
+    }
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_min_final(int32_t n, const double *__restrict__ part,
+                                                      double *__restrict__ out, int32_t *errw) {
+    __shared__ double red[kBlock];
+    double d = 3e200;
+    for (int32_t k = threadIdx.x; k < n; k += kBlock)
+        d = fmin(d, part[k]);
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0];
+        if (!isfinite(red[0]))
+            atomicOr(errw, ERRBIT_NAN);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_from(int32_t n, const double *__restrict__ src,
+                                                      double *__restrict__ dst) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[0];
+}
+
+// EvaluateRHS face loop (DNDS_FV_EulerEvaluator.cpp:215-492): left/right
+// states at the three LINE_3 points, Riemann flux, flux-balance fluxes,
+// Gauss sum.  Stores the integrated F, FLfix, FRfix per face.  BOUNDARY
+// selects the boundary-face instance (run over the list of boundary faces);
+// the inner instance skips boundary faces, so neither carries the other's
+// registers.
+template <bool BOUNDARY>
+__global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int64_t ldc,
+                                                     const int32_t *__restrict__ list,
+                                                     const int32_t *__restrict__ f2c,
+                                                     const int32_t *__restrict__ fbc, const double *__restrict__ fd0,
+                                                     const double *__restrict__ fn, const double *__restrict__ fctr,
+                                                     const double *__restrict__ dlf, const double *__restrict__ u,
+                                                     const double *__restrict__ x, GasParams gp, double t,
+                                                     double *__restrict__ fflux, int32_t *errw) {
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n)
+        return;
+    const int32_t f = BOUNDARY ? list[tid] : tid;
+    const int32_t L = f2c[f], R = f2c[ldf + f];
+    if (!BOUNDARY && R < 0)
+        return;
+    double uL[4], uR[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        uL[k] = u[4 * int64_t(L) + k];
+    // reconstructed increments D0 . x at the 3 points, both sides
+    double incL[3][4], incR[3][4];
+#pragma unroll
+    for (int g = 0; g < 3; g++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            incL[g][v] = 0.0, incR[g][v] = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < 9; c++) {
+        double xc[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            xc[v] = x[(c + 9 * v) * ldc + L];
+#pragma unroll
+        for (int g = 0; g < 3; g++) {
+            const double d = fd0[(g * 18 + c) * ldf + f];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                incL[g][v] += d * xc[v];
+        }
+    }
+    if (!BOUNDARY) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            uR[k] = u[4 * int64_t(R) + k];
+#pragma unroll 1
+        for (int c = 0; c < 9; c++) {
+            double xc[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                xc[v] = x[(c + 9 * v) * ldc + R];
+#pragma unroll
+            for (int g = 0; g < 3; g++) {
+                const double d = fd0[(g * 18 + 9 + c) * ldf + f];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    incR[g][v] += d * xc[v];
+            }
+        }
+    }
+    const double dl = dlf[f];
+    const int32_t bc = BOUNDARY ? fbc[f] : BC_INNER;
+    double acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        acc[k] = 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int g = 0; g < 3; g++) {
+        const double nx = fn[(2 * g) * ldf + f], ny = fn[(2 * g + 1) * ldf + f];
+        double n0, n1;
+        normalized(nx, ny, n0, n1);
+        double ULxy[4], URxy[4], URm[4];
+        compress_rec_part(uL, incL[g], ULxy);
+        if (!BOUNDARY) {
+            compress_rec_part(uR, incR[g], URxy);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                URm[k] = uR[k];
+        } else {
+            const double px = fctr[f], py = fctr[ldf + f];
+            ok &= boundary_value(gp, ULxy, n0, n1, px, py, t, bc, URxy);
+            ok &= boundary_value(gp, uL, n0, n1, px, py, t, bc, URm);
+        }
+        double F[4], FL[4], FR[4];
+        ok &= ok && flux_face(gp, ULxy, URxy, uL, URm, n0, n1, dl, F, FL, FR);
+        const double nl = sqrt(nx * nx + ny * ny);
+        const double w = g == 1 ? kLine3W1 : kLine3W0;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double a = F[v] * nl, b = FL[v] * nl, c = FR[v] * nl;
+            acc[v] += a * w;
+            acc[4 + v] += b * w;
+            acc[8 + v] += c * w;
+        }
+    }
+    if (!ok)
+        atomicOr(errw, ERRBIT_NONPOSITIVE);
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+        fflux[k * ldf + f] = acc[k];
+}
+
+// EvaluateRHS scatter as a per-cell gather in ascending face order
+// (DNDS_FV_EulerEvaluator.cpp:501-508), optionally fused with the SSP-RK3
+// stage update of ExplicitSSPRK3LocalDt::Step (DNDS_ODE.hpp:136-157):
+//   stage 0: rhs = gathered value
+//   stage 1: xLast = x; x = x + rhs dt
+//   stage 2/3: x = ((x + rhs dt) c/(1-c) + xLast)(1-c), c = 1/4, 2/3
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, int64_t ldf,
+                                                     const int32_t *__restrict__ gat,
+                                                     const double *__restrict__ fflux,
+                                                     const double *__restrict__ vol, double *__restrict__ rhs,
+                                                     int stage, double *__restrict__ x,
+                                                     double *__restrict__ xlast, const double *__restrict__ dt,
+                                                     const double *__restrict__ dtmin, double coefA, double coefB,
+                                                     int32_t *errw) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    const double V = vol[i];
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+        const int32_t g = gat[s * ldc + i];
+        if (g < 0)
+            continue;
+        const int32_t f = g >> 1;
+        if ((g & 1) == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                r[v] += fflux[v * ldf + f] / V;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                r[v] -= fflux[(4 + v) * ldf + f] / V;
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                r[v] -= fflux[v * ldf + f] / V;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                r[v] += fflux[(8 + v) * ldf + f] / V;
+        }
+    }
+    bool finite = true;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        finite &= isfinite(r[v]);
+    if (!finite)
+        atomicOr(errw, ERRBIT_NAN);
+    if (stage == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            rhs[4 * int64_t(i) + v] = r[v];
+        return;
+    }
+    const double d = dt ? dt[i] : dtmin[0];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const double xo = x[4 * int64_t(i) + v];
+        double xv = xo + r[v] * d;
+        if (stage == 1) {
+            xlast[4 * int64_t(i) + v] = xo;
+        } else {
+            xv *= coefA;
+            xv += xlast[4 * int64_t(i) + v];
+            xv *= coefB;
+        }
+        x[4 * int64_t(i) + v] = xv;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_set(int64_t n, double *__restrict__ p, double v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_int(int32_t *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        p[0] = 0;
+}
+
+// ========================================================================
+// host side
+// ========================================================================
+
+namespace {
+
+int check_errw(dndsmi_ctx *c) {
+    int32_t h = 0;
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(&h, c->errw, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync"));
+    DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    if (h == 0)
+        return DNDSMI_OK;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->errw);
+    if (h & ERRBIT_NONPOSITIVE) {
+        c->err = "non-positive density or sound speed in a flux or boundary state";
+        return DNDSMI_ERR_NONPOSITIVE_STATE;
+    }
+    if (h & ERRBIT_NAN) {
+        c->err = "non-finite value in the right-hand side or time step";
+        return DNDSMI_ERR_NAN;
+    }
+    c->err = "boundary face of an unsupported kind";
+    return DNDSMI_ERR_UNSUPPORTED;
+}
+
+bool same_ctx(const dndsmi_ctx *c, const dndsmi_array *a, int32_t kind) { return a && a->ctx == c && a->kind == kind; }
+
+} // namespace
+
+extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_view *vr,
+                                 const dndsmi_vr_setting *vs, const dndsmi_euler_setting *es, void *stream,
+                                 dndsmi_ctx **out) {
+    if (!mesh || !vr || !vs || !es || !out)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (vr->n_cell != mesh->n_cell || vr->n_face != mesh->n_face || mesh->n_cell < 1 ||
+        mesh->n_cell > (int64_t(1) << 30) || mesh->n_face > (int64_t(1) << 30))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC)
+        return DNDSMI_ERR_UNSUPPORTED;
+    dndsmi_ctx *c = new (std::nothrow) dndsmi_ctx();
+    if (!c)
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    auto fail = [&](int rc) {
+        dndsmi_ctx_destroy(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess)
+        return fail(DNDSMI_ERR_HIP);
+    if (stream) {
+        c->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(DNDSMI_ERR_HIP);
+        c->own_stream = true;
+    }
+    const int64_t nc = mesh->n_cell, nf = mesh->n_face;
+    c->nc = nc, c->nf = nf, c->ldc = pad64(nc), c->ldf = pad64(nf);
+    c->relax = vs->jacobi_relax;
+    c->gas.gamma = es->gamma;
+    for (int k = 0; k < 4; k++)
+        c->gas.far[k] = es->far_field[k];
+    c->gas.rs = es->riemann_solver;
+    int nfmax = 0;
+    for (int64_t i = 0; i < nc; i++)
+        nfmax = std::max<int>(nfmax, int(mesh->cell_ptr[i + 1] - mesh->cell_ptr[i]));
+    if (nfmax != 3 && nfmax != 4)
+        return fail(DNDSMI_ERR_INVALID_ARGUMENT);
+    c->nfmax = nfmax;
+    const int64_t ldc = c->ldc, ldf = c->ldf;
+    try {
+        // ---- cell streams
+        std::vector<int32_t> nbr(size_t(nfmax) * ldc, -2), sbc(size_t(nfmax) * ldc, 0), sface(size_t(nfmax) * ldc, -1);
+        std::vector<int32_t> gat(size_t(nfmax) * ldc, -1), widx(size_t(nfmax) * ldc, -1);
+        std::vector<double> M(size_t(nfmax) * 81 * ldc, 0.0), bv(size_t(nfmax) * 9 * ldc, 0.0), vol(ldc, 1.0);
+        std::vector<int64_t> wall_cell, wall_face, wall_c2f;
+        for (int64_t i = 0; i < nc; i++) {
+            const int64_t b0 = mesh->cell_ptr[i];
+            const int n = int(mesh->cell_ptr[i + 1] - b0);
+            vol[i] = vr->volume[i];
+            std::vector<std::pair<int64_t, int>> order;
+            for (int s = 0; s < n; s++) {
+                const int64_t f = mesh->cell2face[b0 + s];
+                const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
+                const int side = L == i ? 0 : 1;
+                const int64_t j = side == 0 ? R : L;
+                nbr[s * ldc + i] = j == DNDSMI_NO_CELL ? -1 : int32_t(j);
+                sbc[s * ldc + i] = mesh->face_bc[f];
+                sface[s * ldc + i] = int32_t(f);
+                for (int k = 0; k < 81; k++)
+                    M[(size_t(s) * 81 + k) * ldc + i] = vr->ainv_b[81 * (b0 + s) + k];
+                for (int k = 0; k < 9; k++)
+                    bv[(size_t(s) * 9 + k) * ldc + i] = vr->ainv_bvec[9 * (b0 + s) + k];
+                order.push_back({f, side});
+                if (j == DNDSMI_NO_CELL &&
+                    (mesh->face_bc[f] == DNDSMI_BC_WALL_EULER || mesh->face_bc[f] == DNDSMI_BC_WALL_NOSLIP)) {
+                    widx[s * ldc + i] = int32_t(wall_cell.size());
+                    wall_cell.push_back(i), wall_face.push_back(f), wall_c2f.push_back(b0 + s);
+                }
+            }
+            std::sort(order.begin(), order.end());
+            for (int s = 0; s < n; s++)
+                gat[s * ldc + i] = int32_t((order[s].first << 1) | order[s].second);
+        }
+        DNDSMI_TRY(c->alloc(c->nbr, nfmax * ldc));
+        DNDSMI_TRY(c->upload(c->nbr, nbr));
+        DNDSMI_TRY(c->alloc(c->sbc, nfmax * ldc));
+        DNDSMI_TRY(c->upload(c->sbc, sbc));
+        DNDSMI_TRY(c->alloc(c->sface, nfmax * ldc));
+        DNDSMI_TRY(c->upload(c->sface, sface));
+        DNDSMI_TRY(c->alloc(c->gat, nfmax * ldc));
+        DNDSMI_TRY(c->upload(c->gat, gat));
+        DNDSMI_TRY(c->alloc(c->wall_idx, nfmax * ldc));
+        DNDSMI_TRY(c->upload(c->wall_idx, widx));
+        DNDSMI_TRY(c->alloc(c->M, int64_t(nfmax) * 81 * ldc));
+        DNDSMI_TRY(c->upload(c->M, M));
+        M.clear(), M.shrink_to_fit();
+        DNDSMI_TRY(c->alloc(c->bv, int64_t(nfmax) * 9 * ldc));
+        DNDSMI_TRY(c->upload(c->bv, bv));
+        DNDSMI_TRY(c->alloc(c->vol, ldc));
+        DNDSMI_TRY(c->upload(c->vol, vol));
+        // wall faces (Wall_Euler / Wall_NoSlip): compact arrays
+        const int64_t nw = int64_t(wall_cell.size());
+        c->n_wall = std::max<int64_t>(nw, 1);
+        {
+            std::vector<double> wa(81 * size_t(c->n_wall), 0.0), wd(27 * size_t(c->n_wall), 0.0),
+                wn(6 * size_t(c->n_wall), 0.0), w0(size_t(c->n_wall), 0.0);
+            for (int64_t w = 0; w < nw; w++) {
+                const int64_t i = wall_cell[w], f = wall_face[w];
+                for (int k = 0; k < 81; k++)
+                    wa[k * c->n_wall + w] = vr->ainv[81 * i + k];
+                for (int g = 0; g < 3; g++)
+                    for (int k = 0; k < 9; k++)
+                        wd[(g * 9 + k) * c->n_wall + w] = vr->face_d0[54 * f + 18 * g + k];
+                for (int k = 0; k < 6; k++)
+                    wn[k * c->n_wall + w] = vr->face_norm[6 * f + k];
+                w0[w] = vr->face_weight[10 * f];
+            }
+            DNDSMI_TRY(c->alloc(c->wall_ainv, 81 * c->n_wall));
+            DNDSMI_TRY(c->upload(c->wall_ainv, wa));
+            DNDSMI_TRY(c->alloc(c->wall_d0, 27 * c->n_wall));
+            DNDSMI_TRY(c->upload(c->wall_d0, wd));
+            DNDSMI_TRY(c->alloc(c->wall_nrm, 6 * c->n_wall));
+            DNDSMI_TRY(c->upload(c->wall_nrm, wn));
+            DNDSMI_TRY(c->alloc(c->wall_w0, c->n_wall));
+            DNDSMI_TRY(c->upload(c->wall_w0, w0));
+            std::vector<int32_t> wc(size_t(c->n_wall), 0), wb(size_t(c->n_wall), 0);
+            for (int64_t w = 0; w < nw; w++)
+                wc[w] = int32_t(wall_cell[w]), wb[w] = mesh->face_bc[wall_face[w]];
+            DNDSMI_TRY(c->alloc(c->wall_cell, c->n_wall));
+            DNDSMI_TRY(c->upload(c->wall_cell, wc));
+            DNDSMI_TRY(c->alloc(c->wall_bc, c->n_wall));
+            DNDSMI_TRY(c->upload(c->wall_bc, wb));
+            DNDSMI_TRY(c->alloc(c->wall_corr, 36 * c->n_wall));
+            c->n_wall_faces = nw;
+        }
+        // ---- face streams
+        std::vector<int32_t> f2c(2 * size_t(ldf), -1), fbc(ldf, DNDSMI_BC_INNER);
+        std::vector<double> fd0(54 * size_t(ldf), 0.0), fn(6 * size_t(ldf), 0.0), fnc(2 * size_t(ldf), 0.0),
+            farea(ldf, 0.0), fctr(2 * size_t(ldf), 0.0);
+        for (int64_t f = 0; f < nf; f++) {
+            f2c[f] = int32_t(mesh->face2cell[2 * f]);
+            const int64_t R = mesh->face2cell[2 * f + 1];
+            f2c[ldf + f] = R == DNDSMI_NO_CELL ? -1 : int32_t(R);
+            fbc[f] = mesh->face_bc[f];
+            for (int k = 0; k < 54; k++)
+                fd0[k * ldf + f] = vr->face_d0[54 * f + k];
+            for (int k = 0; k < 6; k++)
+                fn[k * ldf + f] = vr->face_norm[6 * f + k];
+            fnc[f] = vr->face_norm_center[2 * f], fnc[ldf + f] = vr->face_norm_center[2 * f + 1];
+            farea[f] = vr->face_area[f];
+            fctr[f] = vr->face_center[2 * f], fctr[ldf + f] = vr->face_center[2 * f + 1];
+        }
+        DNDSMI_TRY(c->alloc(c->f2c, 2 * ldf));
+        DNDSMI_TRY(c->upload(c->f2c, f2c));
+        DNDSMI_TRY(c->alloc(c->fbc, ldf));
+        DNDSMI_TRY(c->upload(c->fbc, fbc));
+        DNDSMI_TRY(c->alloc(c->fd0, 54 * ldf));
+        DNDSMI_TRY(c->upload(c->fd0, fd0));
+        DNDSMI_TRY(c->alloc(c->fn, 6 * ldf));
+        DNDSMI_TRY(c->upload(c->fn, fn));
+        DNDSMI_TRY(c->alloc(c->fnc, 2 * ldf));
+        DNDSMI_TRY(c->upload(c->fnc, fnc));
+        DNDSMI_TRY(c->alloc(c->farea, ldf));
+        DNDSMI_TRY(c->upload(c->farea, farea));
+        DNDSMI_TRY(c->alloc(c->fctr, 2 * ldf));
+        DNDSMI_TRY(c->upload(c->fctr, fctr));
+        std::vector<int32_t> bfl;
+        for (int64_t f = 0; f < nf; f++)
+            if (mesh->face2cell[2 * f + 1] == DNDSMI_NO_CELL)
+                bfl.push_back(int32_t(f));
+        c->n_bface = int64_t(bfl.size());
+        if (bfl.empty())
+            bfl.push_back(0);
+        DNDSMI_TRY(c->alloc(c->bfaces, int64_t(bfl.size())));
+        DNDSMI_TRY(c->upload(c->bfaces, bfl));
+        // ---- scratch
+        DNDSMI_TRY(c->alloc(c->lambda_face, ldf));
+        DNDSMI_TRY(c->alloc(c->dlambda_face, ldf));
+        DNDSMI_TRY(c->alloc(c->fflux, 12 * ldf));
+        DNDSMI_TRY(c->alloc(c->dt_part, int64_t(grid_for(nc)) + 1));
+        DNDSMI_TRY(c->alloc(c->dt_min, 1));
+        DNDSMI_TRY(c->alloc(c->errw, 1));
+        DNDSMI_TRY(c->alloc(c->xlast, 4 * ldc));
+        DNDSMI_TRY(c->alloc(c->dt_tmp, ldc));
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->errw);
+        hipLaunchKernelGGL(k_set, dim3(grid_for(ldf)), dim3(kBlock), 0, c->stream, ldf, c->dlambda_face, 0.0);
+        DNDSMI_TRY(c->launch_check("init kernels"));
+        DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    } catch (const std::bad_alloc &) {
+        return fail(DNDSMI_ERR_OUT_OF_MEMORY);
+    }
+    *out = c;
+    return DNDSMI_OK;
+}
+
+extern "C" void dndsmi_ctx_destroy(dndsmi_ctx *c) {
+    if (!c)
+        return;
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs)
+        (void)hipFree(p);
+    for (hipEvent_t e : c->ev_pool)
+        (void)hipEventDestroy(e);
+    if (c->own_stream && c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int dndsmi_last_error(const dndsmi_ctx *c, char *buf, size_t n) {
+    if (!c || !buf || n == 0)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    std::snprintf(buf, n, "%s", c->err.c_str());
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_sync(dndsmi_ctx *c) {
+    if (!c)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return check_errw(c);
+}
+
+extern "C" int64_t dndsmi_ctx_device_bytes(const dndsmi_ctx *c) { return c ? c->bytes : -1; }
+
+extern "C" int dndsmi_array_create(dndsmi_ctx *c, int32_t kind, dndsmi_array **out) {
+    if (!c || !out || kind < DNDSMI_ARRAY_DOF || kind > DNDSMI_ARRAY_SCALAR)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    dndsmi_array *a = new (std::nothrow) dndsmi_array();
+    if (!a)
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    a->ctx = c;
+    a->kind = kind;
+    a->elems = kind == DNDSMI_ARRAY_DOF ? 4 * c->ldc : (kind == DNDSMI_ARRAY_REC ? 36 * c->ldc : c->ldc);
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * a->elems) != hipSuccess) {
+        delete a;
+        c->err = "hipMalloc of an array failed";
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    }
+    a->d = static_cast<double *>(p);
+    hipLaunchKernelGGL(k_set, dim3(grid_for(a->elems)), dim3(kBlock), 0, c->stream, a->elems, a->d, 0.0);
+    int rc = c->launch_check("k_set");
+    if (rc) {
+        (void)hipFree(a->d);
+        delete a;
+        return rc;
+    }
+    *out = a;
+    return DNDSMI_OK;
+}
+
+extern "C" void dndsmi_array_destroy(dndsmi_array *a) {
+    if (!a)
+        return;
+    (void)hipStreamSynchronize(a->ctx->stream);
+    (void)hipFree(a->d);
+    delete a;
+}
+
+extern "C" int dndsmi_array_upload(dndsmi_array *a, const double *host) {
+    if (!a || !host)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    dndsmi_ctx *c = a->ctx;
+    const int64_t nc = c->nc, ldc = c->ldc;
+    if (a->kind == DNDSMI_ARRAY_REC) {
+        std::vector<double> buf(36 * size_t(ldc), 0.0);
+        for (int64_t i = 0; i < nc; i++)
+            for (int k = 0; k < 36; k++)
+                buf[k * ldc + i] = host[36 * i + k];
+        return c->upload(a->d, buf);
+    }
+    const int64_t n = a->kind == DNDSMI_ARRAY_DOF ? 4 * nc : nc;
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(a->d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream), "upload"));
+    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+extern "C" int dndsmi_array_download(const dndsmi_array *a, double *host) {
+    if (!a || !host)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    dndsmi_ctx *c = a->ctx;
+    const int64_t nc = c->nc, ldc = c->ldc;
+    if (a->kind == DNDSMI_ARRAY_REC) {
+        std::vector<double> buf(36 * size_t(ldc));
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(buf.data(), a->d, sizeof(double) * buf.size(), hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "download"));
+        DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+        for (int64_t i = 0; i < nc; i++)
+            for (int k = 0; k < 36; k++)
+                host[36 * i + k] = buf[k * ldc + i];
+        return DNDSMI_OK;
+    }
+    const int64_t n = a->kind == DNDSMI_ARRAY_DOF ? 4 * nc : nc;
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(host, a->d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "download"));
+    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+extern "C" int dndsmi_array_copy(dndsmi_array *dst, const dndsmi_array *src) {
+    if (!dst || !src || dst->ctx != src->ctx || dst->kind != src->kind)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    return dst->ctx->hip(hipMemcpyAsync(dst->d, src->d, sizeof(double) * dst->elems, hipMemcpyDeviceToDevice,
+                                        dst->ctx->stream),
+                         "copy");
+}
+
+extern "C" int dndsmi_array_set_constant(dndsmi_array *a, double value) {
+    if (!a)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(k_set, dim3(grid_for(a->elems)), dim3(kBlock), 0, a->ctx->stream, a->elems, a->d, value);
+    return a->ctx->launch_check("k_set");
+}
+
+namespace {
+
+int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t) {
+    const int nc = int(c->nc);
+    if (c->n_wall_faces > 0) {
+        prof_begin(c, DNDSMI_K_WALL_CORR);
+        hipLaunchKernelGGL(k_wall_corr, dim3(grid_for(c->n_wall_faces)), dim3(kBlock), 0, c->stream,
+                           int32_t(c->n_wall_faces), c->ldc, c->wall_cell, c->wall_bc, c->wall_ainv, c->wall_d0,
+                           c->wall_nrm, c->wall_w0, c->n_wall, u, x, c->relax, c->gas, t, c->wall_corr, c->errw);
+        prof_end(c);
+        DNDSMI_TRY(c->launch_check("k_wall_corr"));
+    }
+    prof_begin(c, DNDSMI_K_JACOBI);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_jacobi<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr, c->sbc,
+                           c->wall_idx, c->M, c->bv, u, x, xn, c->relax, c->wall_corr, c->n_wall, c->errw);
+    else
+        hipLaunchKernelGGL(k_jacobi<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr, c->sbc,
+                           c->wall_idx, c->M, c->bv, u, x, xn, c->relax, c->wall_corr, c->n_wall, c->errw);
+    prof_end(c);
+    return c->launch_check("k_jacobi");
+}
+
+int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double *dt_out) {
+    const int nc = int(c->nc), nf = int(c->nf);
+    prof_begin(c, DNDSMI_K_DT_FACE);
+    hipLaunchKernelGGL(k_dt_face, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->f2c, c->fnc, u,
+                       c->gas.gamma, c->lambda_face, c->dlambda_face, c->errw);
+    prof_end(c);
+    const unsigned g = grid_for(nc);
+    prof_begin(c, DNDSMI_K_DT_CELL);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_dt_cell<3>, dim3(g), dim3(kBlock), 0, c->stream, nc, c->ldc, c->gat, c->lambda_face,
+                           c->farea, c->vol, cfl, max_dt, dt_out, c->dt_part);
+    else
+        hipLaunchKernelGGL(k_dt_cell<4>, dim3(g), dim3(kBlock), 0, c->stream, nc, c->ldc, c->gat, c->lambda_face,
+                           c->farea, c->vol, cfl, max_dt, dt_out, c->dt_part);
+    prof_end(c);
+    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kBlock), 0, c->stream, int32_t(g), c->dt_part, c->dt_min, c->errw);
+    return c->launch_check("dt kernels");
+}
+
+int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
+               const double *dt, double coefA, double coefB) {
+    const int nc = int(c->nc), nf = int(c->nf);
+    prof_begin(c, DNDSMI_K_RHS_FACE);
+    hipLaunchKernelGGL(k_rhs_face<false>, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc,
+                       c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fflux,
+                       c->errw);
+    prof_end(c);
+    if (c->n_bface > 0)
+        hipLaunchKernelGGL(k_rhs_face<true>, dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
+                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr,
+                           c->dlambda_face, u, x, c->gas, t, c->fflux, c->errw);
+    prof_begin(c, DNDSMI_K_RHS_CELL);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_rhs_cell<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->gat,
+                           c->fflux, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB, c->errw);
+    else
+        hipLaunchKernelGGL(k_rhs_cell<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->gat,
+                           c->fflux, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB, c->errw);
+    prof_end(c);
+    return c->launch_check("rhs kernels");
+}
+
+} // namespace
+
+extern "C" int dndsmi_reconstruction_jacobi_step(dndsmi_ctx *c, const dndsmi_array *u, dndsmi_array *uRec,
+                                                 dndsmi_array *uRecNewBuf, double t) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNewBuf, DNDSMI_ARRAY_REC) || uRec == uRecNewBuf)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNewBuf->d, t));
+    // uRec = uRecNewBuf (DNDS_FV_VR.hpp:1707-1711) by exchanging the storage:
+    // uRec holds the new iterate, uRecNewBuf the previous one.
+    std::swap(uRec->d, uRecNewBuf->d);
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_evaluate_dt(dndsmi_ctx *c, dndsmi_array *dt, const dndsmi_array *u, double CFL,
+                                  double *dtMinall, double MaxDt, int32_t UseLocaldt) {
+    if (!c || !same_ctx(c, dt, DNDSMI_ARRAY_SCALAR) || !same_ctx(c, u, DNDSMI_ARRAY_DOF))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(launch_dt(c, u->d, CFL, MaxDt, dt->d));
+    if (!UseLocaldt) {
+        hipLaunchKernelGGL(k_fill_from, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, int32_t(c->nc), c->dt_min,
+                           dt->d);
+        DNDSMI_TRY(c->launch_check("k_fill_from"));
+    }
+    if (dtMinall) {
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(dtMinall, c->dt_min, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                          "hipMemcpyAsync"));
+        DNDSMI_TRY(check_errw(c));
+    }
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_evaluate_rhs(dndsmi_ctx *c, dndsmi_array *rhs, const dndsmi_array *u, const dndsmi_array *uRec,
+                                   double t) {
+    if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !same_ctx(c, u, DNDSMI_ARRAY_DOF) ||
+        !same_ctx(c, uRec, DNDSMI_ARRAY_REC))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    return launch_rhs(c, u->d, uRec->d, t, rhs->d, 0, nullptr, nullptr, 0, 0);
+}
+
+extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                                  int32_t n_sweeps, double CFL, double MaxDt, int32_t UseLocaldt, double t,
+                                  double *dt_out) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNew, DNDSMI_ARRAY_REC) || uRec == uRecNew || n_sweeps < 0)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    // fdt: EvaluateDt; the stage update reads dt_i (local) or the device minimum
+    DNDSMI_TRY(launch_dt(c, u->d, CFL, MaxDt, c->dt_tmp));
+    // the global minimum sets the stage times (one host read per step)
+    double dtmin = 0;
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(&dtmin, c->dt_min, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync"));
+    DNDSMI_TRY(check_errw(c));
+    if (dt_out)
+        *dt_out = dtmin;
+    const double Coef[3] = {1.0, 0.25, 2.0 / 3.0};
+    const double ct[3] = {1, 0.25, 2. / 3.};
+    for (int stage = 0; stage < 3; stage++) {
+        const double ts = t + ct[stage] * dtmin;
+        for (int k = 0; k < n_sweeps; k++) {
+            DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, ts));
+            std::swap(uRec->d, uRecNew->d);
+        }
+        const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
+        const double cB = stage ? 1 - Coef[stage] : 0.0;
+        DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,
+                              cB));
+    }
+    return DNDSMI_OK;
+}
+
+// ---- profiling ---------------------------------------------------------
+
+extern "C" int dndsmi_profile_enable(dndsmi_ctx *c, int32_t on) {
+    if (!c)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    c->prof_on = on != 0;
+    c->prof_recs.clear();
+    c->ev_used = 0;
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_profile_query(dndsmi_ctx *c, int32_t kernel, int64_t *count, double *total_ms) {
+    if (!c || !count || !total_ms || kernel < 0 || kernel >= DNDSMI_K_COUNT)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    int64_t n = 0;
+    double tot = 0;
+    for (auto &r : c->prof_recs) {
+        if (r.first != kernel)
+            continue;
+        float ms = 0;
+        DNDSMI_TRY(c->hip(hipEventElapsedTime(&ms, c->ev_pool[r.second], c->ev_pool[r.second + 1]),
+                          "hipEventElapsedTime"));
+        tot += ms;
+        n++;
+    }
+    *count = n;
+    *total_ms = tot;
+    return DNDSMI_OK;
+}
+
+// Algorithmic (unique) HBM bytes of one launch of a kernel: every stream the
+// kernel needs read once and every output written once, from mesh counts.
+extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
+    if (!c)
+        return -1;
+    const int64_t nc = c->nc, nf = c->nf, nif = nf - c->n_bface, nbf = c->n_bface;
+    int64_t c2f = 0; // sum of faces per cell = 2 nif + nbf
+    c2f = 2 * nif + nbf;
+    switch (kernel) {
+    case DNDSMI_K_JACOBI: // M + b per slot, slot neighbour index, x read + x_new write, u
+        return c2f * (81 + 9) * 8 + c2f * 4 + nc * (36 * 8 * 2 + 4 * 8);
+    case DNDSMI_K_DT_FACE: // f2c, centre normal, lambda + dlambda out; u once per cell
+        return nf * (8 + 16 + 16) + nc * 32;
+    case DNDSMI_K_DT_CELL: // gather index, lambda + area per slot, volume, dt out
+        return c2f * (4 + 16) + nc * (8 + 8);
+    case DNDSMI_K_RHS_FACE: // f2c, D0 rows, normals, dlambda, 12 flux values out; x and u once per cell
+        return nif * (8 + 54 * 8 + 48 + 8 + 96) + nc * (36 * 8 + 32);
+    case DNDSMI_K_RHS_CELL: // gather index, F + side fix per slot, volume, x read+write, xLast
+        return c2f * (4 + 64) + nc * (8 + 32 * 3);
+    default:
+        return 0;
+    }
+}

@@ -1,0 +1,30 @@
+// Internal host-side types shared by the mesh builder, the VR setup and the
+// device context.  Not part of the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dndsmi.h"
+
+struct dndsmi_mesh {
+    int64_t n_node = 0, n_cell = 0, n_face = 0;
+    std::vector<double> node_xy;      // [n_node][2]
+    std::vector<int64_t> cell_ptr;    // [n_cell+1]
+    std::vector<int64_t> cell2node;
+    std::vector<int64_t> cell2face;
+    std::vector<int64_t> face2node;   // [n_face][2]
+    std::vector<int64_t> face2cell;   // [n_face][2]
+    std::vector<int32_t> face2cell_ref;
+    std::vector<int32_t> face_bc;
+    std::vector<double> face_shift;   // [n_face][2]
+};
+
+struct dndsmi_vr {
+    int64_t n_cell = 0, n_face = 0, n_c2f = 0;
+    std::vector<double> volume, face_area, cell_bary, base_moment;
+    std::vector<double> face_center, face_norm_center, face_norm, face_weight;
+    std::vector<double> face_d0, face_dibj_g0, matrix_secondary;
+    std::vector<double> ainv, ainv_b, ainv_bvec;
+};

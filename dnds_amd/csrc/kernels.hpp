@@ -1,0 +1,387 @@
+// gfx950 device code of the FP64 hot path.
+//
+// Layouts (all device arrays padded to ld = n rounded up to 64):
+//   DOF arrays   u[cell][4]             (AoS, 32 B per cell)
+//   REC arrays   x[k][cell], k = r + 9 v (SoA: lane-contiguous per element)
+//   A^-1 B       M[(slot*81 + r*9 + c)][cell]
+//   A^-1 b       bv[(slot*9 + r)][cell]
+//   face rows    d0[(g*18 + side*9 + c)][face]
+// One thread owns one cell (sweep, gather) or one face (fluxes); the cell
+// gathers visit a cell's faces in ascending face index, which reproduces the
+// reference's face-loop scatter order without atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dndsmi_dev {
+
+constexpr double kVerySmallReal = 1e-200;
+constexpr double kLine3W0 = 5. / 9., kLine3W1 = 8. / 9., kLine3W2 = 5. / 9.;
+
+enum : int32_t {
+    BC_INNER = -1, BC_WALL = 1, BC_FARFIELD = 2, BC_WALL_EULER = 3, BC_WALL_NOSLIP = 4,
+    BC_DMRFAR = -2, BC_RTFAR = -3, BC_IVFAR = -4, BC_2DRIEMANNFAR = -5
+};
+// bits of the device error word
+enum : int32_t { ERRBIT_NONPOSITIVE = 1, ERRBIT_NAN = 2, ERRBIT_BC = 4 };
+
+__device__ inline bool is_far(int32_t bc) {
+    return bc == BC_FARFIELD || bc == BC_DMRFAR || bc == BC_RTFAR || bc == BC_IVFAR || bc == BC_2DRIEMANNFAR;
+}
+
+__device__ inline double sqr(double x) { return x * x; }
+
+struct GasParams {
+    double gamma;
+    double far[4];
+    int32_t rs; // 1 Roe, 2 HLLC
+};
+
+// ---------------------------------------------------------------- gas ----
+__device__ inline void ideal_gas_thermal(double E, double rho, double vSqr, double gamma, double &p, double &asqr,
+                                         double &H) {
+    p = (gamma - 1) * (E - rho * 0.5 * vSqr);
+    asqr = gamma * p / rho;
+    H = (E + p) / rho;
+}
+
+__device__ inline void cons2prim(const double U[4], double prim[4], double gamma) {
+    for (int k = 0; k < 4; k++)
+        prim[k] = U[k] / U[0];
+    double v0 = U[1] / U[0], v1 = U[2] / U[0];
+    double vSqr = v0 * v0 + v1 * v1;
+    double p = (gamma - 1) * (U[3] - U[0] * 0.5 * vSqr);
+    prim[0] = U[0];
+    prim[3] = p;
+}
+
+__device__ inline void prim2cons(const double prim[4], double U[4], double gamma) {
+    for (int k = 0; k < 4; k++)
+        U[k] = prim[k] * prim[0];
+    double vSqr = prim[1] * prim[1] + prim[2] * prim[2];
+    double E = prim[3] / (gamma - 1) + prim[0] * 0.5 * vSqr;
+    U[0] = prim[0];
+    U[3] = E;
+}
+
+__device__ inline void inviscid_flux(const double U[4], double vx, double p, double F[4]) {
+    for (int k = 0; k < 4; k++)
+        F[k] = U[k] * vx;
+    F[1] += p;
+    F[3] += vx * p;
+}
+
+// Roe flux with the Harten-Yee entropy fix, face frame (DNDS_Gas.hpp:405-603)
+__device__ inline bool roe_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4]) {
+    if (!(UL[0] > 0 && UR[0] > 0))
+        return false;
+    const double vL0 = UL[1] / UL[0], vL1 = UL[2] / UL[0];
+    const double vR0 = UR[1] / UR[0], vR1 = UR[2] / UR[0];
+    double asqrL, asqrR, pL, pR, HL, HR;
+    ideal_gas_thermal(UL[3], UL[0], vL0 * vL0 + vL1 * vL1, gamma, pL, asqrL, HL);
+    ideal_gas_thermal(UR[3], UR[0], vR0 * vR0 + vR1 * vR1, gamma, pR, asqrR, HR);
+    const double sL = sqrt(UL[0]), sR = sqrt(UR[0]);
+    const double vRoe0 = (sL * vL0 + sR * vR0) / (sL + sR);
+    const double vRoe1 = (sL * vL1 + sR * vR1) / (sL + sR);
+    const double vsqrRoe = vRoe0 * vRoe0 + vRoe1 * vRoe1;
+    const double HRoe = (sL * HL + sR * HR) / (sL + sR);
+    const double asqrRoe = (gamma - 1) * (HRoe - 0.5 * vsqrRoe);
+    double FL[4], FR[4];
+    inviscid_flux(UL, vL0, pL, FL);
+    inviscid_flux(UR, vR0, pR, FR);
+    if (!(asqrRoe > 0))
+        return false;
+    const double aRoe = sqrt(asqrRoe);
+    double lam0 = fabs(vRoe0 - aRoe);
+    const double lam123 = fabs(vRoe0);
+    double lam4 = fabs(vRoe0 + aRoe);
+    const double thr = fmax(0.05 * (sqrt(vsqrRoe) + aRoe), dLambda);
+    const double thrS = thr * thr;
+    if (lam0 < thr)
+        lam0 = (sqr(lam0) + thrS) / (2 * thr);
+    if (lam4 < thr)
+        lam4 = (sqr(lam4) + thrS) / (2 * thr);
+    double inc[4];
+    for (int k = 0; k < 4; k++)
+        inc[k] = UR[k] - UL[k];
+    double a2 = inc[2] - vRoe1 * inc[0];
+    double inc4b = inc[3] - a2 * vRoe1;
+    double a1 = (gamma - 1) / asqrRoe * (inc[0] * (HRoe - vRoe0 * vRoe0) + vRoe0 * inc[1] - inc4b);
+    double a0 = (inc[0] * (vRoe0 + aRoe) - inc[1] - aRoe * a1) / (2 * aRoe);
+    double a3 = inc[0] - (a0 + a1);
+    const double la0 = lam0 * a0, la1 = lam123 * a1, la2 = lam123 * a2, la3 = lam4 * a3;
+    // right eigenvectors (EulerGasRightEigenVector), rows dotted with lam*alpha in column order
+    double r;
+    r = 0.0;
+    r += 1.0 * la0, r += 1.0 * la1, r += 0.0 * la2, r += 1.0 * la3;
+    F[0] = (FL[0] + FR[0]) * 0.5 - 0.5 * r;
+    r = 0.0;
+    r += (vRoe0 - aRoe) * la0, r += vRoe0 * la1, r += 0.0 * la2, r += (vRoe0 + aRoe) * la3;
+    F[1] = (FL[1] + FR[1]) * 0.5 - 0.5 * r;
+    r = 0.0;
+    r += vRoe1 * la0, r += vRoe1 * la1, r += 1.0 * la2, r += vRoe1 * la3;
+    F[2] = (FL[2] + FR[2]) * 0.5 - 0.5 * r;
+    r = 0.0;
+    r += (HRoe - vRoe0 * aRoe) * la0, r += (0.5 * vsqrRoe) * la1, r += vRoe1 * la2, r += (HRoe + vRoe0 * aRoe) * la3;
+    F[3] = (FL[3] + FR[3]) * 0.5 - 0.5 * r;
+    return true;
+}
+
+// HLLC flux with Einfeldt speeds, face frame (DNDS_Gas.hpp:312-403)
+__device__ inline bool hllc_flux(const double UL[4], const double UR[4], double gamma, double F[4]) {
+    if (!(UL[0] > 0 && UR[0] > 0))
+        return false;
+    const double vL0 = UL[1] / UL[0], vL1 = UL[2] / UL[0];
+    const double vR0 = UR[1] / UR[0], vR1 = UR[2] / UR[0];
+    double asqrL, asqrR, pL, pR, HL, HR;
+    ideal_gas_thermal(UL[3], UL[0], vL0 * vL0 + vL1 * vL1, gamma, pL, asqrL, HL);
+    ideal_gas_thermal(UR[3], UR[0], vR0 * vR0 + vR1 * vR1, gamma, pR, asqrR, HR);
+    const double sL = sqrt(UL[0]), sR = sqrt(UR[0]);
+    const double vRoe0 = (sL * vL0 + sR * vR0) / (sL + sR);
+    const double eta2 = 0.5 * (sL * sR) / sqr(sL + sR);
+    const double dsqr = (asqrL * sL + asqrR * sR) / (sL + sR) + eta2 * sqr(vR0 - vL0);
+    if (!(dsqr > 0))
+        return false;
+    const double SL = vRoe0 - sqrt(dsqr);
+    const double SR = vRoe0 + sqrt(dsqr);
+    double FL[4], FR[4];
+    inviscid_flux(UL, vL0, pL, FL);
+    inviscid_flux(UR, vR0, pR, FR);
+    if (0 <= SL) {
+        for (int k = 0; k < 4; k++)
+            F[k] = FL[k];
+        return true;
+    }
+    if (SR <= 0) {
+        for (int k = 0; k < 4; k++)
+            F[k] = FR[k];
+        return true;
+    }
+    double SS = 0;
+    const double div = (UL[0] * (SL - vL0) - UR[0] * (SR - vR0));
+    if (fabs(div) > kVerySmallReal)
+        SS = (pR - pL + UL[1] * (SL - vL0) - UR[1] * (SR - vR0)) / div;
+    const double DS[4] = {0, 1, 0, SS};
+    if (SS >= 0) {
+        const double dv = SL - SS;
+        if (fabs(dv) < kVerySmallReal) {
+            for (int k = 0; k < 4; k++)
+                F[k] = FL[k];
+        } else {
+            const double X = (pL + UL[0] * (SL - vL0) * (SS - vL0)) * SL;
+            for (int k = 0; k < 4; k++)
+                F[k] = ((UL[k] * SL - FL[k]) * SS + DS[k] * X) / dv;
+        }
+    } else {
+        const double dv = SR - SS;
+        if (fabs(dv) < kVerySmallReal) {
+            for (int k = 0; k < 4; k++)
+                F[k] = FR[k];
+        } else {
+            const double X = (pR + UR[0] * (SR - vR0) * (SS - vR0)) * SR;
+            for (int k = 0; k < 4; k++)
+                F[k] = ((UR[k] * SR - FR[k]) * SS + DS[k] * X) / dv;
+        }
+    }
+    return true;
+}
+
+// generateBoundaryValue (DNDS_FV_EulerEvaluator.hpp:958-1275); returns false
+// on a state or position the reference asserts on.
+__device__ inline bool boundary_value(const GasParams &gp, const double UL[4], double n0, double n1, double px,
+                                      double py, double t, int32_t bc, double UR[4]) {
+    if (!(UL[0] > 0))
+        return false;
+    const double gamma = gp.gamma;
+    if (bc == BC_FARFIELD) {
+        const double un = (UL[1] * n0 + UL[2] * n1) / UL[0];
+        const double vsqr = sqr(UL[1] / UL[0]) + sqr(UL[2] / UL[0]);
+        double asqr, H, p;
+        ideal_gas_thermal(UL[3], UL[0], vsqr, gamma, p, asqr, H);
+        if (!(asqr >= 0))
+            return false;
+        const double a = sqrt(asqr);
+        if (un - a > 0) {
+            for (int k = 0; k < 4; k++)
+                UR[k] = UL[k];
+        } else if (un > 0) {
+            double fp[4], lp[4];
+            cons2prim(gp.far, fp, gamma);
+            cons2prim(UL, lp, gamma);
+            lp[3] = fp[3];
+            prim2cons(lp, UR, gamma);
+        } else if (un + a > 0) {
+            double fp[4], lp[4];
+            cons2prim(gp.far, fp, gamma);
+            cons2prim(UL, lp, gamma);
+            fp[3] = lp[3];
+            prim2cons(fp, UR, gamma);
+        } else {
+            for (int k = 0; k < 4; k++)
+                UR[k] = gp.far[k];
+        }
+        return true;
+    }
+    if (bc == BC_DMRFAR) {
+        const double pi = 3.141592653589793;
+        if (((px - 10.0 / sin(pi / 3) * t - 1. / 6.) - py / tan(pi / 3)) > 0) {
+            UR[0] = 1.4, UR[1] = 0, UR[2] = 0, UR[3] = 2.5;
+        } else {
+            UR[0] = 8, UR[1] = 57.157676649772960, UR[2] = -33, UR[3] = 5.635e2;
+        }
+        return true;
+    }
+    if (bc == BC_RTFAR) {
+        const double pi = 3.141592653589793;
+        const double vsqr = sqr(UL[1] / UL[0]) + sqr(UL[2] / UL[0]);
+        double asqr, H, p;
+        ideal_gas_thermal(UL[3], UL[0], vsqr, gamma, p, asqr, H);
+        if (!(asqr >= 0))
+            return false;
+        const double a = sqrt(asqr);
+        const double v = -0.025 * a * cos(px * 8 * pi);
+        const double rho = py < 0.5 ? 2 : 1;
+        const double pr = py < 0.5 ? 1 : 2.5;
+        UR[0] = rho, UR[1] = 0, UR[2] = rho * v, UR[3] = 0.5 * rho * sqr(v) + pr / (gamma - 1);
+        return true;
+    }
+    if (bc == BC_IVFAR) {
+        const double pi = 3.141592653589793;
+        const double chi = 5, xc = 5 + t, yc = 5 + t;
+        const double r = sqrt(sqr(px - xc) + sqr(py - yc));
+        const double dT = -(gamma - 1) / (8 * gamma * sqr(pi)) * sqr(chi) * exp(1 - sqr(r));
+        const double dux = chi / 2 / pi * exp((1 - sqr(r)) / 2) * -(py - xc);
+        const double duy = chi / 2 / pi * exp((1 - sqr(r)) / 2) * +(px - yc);
+        const double T = dT + 1, ux = dux + 1, uy = duy + 1;
+        const double rho = pow(T / 1.0, 1 / (gamma - 1));
+        const double pr = T * rho;
+        UR[0] = rho, UR[1] = rho * ux, UR[2] = rho * uy;
+        UR[3] = 0.5 * (sqr(ux) + sqr(uy)) * rho + pr / (gamma - 1);
+        return true;
+    }
+    if (bc == BC_2DRIEMANNFAR) {
+        const double phi2 = -0.422115882408869;
+        const double p2 = 0.8 + phi2 * t;
+        double lp[4];
+        cons2prim(UL, lp, gamma);
+        double fp[4] = {lp[0], lp[1], lp[2], lp[3]};
+        const double bTol = 1e-9;
+        double rho, u, v, pre;
+        if (fabs(px - 0.0) < bTol) {
+            if (py <= p2)
+                rho = 0.137992831541219, u = 1.206045378311055, v = 1.206045378311055, pre = 0.029032258064516;
+            else
+                rho = 0.532258064516129, u = 1.206045378311055, v = 0.0, pre = 0.3;
+        } else if (fabs(px - 1.0) < bTol) {
+            rho = lp[0], u = -lp[1], v = lp[2], pre = lp[3];
+        } else if (fabs(py - 1.0) < bTol) {
+            rho = lp[0], u = lp[1], v = -lp[2], pre = lp[3];
+        } else if (fabs(py - 0.0) < bTol) {
+            if (px <= p2)
+                rho = 0.137992831541219, u = 1.206045378311055, v = 1.206045378311055, pre = 0.029032258064516;
+            else
+                rho = 0.532258064516129, u = 0.0, v = 1.206045378311055, pre = 0.3;
+        } else {
+            return false;
+        }
+        fp[0] = rho, fp[1] = u, fp[2] = v, fp[3] = pre;
+        prim2cons(fp, UR, gamma);
+        return true;
+    }
+    if (bc == BC_WALL_EULER) {
+        for (int k = 0; k < 4; k++)
+            UR[k] = UL[k];
+        const double d = UR[1] * n0 + UR[2] * n1;
+        UR[1] -= d * n0;
+        UR[2] -= d * n1;
+        return true;
+    }
+    if (bc == BC_WALL_NOSLIP) {
+        for (int k = 0; k < 4; k++)
+            UR[k] = UL[k];
+        UR[1] *= -1;
+        UR[2] *= -1;
+        return true;
+    }
+    return false;
+}
+
+// CompressRecPart (DNDS_FV_EulerEvaluator.hpp:1277-1360)
+__device__ inline void compress_rec_part(const double um[4], const double inc[4], double ret[4]) {
+    for (int k = 0; k < 4; k++)
+        ret[k] = um[k] + inc[k];
+    if (ret[0] < 0)
+        for (int k = 0; k < 4; k++)
+            ret[k] = um[k];
+    const double eK = (ret[1] * ret[1] + ret[2] * ret[2]) * 0.5 / (kVerySmallReal + ret[0]);
+    const double e = ret[3] - eK;
+    if (e < 0)
+        for (int k = 0; k < 4; k++)
+            ret[k] = um[k];
+}
+
+__device__ inline void normalized(double x, double y, double &ox, double &oy) {
+    const double z = x * x + y * y;
+    if (z > 0) {
+        const double s = sqrt(z);
+        ox = x / s, oy = y / s;
+    } else
+        ox = x, oy = y;
+}
+
+__device__ inline void stable_normalized(double x, double y, double &ox, double &oy) {
+    const double w = fmax(fabs(x), fabs(y));
+    const double zx = x / w, zy = y / w;
+    const double z = zx * zx + zy * zy;
+    if (z > 0) {
+        const double s = sqrt(z);
+        ox = (x / s) / w, oy = (y / s) / w;
+    } else
+        ox = x, oy = y;
+}
+
+// fluxFace (DNDS_FV_EulerEvaluator.hpp:371-591) for muGas = 0: the Riemann
+// flux in the face frame, rotated back and negated, and the flux-balance
+// fluxes of the two cell means.
+__device__ inline bool flux_face(const GasParams &gp, const double ULxy[4], const double URxy[4],
+                                 const double ULm[4], const double URm[4], double n0, double n1, double dLambda,
+                                 double out[4], double FLfix[4], double FRfix[4]) {
+    auto rotT = [&](const double in[4], double o[4]) {
+        o[0] = in[0];
+        o[1] = n0 * in[1] + n1 * in[2];
+        o[2] = (-n1) * in[1] + n0 * in[2];
+        o[3] = in[3];
+    };
+    auto rot = [&](double v[4]) {
+        const double a = n0 * v[1] + (-n1) * v[2];
+        const double b = n1 * v[1] + n0 * v[2];
+        v[1] = a, v[2] = b;
+    };
+    double UL[4], UR[4];
+    rotT(ULxy, UL);
+    rotT(URxy, UR);
+    {
+        double Lm[4], Rm[4], wL[4], wR[4];
+        rotT(ULm, Lm);
+        rotT(URm, Rm);
+        if (!(Lm[0] > 0) || !(Rm[0] > 0))
+            return false;
+        cons2prim(Lm, wL, gp.gamma);
+        cons2prim(Rm, wR, gp.gamma);
+        inviscid_flux(Lm, wL[1], wL[3], FLfix);
+        inviscid_flux(Rm, wR[1], wR[3], FRfix);
+        rot(FLfix);
+        rot(FRfix);
+    }
+    double f[4];
+    const bool ok = gp.rs == 2 ? hllc_flux(UL, UR, gp.gamma, f) : roe_flux(UL, UR, gp.gamma, dLambda, f);
+    if (!ok)
+        return false;
+    rot(f);
+    for (int k = 0; k < 4; k++)
+        out[k] = -f[k];
+    return true;
+}
+
+} // namespace dndsmi_dev

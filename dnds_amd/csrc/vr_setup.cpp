@@ -1,0 +1,813 @@
+// Host FP64 geometry and VR setup: the ImplicitFiniteVolume2D constructor
+// (DNDS_FV_VR.hpp:29-102) and VRFiniteVolume2D::Initialization
+// (DNDS_FV_VR.cpp:3-1154) for p = 3 with the global-coordinate basis.
+//
+// The reference keeps 10x10 basis-derivative matrices at every face and cell
+// quadrature point (DNDS_FV_VR.cpp:195-532), about 19 KB per triangle.  Here
+// the per-face Gauss-point matrices are recomputed where they are consumed and
+// only the rows each kernel reads are kept (face_d0, face_dibj_g0).  The
+// arithmetic of every kept quantity follows the reference's expressions and
+// summation order; dense products use ascending index order.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <thread>
+
+#include "internal.hpp"
+
+namespace {
+
+// ---- quadrature tables (DNDS_Elements.hpp:325-404) ----------------------
+// LINE rules on [-1, 1], TRI rules on the unit right triangle, QUAD on [-1,1]^2.
+const double kLine3X[3] = {-0.7745966692414833, 0, 0.7745966692414833};
+const double kLine3W[3] = {5. / 9., 8. / 9., 5. / 9.};
+const double kLine5X[5] = {-0.906179845938664, -0.538469310105683, 0, 0.538469310105683, 0.906179845938664};
+const double kLine5W[5] = {0.236926885056188, 0.478628670499366, 0.568888888888889, 0.478628670499366,
+                           0.236926885056190};
+const double kQuad9X[9] = {-0.7745966692414833, 0, 0.7745966692414833, -0.7745966692414833, 0,
+                           0.7745966692414833, -0.7745966692414833, 0, 0.7745966692414833};
+const double kQuad9Y[9] = {-0.7745966692414833, -0.7745966692414833, -0.7745966692414833, 0, 0, 0,
+                           0.7745966692414833, 0.7745966692414833, 0.7745966692414833};
+const double kQuad9W[9] = {25. / 81., 40. / 81., 25. / 81., 40. / 81., 64. / 81.,
+                           40. / 81., 25. / 81., 40. / 81., 25. / 81.};
+const double kTri4X[4] = {1. / 3., 0.6, 0.2, 0.2};
+const double kTri4Y[4] = {1. / 3., 0.2, 0.6, 0.2};
+const double kTri4W[4] = {-27. / 96., 25. / 96., 25. / 96., 25. / 96.};
+const double kT7A1 = 0.059715871789770, kT7B1 = 0.470142064105115, kT7W1 = 0.132394152788506 * 0.5;
+const double kT7A2 = 0.797426985353087, kT7B2 = 0.101286507323456, kT7W2 = 0.125939180544827 * 0.5;
+const double kTri7X[7] = {1. / 3., kT7A1, kT7B1, kT7B1, kT7A2, kT7B2, kT7B2};
+const double kTri7Y[7] = {1. / 3., kT7B1, kT7A1, kT7B1, kT7B2, kT7A2, kT7B2};
+const double kTri7W[7] = {9. / 80., kT7W1, kT7W1, kT7W1, kT7W2, kT7W2, kT7W2};
+
+// 2D derivative order list (diffOperatorOrderList2D, DNDS_Elements.hpp:469-480)
+const int kDiffX[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
+const int kDiffY[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+const int kNCombs[10] = {1, 1, 1, 1, 2, 1, 1, 3, 3, 1};     // diffNCombs2D
+const int kFactorial[10] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880};
+const int kDFact[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 2, 0}, {1, 3, 6, 6}}; // dFactorials
+const double kDirWeight3HQM[4] = {1, .5295, .2117 * 1.0, .2117 * 1.0};             // dirWeight3_HQM (sqrt(1.) = 1)
+
+constexpr double kVerySmallReal = 1e-200;
+
+inline double sqr(double x) { return x * x; }
+
+struct Cell {
+    int nv = 0;
+    double x[4] = {0, 0, 0, 0}, y[4] = {0, 0, 0, 0};
+};
+
+// Shape functions and first parametric derivatives (GetDiNj, DNDS_Elements.hpp:992-1090).
+inline void shape(int nv, double xi, double eta, double N[4], double Nx[4], double Ny[4]) {
+    if (nv == 3) {
+        N[0] = 1 - xi - eta, N[1] = xi, N[2] = eta, N[3] = 0;
+        Nx[0] = -1, Nx[1] = 1, Nx[2] = 0, Nx[3] = 0;
+        Ny[0] = -1, Ny[1] = 0, Ny[2] = 1, Ny[3] = 0;
+    } else {
+        N[0] = (1 - xi) * (1 - eta) * 0.25;
+        N[1] = (1 + xi) * (1 - eta) * 0.25;
+        N[2] = (1 + xi) * (1 + eta) * 0.25;
+        N[3] = (1 - xi) * (1 + eta) * 0.25;
+        Nx[0] = (-1) * (1 - eta) * 0.25;
+        Nx[1] = (1) * (1 - eta) * 0.25;
+        Nx[2] = (1) * (1 + eta) * 0.25;
+        Nx[3] = (-1) * (1 + eta) * 0.25;
+        Ny[0] = (1 - xi) * (-1) * 0.25;
+        Ny[1] = (1 + xi) * (-1) * 0.25;
+        Ny[2] = (1 + xi) * (1) * 0.25;
+        Ny[3] = (1 - xi) * (1) * 0.25;
+    }
+}
+
+// point and Jacobian determinant at a parametric point (DiNj2Jacobi, DNDS_Elements.hpp:1441-1445)
+inline void map_point(const Cell &c, double xi, double eta, double &px, double &py, double &detJ) {
+    double N[4], Nx[4], Ny[4];
+    shape(c.nv, xi, eta, N, Nx, Ny);
+    px = 0, py = 0;
+    double j00 = 0, j01 = 0, j10 = 0, j11 = 0;
+    for (int n = 0; n < c.nv; n++) {
+        px += c.x[n] * N[n];
+        py += c.y[n] * N[n];
+        j00 += Nx[n] * c.x[n];
+        j01 += Nx[n] * c.y[n];
+        j10 += Ny[n] * c.x[n];
+        j11 += Ny[n] * c.y[n];
+    }
+    detJ = j00 * j11 - j10 * j01;
+}
+
+// face normal (dy/dxi, -dx/dxi) of a straight 2-node face (Jacobi2LineNorm2D, DNDS_Elements.hpp:1447-1454)
+inline void line_norm(const double fx[2], const double fy[2], double &nx, double &ny) {
+    double j00 = (-1) * 0.5 * fx[0] + (1) * 0.5 * fx[1];
+    double j01 = (-1) * 0.5 * fy[0] + (1) * 0.5 * fy[1];
+    nx = j01, ny = -j00;
+}
+
+inline double norm2(double x, double y) { return std::sqrt(x * x + y * y); }
+
+// Eigen's stableNormalized for a 2-vector (z component 0)
+inline void stable_normalized(double x, double y, double &ox, double &oy) {
+    double w = std::max(std::fabs(x), std::fabs(y));
+    double zx = x / w, zy = y / w;
+    double z = zx * zx + zy * zy;
+    if (z > 0) {
+        double s = std::sqrt(z);
+        ox = (x / s) / w, oy = (y / s) / w;
+    } else
+        ox = x, oy = y;
+}
+
+// FaceSpace2VolSpace (DNDS_Elements.hpp:845-941): parametric point of a face
+// point xi in the cell, for the cell's local face `lf`, `inv` = face node
+// order opposite to the cell's local order.
+inline void face2vol(int nv, int lf, bool inv, double xi, double &px, double &py) {
+    if (nv == 3) {
+        if (!inv) {
+            switch (lf) {
+            case 0: px = 0.5 * (xi + 1), py = 0.; break;
+            case 1: px = 0.5 * (-xi + 1), py = 0.5 * (xi + 1); break;
+            default: px = 0, py = 0.5 * (-xi + 1); break;
+            }
+        } else {
+            switch (lf) {
+            case 0: px = 0.5 * (-xi + 1), py = 0.; break;
+            case 1: px = 0.5 * (xi + 1), py = 0.5 * (-xi + 1); break;
+            default: px = 0, py = 0.5 * (xi + 1); break;
+            }
+        }
+    } else {
+        if (!inv) {
+            switch (lf) {
+            case 0: px = xi, py = -1; break;
+            case 1: px = 1, py = xi; break;
+            case 2: px = -xi, py = 1; break;
+            default: px = -1, py = -xi; break;
+            }
+        } else {
+            switch (lf) {
+            case 0: px = -xi, py = -1; break;
+            case 1: px = 1, py = -xi; break;
+            case 2: px = xi, py = 1; break;
+            default: px = -1, py = xi; break;
+            }
+        }
+    }
+}
+
+struct Ctx {
+    const dndsmi_mesh_view *m;
+    const dndsmi_vr_setting *s;
+    dndsmi_vr *out;
+    std::vector<double> scale; // [n_cell][2] CoordMinMaxScale (after the isotropic option)
+};
+
+Cell load_cell(const dndsmi_mesh_view &m, int64_t ic, double shx = 0, double shy = 0) {
+    Cell c;
+    int64_t b = m.cell_ptr[ic];
+    c.nv = int(m.cell_ptr[ic + 1] - b);
+    for (int n = 0; n < c.nv; n++) {
+        int64_t nd = m.cell2node[b + n];
+        c.x[n] = m.node_xy[2 * nd] + shx;
+        c.y[n] = m.node_xy[2 * nd + 1] + shy;
+    }
+    return c;
+}
+
+// FDiffBaseValue, global coordinates (DNDS_FV_VR.hpp:300-356): D[idiff][ibase],
+// 10x10, derivatives of the scaled cubic monomials about (cx, cy), row 0
+// minus the base moments.
+void diff_base(double px, double py, double cx, double cy, double sx, double sy, const double *moment, double D[10][10]) {
+    double X = (px - cx) / sx;
+    double Y = (py - cy) / sy;
+    double powX[4], powY[4];
+    for (int k = 0; k < 4; k++) {
+        powX[k] = std::pow(X, double(k));
+        powY[k] = std::pow(Y, double(k));
+    }
+    double sPowX[4], sPowY[4];
+    for (int k = 0; k < 4; k++) {
+        sPowX[k] = std::pow(sx, double(k));
+        sPowY[k] = std::pow(sy, double(k));
+    }
+    for (int id = 0; id < 10; id++) {
+        const int ndx = kDiffX[id], ndy = kDiffY[id];
+        const double den = sPowX[ndx] * sPowY[ndy] * 1.0; // pow(1, ndz) = 1
+        for (int ib = 0; ib < 10; ib++) {
+            const int bx = kDiffX[ib], by = kDiffY[ib];
+            // FPolynomial3D (DNDS_FV_VR.hpp:254-259); z = 0 with pz = dz = 0 gives pow(0, 0) = 1
+            int c = (ndx <= bx ? kDFact[bx][ndx] : 0) * (ndy <= by ? kDFact[by][ndy] : 0) * 1;
+            double v = c ? double(c) * powX[bx - ndx] * powY[by - ndy] * 1.0 : 0.;
+            D[id][ib] = v / den;
+        }
+    }
+    for (int ib = 0; ib < 10; ib++)
+        D[0][ib] -= moment[ib];
+}
+
+// Scale factors of FDiffBaseValue: CoordMinMaxScale, made isotropic unless anisotropicLengths.
+void cell_scale(const Cell &c, bool aniso, double &sx, double &sy) {
+    double xmin = c.x[0], xmax = c.x[0], ymin = c.y[0], ymax = c.y[0];
+    for (int n = 1; n < c.nv; n++) {
+        xmin = std::min(xmin, c.x[n]), xmax = std::max(xmax, c.x[n]);
+        ymin = std::min(ymin, c.y[n]), ymax = std::max(ymax, c.y[n]);
+    }
+    sx = (xmax - xmin) * 0.5, sy = (ymax - ymin) * 0.5;
+    if (!aniso) {
+        double n = std::sqrt(sx * sx + sy * sy + 0.0);
+        sx = sy = n;
+    }
+}
+
+// The cell and its parametric point for side `side` of face f at face coordinate xi.
+struct FaceSide {
+    Cell c;
+    int64_t cell;
+    double cx, cy; // base centre (barycentre, shifted with the cell)
+};
+
+FaceSide face_side(const Ctx &C, int64_t f, int side) {
+    const dndsmi_mesh_view &m = *C.m;
+    FaceSide fs;
+    fs.cell = m.face2cell[2 * f + side];
+    double shx = side ? m.face_shift[2 * f] : 0, shy = side ? m.face_shift[2 * f + 1] : 0;
+    fs.c = load_cell(m, fs.cell, shx, shy);
+    fs.cx = C.out->cell_bary[2 * fs.cell] + shx;
+    fs.cy = C.out->cell_bary[2 * fs.cell + 1] + shy;
+    return fs;
+}
+
+// Does the face's node order run against the cell's local face order?
+bool face_inverted(const Ctx &C, int64_t f, int side) {
+    const dndsmi_mesh_view &m = *C.m;
+    const int64_t ic = m.face2cell[2 * f + side];
+    const int64_t b = m.cell_ptr[ic];
+    const int nv = int(m.cell_ptr[ic + 1] - b);
+    const int lf = m.face2cell_ref[2 * f + side];
+    const int64_t s0 = m.cell2node[b + lf];
+    const int64_t f0 = m.face2node[2 * f];
+    if (side == 0 || (m.face_shift[2 * f] == 0 && m.face_shift[2 * f + 1] == 0))
+        return s0 != f0;
+    // periodic face: match the shifted node positions
+    const int64_t f1 = m.face2node[2 * f + 1];
+    double sx = m.node_xy[2 * s0] + m.face_shift[2 * f], sy = m.node_xy[2 * s0 + 1] + m.face_shift[2 * f + 1];
+    double d0 = norm2(sx - m.node_xy[2 * f0], sy - m.node_xy[2 * f0 + 1]);
+    double d1 = norm2(sx - m.node_xy[2 * f1], sy - m.node_xy[2 * f1 + 1]);
+    (void)nv;
+    return !(d0 < d1);
+}
+
+// 10x10 DiBj of side `side` of face f at face coordinate xi.
+void face_dibj(const Ctx &C, int64_t f, int side, double xi, double D[10][10]) {
+    const dndsmi_mesh_view &m = *C.m;
+    FaceSide fs = face_side(C, f, side);
+    double pxi, peta;
+    face2vol(fs.c.nv, m.face2cell_ref[2 * f + side], face_inverted(C, f, side), xi, pxi, peta);
+    double px, py, detJ;
+    map_point(fs.c, pxi, peta, px, py, detJ);
+    diff_base(px, py, fs.cx, fs.cy, C.scale[2 * fs.cell], C.scale[2 * fs.cell + 1],
+              &C.out->base_moment[10 * fs.cell], D);
+}
+
+inline bool is_far(int32_t bc) {
+    return bc == DNDSMI_BC_FARFIELD || bc == DNDSMI_BC_SPECIAL_DMRFAR || bc == DNDSMI_BC_SPECIAL_RTFAR ||
+           bc == DNDSMI_BC_SPECIAL_IVFAR || bc == DNDSMI_BC_SPECIAL_2DRIEMANNFAR;
+}
+inline bool is_wall(int32_t bc) {
+    return bc == DNDSMI_BC_WALL || bc == DNDSMI_BC_WALL_EULER || bc == DNDSMI_BC_WALL_NOSLIP;
+}
+
+// Projections of the columns of a 10-row derivative block used by the norm
+// functional (FFaceFunctional with USE_NORM_FUNCTIONAL, DNDS_FV_VR.hpp:1071-1210):
+// P[k][col] for the 10 terms, in the reference's evaluation order.
+struct FFParams {
+    double W0, n0, n1, t0, t1, w2r, w3r;
+};
+
+FFParams ff_params(const Ctx &C, int64_t f, int g, const double *W) {
+    FFParams p;
+    const double epsR = 1e-50;
+    double w2r, w3r, length;
+    if (std::fabs(W[1]) >= epsR || std::fabs(W[2]) >= epsR) {
+        w2r = std::fabs(W[1]) >= std::fabs(W[2]) ? W[3] / (W[1] * W[1]) : W[5] / (W[2] * W[2]);
+        w3r = std::fabs(W[1]) >= std::fabs(W[2]) ? W[6] / (W[1] * W[1] * W[1]) : W[9] / (W[2] * W[2] * W[2]);
+        length = std::sqrt(W[1] * W[1] + W[2] * W[2]);
+    } else {
+        w2r = w3r = 0.0;
+        length = 0;
+    }
+    double fn0, fn1;
+    if (C.s->tang_weight_direction == 0) {
+        const double *nv = &C.out->face_norm[6 * f + 2 * g];
+        double ux, uy;
+        stable_normalized(nv[0], nv[1], ux, uy);
+        fn0 = ux * length, fn1 = uy * length;
+    } else {
+        fn0 = W[1], fn1 = W[2];
+    }
+    // tangModifier = 1 for tangWeightModMin = 1, so its factors are exact 1s
+    double ft0 = -fn1 * (C.s->tang_weight * 1.0), ft1 = fn0 * (C.s->tang_weight * 1.0);
+    p.W0 = W[0];
+    p.n0 = fn0, p.n1 = fn1, p.t0 = ft0, p.t1 = ft1;
+    p.w2r = w2r, p.w3r = w3r;
+    return p;
+}
+
+// per-column projected quantities a_k(col), k = 1..9 (k = 0 handled apart)
+void ff_project(const FFParams &p, const double D[10][10], int col, double a[10]) {
+    const double n0 = p.n0, n1 = p.n1, t0 = p.t0, t1 = p.t1, w2r = p.w2r, w3r = p.w3r;
+    const double d1 = D[1][col], d2 = D[2][col], d3 = D[3][col], d4 = D[4][col], d5 = D[5][col];
+    const double d6 = D[6][col], d7 = D[7][col], d8 = D[8][col], d9 = D[9][col];
+    a[0] = D[0][col];
+    a[1] = d1 * n0 + d2 * n1;
+    a[2] = d1 * t0 + d2 * t1;
+    a[3] = (d3 * n0 * n0 + d4 * n0 * n1 * 2 + d5 * n1 * n1) * w2r;
+    a[4] = (d3 * n0 * t0 + d4 * (n0 * t1 + n1 * t0) + d5 * n1 * t1) * w2r;
+    a[5] = (d3 * t0 * t0 + d4 * t0 * t1 * 2 + d5 * t1 * t1) * w2r;
+    a[6] = (d6 * n0 * n0 * n0 + d7 * n0 * n0 * n1 * 3 + d8 * n0 * n1 * n1 * 3 + d9 * n1 * n1 * n1) * w3r;
+    a[7] = (d6 * n0 * n0 * t0 + d7 * (n0 * n0 * t1 + n0 * t0 * n1 + t0 * n0 * n1) +
+            d8 * (n0 * n1 * t1 + n0 * t1 * n1 + t0 * n1 * n1) + d9 * n1 * n1 * t1) *
+           w3r;
+    a[8] = (d6 * n0 * t0 * t0 + d7 * (t0 * n0 * t1 + n0 * t0 * t1 + t0 * t0 * n1) +
+            d8 * (t0 * n1 * t1 + n0 * t1 * t1 + t0 * t1 * n1) + d9 * n1 * t1 * t1) *
+           w3r;
+    a[9] = (d6 * t0 * t0 * t0 + d7 * t0 * t0 * t1 * 3 + d8 * t0 * t1 * t1 * 3 + d9 * t1 * t1 * t1) * w3r;
+}
+
+// Conj(i, j) of FFaceFunctional from the projections (same operation order)
+inline double ff_conj(const FFParams &p, const double a[10], const double b[10]) {
+    double c = 0;
+    c += a[0] * b[0] * p.W0 * p.W0;
+    c += a[1] * b[1];
+    c += a[2] * b[2] * 1.0;
+    c += a[3] * b[3];
+    c += a[4] * b[4] * 2 * 1.0;
+    c += a[5] * b[5] * 1.0;
+    c += a[6] * b[6];
+    c += a[7] * b[7] * 3 * 1.0;
+    c += a[8] * b[8] * 3 * 1.0;
+    c += a[9] * b[9] * 1.0;
+    return c;
+}
+
+// One-sided Jacobi SVD of an n x n matrix (n <= 9), then the least-squares
+// solution X = A^+ B with the rank rule of Eigen's JacobiSVD::solve: singular
+// values <= max(sv) * n * eps are dropped (DNDS_HardEigen.cpp:10-16, 53-57).
+void lstsq(int n, const double *A, const double *B, int nrhs, double *X) {
+    double U[81], V[81];
+    for (int i = 0; i < n * n; i++)
+        U[i] = A[i];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            V[i * n + j] = i == j ? 1.0 : 0.0;
+    const double eps = 2.220446049250313e-16;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double alpha = 0, beta = 0, gamma = 0;
+                for (int i = 0; i < n; i++) {
+                    alpha += U[i * n + p] * U[i * n + p];
+                    beta += U[i * n + q] * U[i * n + q];
+                    gamma += U[i * n + p] * U[i * n + q];
+                }
+                if (gamma == 0 || std::fabs(gamma) <= eps * std::sqrt(alpha * beta))
+                    continue;
+                off = std::max(off, std::fabs(gamma) / std::sqrt(alpha * beta));
+                double zeta = (beta - alpha) / (2 * gamma);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+                double c = 1 / std::sqrt(1 + t * t), s = c * t;
+                for (int i = 0; i < n; i++) {
+                    double up = U[i * n + p], uq = U[i * n + q];
+                    U[i * n + p] = c * up - s * uq;
+                    U[i * n + q] = s * up + c * uq;
+                    double vp = V[i * n + p], vq = V[i * n + q];
+                    V[i * n + p] = c * vp - s * vq;
+                    V[i * n + q] = s * vp + c * vq;
+                }
+            }
+        if (off <= eps)
+            break;
+    }
+    double sv[9], smax = 0;
+    for (int k = 0; k < n; k++) {
+        double s2 = 0;
+        for (int i = 0; i < n; i++)
+            s2 += U[i * n + k] * U[i * n + k];
+        sv[k] = std::sqrt(s2);
+        smax = std::max(smax, sv[k]);
+    }
+    const double thr = std::max(smax * double(n) * eps, std::numeric_limits<double>::min());
+    // X = V diag(1/sv) U_normalized^T B
+    double T[81];
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < nrhs; j++) {
+            double acc = 0;
+            if (sv[k] > thr) {
+                for (int i = 0; i < n; i++)
+                    acc += (U[i * n + k] / sv[k]) * B[i * nrhs + j];
+                acc /= sv[k];
+            }
+            T[k * nrhs + j] = acc;
+        }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < nrhs; j++) {
+            double acc = 0;
+            for (int k = 0; k < n; k++)
+                acc += V[i * n + k] * T[k * nrhs + j];
+            X[i * nrhs + j] = acc;
+        }
+}
+
+template <class F>
+void parallel_for(int64_t n, int nthreads, F &&f) {
+    if (nthreads <= 1 || n < 1024) {
+        for (int64_t i = 0; i < n; i++)
+            f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    const int64_t chunk = 256;
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nthreads; t++)
+        ts.emplace_back([&]() {
+            for (;;) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n)
+                    break;
+                int64_t e = std::min(n, b + chunk);
+                for (int64_t i = b; i < e; i++)
+                    f(i);
+            }
+        });
+    for (auto &t : ts)
+        t.join();
+}
+
+struct SetupError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+} // namespace
+
+extern "C" void dndsmi_vr_setting_default(dndsmi_vr_setting *s) {
+    if (!s)
+        return;
+    std::memset(s, 0, sizeof(*s));
+    s->rec_order = 3;
+    s->jacobi_relax = 1.0;
+    s->wall_weight = 1.0;
+    s->far_weight = 0.0;
+    s->tang_weight = 1;
+    s->tang_weight_mod_min = 1;
+    s->tang_weight_direction = 1; // TWD_Bary
+    s->anisotropic_lengths = 0;
+    s->weight_scheme_geom = 0;
+    s->weight_scheme_dir = 0;
+    s->weight_len_scale = 1;
+    s->wbap_smooth_indicator_scale = 1e-10;
+    s->wbap_n_std = 10.0;
+    s->use_local_coord = 0;
+    s->orthogonalize_base = 0;
+    s->norm_wbap = 0;
+    s->build_limiter_data = 0;
+}
+
+extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_setting *setting, int32_t nthreads,
+                                dndsmi_vr **outp) {
+    if (!mesh || !setting || !outp)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    *outp = nullptr;
+    if (setting->rec_order != 3 || setting->use_local_coord || setting->orthogonalize_base || setting->norm_wbap ||
+        setting->tang_weight_mod_min != 1.0 || setting->weight_scheme_geom < 0 || setting->weight_scheme_geom > 3 ||
+        setting->weight_scheme_dir < 0 || setting->weight_scheme_dir > 1)
+        return DNDSMI_ERR_UNSUPPORTED;
+    if (nthreads <= 0)
+        nthreads = int32_t(std::max(1u, std::thread::hardware_concurrency()));
+    const dndsmi_mesh_view &m = *mesh;
+    dndsmi_vr *out = new (std::nothrow) dndsmi_vr();
+    if (!out)
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    Ctx C{mesh, setting, out, {}};
+    const int64_t nc = m.n_cell, nf = m.n_face, nc2f = m.cell_ptr[nc];
+    try {
+        out->n_cell = nc, out->n_face = nf, out->n_c2f = nc2f;
+        out->volume.assign(nc, 0.0);
+        out->face_area.assign(nf, 0.0);
+        out->cell_bary.assign(2 * nc, 0.0);
+        out->base_moment.assign(10 * nc, 0.0);
+        out->face_center.assign(2 * nf, 0.0);
+        out->face_norm_center.assign(2 * nf, 0.0);
+        out->face_norm.assign(6 * nf, 0.0);
+        out->face_weight.assign(10 * nf, 0.0);
+        out->face_d0.assign(size_t(54) * nf, 0.0);
+        out->ainv.assign(size_t(81) * nc, 0.0);
+        out->ainv_b.assign(size_t(81) * nc2f, 0.0);
+        out->ainv_bvec.assign(size_t(9) * nc2f, 0.0);
+        if (setting->build_limiter_data) {
+            out->face_dibj_g0.assign(size_t(180) * nf, 0.0);
+            out->matrix_secondary.assign(size_t(162) * nf, 0.0);
+        }
+        C.scale.assign(2 * nc, 0.0);
+        std::atomic<int> failed{0};
+
+        // ---- pass 1: cells -- volume (TRI_7 / QUAD_9), barycentre and
+        // base moments (TRI_4 / QUAD_9), DNDS_FV_VR.hpp:40-74, DNDS_FV_VR.cpp:82-179
+        parallel_for(nc, nthreads, [&](int64_t ic) {
+            Cell c = load_cell(m, ic);
+            const bool tri = c.nv == 3;
+            double v = 0.0;
+            const int nvq = tri ? 7 : 9;
+            for (int g = 0; g < nvq; g++) {
+                double px, py, det;
+                map_point(c, tri ? kTri7X[g] : kQuad9X[g], tri ? kTri7Y[g] : kQuad9Y[g], px, py, det);
+                if (!(det > 0))
+                    failed = 1;
+                v += det * (tri ? kTri7W[g] : kQuad9W[g]);
+            }
+            out->volume[ic] = v;
+            const int ng = tri ? 4 : 9;
+            const double *gx = tri ? kTri4X : kQuad9X, *gy = tri ? kTri4Y : kQuad9Y, *gw = tri ? kTri4W : kQuad9W;
+            double bx = 0, by = 0;
+            for (int g = 0; g < ng; g++) {
+                double px, py, det;
+                map_point(c, gx[g], gy[g], px, py, det);
+                double ix = px * det, iy = py * det;
+                bx += ix * gw[g];
+                by += iy * gw[g];
+            }
+            bx /= v, by /= v;
+            out->cell_bary[2 * ic] = bx, out->cell_bary[2 * ic + 1] = by;
+            double sx, sy;
+            cell_scale(c, setting->anisotropic_lengths != 0, sx, sy);
+            C.scale[2 * ic] = sx, C.scale[2 * ic + 1] = sy;
+            double zero[10] = {0};
+            double mom[10] = {0};
+            for (int g = 0; g < ng; g++) {
+                double px, py, det;
+                map_point(c, gx[g], gy[g], px, py, det);
+                double D[10][10];
+                diff_base(px, py, bx, by, sx, sy, zero, D);
+                for (int k = 0; k < 10; k++)
+                    mom[k] += (D[0][k] * det) * gw[g];
+            }
+            for (int k = 0; k < 10; k++)
+                out->base_moment[10 * ic + k] = mom[k] / v;
+            out->base_moment[10 * ic] = 1.; // DNDS_FV_VR.cpp:178
+        });
+        if (failed)
+            throw SetupError("non-positive cell Jacobian (orientation wrong or distorted)");
+
+        // ---- pass 2: faces -- area (LINE_5), centre and normals, weights,
+        // row 0 of the Gauss-point bases, limiter data (DNDS_FV_VR.hpp:75-101,
+        // DNDS_FV_VR.cpp:534-887)
+        parallel_for(nf, nthreads, [&](int64_t f) {
+            const int64_t n0 = m.face2node[2 * f], n1 = m.face2node[2 * f + 1];
+            const double fx[2] = {m.node_xy[2 * n0], m.node_xy[2 * n1]};
+            const double fy[2] = {m.node_xy[2 * n0 + 1], m.node_xy[2 * n1 + 1]};
+            double nx, ny;
+            line_norm(fx, fy, nx, ny);
+            double area = 0.0;
+            for (int g = 0; g < 5; g++)
+                area += norm2(nx, ny) * kLine5W[g];
+            out->face_area[f] = area;
+            const double cxf = fx[0] * ((1 - 0.0) * 0.5) + fx[1] * ((1 + 0.0) * 0.5);
+            const double cyf = fy[0] * ((1 - 0.0) * 0.5) + fy[1] * ((1 + 0.0) * 0.5);
+            out->face_center[2 * f] = cxf, out->face_center[2 * f + 1] = cyf;
+            out->face_norm_center[2 * f] = nx, out->face_norm_center[2 * f + 1] = ny;
+            for (int g = 0; g < 3; g++)
+                out->face_norm[6 * f + 2 * g] = nx, out->face_norm[6 * f + 2 * g + 1] = ny;
+            // orientation check: normal points out of f2c[0] (DNDS_FV_VR.cpp:579)
+            {
+                const int64_t L = m.face2cell[2 * f];
+                Cell c = load_cell(m, L);
+                double px, py, det;
+                map_point(c, c.nv == 3 ? 1. / 3. : 0., c.nv == 3 ? 1. / 3. : 0., px, py, det);
+                if (!(nx * (px - cxf) + ny * (py - cyf) < 0.0))
+                    failed = 1;
+            }
+            const int64_t R = m.face2cell[2 * f + 1];
+            const int32_t bc = m.face_bc[f];
+            const int64_t L = m.face2cell[2 * f];
+            double W[10];
+            double dx, dy;
+            if (R != DNDSMI_NO_CELL) {
+                dx = (out->cell_bary[2 * R] + m.face_shift[2 * f]) - out->cell_bary[2 * L];
+                dy = (out->cell_bary[2 * R + 1] + m.face_shift[2 * f + 1]) - out->cell_bary[2 * L + 1];
+                for (int k = 0; k < 10; k++)
+                    W[k] = 1.0;
+            } else {
+                for (int k = 0; k < 10; k++)
+                    W[k] = 0.0;
+                dx = cxf - out->cell_bary[2 * L];
+                dy = cyf - out->cell_bary[2 * L + 1];
+                if (bc == DNDSMI_BC_WALL || bc == DNDSMI_BC_WALL_EULER)
+                    W[0] = setting->wall_weight;
+                else if (is_far(bc))
+                    W[0] = setting->far_weight;
+                else if (bc == DNDSMI_BC_WALL_NOSLIP) {
+                    W[0] = setting->wall_weight;
+                    // USE_FIRST_ORDER_VISCOUS_WALL_DELTA_IN_VR_WEIGHT: project delta on the normal
+                    double s = norm2(nx, ny);
+                    double ux = nx / s, uy = ny / s;
+                    double d = ux * dx + uy * dy;
+                    dx = ux * d, dy = uy * d;
+                } else
+                    failed = 2;
+            }
+            const double D = norm2(dx, dy);
+            const double S = area;
+            double GW = 1;
+            switch (setting->weight_scheme_geom) {
+            case 1: GW = std::pow(1. / D, 0.5); break;
+            case 2: GW = std::pow(1. / S, 0.5); break;
+            case 3: GW = std::pow(S / D, 0.5 / 2); break;
+            default: break;
+            }
+            if (setting->tang_weight_direction == 0) {
+                double ux, uy;
+                stable_normalized(nx, ny, ux, uy);
+                const double dn = norm2(dx, dy);
+                dx = ux * dn, dy = uy * dn;
+            }
+            dx *= setting->weight_len_scale, dy *= setting->weight_len_scale;
+            for (int id = 0; id < 10; id++) {
+                const int ndx = kDiffX[id], ndy = kDiffY[id];
+                if (setting->weight_scheme_dir == 0)
+                    W[id] *= GW * std::pow(dx, double(ndx)) * std::pow(dy, double(ndy)) * double(kNCombs[id]) /
+                             double(kFactorial[ndx + ndy]);
+                else
+                    W[id] *= GW * std::pow(dx, double(ndx)) * std::pow(dy, double(ndy)) * 1 * kDirWeight3HQM[ndx + ndy];
+            }
+            for (int k = 0; k < 10; k++)
+                out->face_weight[10 * f + k] = W[k];
+        });
+        if (failed == 1)
+            throw SetupError("face normal does not point out of its first cell");
+        if (failed == 2)
+            throw SetupError("boundary face of unsupported kind");
+
+        // Gauss-point bases of faces (needs face normals only for the limiter data)
+        parallel_for(nf, nthreads, [&](int64_t f) {
+            const int64_t R = m.face2cell[2 * f + 1];
+            double D[10][10];
+            for (int g = 0; g < 3; g++)
+                for (int side = 0; side < (R != DNDSMI_NO_CELL ? 2 : 1); side++) {
+                    face_dibj(C, f, side, kLine3X[g], D);
+                    for (int c = 0; c < 9; c++)
+                        out->face_d0[54 * f + 18 * g + 9 * side + c] = D[0][c + 1];
+                    if (setting->build_limiter_data && g == 0)
+                        for (int r = 0; r < 10; r++)
+                            for (int c = 0; c < 9; c++)
+                                out->face_dibj_g0[180 * f + 90 * side + 9 * r + c] = D[r][c + 1];
+                }
+            if (setting->build_limiter_data && R != DNDSMI_NO_CELL) {
+                // matrixSecondaryBatch from the face-centre bases (DNDS_FV_VR.cpp:686-712)
+                double DL[10][10], DR[10][10];
+                face_dibj(C, f, 0, 0.0, DL);
+                face_dibj(C, f, 1, 0.0, DR);
+                double AL[81], AR[81];
+                for (int r = 0; r < 9; r++)
+                    for (int c = 0; c < 9; c++)
+                        AL[9 * r + c] = DL[r + 1][c + 1], AR[9 * r + c] = DR[r + 1][c + 1];
+                lstsq(9, AL, AR, 9, &out->matrix_secondary[162 * f]);      // msR2L
+                lstsq(9, AR, AL, 9, &out->matrix_secondary[162 * f + 81]); // msL2R
+            }
+        });
+
+        // ---- pass 3: cells -- A_ii, its pseudo-inverse, A^-1 B and A^-1 b
+        // (initReconstructionMatVec, DNDS_FV_VR.cpp:983-1153)
+        parallel_for(nc, nthreads, [&](int64_t ic) {
+            const int64_t b0 = m.cell_ptr[ic];
+            const int nfc = int(m.cell_ptr[ic + 1] - b0);
+            double A[81] = {0};
+            // DI[face][g][10][10] of this cell, with the functional parameters
+            double DI[4][3][10][10];
+            FFParams P[4][3];
+            for (int lf = 0; lf < nfc; lf++) {
+                const int64_t f = m.cell2face[b0 + lf];
+                const int side = m.face2cell[2 * f] == ic ? 0 : 1;
+                for (int g = 0; g < 3; g++) {
+                    face_dibj(C, f, side, kLine3X[g], DI[lf][g]);
+                    P[lf][g] = ff_params(C, f, g, &out->face_weight[10 * f]);
+                }
+            }
+            for (int lf = 0; lf < nfc; lf++) {
+                const int64_t f = m.cell2face[b0 + lf];
+                const double nl = norm2(out->face_norm[6 * f], out->face_norm[6 * f + 1]);
+                for (int g = 0; g < 3; g++) {
+                    double a[10][10];
+                    for (int i = 0; i < 10; i++)
+                        ff_project(P[lf][g], DI[lf][g], i, a[i]);
+                    for (int i = 1; i < 10; i++)
+                        for (int j = 1; j < 10; j++) {
+                            double inc = ff_conj(P[lf][g], a[i], a[j]) * nl;
+                            A[9 * (i - 1) + (j - 1)] += inc * kLine3W[g];
+                        }
+                }
+            }
+            double I9[81];
+            for (int i = 0; i < 81; i++)
+                I9[i] = (i % 10 == 0) ? 1.0 : 0.0;
+            double Ainv[81];
+            lstsq(9, A, I9, 9, Ainv);
+            std::memcpy(&out->ainv[81 * ic], Ainv, sizeof(Ainv));
+            double rows[4][9];
+            for (int lf = 0; lf < nfc; lf++) {
+                const int64_t f = m.cell2face[b0 + lf];
+                const int side = m.face2cell[2 * f] == ic ? 0 : 1;
+                const int64_t other = m.face2cell[2 * f + 1 - side];
+                const int32_t bc = m.face_bc[f];
+                const double nl = norm2(out->face_norm[6 * f], out->face_norm[6 * f + 1]);
+                double *M = &out->ainv_b[81 * (b0 + lf)];
+                if (other != DNDSMI_NO_CELL || is_far(bc)) {
+                    double B[81] = {0};
+                    for (int g = 0; g < 3; g++) {
+                        double DJ[10][10];
+                        if (other != DNDSMI_NO_CELL)
+                            face_dibj(C, f, 1 - side, kLine3X[g], DJ);
+                        else
+                            std::memcpy(DJ, DI[lf][g], sizeof(DJ));
+                        double a[10][10], bb[10][10];
+                        for (int i = 0; i < 10; i++) {
+                            ff_project(P[lf][g], DI[lf][g], i, a[i]);
+                            ff_project(P[lf][g], DJ, i, bb[i]);
+                        }
+                        for (int i = 1; i < 10; i++)
+                            for (int j = 1; j < 10; j++) {
+                                double inc = ff_conj(P[lf][g], a[i], bb[j]) * nl;
+                                B[9 * (i - 1) + (j - 1)] += inc * kLine3W[g];
+                            }
+                    }
+                    for (int r = 0; r < 9; r++)
+                        for (int c = 0; c < 9; c++) {
+                            double acc = 0;
+                            for (int k = 0; k < 9; k++)
+                                acc += Ainv[9 * r + k] * B[9 * k + c];
+                            M[9 * r + c] = acc;
+                        }
+                } else if (is_wall(bc)) {
+                    for (int k = 0; k < 81; k++)
+                        M[k] = 0.0;
+                } else {
+                    failed = 3;
+                }
+                // b row: FFaceFunctional(Ones(1,1), row 0, weight 0) (DNDS_FV_VR.cpp:1118-1138)
+                const double w0 = out->face_weight[10 * f];
+                double row[9] = {0};
+                for (int g = 0; g < 3; g++)
+                    for (int c = 0; c < 9; c++) {
+                        double inc = ((1.0 * w0) * w0) * DI[lf][g][0][c + 1];
+                        inc *= nl;
+                        row[c] += inc * kLine3W[g];
+                    }
+                for (int c = 0; c < 9; c++)
+                    rows[lf][c] = row[c];
+            }
+            // vectorBatch = vectorBatch * Ainv^T
+            for (int lf = 0; lf < nfc; lf++)
+                for (int r = 0; r < 9; r++) {
+                    double acc = 0;
+                    for (int k = 0; k < 9; k++)
+                        acc += rows[lf][k] * Ainv[9 * r + k];
+                    out->ainv_bvec[9 * (b0 + lf) + r] = acc;
+                }
+        });
+        if (failed == 3)
+            throw SetupError("inner boundary face without a second cell");
+    } catch (const SetupError &) {
+        delete out;
+        return DNDSMI_ERR_SETUP;
+    } catch (const std::bad_alloc &) {
+        delete out;
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    }
+    *outp = out;
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_vr_get_view(const dndsmi_vr *vr, dndsmi_vr_view *v) {
+    if (!vr || !v)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    v->n_cell = vr->n_cell, v->n_face = vr->n_face, v->n_c2f = vr->n_c2f;
+    v->volume = vr->volume.data();
+    v->face_area = vr->face_area.data();
+    v->cell_bary = vr->cell_bary.data();
+    v->base_moment = vr->base_moment.data();
+    v->face_center = vr->face_center.data();
+    v->face_norm_center = vr->face_norm_center.data();
+    v->face_norm = vr->face_norm.data();
+    v->face_weight = vr->face_weight.data();
+    v->face_d0 = vr->face_d0.data();
+    v->face_dibj_g0 = vr->face_dibj_g0.empty() ? nullptr : vr->face_dibj_g0.data();
+    v->matrix_secondary = vr->matrix_secondary.empty() ? nullptr : vr->matrix_secondary.data();
+    v->ainv = vr->ainv.data();
+    v->ainv_b = vr->ainv_b.data();
+    v->ainv_bvec = vr->ainv_bvec.data();
+    return DNDSMI_OK;
+}
+
+extern "C" void dndsmi_vr_destroy(dndsmi_vr *vr) { delete vr; }

@@ -1,0 +1,324 @@
+"""Host-side mirror of the reference's hot-path interfaces over the C ABI.
+
+Names, argument order and meaning follow the reference:
+
+* `CompactFacedMeshSerialRW`  -- DNDS_Mesh.hpp:634-709 (topology and its
+  conventions; built by the seeded generator or from raw cells).
+* `VRFiniteVolume2D`          -- DNDS_FV_VR.hpp:125-1713: `Initialization`
+  (DNDS_FV_VR.cpp:3-13), `BuildRec`, `ReconstructionJacobiStep`.
+* `EulerEvaluator`            -- DNDS_FV_EulerEvaluator.hpp:71-1462: `BuildMean`,
+  `EvaluateDt`, `EvaluateRHS`.
+* `ExplicitSSPRK3LocalDt`     -- DNDS_ODE.hpp:95-179: `Step`.
+
+Errors: the reference aborts through DNDS_assert (DNDS_Defines.h:31-44); here
+every failing call raises `DNDSError` carrying the ABI status code and the
+library's message.  Device arrays (`ArrayDOFV`, `ArrayRecV`, `ArrayScalar`)
+live on the GPU; `upload` / `download` convert to and from the reference's
+host layouts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+# vfvSetting of the reference's 2D Riemann configuration
+# (data/euler_config_2DRiemann.json: anisotropicLengths, tangWeight 1 along
+# the face normal, S geometric weights, weightLenScale 0.5, farWeight 0,
+# wallWeight 0.5, WBAP_SmoothIndicatorScale 0.1, global coordinates).
+RIEMANN_VR_SETTING = dict(anisotropic_lengths=1, tang_weight=1.0, tang_weight_direction=0, wall_weight=0.5,
+                          far_weight=0.0, weight_len_scale=0.5, weight_scheme_geom=2, weight_scheme_dir=0,
+                          wbap_smooth_indicator_scale=0.1, jacobi_relax=1.0)
+
+
+class DNDSError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dndsmi error {code}: {msg}")
+        self.code = code
+
+
+def _lib():
+    return A.load()
+
+
+def _check(rc: int, what: str, ctx=None):
+    if rc != A.OK:
+        msg = what
+        if ctx:
+            buf = C.create_string_buffer(512)
+            _lib().dndsmi_last_error(ctx, buf, 512)
+            if buf.value:
+                msg = f"{what}: {buf.value.decode()}"
+        raise DNDSError(rc, msg)
+
+
+class CompactFacedMeshSerialRW:
+    """A 2D mesh with the reference's topology conventions (owns the C handle)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        self.view = A.MeshView()
+        _check(_lib().dndsmi_mesh_get_view(self._h, C.byref(self.view)), "mesh view")
+        self._arrays = None
+
+    @classmethod
+    def generate(cls, kind: int, nx: int, ny: int, x0=0.0, x1=1.0, y0=0.0, y1=1.0, periodic=(False, False),
+                 jitter=0.0, y_ratio=1.0, seed=0, bc=(A.BC_FARFIELD,) * 4):
+        """Seeded structured-as-unstructured mesh; bc = (xmin, xmax, ymin, ymax)."""
+        spec = A.MeshSpec(kind=kind, periodic_x=int(periodic[0]), periodic_y=int(periodic[1]), nx=nx, ny=ny,
+                          x0=x0, x1=x1, y0=y0, y1=y1, jitter=jitter, y_ratio=y_ratio, seed=seed,
+                          bc_xmin=bc[0], bc_xmax=bc[1], bc_ymin=bc[2], bc_ymax=bc[3])
+        h = C.c_void_p()
+        _check(_lib().dndsmi_mesh_generate(C.byref(spec), C.byref(h)), "mesh generate")
+        return cls(h)
+
+    @classmethod
+    def from_cells(cls, node_xy, cell_ptr, cell2node, bnd2node, bnd_bc):
+        """Topology from raw cells and boundary edges (the Gmsh-reader path)."""
+        node_xy = np.ascontiguousarray(node_xy, dtype=np.float64)
+        cell_ptr = np.ascontiguousarray(cell_ptr, dtype=np.int64)
+        cell2node = np.ascontiguousarray(cell2node, dtype=np.int64)
+        bnd2node = np.ascontiguousarray(bnd2node, dtype=np.int64).reshape(-1)
+        bnd_bc = np.ascontiguousarray(bnd_bc, dtype=np.int32)
+        h = C.c_void_p()
+        _check(_lib().dndsmi_mesh_from_cells(node_xy.shape[0], A.dptr(node_xy), cell_ptr.shape[0] - 1,
+                                             A.i64ptr(cell_ptr), A.i64ptr(cell2node), bnd_bc.shape[0],
+                                             A.i64ptr(bnd2node), A.i32ptr(bnd_bc), C.byref(h)), "mesh from cells")
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().dndsmi_mesh_destroy(self._h)
+            self._h = None
+
+    @property
+    def n_cell(self):
+        return int(self.view.n_cell)
+
+    @property
+    def n_face(self):
+        return int(self.view.n_face)
+
+    def arrays(self) -> dict:
+        """numpy copies of the topology (keys = dndsmi_mesh_view fields)."""
+        if self._arrays is None:
+            v = self.view
+            nc, nf = int(v.n_cell), int(v.n_face)
+            cp = A.as_array(v.cell_ptr, nc + 1, np.int64)
+            self._arrays = dict(
+                n_node=int(v.n_node), n_cell=nc, n_face=nf,
+                node_xy=A.as_array(v.node_xy, 2 * v.n_node).reshape(-1, 2),
+                cell_ptr=cp,
+                cell2node=A.as_array(v.cell2node, cp[-1], np.int64),
+                cell2face=A.as_array(v.cell2face, cp[-1], np.int64),
+                face2node=A.as_array(v.face2node, 2 * nf, np.int64).reshape(-1, 2),
+                face2cell=A.as_array(v.face2cell, 2 * nf, np.int64).reshape(-1, 2),
+                face2cell_ref=A.as_array(v.face2cell_ref, 2 * nf, np.int32).reshape(-1, 2),
+                face_bc=A.as_array(v.face_bc, nf, np.int32),
+                face_shift=A.as_array(v.face_shift, 2 * nf).reshape(-1, 2),
+            )
+        return self._arrays
+
+
+def vr_setting(**overrides) -> A.VRSetting:
+    """VRFiniteVolume2D::Setting: the reference defaults, then `overrides`."""
+    s = A.VRSetting()
+    _lib().dndsmi_vr_setting_default(C.byref(s))
+    for k, v in overrides.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+SETUP_SIZES = {  # name -> (per, count) where per is "cell", "face" or "c2f"
+    "volume": ("cell", 1), "face_area": ("face", 1), "cell_bary": ("cell", 2), "base_moment": ("cell", 10),
+    "face_center": ("face", 2), "face_norm_center": ("face", 2), "face_norm": ("face", 6),
+    "face_weight": ("face", 10), "face_d0": ("face", 54), "face_dibj_g0": ("face", 180),
+    "matrix_secondary": ("face", 162), "ainv": ("cell", 81), "ainv_b": ("c2f", 81), "ainv_bvec": ("c2f", 9),
+}
+
+
+class VRFiniteVolume2D:
+    """Geometry + VR setup (host FP64) and the Jacobi sweep (device)."""
+
+    def __init__(self, mesh: CompactFacedMeshSerialRW, setting: A.VRSetting | None = None, P_ORDER: int = 3):
+        self.mesh = mesh
+        self.setting = setting if setting is not None else vr_setting()
+        self.setting.rec_order = P_ORDER
+        self.P_ORDER = P_ORDER
+        self._h = None
+        self.view = None
+        self.device = None  # set by EulerEvaluator
+
+    def Initialization(self, nthreads: int = 0):
+        h = C.c_void_p()
+        _check(_lib().dndsmi_vr_create(C.byref(self.mesh.view), C.byref(self.setting), nthreads, C.byref(h)),
+               "VR setup")
+        self._h = h
+        self.view = A.VRView()
+        _check(_lib().dndsmi_vr_get_view(h, C.byref(self.view)), "VR view")
+        return self
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().dndsmi_vr_destroy(self._h)
+            self._h = None
+
+    def arrays(self, names=None) -> dict:
+        """numpy copies of setup arrays (keys = dndsmi_vr_view fields)."""
+        v = self.view
+        n = {"cell": int(v.n_cell), "face": int(v.n_face), "c2f": int(v.n_c2f)}
+        out = {}
+        for name, (per, cnt) in SETUP_SIZES.items():
+            if names is not None and name not in names:
+                continue
+            ptr = getattr(v, name)
+            out[name] = A.as_array(ptr, n[per] * cnt) if ptr else None
+        return out
+
+    def BuildRec(self):
+        return ArrayRecV(self.device)
+
+    def ReconstructionJacobiStep(self, u: "ArrayDOFV", uRec: "ArrayRecV", uRecNewBuf: "ArrayRecV", t: float = 0.0):
+        """One Jacobi sweep with SOR_Instead = false; afterwards uRec holds the
+        new iterate and uRecNewBuf the previous one (its storage is swapped)."""
+        d = self.device
+        _check(_lib().dndsmi_reconstruction_jacobi_step(d.ctx, u.h, uRec.h, uRecNewBuf.h, t), "Jacobi step", d.ctx)
+
+
+class Device:
+    """The device context: mesh + setup on the GPU (one per evaluator)."""
+
+    def __init__(self, mesh: CompactFacedMeshSerialRW, vfv: VRFiniteVolume2D, esetting: A.EulerSetting,
+                 stream: int | None = None):
+        h = C.c_void_p()
+        _check(_lib().dndsmi_ctx_create(C.byref(mesh.view), C.byref(vfv.view), C.byref(vfv.setting),
+                                        C.byref(esetting), C.c_void_p(stream) if stream else None, C.byref(h)),
+               "device context")
+        self.ctx = h
+        self.n_cell = mesh.n_cell
+
+    def sync(self):
+        _check(_lib().dndsmi_sync(self.ctx), "sync", self.ctx)
+
+    def device_bytes(self) -> int:
+        return int(_lib().dndsmi_ctx_device_bytes(self.ctx))
+
+    def __del__(self):
+        if getattr(self, "ctx", None):
+            _lib().dndsmi_ctx_destroy(self.ctx)
+            self.ctx = None
+
+
+class _DeviceArray:
+    KIND = None
+    WIDTH = None
+
+    def __init__(self, device: Device):
+        self.device = device
+        h = C.c_void_p()
+        _check(_lib().dndsmi_array_create(device.ctx, self.KIND, C.byref(h)), "array create", device.ctx)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib().dndsmi_array_destroy(self.h)
+            self.h = None
+
+    def shape(self):
+        return (self.device.n_cell, self.WIDTH) if self.WIDTH > 1 else (self.device.n_cell,)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape())
+        _check(_lib().dndsmi_array_upload(self.h, A.dptr(a)), "upload", self.device.ctx)
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape(), dtype=np.float64)
+        _check(_lib().dndsmi_array_download(self.h, A.dptr(out)), "download", self.device.ctx)
+        return out
+
+    def copy_from(self, other):
+        _check(_lib().dndsmi_array_copy(self.h, other.h), "copy", self.device.ctx)
+        return self
+
+    def setConstant(self, value: float):
+        _check(_lib().dndsmi_array_set_constant(self.h, float(value)), "set constant", self.device.ctx)
+        return self
+
+
+class ArrayDOFV(_DeviceArray):
+    """ArrayDOFV<4>: 4 conservative variables per cell, host layout [n_cell][4]."""
+    KIND = A.ARRAY_DOF
+    WIDTH = 4
+
+
+class ArrayRecV(_DeviceArray):
+    """ArrayRecV: 9 x 4 rec coefficients per cell, host layout [n_cell][36]
+    with element (r, v) at r + 9 v (column-major per cell)."""
+    KIND = A.ARRAY_REC
+    WIDTH = 36
+
+
+class ArrayScalar(_DeviceArray):
+    """One double per cell (dt, ifUseLimiter)."""
+    KIND = A.ARRAY_SCALAR
+    WIDTH = 1
+
+
+RS = {"Roe": A.RS_ROE, "HLLC": A.RS_HLLC}
+
+
+class EulerEvaluator:
+    """EulerEvaluator<NS_2D> (Euler flux, muGas = 0)."""
+
+    def __init__(self, mesh: CompactFacedMeshSerialRW, vfv: VRFiniteVolume2D, riemannSolverType: str = "Roe",
+                 gamma: float = 1.4, farFieldStaticValue=(1.0, 0.0, 0.0, 2.5), stream: int | None = None):
+        self.mesh = mesh
+        self.vfv = vfv
+        self.settings = A.EulerSetting(riemann_solver=RS[riemannSolverType], gamma=gamma,
+                                       far_field=(C.c_double * 4)(*farFieldStaticValue))
+        self.device = Device(mesh, vfv, self.settings, stream)
+        vfv.device = self.device
+
+    def BuildMean(self):
+        return ArrayDOFV(self.device)
+
+    def BuildScalar(self):
+        return ArrayScalar(self.device)
+
+    def EvaluateDt(self, dt: ArrayScalar, u: ArrayDOFV, CFL: float, MaxDt: float = 1.0,
+                   UseLocaldt: bool = False) -> float:
+        """Returns dtMinall (the reference's output argument)."""
+        dtmin = C.c_double()
+        _check(_lib().dndsmi_evaluate_dt(self.device.ctx, dt.h, u.h, CFL, C.byref(dtmin), MaxDt, int(UseLocaldt)),
+               "EvaluateDt", self.device.ctx)
+        return dtmin.value
+
+    def EvaluateRHS(self, rhs: ArrayDOFV, u: ArrayDOFV, uRec: ArrayRecV, t: float):
+        _check(_lib().dndsmi_evaluate_rhs(self.device.ctx, rhs.h, u.h, uRec.h, t), "EvaluateRHS", self.device.ctx)
+
+
+class ExplicitSSPRK3LocalDt:
+    """ODE::ExplicitSSPRK3LocalDt with frhs = nInternalRecStep Jacobi sweeps +
+    EvaluateRHS and fdt = EvaluateDt (DNDS_ODE.hpp:131-178); the stage times
+    follow the reference's frhs arguments (t + dt, t + dt/4, t + 2dt/3)."""
+
+    def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, MaxDt: float = 1e100,
+                 UseLocaldt: bool = False):
+        self.ev = evaluator
+        self.n_sweeps = nInternalRecStep
+        self.CFL = CFL
+        self.MaxDt = MaxDt
+        self.UseLocaldt = UseLocaldt
+
+    def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, t: float) -> float:
+        """Advances u in place; returns the global minimum dt of the step."""
+        dt = C.c_double()
+        d = self.ev.device
+        _check(_lib().dndsmi_ssprk3_step(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, self.CFL, self.MaxDt,
+                                         int(self.UseLocaldt), t, C.byref(dt)), "SSP-RK3 step", d.ctx)
+        return dt.value

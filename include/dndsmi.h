@@ -1,0 +1,271 @@
+/*
+ * dndsmi.h -- C ABI of the MI355X (gfx950) implementation of DNDS's 2D
+ * variational-reconstruction finite-volume hot path.
+ *
+ * Everything here is plain C: pointers, sizes and status codes.  No HIP,
+ * torch or C++ type crosses this boundary.  Host arrays are caller-owned and
+ * are only read or written during the call that receives them; device state
+ * lives in objects owned by the library (dndsmi_ctx, dndsmi_array).
+ *
+ * Reference interfaces replaced (paths relative to the DNDS source tree):
+ *   mesh topology       CompactFacedMeshSerialRW, InterpolateTopology
+ *                       DNDS_Mesh.hpp:238-350, 634-709, 1330-1361
+ *   geometry + VR setup ImplicitFiniteVolume2D ctor DNDS_FV_VR.hpp:29-102,
+ *                       VRFiniteVolume2D::Initialization DNDS_FV_VR.cpp:3-13
+ *   Jacobi sweep        VRFiniteVolume2D::ReconstructionJacobiStep
+ *                       DNDS_FV_VR.hpp:1474-1713
+ *   time step           EulerEvaluator::EvaluateDt DNDS_FV_EulerEvaluator.cpp:14-183
+ *   right-hand side     EulerEvaluator::EvaluateRHS DNDS_FV_EulerEvaluator.cpp:197-640
+ *   SSP-RK3 step        ODE::ExplicitSSPRK3LocalDt::Step DNDS_ODE.hpp:131-178
+ *
+ * Every function returns DNDSMI_OK (0) or one of the error codes below; the
+ * message of the last failure of a context is returned by dndsmi_last_error.
+ * The reference aborts on the same conditions (DNDS_assert, DNDS_Defines.h:31-44);
+ * the C++ facade in dndsmi.hpp restores that behaviour.
+ */
+#ifndef DNDSMI_H
+#define DNDSMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    DNDSMI_OK = 0,
+    DNDSMI_ERR_INVALID_ARGUMENT = 1,
+    DNDSMI_ERR_HIP = 2,
+    DNDSMI_ERR_NAN = 3,              /* non-finite value detected (DNDS_FV_EulerEvaluator.hpp:576-588) */
+    DNDSMI_ERR_NONPOSITIVE_STATE = 4,/* rho <= 0 or negative sound speed squared in a flux (DNDS_Gas.hpp:322, 438) */
+    DNDSMI_ERR_UNSUPPORTED = 5,      /* option outside the implemented set */
+    DNDSMI_ERR_OUT_OF_MEMORY = 6,
+    DNDSMI_ERR_SETUP = 7             /* degenerate geometry during setup (DNDS_FV_VR.hpp:67-69, DNDS_FV_VR.cpp:579) */
+};
+
+/* ---- boundary kinds: the values of BoundaryType, DNDS_Mesh.hpp:21-35 ----- */
+enum {
+    DNDSMI_BC_INNER = -1,
+    DNDSMI_BC_WALL = 1,
+    DNDSMI_BC_FARFIELD = 2,
+    DNDSMI_BC_WALL_EULER = 3,
+    DNDSMI_BC_WALL_NOSLIP = 4,
+    DNDSMI_BC_SPECIAL_DMRFAR = -2,
+    DNDSMI_BC_SPECIAL_RTFAR = -3,
+    DNDSMI_BC_SPECIAL_IVFAR = -4,
+    DNDSMI_BC_SPECIAL_2DRIEMANNFAR = -5
+};
+
+/* second entry of face2cell for a boundary face (FACE_2_VOL_EMPTY, DNDS_Mesh.hpp:19) */
+#define DNDSMI_NO_CELL INT64_MIN
+
+/* ======================================================================== */
+/* Mesh                                                                     */
+/* ======================================================================== */
+
+/*
+ * A 2D mesh of linear triangles (3 nodes) and quadrilaterals (4 nodes),
+ * counter-clockwise, with the reference's conventions (DNDS_Mesh.hpp:238-350):
+ * faces are numbered in order of first appearance when walking the cells in
+ * order and each cell's faces in local order (tri 01,12,20; quad 01,12,23,30),
+ * the face's nodes and its first cell f2c[0] come from that first walk, so the
+ * face normal (dy, -dx) points out of f2c[0].  A periodic face is an inner
+ * face whose second cell lies across the domain; face_shift[f] is the
+ * translation that moves f2c[1]'s nodes next to the face (zero otherwise).
+ */
+typedef struct dndsmi_mesh_view {
+    int64_t n_node, n_cell, n_face;
+    const double *node_xy;         /* [n_node][2]                            */
+    const int64_t *cell_ptr;       /* [n_cell+1] offsets into cell2node and cell2face */
+    const int64_t *cell2node;      /* CCW node lists                          */
+    const int64_t *cell2face;      /* local face order                        */
+    const int64_t *face2node;      /* [n_face][2]                            */
+    const int64_t *face2cell;      /* [n_face][2], DNDSMI_NO_CELL if boundary */
+    const int32_t *face2cell_ref;  /* [n_face][2] local face index in each cell (face2cellRefLocal) */
+    const int32_t *face_bc;        /* [n_face] DNDSMI_BC_*                    */
+    const double *face_shift;      /* [n_face][2]                            */
+} dndsmi_mesh_view;
+
+enum {
+    DNDSMI_MESH_TRI_SPLIT = 0, /* each quad cut along a diagonal chosen by a seeded coin flip */
+    DNDSMI_MESH_QUAD = 1,      /* quads only */
+    DNDSMI_MESH_MIXED = 2      /* each quad kept (p = 1/2, seeded) or cut as in TRI_SPLIT */
+};
+
+typedef struct dndsmi_mesh_spec {
+    int32_t kind;                  /* DNDSMI_MESH_*                           */
+    int32_t periodic_x, periodic_y;
+    int64_t nx, ny;                /* quads per direction                      */
+    double x0, x1, y0, y1;         /* domain                                   */
+    double jitter;                 /* interior nodes moved by U(-j, j) * local spacing, per axis */
+    double y_ratio;                /* geometric ratio of successive y spacings (1 = uniform) */
+    uint64_t seed;
+    int32_t bc_xmin, bc_xmax, bc_ymin, bc_ymax; /* DNDSMI_BC_* of non-periodic sides */
+} dndsmi_mesh_spec;
+
+typedef struct dndsmi_mesh dndsmi_mesh;
+
+/* Deterministic seeded generator of structured-as-unstructured meshes. */
+int dndsmi_mesh_generate(const dndsmi_mesh_spec *spec, dndsmi_mesh **out);
+/* Topology from raw cells (the Gmsh-reader path): nodes, CCW cells, and the
+ * boundary edges with their DNDSMI_BC_* kind; faces are built as the
+ * reference's InterpolateTopology does (DNDS_Mesh.hpp:238-350). */
+int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const int64_t *cell_ptr,
+                           const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node, const int32_t *bnd_bc,
+                           dndsmi_mesh **out);
+int dndsmi_mesh_get_view(const dndsmi_mesh *mesh, dndsmi_mesh_view *view);
+void dndsmi_mesh_destroy(dndsmi_mesh *mesh);
+
+/* ======================================================================== */
+/* VR setup (host, FP64)                                                    */
+/* ======================================================================== */
+
+/* VRFiniteVolume2D::Setting, DNDS_FV_VR.hpp:170-230 (names kept) */
+typedef struct dndsmi_vr_setting {
+    int32_t rec_order;              /* P_ORDER, only 3 is implemented        */
+    double jacobi_relax;            /* JacobiRelax                            */
+    double wall_weight, far_weight; /* wallWeight, farWeight                  */
+    double tang_weight;             /* tangWeight                             */
+    double tang_weight_mod_min;     /* tangWeightModMin (only 1 implemented)   */
+    int32_t tang_weight_direction;  /* 0 = TWD_Norm, 1 = TWD_Bary             */
+    int32_t anisotropic_lengths;    /* anisotropicLengths                     */
+    int32_t weight_scheme_geom;     /* 0 None, 1 D, 2 S, 3 SDHQM              */
+    int32_t weight_scheme_dir;      /* 0 None, 1 optHQM                       */
+    double weight_len_scale;        /* weightLenScale                         */
+    double wbap_smooth_indicator_scale; /* WBAP_SmoothIndicatorScale          */
+    double wbap_n_std;              /* WBAP_nStd                              */
+    int32_t use_local_coord;        /* must be 0                              */
+    int32_t orthogonalize_base;     /* must be 0                              */
+    int32_t norm_wbap;              /* must be 0                              */
+    int32_t build_limiter_data;     /* also build the WBAP face data          */
+} dndsmi_vr_setting;
+
+/* The reference's defaults, DNDS_FV_VR.hpp:172-228. */
+void dndsmi_vr_setting_default(dndsmi_vr_setting *s);
+
+/*
+ * Setup products, host FP64, flattened in the reference's logical layout.
+ * Row-major small matrices; 9 = NDOF-1 rec dofs at p=3, 10 = NDIFF.
+ */
+typedef struct dndsmi_vr_view {
+    int64_t n_cell, n_face, n_c2f;    /* n_c2f = cell_ptr[n_cell]                  */
+    const double *volume;             /* [n_cell] volumeLocal (TRI_7 / QUAD_9)     */
+    const double *face_area;          /* [n_face] faceArea (LINE_5)                */
+    const double *cell_bary;          /* [n_cell][2] cellBaries (TRI_4 / QUAD_9)   */
+    const double *base_moment;        /* [n_cell][10] baseMoments                  */
+    const double *face_center;        /* [n_face][2] faceCenters                   */
+    const double *face_norm_center;   /* [n_face][2] faceNormCenter                */
+    const double *face_norm;          /* [n_face][3][2] faceNorms at LINE_3 points */
+    const double *face_weight;        /* [n_face][10] faceWeights                  */
+    const double *face_d0;            /* [n_face][3][2][9] row 0 of faceDiBjGaussBatch, cols 1..9 */
+    const double *face_dibj_g0;       /* [n_face][2][10][9] all rows at LINE_3 point 0 (NULL unless limiter data) */
+    const double *matrix_secondary;   /* [n_face][2][9][9] matrixSecondaryBatch (NULL unless limiter data) */
+    const double *ainv;               /* [n_cell][9][9] matrixBatch.m(0)          */
+    const double *ainv_b;             /* [n_c2f][9][9] matrixBatch.m(ic2f+1)      */
+    const double *ainv_bvec;          /* [n_c2f][9] vectorBatch rows              */
+} dndsmi_vr_view;
+
+typedef struct dndsmi_vr dndsmi_vr;
+
+/* ImplicitFiniteVolume2D ctor + VRFiniteVolume2D::Initialization on the host,
+ * nthreads worker threads (0 = hardware concurrency). */
+int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_setting *setting, int32_t nthreads,
+                     dndsmi_vr **out);
+int dndsmi_vr_get_view(const dndsmi_vr *vr, dndsmi_vr_view *view);
+void dndsmi_vr_destroy(dndsmi_vr *vr);
+
+/* ======================================================================== */
+/* Device context and arrays                                                */
+/* ======================================================================== */
+
+/* EulerEvaluator<NS_2D>::Setting, DNDS_FV_EulerEvaluator.hpp:139-196 (Euler subset) */
+typedef struct dndsmi_euler_setting {
+    int32_t riemann_solver;           /* 1 Roe, 2 HLLC (RiemannSolverType)      */
+    double gamma;                     /* idealGasProperty.gamma                 */
+    double far_field[4];              /* farFieldStaticValue (rho, rhou, rhov, E) */
+} dndsmi_euler_setting;
+
+enum { DNDSMI_RS_ROE = 1, DNDSMI_RS_HLLC = 2 };
+
+typedef struct dndsmi_ctx dndsmi_ctx;
+typedef struct dndsmi_array dndsmi_array;
+
+/* Array kinds: DOF = nvar doubles per cell (ArrayDOFV<4>), REC = 9 x nvar per
+ * cell (ArrayRecV), SCALAR = one double per cell (dt, ifUseLimiter). */
+enum { DNDSMI_ARRAY_DOF = 0, DNDSMI_ARRAY_REC = 1, DNDSMI_ARRAY_SCALAR = 2 };
+
+/*
+ * Uploads the mesh and the setup to the current HIP device.  `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream); every call on the
+ * context is asynchronous on it unless it returns host data.
+ */
+int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_view *vr, const dndsmi_vr_setting *vsetting,
+                      const dndsmi_euler_setting *esetting, void *stream, dndsmi_ctx **out);
+void dndsmi_ctx_destroy(dndsmi_ctx *ctx);
+int dndsmi_last_error(const dndsmi_ctx *ctx, char *buf, size_t n);
+int dndsmi_sync(dndsmi_ctx *ctx);
+/* Bytes of device memory held by the context (setup + scratch, not arrays). */
+int64_t dndsmi_ctx_device_bytes(const dndsmi_ctx *ctx);
+
+int dndsmi_array_create(dndsmi_ctx *ctx, int32_t kind, dndsmi_array **out);
+void dndsmi_array_destroy(dndsmi_array *a);
+/* Host data in the reference layout: DOF [n_cell][4]; REC per cell column-major
+ * 9x4, element (r, v) at [cell*36 + r + 9*v] (VarVector::m_by_ncol); SCALAR [n_cell]. */
+int dndsmi_array_upload(dndsmi_array *a, const double *host);
+int dndsmi_array_download(const dndsmi_array *a, double *host);
+int dndsmi_array_copy(dndsmi_array *dst, const dndsmi_array *src);
+int dndsmi_array_set_constant(dndsmi_array *a, double value);
+
+/* ReconstructionJacobiStep(u, uRec, uRecNewBuf, FBoundary) with SOR_Instead =
+ * false: uRecNewBuf receives the sweep, then uRec = uRecNewBuf.  t is the time
+ * handed to the boundary function of Wall_NoSlip faces. */
+int dndsmi_reconstruction_jacobi_step(dndsmi_ctx *ctx, const dndsmi_array *u, dndsmi_array *uRec,
+                                      dndsmi_array *uRecNewBuf, double t);
+
+/* EvaluateDt(dt, u, CFL, dtMinall, MaxDt, UseLocaldt); also refreshes the
+ * evaluator's lambdaFace / deltaLambdaFace.  dtMinall may be NULL (no host sync). */
+int dndsmi_evaluate_dt(dndsmi_ctx *ctx, dndsmi_array *dt, const dndsmi_array *u, double CFL, double *dtMinall,
+                       double MaxDt, int32_t UseLocaldt);
+
+/* EvaluateRHS(rhs, u, uRec, t). */
+int dndsmi_evaluate_rhs(dndsmi_ctx *ctx, dndsmi_array *rhs, const dndsmi_array *u, const dndsmi_array *uRec,
+                        double t);
+
+/* One ExplicitSSPRK3LocalDt::Step over frhs = n_sweeps Jacobi sweeps then
+ * EvaluateRHS, fdt = EvaluateDt(CFL, MaxDt, UseLocaldt).  Stage times follow
+ * DNDS_ODE.hpp:138-151 (t + dt, t + dt/4, t + 2dt/3).  `t` is advanced by the
+ * global minimum dt when dt_out is not NULL (*dt_out receives it). */
+int dndsmi_ssprk3_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                       int32_t n_sweeps, double CFL, double MaxDt, int32_t UseLocaldt, double t, double *dt_out);
+
+/* ======================================================================== */
+/* Measurement                                                              */
+/* ======================================================================== */
+
+/* Kernels of the hot path, for per-kernel timing and byte counts. */
+enum {
+    DNDSMI_K_JACOBI = 0,     /* Jacobi sweep                                 */
+    DNDSMI_K_WALL_CORR = 1,  /* Wall_Euler / Wall_NoSlip correction of a sweep */
+    DNDSMI_K_DT_FACE = 2,    /* EvaluateDt, face part                         */
+    DNDSMI_K_DT_CELL = 3,    /* EvaluateDt, cell part + block minima          */
+    DNDSMI_K_RHS_FACE = 4,   /* EvaluateRHS, inner-face fluxes                */
+    DNDSMI_K_RHS_CELL = 5,   /* EvaluateRHS gather (+ SSP-RK3 stage update)   */
+    DNDSMI_K_COUNT = 6
+};
+
+/* Turns HIP-event brackets around every launch on or off and clears the
+ * records (synchronises the stream).  No counterpart in the reference, which
+ * times with MPI_Wtime (DNDS_Profiling.h:7-37). */
+int dndsmi_profile_enable(dndsmi_ctx *ctx, int32_t on);
+/* Number of recorded launches of `kernel` and their summed duration (ms). */
+int dndsmi_profile_query(dndsmi_ctx *ctx, int32_t kernel, int64_t *count, double *total_ms);
+/* Algorithmic HBM bytes of one launch of `kernel` on this context's mesh. */
+int64_t dndsmi_kernel_bytes(const dndsmi_ctx *ctx, int32_t kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNDSMI_H */

@@ -1,0 +1,766 @@
+// CPU FP64 restatement of DNDS's 2D VR finite-volume hot path.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker for the HIP kernels and the
+// CPU baseline of the benchmark; no product code uses it.  Every function
+// cites the reference lines it restates; loops run in the reference's order
+// (face loop with scatter in face order for the RHS and time step, cell loop
+// in cell2face order for the Jacobi sweep) and expressions keep the
+// reference's operation order.  Dense products whose order is fixed by Eigen
+// internals in the reference (GEMM blocking, vectorised reductions) are
+// summed in ascending index order here.
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr double kVerySmallReal = 1e-200;   // DNDS_Defines.h:70
+constexpr double kVeryLargeReal = 3e200;    // DNDS_Defines.h:67
+const double kPi = std::acos(-1);
+const double kLine3W[3] = {5. / 9., 8. / 9., 5. / 9.};
+constexpr int64_t kNoCell = INT64_MIN;
+
+enum {
+    BC_INNER = -1, BC_WALL = 1, BC_FARFIELD = 2, BC_WALL_EULER = 3, BC_WALL_NOSLIP = 4,
+    BC_DMRFAR = -2, BC_RTFAR = -3, BC_IVFAR = -4, BC_2DRIEMANNFAR = -5
+};
+enum { ERR_OK = 0, ERR_INVALID = 1, ERR_NAN = 3, ERR_NONPOSITIVE = 4, ERR_UNSUPPORTED = 5 };
+
+inline double sqr(double x) { return x * x; }
+
+bool is_far(int32_t bc) {
+    return bc == BC_FARFIELD || bc == BC_DMRFAR || bc == BC_RTFAR || bc == BC_IVFAR || bc == BC_2DRIEMANNFAR;
+}
+
+// Gas::IdealGasThermal, DNDS_Gas.hpp:73-79
+inline void ideal_gas_thermal(double E, double rho, double vSqr, double gamma, double &p, double &asqr, double &H) {
+    p = (gamma - 1) * (E - rho * 0.5 * vSqr);
+    asqr = gamma * p / rho;
+    H = (E + p) / rho;
+}
+
+// IdealGasThermalConservative2Primitive, DNDS_Gas.hpp:81-92
+inline void cons2prim(const double U[4], double prim[4], double gamma) {
+    for (int k = 0; k < 4; k++)
+        prim[k] = U[k] / U[0];
+    double v0 = U[1] / U[0], v1 = U[2] / U[0];
+    double vSqr = v0 * v0 + v1 * v1;
+    double rho = U[0];
+    double E = U[3];
+    double p = (gamma - 1) * (E - rho * 0.5 * vSqr);
+    prim[0] = rho;
+    prim[3] = p;
+}
+
+// IdealGasThermalPrimitive2Conservative, DNDS_Gas.hpp:94-105
+inline void prim2cons(const double prim[4], double U[4], double gamma) {
+    for (int k = 0; k < 4; k++)
+        U[k] = prim[k] * prim[0];
+    double vSqr = prim[1] * prim[1] + prim[2] * prim[2];
+    double rho = prim[0];
+    double p = prim[3];
+    double E = p / (gamma - 1) + rho * 0.5 * vSqr;
+    U[0] = rho;
+    U[3] = E;
+}
+
+// GasInviscidFlux, DNDS_Gas.hpp:113-119 (x direction)
+inline void inviscid_flux(const double U[4], double vx, double p, double F[4]) {
+    for (int k = 0; k < 4; k++)
+        F[k] = U[k] * vx;
+    F[1] += p;
+    F[3] += vx * p;
+}
+
+// EulerGasRightEigenVector / EulerGasLeftEigenVector, DNDS_Gas.hpp:17-71, dim = 2
+void right_eigen(const double velo[2], double vsqr, double H, double a, double R[4][4]) {
+    std::memset(R, 0, sizeof(double) * 16);
+    R[0][0] = R[0][1] = R[0][3] = 1;
+    R[2][2] = 1;
+    R[1][0] = velo[0], R[1][1] = velo[0], R[1][3] = velo[0];
+    R[2][0] = velo[1], R[2][1] = velo[1], R[2][3] = velo[1];
+    R[1][0] -= a;
+    R[1][3] += a;
+    R[3][0] = H - velo[0] * a;
+    R[3][3] = H + velo[0] * a;
+    R[3][1] = 0.5 * vsqr;
+    R[3][2] = velo[1];
+}
+
+void left_eigen(const double velo[2], double vsqr, double H, double a, double gamma, double L[4][4]) {
+    (void)vsqr;
+    std::memset(L, 0, sizeof(double) * 16);
+    double gammaBar = gamma - 1;
+    L[0][0] = H + a / gammaBar * (velo[0] - a);
+    L[0][1] = -velo[0] - a / gammaBar;
+    L[0][2] = -velo[1];
+    L[0][3] = 1;
+    L[1][0] = -2 * H + 4 / gammaBar * (a * a);
+    L[1][1] = velo[0] * 2;
+    L[1][2] = velo[1] * 2;
+    L[1][3] = -2;
+    L[2][0] = velo[1] * (-2 * (a * a) / gammaBar);
+    L[2][2] = 2 * (a * a) / gammaBar;
+    L[3][0] = H - a / gammaBar * (velo[0] + a);
+    L[3][1] = -velo[0] + a / gammaBar;
+    L[3][2] = -velo[1];
+    L[3][3] = 1;
+    const double sc = gammaBar / (2 * a * a);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            L[i][j] *= sc;
+}
+
+// Gas::RoeFlux_IdealGas_HartenYee<2, 0>, DNDS_Gas.hpp:405-603
+int roe_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4]) {
+    const double scaleHartenYee = 0.05;
+    if (!(UL[0] > 0 && UR[0] > 0))
+        return ERR_NONPOSITIVE;
+    double veloL[2] = {UL[1] / UL[0], UL[2] / UL[0]};
+    double veloR[2] = {UR[1] / UR[0], UR[2] / UR[0]};
+    double asqrL, asqrR, pL, pR, HL, HR;
+    double vsqrL = veloL[0] * veloL[0] + veloL[1] * veloL[1];
+    double vsqrR = veloR[0] * veloR[0] + veloR[1] * veloR[1];
+    ideal_gas_thermal(UL[3], UL[0], vsqrL, gamma, pL, asqrL, HL);
+    ideal_gas_thermal(UR[3], UR[0], vsqrR, gamma, pR, asqrR, HR);
+    double sqrtRhoL = std::sqrt(UL[0]);
+    double sqrtRhoR = std::sqrt(UR[0]);
+    double veloRoe[2];
+    for (int k = 0; k < 2; k++)
+        veloRoe[k] = (sqrtRhoL * veloL[k] + sqrtRhoR * veloR[k]) / (sqrtRhoL + sqrtRhoR);
+    double vsqrRoe = veloRoe[0] * veloRoe[0] + veloRoe[1] * veloRoe[1];
+    double HRoe = (sqrtRhoL * HL + sqrtRhoR * HR) / (sqrtRhoL + sqrtRhoR);
+    double asqrRoe = (gamma - 1) * (HRoe - 0.5 * vsqrRoe);
+    double FL[4], FR[4];
+    inviscid_flux(UL, veloL[0], pL, FL);
+    inviscid_flux(UR, veloR[0], pR, FR);
+    if (!(asqrRoe > 0))
+        return ERR_NONPOSITIVE;
+    double aRoe = std::sqrt(asqrRoe);
+    double lam0 = std::abs(veloRoe[0] - aRoe);
+    double lam123 = std::abs(veloRoe[0]);
+    double lam4 = std::abs(veloRoe[0] + aRoe);
+    double thresholdHartenYee = std::max(scaleHartenYee * (std::sqrt(vsqrRoe) + aRoe), dLambda);
+    double thresholdHartenYeeS = thresholdHartenYee * thresholdHartenYee;
+    if (lam0 < thresholdHartenYee)
+        lam0 = (sqr(lam0) + thresholdHartenYeeS) / (2 * thresholdHartenYee);
+    if (lam4 < thresholdHartenYee)
+        lam4 = (sqr(lam4) + thresholdHartenYeeS) / (2 * thresholdHartenYee);
+    double lam[4] = {lam0, lam123, lam123, lam4};
+    double R[4][4];
+    right_eigen(veloRoe, vsqrRoe, HRoe, aRoe, R);
+    double incU[4];
+    for (int k = 0; k < 4; k++)
+        incU[k] = UR[k] - UL[k];
+    double alpha[4];
+    alpha[2] = incU[2] - veloRoe[1] * incU[0];
+    double incU4b = incU[3] - alpha[2] * veloRoe[1];
+    alpha[1] = (gamma - 1) / asqrRoe * (incU[0] * (HRoe - veloRoe[0] * veloRoe[0]) + veloRoe[0] * incU[1] - incU4b);
+    alpha[0] = (incU[0] * (veloRoe[0] + aRoe) - incU[1] - aRoe * alpha[1]) / (2 * aRoe);
+    alpha[3] = incU[0] - (alpha[0] + alpha[1]);
+    double la[4];
+    for (int k = 0; k < 4; k++)
+        la[k] = lam[k] * alpha[k];
+    for (int r = 0; r < 4; r++) {
+        double incF = 0;
+        for (int c = 0; c < 4; c++)
+            incF += R[r][c] * la[c];
+        F[r] = (FL[r] + FR[r]) * 0.5 - 0.5 * incF;
+    }
+    return ERR_OK;
+}
+
+// Gas::HLLCFlux_IdealGas_HartenYee<2>, DNDS_Gas.hpp:312-403
+int hllc_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4]) {
+    (void)dLambda;
+    if (!(UL[0] > 0 && UR[0] > 0))
+        return ERR_NONPOSITIVE;
+    double veloL[2] = {UL[1] / UL[0], UL[2] / UL[0]};
+    double veloR[2] = {UR[1] / UR[0], UR[2] / UR[0]};
+    double asqrL, asqrR, pL, pR, HL, HR;
+    double vsqrL = veloL[0] * veloL[0] + veloL[1] * veloL[1];
+    double vsqrR = veloR[0] * veloR[0] + veloR[1] * veloR[1];
+    ideal_gas_thermal(UL[3], UL[0], vsqrL, gamma, pL, asqrL, HL);
+    ideal_gas_thermal(UR[3], UR[0], vsqrR, gamma, pR, asqrR, HR);
+    double sqrtRhoL = std::sqrt(UL[0]);
+    double sqrtRhoR = std::sqrt(UR[0]);
+    double veloRoe0 = (sqrtRhoL * veloL[0] + sqrtRhoR * veloR[0]) / (sqrtRhoL + sqrtRhoR);
+    double eta2 = 0.5 * (sqrtRhoL * sqrtRhoR) / sqr(sqrtRhoL + sqrtRhoR);
+    double dsqr = (asqrL * sqrtRhoL + asqrR * sqrtRhoR) / (sqrtRhoL + sqrtRhoR) + eta2 * sqr(veloR[0] - veloL[0]);
+    if (!(dsqr > 0))
+        return ERR_NONPOSITIVE;
+    double SL = veloRoe0 - std::sqrt(dsqr);
+    double SR = veloRoe0 + std::sqrt(dsqr);
+    double FL[4], FR[4];
+    inviscid_flux(UL, veloL[0], pL, FL);
+    inviscid_flux(UR, veloR[0], pR, FR);
+    if (0 <= SL) {
+        for (int k = 0; k < 4; k++)
+            F[k] = FL[k];
+        return ERR_OK;
+    }
+    if (SR <= 0) {
+        for (int k = 0; k < 4; k++)
+            F[k] = FR[k];
+        return ERR_OK;
+    }
+    double SS = 0;
+    double div = (UL[0] * (SL - veloL[0]) - UR[0] * (SR - veloR[0]));
+    if (std::abs(div) > kVerySmallReal)
+        SS = (pR - pL + UL[1] * (SL - veloL[0]) - UR[1] * (SR - veloR[0])) / div;
+    double DS[4] = {0, 1, 0, SS};
+    if (SS >= 0) {
+        double dv = SL - SS;
+        if (std::abs(dv) < kVerySmallReal)
+            for (int k = 0; k < 4; k++)
+                F[k] = FL[k];
+        else {
+            double X = (pL + UL[0] * (SL - veloL[0]) * (SS - veloL[0])) * SL;
+            for (int k = 0; k < 4; k++)
+                F[k] = ((UL[k] * SL - FL[k]) * SS + DS[k] * X) / dv;
+        }
+    } else {
+        double dv = SR - SS;
+        if (std::abs(dv) < kVerySmallReal)
+            for (int k = 0; k < 4; k++)
+                F[k] = FR[k];
+        else {
+            double X = (pR + UR[0] * (SR - veloR[0]) * (SS - veloR[0])) * SR;
+            for (int k = 0; k < 4; k++)
+                F[k] = ((UR[k] * SR - FR[k]) * SS + DS[k] * X) / dv;
+        }
+    }
+    return ERR_OK;
+}
+
+// generateBoundaryValue, DNDS_FV_EulerEvaluator.hpp:958-1275 (dim = 2)
+int boundary_value(const oracle_setting &cfg, const double UL[4], const double n[2], const double p[2], double t,
+                   int32_t bc, double UR[4]) {
+    if (!(UL[0] > 0))
+        return ERR_NONPOSITIVE;
+    const double gamma = cfg.gamma;
+    if (bc == BC_FARFIELD) {
+        const double *far = cfg.far_field;
+        double un = (UL[1] * n[0] + UL[2] * n[1]) / UL[0];
+        double vsqr = sqr(UL[1] / UL[0]) + sqr(UL[2] / UL[0]);
+        double asqr, H, pp;
+        ideal_gas_thermal(UL[3], UL[0], vsqr, gamma, pp, asqr, H);
+        if (!(asqr >= 0))
+            return ERR_NONPOSITIVE;
+        double a = std::sqrt(asqr);
+        if (un - a > 0) {
+            for (int k = 0; k < 4; k++)
+                UR[k] = UL[k];
+        } else if (un > 0) {
+            double farPrim[4], ULPrim[4];
+            cons2prim(far, farPrim, gamma);
+            cons2prim(UL, ULPrim, gamma);
+            ULPrim[3] = farPrim[3];
+            prim2cons(ULPrim, UR, gamma);
+        } else if (un + a > 0) {
+            double farPrim[4], ULPrim[4];
+            cons2prim(far, farPrim, gamma);
+            cons2prim(UL, ULPrim, gamma);
+            farPrim[3] = ULPrim[3];
+            prim2cons(farPrim, UR, gamma);
+        } else {
+            for (int k = 0; k < 4; k++)
+                UR[k] = far[k];
+        }
+        return ERR_OK;
+    }
+    if (bc == BC_DMRFAR) {
+        const double uShock = 10;
+        if (((p[0] - uShock / std::sin(kPi / 3) * t - 1. / 6.) - p[1] / std::tan(kPi / 3)) > 0) {
+            UR[0] = 1.4, UR[1] = 0, UR[2] = 0, UR[3] = 2.5;
+        } else {
+            UR[0] = 8, UR[1] = 57.157676649772960, UR[2] = -33, UR[3] = 5.635e2;
+        }
+        return ERR_OK;
+    }
+    if (bc == BC_RTFAR) {
+        double far[4] = {cfg.far_field[0], cfg.far_field[1], cfg.far_field[2], cfg.far_field[3]};
+        double vsqr = sqr(UL[1] / UL[0]) + sqr(UL[2] / UL[0]);
+        double asqr, H, pp;
+        ideal_gas_thermal(UL[3], UL[0], vsqr, gamma, pp, asqr, H);
+        if (!(asqr >= 0))
+            return ERR_NONPOSITIVE;
+        double a = std::sqrt(asqr);
+        double v = -0.025 * a * std::cos(p[0] * 8 * kPi);
+        double rho = p[1] < 0.5 ? 2 : 1;
+        double pr = p[1] < 0.5 ? 1 : 2.5;
+        far[0] = rho;
+        far[1] = 0;
+        far[2] = rho * v;
+        far[3] = 0.5 * rho * sqr(v) + pr / (gamma - 1);
+        for (int k = 0; k < 4; k++) // the reference overrides every branch with `far`
+            UR[k] = far[k];
+        return ERR_OK;
+    }
+    if (bc == BC_IVFAR) {
+        double chi = 5;
+        double xc = 5 + t;
+        double yc = 5 + t;
+        double r = std::sqrt(sqr(p[0] - xc) + sqr(p[1] - yc));
+        double dT = -(gamma - 1) / (8 * gamma * sqr(kPi)) * sqr(chi) * std::exp(1 - sqr(r));
+        double dux = chi / 2 / kPi * std::exp((1 - sqr(r)) / 2) * -(p[1] - xc);
+        double duy = chi / 2 / kPi * std::exp((1 - sqr(r)) / 2) * +(p[0] - yc);
+        double T = dT + 1;
+        double ux = dux + 1;
+        double uy = duy + 1;
+        double S = 1;
+        double rho = std::pow(T / S, 1 / (gamma - 1));
+        double pr = T * rho;
+        double E = 0.5 * (sqr(ux) + sqr(uy)) * rho + pr / (gamma - 1);
+        UR[0] = rho, UR[1] = rho * ux, UR[2] = rho * uy, UR[3] = E;
+        return ERR_OK;
+    }
+    if (bc == BC_2DRIEMANNFAR) {
+        const double bdL = 0.0, bdR = 1.0, bdD = 0.0, bdU = 1.0;
+        const double phi1 = -0.663324958071080;
+        const double phi2 = -0.422115882408869;
+        const double location = 0.8;
+        double p1 = location + phi1 * t;
+        double p2 = location + phi2 * t;
+        double rho, u, v, pre;
+        double ULPrim[4];
+        cons2prim(UL, ULPrim, gamma);
+        double rhoL = ULPrim[0], uL = ULPrim[1], vL = ULPrim[2], preL = ULPrim[3];
+        double farPrim[4] = {ULPrim[0], ULPrim[1], ULPrim[2], ULPrim[3]};
+        const double bTol = 1e-9;
+        if (std::abs(p[0] - bdL) < bTol) {
+            if (p[1] <= p2)
+                rho = 0.137992831541219, u = 1.206045378311055, v = 1.206045378311055, pre = 0.029032258064516;
+            else
+                rho = 0.532258064516129, u = 1.206045378311055, v = 0.0, pre = 0.3;
+        } else if (std::abs(p[0] - bdR) < bTol) {
+            rho = rhoL, u = -uL, v = vL, pre = preL;
+        } else if (std::abs(p[1] - bdU) < bTol) {
+            rho = rhoL, u = uL, v = -vL, pre = preL;
+        } else if (std::abs(p[1] - bdD) < bTol) {
+            if (p[0] <= p2)
+                rho = 0.137992831541219, u = 1.206045378311055, v = 1.206045378311055, pre = 0.029032258064516;
+            else
+                rho = 0.532258064516129, u = 0.0, v = 1.206045378311055, pre = 0.3;
+        } else {
+            return ERR_INVALID; // not a valid boundary position
+        }
+        (void)p1;
+        farPrim[0] = rho;
+        farPrim[1] = u, farPrim[2] = v;
+        farPrim[3] = pre;
+        prim2cons(farPrim, UR, gamma);
+        return ERR_OK;
+    }
+    if (bc == BC_WALL_EULER) {
+        for (int k = 0; k < 4; k++)
+            UR[k] = UL[k];
+        double d = UR[1] * n[0] + UR[2] * n[1];
+        UR[1] -= d * n[0];
+        UR[2] -= d * n[1];
+        return ERR_OK;
+    }
+    if (bc == BC_WALL_NOSLIP) {
+        for (int k = 0; k < 4; k++)
+            UR[k] = UL[k];
+        UR[1] *= -1;
+        UR[2] *= -1;
+        return ERR_OK;
+    }
+    return ERR_UNSUPPORTED; // Wall is not a proper flux boundary (DNDS_FV_EulerEvaluator.hpp:1265-1269)
+}
+
+// CompressRecPart, DNDS_FV_EulerEvaluator.hpp:1277-1360
+inline void compress_rec_part(const double umean[4], const double inc[4], double ret[4]) {
+    for (int k = 0; k < 4; k++)
+        ret[k] = umean[k] + inc[k];
+    if (ret[0] < 0)
+        for (int k = 0; k < 4; k++)
+            ret[k] = umean[k];
+    double eK = (ret[1] * ret[1] + ret[2] * ret[2]) * 0.5 / (kVerySmallReal + ret[0]);
+    double e = ret[3] - eK;
+    if (e < 0)
+        for (int k = 0; k < 4; k++)
+            ret[k] = umean[k];
+}
+
+// Eigen's stableNormalized of a 2-vector
+inline void stable_normalized(double x, double y, double &ox, double &oy) {
+    double w = std::max(std::fabs(x), std::fabs(y));
+    double zx = x / w, zy = y / w;
+    double z = zx * zx + zy * zy;
+    if (z > 0) {
+        double s = std::sqrt(z);
+        ox = (x / s) / w, oy = (y / s) / w;
+    } else
+        ox = x, oy = y;
+}
+
+// Eigen's normalized(): v / sqrt(squaredNorm)
+inline void normalized(double x, double y, double &ox, double &oy) {
+    double z = x * x + y * y;
+    if (z > 0) {
+        double s = std::sqrt(z);
+        ox = x / s, oy = y / s;
+    } else
+        ox = x, oy = y;
+}
+
+// EulerEvaluator::fluxFace, DNDS_FV_EulerEvaluator.hpp:371-591, Euler model
+// (the viscous flux vanishes for muGas = 0).  Returns -F and the
+// flux-balance fluxes FLfix / FRfix of the cell means.
+int flux_face(const oracle_setting &cfg, const double ULxy[4], const double URxy[4], const double ULMeanXy[4],
+              const double URMeanXy[4], const double n[2], double dLambda, double out[4], double FLfix[4],
+              double FRfix[4]) {
+    const double gamma = cfg.gamma;
+    // normBase = [n, (-n1, n0)] (NormBuildLocalBaseV, DNDS_Elements.hpp:1466-1488)
+    auto rotT = [&](const double in[4], double o[4]) {
+        o[0] = in[0];
+        o[1] = n[0] * in[1] + n[1] * in[2];
+        o[2] = (-n[1]) * in[1] + n[0] * in[2];
+        o[3] = in[3];
+    };
+    auto rot = [&](double v[4]) {
+        double a = n[0] * v[1] + (-n[1]) * v[2];
+        double b = n[1] * v[1] + n[0] * v[2];
+        v[1] = a, v[2] = b;
+    };
+    double UL[4], UR[4];
+    rotT(ULxy, UL);
+    rotT(URxy, UR);
+    {
+        double ULMean[4], URMean[4], wL[4], wR[4];
+        rotT(ULMeanXy, ULMean);
+        rotT(URMeanXy, URMean);
+        cons2prim(ULMean, wL, gamma);
+        cons2prim(URMean, wR, gamma);
+        if (!(ULMean[0] > 0) || !(URMean[0] > 0))
+            return ERR_NONPOSITIVE;
+        inviscid_flux(ULMean, wL[1], wL[3], FLfix);
+        inviscid_flux(URMean, wR[1], wR[3], FRfix);
+        rot(FLfix);
+        rot(FRfix);
+    }
+    double finc[4];
+    int rc = cfg.riemann_solver == 2 ? hllc_flux(UL, UR, gamma, dLambda, finc)
+                                     : (cfg.riemann_solver == 1 ? roe_flux(UL, UR, gamma, dLambda, finc) : ERR_UNSUPPORTED);
+    if (rc)
+        return rc;
+    rot(finc);
+    for (int k = 0; k < 4; k++) {
+        if (!std::isfinite(finc[k]))
+            return ERR_NAN;
+        out[k] = -finc[k];
+    }
+    return ERR_OK;
+}
+
+} // namespace
+
+extern "C" int oracle_roe_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F) {
+    return roe_flux(UL, UR, gamma, dlambda, F);
+}
+extern "C" int oracle_hllc_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F) {
+    return hllc_flux(UL, UR, gamma, dlambda, F);
+}
+extern "C" int oracle_eigenvectors(const double *U, double gamma, double *L, double *R) {
+    if (!(U[0] > 0))
+        return ERR_NONPOSITIVE;
+    double velo[2] = {U[1] / U[0], U[2] / U[0]};
+    double vsqr = velo[0] * velo[0] + velo[1] * velo[1];
+    double asqr, p, H;
+    ideal_gas_thermal(U[3], U[0], vsqr, gamma, p, asqr, H);
+    if (!(asqr >= 0))
+        return ERR_NONPOSITIVE;
+    double a = std::sqrt(asqr);
+    right_eigen(velo, vsqr, H, a, reinterpret_cast<double(*)[4]>(R));
+    left_eigen(velo, vsqr, H, a, gamma, reinterpret_cast<double(*)[4]>(L));
+    return ERR_OK;
+}
+extern "C" int oracle_boundary_value(const oracle_setting *cfg, double *UL, const double *unit_norm, const double *p,
+                                     double t, int32_t bc, int32_t fix_ul, double *UR) {
+    (void)fix_ul;
+    return boundary_value(*cfg, UL, unit_norm, p, t, bc, UR);
+}
+
+// ReconstructionJacobiStep, DNDS_FV_VR.hpp:1474-1713, SOR_Instead = false
+extern "C" int oracle_jacobi_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                  const double *u, double *urec, double *urec_new, double t) {
+    (void)t;
+    const double relax = cfg->jacobi_relax;
+    for (int64_t i = 0; i < m->n_cell; i++) {
+        double *nw = urec_new + 36 * i;
+        const double *xi = urec + 36 * i;
+        for (int k = 0; k < 36; k++)
+            nw[k] = (1 - relax) * xi[k];
+        const int64_t b0 = m->cell_ptr[i];
+        const int nf = int(m->cell_ptr[i + 1] - b0);
+        for (int lf = 0; lf < nf; lf++) {
+            const int64_t f = m->cell2face[b0 + lf];
+            const int64_t j = m->face2cell[2 * f] == i ? m->face2cell[2 * f + 1] : m->face2cell[2 * f];
+            const double *M = s->ainv_b + 81 * (b0 + lf);
+            const double *bv = s->ainv_bvec + 9 * (b0 + lf);
+            const int32_t bc = m->face_bc[f];
+            if (j != kNoCell) {
+                const double *xj = urec + 36 * j;
+                double du[4];
+                for (int v = 0; v < 4; v++)
+                    du[v] = u[4 * j + v] - u[4 * i + v];
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double tt = 0;
+                        for (int c = 0; c < 9; c++)
+                            tt += M[9 * r + c] * xj[c + 9 * v];
+                        nw[r + 9 * v] += relax * (tt + du[v] * bv[r]);
+                    }
+            } else if (bc == BC_WALL) {
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++)
+                        nw[r + 9 * v] += relax * ((0.0 - u[4 * i + v]) * bv[r]);
+            } else if (is_far(bc)) {
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double tt = 0;
+                        for (int c = 0; c < 9; c++)
+                            tt += M[9 * r + c] * xi[c + 9 * v];
+                        nw[r + 9 * v] += relax * tt;
+                    }
+            } else if (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) {
+                // face-Gauss boundary correction, DNDS_FV_VR.hpp:1577-1657
+                double BC[36] = {0};
+                const double w0 = s->face_weight[10 * f];
+                for (int g = 0; g < 3; g++) {
+                    const double *d = s->face_d0 + 54 * f + 18 * g; // left side (the cell itself)
+                    const double nx = s->face_norm[6 * f + 2 * g], ny = s->face_norm[6 * f + 2 * g + 1];
+                    double uBL[4];
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += d[c] * xi[c + 9 * v];
+                        uBL[v] = acc;
+                    }
+                    for (int v = 0; v < 4; v++)
+                        uBL[v] += u[4 * i + v];
+                    double uBV[4];
+                    double n0, n1;
+                    stable_normalized(nx, ny, n0, n1);
+                    if (bc == BC_WALL_EULER) {
+                        for (int v = 0; v < 4; v++)
+                            uBV[v] = uBL[v];
+                        uBV[0] = uBL[0];
+                        double dd = n0 * uBV[1] + n1 * uBV[2];
+                        uBV[1] = uBL[1] - n0 * dd;
+                        uBV[2] = uBL[2] - n1 * dd;
+                        uBV[3] = uBL[3];
+                    } else {
+                        double nrm[2] = {n0, n1}, pp[2] = {0, 0};
+                        int rc = boundary_value(*cfg, uBL, nrm, pp, t, bc, uBV);
+                        if (rc)
+                            return rc;
+                    }
+                    const double nl = std::sqrt(nx * nx + ny * ny);
+                    for (int r = 0; r < 9; r++)
+                        for (int v = 0; v < 4; v++) {
+                            double inc = d[r] * w0 * w0 * (uBV[v] - u[4 * i + v]);
+                            inc *= nl;
+                            BC[r + 9 * v] += inc * kLine3W[g];
+                        }
+                }
+                const double *Ai = s->ainv + 81 * i;
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += Ai[9 * r + c] * BC[c + 9 * v];
+                        nw[r + 9 * v] += relax * acc;
+                    }
+            } else {
+                return ERR_UNSUPPORTED;
+            }
+        }
+    }
+    std::memcpy(urec, urec_new, sizeof(double) * 36 * m->n_cell);
+    return ERR_OK;
+}
+
+// EvaluateDt, DNDS_FV_EulerEvaluator.cpp:14-183 (nTimeFilterPass = 0, muGas = 0)
+extern "C" int oracle_evaluate_dt(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                  const double *u, double cfl, double max_dt, int32_t use_local_dt, double *dt,
+                                  double *dt_min, double *lambda_face, double *delta_lambda_face) {
+    const double gamma = cfg->gamma;
+    std::vector<double> lambdaCell(m->n_cell, 0.0);
+    for (int64_t f = 0; f < m->n_face; f++) {
+        const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
+        double n0, n1;
+        normalized(s->face_norm_center[2 * f], s->face_norm_center[2 * f + 1], n0, n1);
+        const double *UL = u + 4 * L;
+        double uMean[4] = {UL[0], UL[1], UL[2], UL[3]};
+        double pL, asqrL, HL, pR, asqrR, HR;
+        double vL[2] = {UL[1] / UL[0], UL[2] / UL[0]};
+        double vR[2] = {vL[0], vL[1]};
+        ideal_gas_thermal(UL[3], UL[0], vL[0] * vL[0] + vL[1] * vL[1], gamma, pL, asqrL, HL);
+        pR = pL, HR = HL, asqrR = asqrL;
+        if (R != kNoCell) {
+            const double *UR = u + 4 * R;
+            for (int k = 0; k < 4; k++)
+                uMean[k] = (uMean[k] + UR[k]) * 0.5;
+            vR[0] = UR[1] / UR[0], vR[1] = UR[2] / UR[0];
+            ideal_gas_thermal(UR[3], UR[0], vR[0] * vR[0] + vR[1] * vR[1], gamma, pR, asqrR, HR);
+        }
+        if (!(uMean[0] > 0))
+            return ERR_NONPOSITIVE;
+        double veloNMean = 0.5 * ((vL[0] + vR[0]) * n0 + (vL[1] + vR[1]) * n1);
+        double pMean = (pL + pR) * 0.5;
+        double aMean = std::sqrt(gamma * pMean / uMean[0]);
+        double lambdaConvection = std::abs(veloNMean) + aMean;
+        const double area = s->face_area[f];
+        double lf = lambdaConvection; // + lamVis * area * (...), zero for muGas = 0
+        if (lambda_face)
+            lambda_face[f] = lf;
+        lambdaCell[L] += lf * area;
+        if (R != kNoCell)
+            lambdaCell[R] += lf * area;
+        double dlf = std::abs((vR[0] - vL[0]) * n0 + (vR[1] - vL[1]) * n1) + std::sqrt(std::abs(asqrR - asqrL)) * 0.7071;
+        if (delta_lambda_face)
+            delta_lambda_face[f] = dlf;
+    }
+    double dtMin = kVeryLargeReal;
+    for (int64_t i = 0; i < m->n_cell; i++) {
+        dt[i] = std::min(cfl * s->volume[i] / (lambdaCell[i] + 1e-100), max_dt);
+        dtMin = std::min(dtMin, dt[i]);
+    }
+    if (!std::isfinite(dtMin))
+        return ERR_NAN;
+    if (dt_min)
+        *dt_min = dtMin;
+    if (!use_local_dt)
+        for (int64_t i = 0; i < m->n_cell; i++)
+            dt[i] = dtMin;
+    return ERR_OK;
+}
+
+// EvaluateRHS, DNDS_FV_EulerEvaluator.cpp:197-640 (source term zero for NS_2D
+// with constMassForce = 0)
+extern "C" int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                   const double *delta_lambda_face, const double *u, const double *urec, double t,
+                                   double *rhs) {
+    const int64_t nc = m->n_cell;
+    for (int64_t i = 0; i < nc; i++)
+        for (int v = 0; v < 4; v++)
+            rhs[4 * i + v] = 0.0;
+    for (int64_t f = 0; f < m->n_face; f++) {
+        const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
+        double fluxEs[3][4] = {{0}};
+        for (int g = 0; g < 3; g++) {
+            const double nx = s->face_norm[6 * f + 2 * g], ny = s->face_norm[6 * f + 2 * g + 1];
+            double un[2];
+            normalized(nx, ny, un[0], un[1]);
+            const double *dL = s->face_d0 + 54 * f + 18 * g;
+            double inc[4];
+            for (int v = 0; v < 4; v++) {
+                double acc = 0;
+                for (int c = 0; c < 9; c++)
+                    acc += dL[c] * urec[36 * L + c + 9 * v];
+                inc[v] = acc;
+            }
+            double ULxy[4], URxy[4], URMeanXy[4];
+            compress_rec_part(u + 4 * L, inc, ULxy);
+            const double *ULMeanXy = u + 4 * L;
+            if (R != kNoCell) {
+                const double *dR = s->face_d0 + 54 * f + 18 * g + 9;
+                for (int v = 0; v < 4; v++) {
+                    double acc = 0;
+                    for (int c = 0; c < 9; c++)
+                        acc += dR[c] * urec[36 * R + c + 9 * v];
+                    inc[v] = acc;
+                }
+                compress_rec_part(u + 4 * R, inc, URxy);
+                for (int v = 0; v < 4; v++)
+                    URMeanXy[v] = u[4 * R + v];
+            } else {
+                const double *pc = s->face_center + 2 * f;
+                int rc = boundary_value(*cfg, ULxy, un, pc, t, m->face_bc[f], URxy);
+                if (rc)
+                    return rc;
+                rc = boundary_value(*cfg, ULMeanXy, un, pc, t, m->face_bc[f], URMeanXy);
+                if (rc)
+                    return rc;
+            }
+            double F[4], FL[4], FR[4];
+            int rc = flux_face(*cfg, ULxy, URxy, ULMeanXy, URMeanXy, un, delta_lambda_face[f], F, FL, FR);
+            if (rc)
+                return rc;
+            const double nl = std::sqrt(nx * nx + ny * ny);
+            for (int v = 0; v < 4; v++) {
+                double a = F[v] * nl, b = FL[v] * nl, c = FR[v] * nl;
+                fluxEs[0][v] += a * kLine3W[g];
+                fluxEs[1][v] += b * kLine3W[g];
+                fluxEs[2][v] += c * kLine3W[g];
+            }
+        }
+        const double VL = s->volume[L];
+        for (int v = 0; v < 4; v++)
+            rhs[4 * L + v] += fluxEs[0][v] / VL;
+        if (R != kNoCell) {
+            const double VR = s->volume[R];
+            for (int v = 0; v < 4; v++)
+                rhs[4 * R + v] -= fluxEs[0][v] / VR;
+        }
+        for (int v = 0; v < 4; v++)
+            rhs[4 * L + v] -= fluxEs[1][v] / VL;
+        if (R != kNoCell) {
+            const double VR = s->volume[R];
+            for (int v = 0; v < 4; v++)
+                rhs[4 * R + v] += fluxEs[2][v] / VR;
+        }
+    }
+    for (int64_t i = 0; i < 4 * nc; i++)
+        if (!std::isfinite(rhs[i]))
+            return ERR_NAN;
+    return ERR_OK;
+}
+
+// ExplicitSSPRK3LocalDt::Step, DNDS_ODE.hpp:131-178; frhs as the Eikonal
+// solver composes it (DNDS_FV_EikonalCRSolver.hpp:948-999): sweeps then RHS.
+extern "C" int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                   int32_t n_steps, int32_t n_sweeps, double cfl, double max_dt, int32_t use_local_dt,
+                                   double *u, double *urec, double *t) {
+    const int64_t nc = m->n_cell;
+    std::vector<double> dt(nc), rhs(4 * nc), xLast(4 * nc), urecNew(36 * nc), dlf(m->n_face);
+    const double Coef[3] = {1.0, 0.25, 2.0 / 3.0};
+    for (int step = 0; step < n_steps; step++) {
+        double dtMin;
+        int rc = oracle_evaluate_dt(m, s, cfg, u, cfl, max_dt, use_local_dt, dt.data(), &dtMin, nullptr, dlf.data());
+        if (rc)
+            return rc;
+        std::memcpy(xLast.data(), u, sizeof(double) * 4 * nc);
+        const double ct[3] = {1, 0.25, 2. / 3.};
+        for (int stage = 0; stage < 3; stage++) {
+            const double ts = *t + ct[stage] * dtMin;
+            for (int k = 0; k < n_sweeps; k++) {
+                rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), ts);
+                if (rc)
+                    return rc;
+            }
+            rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, ts, rhs.data());
+            if (rc)
+                return rc;
+            for (int64_t i = 0; i < nc; i++)
+                for (int v = 0; v < 4; v++) {
+                    double r = rhs[4 * i + v] * dt[i];
+                    double x = u[4 * i + v] + r;
+                    if (stage > 0) {
+                        x *= Coef[stage] / (1 - Coef[stage]);
+                        x += xLast[4 * i + v];
+                        x *= 1 - Coef[stage];
+                    }
+                    u[4 * i + v] = x;
+                }
+        }
+        *t += dtMin;
+    }
+    return ERR_OK;
+}

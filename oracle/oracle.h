@@ -1,0 +1,96 @@
+/*
+ * oracle.h -- CPU FP64 restatement of DNDS's 2D VR finite-volume hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * HIP implementation and the CPU baseline of the benchmark; no product code
+ * links, imports or calls it.  It is written independently of the product
+ * sources: it restates the reference's loops (face loop with in-order
+ * scatter, cell loop in cell2face order) and its expression order, compiled
+ * with -ffp-contract=off.
+ *
+ * Array layouts are the same logical layouts the product's C ABI documents
+ * (reference layouts): u [n_cell][4]; uRec [n_cell][36] column-major 9x4 per
+ * cell; setup arrays row-major.
+ */
+#ifndef DNDS_ORACLE_H
+#define DNDS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_mesh {
+    int64_t n_cell, n_face;
+    const int64_t *cell_ptr;     /* [n_cell+1]              */
+    const int64_t *cell2face;    /* local face order        */
+    const int64_t *face2cell;    /* [n_face][2], INT64_MIN  */
+    const int32_t *face_bc;      /* BoundaryType values     */
+} oracle_mesh;
+
+typedef struct oracle_setup {
+    const double *volume;           /* [n_cell]            */
+    const double *face_area;        /* [n_face]            */
+    const double *face_center;      /* [n_face][2]         */
+    const double *face_norm_center; /* [n_face][2]         */
+    const double *face_norm;        /* [n_face][3][2]      */
+    const double *face_weight;      /* [n_face][10]        */
+    const double *face_d0;          /* [n_face][3][2][9]   */
+    const double *face_dibj_g0;     /* [n_face][2][10][9]  */
+    const double *matrix_secondary; /* [n_face][2][9][9]   */
+    const double *ainv;             /* [n_cell][9][9]      */
+    const double *ainv_b;           /* [n_c2f][9][9]       */
+    const double *ainv_bvec;        /* [n_c2f][9]          */
+} oracle_setup;
+
+typedef struct oracle_setting {
+    int32_t riemann_solver;        /* 1 Roe, 2 HLLC          */
+    double gamma;
+    double far_field[4];
+    double jacobi_relax;
+    double tang_weight;            /* for the WBAP functional */
+    int32_t tang_weight_direction; /* 0 Norm, 1 Bary          */
+    double wbap_smooth_indicator_scale;
+    double wbap_n_std;
+} oracle_setting;
+
+/* VRFiniteVolume2D::ReconstructionJacobiStep, SOR_Instead = false
+ * (DNDS_FV_VR.hpp:1474-1713): urec_new receives the sweep, urec is then
+ * overwritten with it.  Returns 0 or a nonzero error code. */
+int oracle_jacobi_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                       double *urec, double *urec_new, double t);
+
+/* EulerEvaluator::EvaluateDt (DNDS_FV_EulerEvaluator.cpp:14-183) for the
+ * Euler model (muGas = 0): fills dt, *dt_min, lambda_face, delta_lambda_face. */
+int oracle_evaluate_dt(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                       double cfl, double max_dt, int32_t use_local_dt, double *dt, double *dt_min,
+                       double *lambda_face, double *delta_lambda_face);
+
+/* EulerEvaluator::EvaluateRHS (DNDS_FV_EulerEvaluator.cpp:197-640) for the
+ * Euler model: face loop, Gauss sum, in-order scatter. */
+int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                        const double *delta_lambda_face, const double *u, const double *urec, double t, double *rhs);
+
+/* n_steps of ExplicitSSPRK3LocalDt::Step (DNDS_ODE.hpp:131-178) with frhs =
+ * n_sweeps Jacobi sweeps + EvaluateRHS and fdt = EvaluateDt.  *t advances
+ * by the global minimum dt of each step. */
+int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, int32_t n_steps,
+                        int32_t n_sweeps, double cfl, double max_dt, int32_t use_local_dt, double *u, double *urec,
+                        double *t);
+
+/* Gas-dynamics kernels exposed for unit tests (rotated frame, 4 variables). */
+int oracle_roe_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F);
+int oracle_hllc_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F);
+/* Left / right eigenvector matrices of the x-direction Euler Jacobian
+ * (EulerGasLeftEigenVector / EulerGasRightEigenVector), row-major 4x4. */
+int oracle_eigenvectors(const double *U, double gamma, double *L, double *R);
+/* generateBoundaryValue (DNDS_FV_EulerEvaluator.hpp:958-1275). */
+int oracle_boundary_value(const oracle_setting *cfg, double *UL, const double *unit_norm, const double *p, double t,
+                          int32_t bc, int32_t fix_ul, double *UR);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,187 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg, always as the checker or the baseline, never
+as the product.  Build with `make -C oracle` (or `oracle.build()`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+PD = C.POINTER(C.c_double)
+PI64 = C.POINTER(C.c_int64)
+PI32 = C.POINTER(C.c_int32)
+
+
+class OMesh(C.Structure):
+    _fields_ = [("n_cell", C.c_int64), ("n_face", C.c_int64), ("cell_ptr", PI64), ("cell2face", PI64),
+                ("face2cell", PI64), ("face_bc", PI32)]
+
+
+class OSetup(C.Structure):
+    _fields_ = [("volume", PD), ("face_area", PD), ("face_center", PD), ("face_norm_center", PD),
+                ("face_norm", PD), ("face_weight", PD), ("face_d0", PD), ("face_dibj_g0", PD),
+                ("matrix_secondary", PD), ("ainv", PD), ("ainv_b", PD), ("ainv_bvec", PD)]
+
+
+class OSetting(C.Structure):
+    _fields_ = [("riemann_solver", C.c_int32), ("gamma", C.c_double), ("far_field", C.c_double * 4),
+                ("jacobi_relax", C.c_double), ("tang_weight", C.c_double), ("tang_weight_direction", C.c_int32),
+                ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double)]
+
+
+def build(quiet: bool = True) -> str:
+    """Compile liboracle.so from oracle.cpp (g++, -O2 -ffp-contract=off)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sig = {
+            "oracle_jacobi_step": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, C.c_double],
+            "oracle_evaluate_dt": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, C.c_double,
+                                   C.c_double, C.c_int32, PD, PD, PD, PD],
+            "oracle_evaluate_rhs": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, C.c_double,
+                                    PD],
+            "oracle_ssprk3_steps": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), C.c_int32, C.c_int32,
+                                    C.c_double, C.c_double, C.c_int32, PD, PD, PD],
+            "oracle_roe_flux": [PD, PD, C.c_double, C.c_double, PD],
+            "oracle_hllc_flux": [PD, PD, C.c_double, C.c_double, PD],
+            "oracle_eigenvectors": [PD, C.c_double, PD, PD],
+            "oracle_boundary_value": [C.POINTER(OSetting), PD, PD, PD, C.c_double, C.c_int32, C.c_int32, PD],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    if a is None:
+        return PD()
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], a.dtype
+    return a.ctypes.data_as(PD)
+
+
+class Oracle:
+    """Holds contiguous copies of the mesh and setup arrays and calls the oracle.
+
+    `mesh` and `setup` are dicts of numpy arrays with the keys of the
+    product's mesh and VR views (see dnds_amd.host), so the same data feeds
+    both the oracle and the device.
+    """
+
+    def __init__(self, mesh: dict, setup: dict, riemann_solver: int = 1, gamma: float = 1.4,
+                 far_field=(1.0, 0.0, 0.0, 2.5), jacobi_relax: float = 1.0, tang_weight: float = 1.0,
+                 tang_weight_direction: int = 0, wbap_smooth_indicator_scale: float = 1e-10,
+                 wbap_n_std: float = 10.0):
+        self._keep = {}
+
+        def keep(name, a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            self._keep[name] = a
+            return a
+
+        self.n_cell = int(mesh["n_cell"])
+        self.n_face = int(mesh["n_face"])
+        self.m = OMesh(self.n_cell, self.n_face,
+                       keep("cell_ptr", mesh["cell_ptr"], np.int64).ctypes.data_as(PI64),
+                       keep("cell2face", mesh["cell2face"], np.int64).ctypes.data_as(PI64),
+                       keep("face2cell", mesh["face2cell"], np.int64).ctypes.data_as(PI64),
+                       keep("face_bc", mesh["face_bc"], np.int32).ctypes.data_as(PI32))
+        fields = {}
+        for name, _ in OSetup._fields_:
+            a = setup.get(name)
+            fields[name] = PD() if a is None or a.size == 0 else keep(name, a, np.float64).ctypes.data_as(PD)
+        self.s = OSetup(**fields)
+        ff = (C.c_double * 4)(*far_field)
+        self.cfg = OSetting(riemann_solver, gamma, ff, jacobi_relax, tang_weight, tang_weight_direction,
+                            wbap_smooth_indicator_scale, wbap_n_std)
+        self.lib = lib()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+    def jacobi_step(self, u, urec, t=0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        urec = np.array(urec, dtype=np.float64, order="C", copy=True)
+        new = np.empty_like(urec)
+        self._check(self.lib.oracle_jacobi_step(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u),
+                                                _d(urec), _d(new), t), "jacobi_step")
+        return urec
+
+    def evaluate_dt(self, u, cfl, max_dt=1e100, use_local_dt=False):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        dt = np.empty(self.n_cell)
+        lf = np.empty(self.n_face)
+        dlf = np.empty(self.n_face)
+        dtmin = C.c_double()
+        self._check(self.lib.oracle_evaluate_dt(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u), cfl,
+                                                max_dt, int(use_local_dt), _d(dt), C.byref(dtmin), _d(lf),
+                                                _d(dlf)), "evaluate_dt")
+        return dt, dtmin.value, lf, dlf
+
+    def evaluate_rhs(self, u, urec, delta_lambda_face, t=0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        urec = np.ascontiguousarray(urec, dtype=np.float64)
+        dlf = np.ascontiguousarray(delta_lambda_face, dtype=np.float64)
+        rhs = np.empty((self.n_cell, 4))
+        self._check(self.lib.oracle_evaluate_rhs(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(dlf),
+                                                 _d(u), _d(urec), t, _d(rhs)), "evaluate_rhs")
+        return rhs
+
+    def ssprk3_steps(self, u, urec, n_steps, n_sweeps, cfl, t=0.0, max_dt=1e100, use_local_dt=False):
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        urec = np.array(urec, dtype=np.float64, order="C", copy=True)
+        tt = C.c_double(t)
+        self._check(self.lib.oracle_ssprk3_steps(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), n_steps,
+                                                 n_sweeps, cfl, max_dt, int(use_local_dt), _d(u), _d(urec),
+                                                 C.byref(tt)), "ssprk3_steps")
+        return u, urec, tt.value
+
+
+def roe_flux(UL, UR, gamma=1.4, dlambda=0.0):
+    F = np.empty(4)
+    rc = lib().oracle_roe_flux(_d(np.ascontiguousarray(UL, float)), _d(np.ascontiguousarray(UR, float)), gamma,
+                               dlambda, _d(F))
+    if rc:
+        raise RuntimeError(f"roe flux failed {rc}")
+    return F
+
+
+def hllc_flux(UL, UR, gamma=1.4, dlambda=0.0):
+    F = np.empty(4)
+    rc = lib().oracle_hllc_flux(_d(np.ascontiguousarray(UL, float)), _d(np.ascontiguousarray(UR, float)), gamma,
+                                dlambda, _d(F))
+    if rc:
+        raise RuntimeError(f"hllc flux failed {rc}")
+    return F
+
+
+def eigenvectors(U, gamma=1.4):
+    L = np.empty((4, 4))
+    R = np.empty((4, 4))
+    rc = lib().oracle_eigenvectors(_d(np.ascontiguousarray(U, float)), gamma, _d(L), _d(R))
+    if rc:
+        raise RuntimeError(f"eigenvectors failed {rc}")
+    return L, R

@@ -1,0 +1,145 @@
+"""GPU (HIP) vs CPU oracle parity of the FP64 hot path, through the C ABI.
+
+Bar: per-call Jacobi sweeps, time steps and right-hand sides agree with the
+oracle to <= 1e-12 relative (max abs difference over the array / max abs of
+the oracle's array); SSP-RK3 trajectories to the same bar after several
+steps.  Both sides consume the same host setup arrays.
+"""
+import numpy as np
+import pytest
+
+from conftest import mesh_and_setup, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+BC_SETS = {
+    "periodic": None,
+    "farfield": (2, 2, 2, 2),
+    "walls": (3, 3, 3, 3),          # Wall_Euler everywhere
+    "mixed_bc": (2, 3, 4, 3),       # Farfield, Wall_Euler, Wall_NoSlip, Wall_Euler
+    "with_wall": (2, 3, 4, 1),      # ... and a Wall face (valid for the sweep only)
+    "riemann": (-5, -5, -5, -5),    # Special_2DRiemannFar
+}
+
+
+def build(kind, bcname, riemann="Roe", nx=9, ny=7, jitter=0.2, seed=5):
+    import oracle
+    from dnds_amd.host import EulerEvaluator
+    periodic = bcname == "periodic"
+    mesh, vfv = mesh_and_setup(kind=kind, nx=nx, ny=ny, periodic=periodic, jitter=0.0 if periodic else jitter,
+                               seed=seed, bc=BC_SETS[bcname])
+    far = (1.0, 0.3, -0.2, 2.6)
+    ev = EulerEvaluator(mesh, vfv, riemannSolverType=riemann, gamma=1.4, farFieldStaticValue=far)
+    orc = oracle.Oracle(mesh.arrays(), vfv.arrays(), riemann_solver=1 if riemann == "Roe" else 2, gamma=1.4,
+                        far_field=far, jacobi_relax=vfv.setting.jacobi_relax)
+    return mesh, vfv, ev, orc
+
+
+def random_state(n, seed, amp=0.2, rec_amp=0.05):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + amp * rng.uniform(-1, 1, n)
+    vx = 0.5 + amp * rng.uniform(-1, 1, n)
+    vy = -0.3 + amp * rng.uniform(-1, 1, n)
+    p = 1.0 + amp * rng.uniform(-1, 1, n)
+    E = p / 0.4 + 0.5 * rho * (vx ** 2 + vy ** 2)
+    u = np.stack([rho, rho * vx, rho * vy, E], axis=1)
+    urec = rec_amp * rng.standard_normal((n, 36))
+    return u, urec
+
+
+CASES = [(k, b) for k in (0, 1, 2) for b in ("periodic", "farfield", "walls", "mixed_bc")] + [(0, "riemann")]
+
+
+@pytest.mark.parametrize("kind,bcname", CASES + [(2, "with_wall")])
+def test_jacobi_sweeps_match_oracle(kind, bcname):
+    mesh, vfv, ev, orc = build(kind, bcname)
+    u, urec = random_state(mesh.n_cell, 11)
+    du, dr, dn = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec()
+    ref = urec
+    for sweep in range(4):
+        vfv.ReconstructionJacobiStep(du, dr, dn, t=0.0)
+        ref = orc.jacobi_step(u, ref)
+        got = dr.download()
+        assert rel_err(got, ref) <= TOL, (sweep, rel_err(got, ref))
+    ev.device.sync()
+
+
+@pytest.mark.parametrize("kind,bcname", CASES)
+@pytest.mark.parametrize("local", [False, True])
+def test_evaluate_dt_matches_oracle(kind, bcname, local):
+    mesh, vfv, ev, orc = build(kind, bcname)
+    u, _ = random_state(mesh.n_cell, 12)
+    du, dt = ev.BuildMean().upload(u), ev.BuildScalar()
+    dtmin = ev.EvaluateDt(dt, du, CFL=0.7, MaxDt=1e100, UseLocaldt=local)
+    rdt, rmin, _, _ = orc.evaluate_dt(u, 0.7, 1e100, local)
+    assert abs(dtmin - rmin) <= TOL * abs(rmin)
+    assert rel_err(dt.download(), rdt) <= TOL
+
+
+@pytest.mark.parametrize("kind,bcname", CASES)
+@pytest.mark.parametrize("riemann", ["Roe", "HLLC"])
+def test_evaluate_rhs_matches_oracle(kind, bcname, riemann):
+    mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
+    u, urec = random_state(mesh.n_cell, 13)
+    du, dr, rhs, dt = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), ev.BuildMean(), ev.BuildScalar()
+    ev.EvaluateDt(dt, du, CFL=0.5)            # refreshes deltaLambdaFace
+    _, _, _, dlf = orc.evaluate_dt(u, 0.5)
+    ev.EvaluateRHS(rhs, du, dr, t=0.05)
+    ref = orc.evaluate_rhs(u, urec, dlf, t=0.05)
+    assert rel_err(rhs.download(), ref) <= TOL
+
+
+@pytest.mark.parametrize("kind,bcname,riemann", [(0, "periodic", "Roe"), (2, "farfield", "HLLC"),
+                                                 (1, "walls", "Roe"), (0, "riemann", "HLLC")])
+def test_ssprk3_steps_match_oracle(kind, bcname, riemann):
+    from dnds_amd.host import ExplicitSSPRK3LocalDt
+    mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
+    u, urec = random_state(mesh.n_cell, 14, amp=0.1, rec_amp=0.0)
+    du, dr, dn = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec()
+    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=2, CFL=0.4)
+    t = 0.0
+    for _ in range(5):
+        t += ode.Step(du, dr, dn, t)
+    ru, rr, rt = orc.ssprk3_steps(u, urec, 5, 2, 0.4)
+    assert abs(t - rt) <= TOL * rt
+    assert rel_err(du.download(), ru) <= TOL
+    assert rel_err(dr.download(), rr) <= TOL
+
+
+def test_vortex_config_full_run_matches_oracle():
+    """The small CPU-runnable configuration end to end: periodic 40x40 split
+    quads (3200 cells), p = 3, Roe, CFL 0.5, 5 sweeps per stage, 100 steps."""
+    import oracle
+    from dnds_amd import cases
+    from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
+    case = cases.isentropic_vortex(40, seed=1)
+    ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType="Roe", farFieldStaticValue=case.far)
+    orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=1, far_field=case.far)
+    urec0 = np.zeros((case.n_cell, 36))
+    du, dr, dn = ev.BuildMean().upload(case.u0), case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=case.sweeps, CFL=case.cfl)
+    t = 0.0
+    for _ in range(case.steps):
+        t += ode.Step(du, dr, dn, t)
+    ru, rr, rt = orc.ssprk3_steps(case.u0, urec0, case.steps, case.sweeps, case.cfl)
+    assert abs(t - rt) <= TOL * rt
+    assert rel_err(du.download(), ru) <= TOL
+    assert rel_err(dr.download(), rr) <= TOL
+
+
+def test_wall_face_in_rhs_is_an_error_on_both_sides():
+    """A `Wall` face is not a flux boundary (the reference asserts,
+    DNDS_FV_EulerEvaluator.hpp:1265-1269): both implementations report it."""
+    from dnds_amd.host import DNDSError
+    mesh, vfv, ev, orc = build(0, "with_wall")
+    u, urec = random_state(mesh.n_cell, 15)
+    du, dr, rhs, dt = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), ev.BuildMean(), ev.BuildScalar()
+    ev.EvaluateDt(dt, du, CFL=0.5)
+    ev.EvaluateRHS(rhs, du, dr, t=0.0)
+    with pytest.raises(DNDSError):
+        ev.device.sync()
+    _, _, _, dlf = orc.evaluate_dt(u, 0.5)
+    with pytest.raises(RuntimeError):
+        orc.evaluate_rhs(u, urec, dlf)
