@@ -2,17 +2,19 @@
 
 Workload (default): the four-quadrant 2D Riemann problem on [0,1]^2,
 1024 x 1024 quads cut along seeded diagonals (2,097,152 triangles), interior
-nodes jittered by U(-0.2h, 0.2h); p = 3 VR, HLLC, SSP-RK3 at CFL 0.5 with a
-global time step, 5 Jacobi sweeps per stage, FP64.  One "step" is one SSP-RK3
-step: EvaluateDt once, then three stages of (5 sweeps + EvaluateRHS + stage
-update), all on the device with the state resident in HBM.
+nodes jittered by U(-0.2h, 0.2h); p = 3 VR, HLLC, WBAP V3 limiter, SSP-RK3 at
+CFL 0.5 with a global time step, 5 Jacobi sweeps per stage, FP64.  One
+"step" is one SSP-RK3 step: EvaluateDt once, then three stages of (5 sweeps +
+WBAP + EvaluateRHS + stage update), all on the device with the state resident
+in HBM.
 
 value = cells x 3 stages x steps / wall time of the timed steps.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
 (the Jacobi sweep: algorithmic bytes per launch / mean launch time from HIP
 events recorded by the library on its stream) and a CPU baseline (the FP64
-oracle, single thread, one SSP-RK3 step on the same mesh).
+oracle, single thread, one SSP-RK3 step on the same mesh).  The default
+workload carries the WBAP V3 limiter (limiterProcedure 1) in every stage.
 """
 from __future__ import annotations
 
@@ -42,8 +44,10 @@ def main():
     ap.add_argument("--n", type=int, default=1024, help="quads per direction")
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--cfl", type=float, default=0.5)
-    ap.add_argument("--threads", type=int, default=0, help="host threads for the setup")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1), help="host threads for the setup")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on one step (0 = skip)")
+    ap.add_argument("--case", default="riemann", choices=["riemann", "vortex"],
+                    help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -54,8 +58,11 @@ def main():
     from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
 
     t0 = time.time()
-    log(f"building mesh + VR setup, {args.n}x{args.n} quads")
-    case = cases.riemann_2d(args.n, nthreads=args.threads, build_limiter_data=False)
+    log(f"building mesh + VR setup, {args.n}x{args.n} quads ({args.case})")
+    if args.case == "riemann":
+        case = cases.riemann_2d(args.n, nthreads=args.threads, build_limiter_data=True)
+    else:
+        case = cases.isentropic_vortex(args.n, nthreads=args.threads)
     nc = case.n_cell
     log(f"setup done: {nc} cells, {case.mesh.n_face} faces, {time.time() - t0:.1f}s")
     ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType=case.riemann, farFieldStaticValue=case.far)
@@ -63,7 +70,7 @@ def main():
     u = ev.BuildMean().upload(case.u0)
     urec0 = np.zeros((nc, 36))
     r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
-    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl)
+    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl, useLimiter=case.limiter)
     t = 0.0
     for _ in range(args.warmup):
         t += ode.Step(u, r, rn, t)
@@ -108,12 +115,13 @@ def main():
     if args.cpu_baseline:
         import oracle
         log("CPU baseline: oracle (1 thread), one SSP-RK3 step on the same mesh")
-        orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(["volume", "face_area", "face_center",
-                                                                  "face_norm_center", "face_norm", "face_weight",
-                                                                  "face_d0", "ainv", "ainv_b", "ainv_bvec"]),
-                            riemann_solver=2 if case.riemann == "HLLC" else 1, far_field=case.far)
+        st = case.vfv.setting
+        orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2 if case.riemann == "HLLC" else 1,
+                            far_field=case.far, tang_weight=st.tang_weight,
+                            tang_weight_direction=st.tang_weight_direction,
+                            wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std)
         c0 = time.perf_counter()
-        orc.ssprk3_steps(case.u0, urec0, 1, args.sweeps, args.cfl)
+        orc.ssprk3_steps(case.u0, urec0, 1, args.sweeps, args.cfl, use_limiter=case.limiter)
         c1 = time.perf_counter()
         cpu = {"value": nc * 3 / (c1 - c0), "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps + RHS) on the same {nc}-cell mesh, "
@@ -132,11 +140,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded mesh, four-quadrant Riemann initial state)",
-        "config": {"workload": f"2D Riemann, {args.n}x{args.n} quads split to {nc} triangles, jitter 0.2h, "
-                               f"p=3 VR, HLLC, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, "
-                               "no limiter",
-                   "cells": nc, "sweeps_per_stage": args.sweeps, "limiter": "none", "precision": "fp64"},
+        "data": "synthetic (seeded mesh, analytic initial state)",
+        "config": {"workload": (f"2D Riemann, {args.n}x{args.n} quads split to {nc} triangles, jitter 0.2h, "
+                                f"p=3 VR, HLLC, WBAP V3 limiter, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
+                                "sweeps/stage")
+                   if args.case == "riemann" else
+                   (f"periodic isentropic vortex, {args.n}x{args.n} quads split to {nc} triangles, "
+                    f"p=3 VR, Roe, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, no limiter"),
+                   "cells": nc, "sweeps_per_stage": args.sweeps,
+                   "limiter": "WBAP V3" if case.limiter else "none", "precision": "fp64"},
         "kernels": prof,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -126,8 +126,10 @@ SIGNATURES = [
     ("dndsmi_reconstruction_jacobi_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     ("dndsmi_evaluate_dt", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, PD, C.c_double, C.c_int32]),
     ("dndsmi_evaluate_rhs", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
-    ("dndsmi_ssprk3_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
-                                     C.c_double, C.c_int32, C.c_double, PD]),
+    ("dndsmi_wbap_limit_facial_v3", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_int32]),
+    ("dndsmi_ssprk3_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                     C.c_double, C.c_double, C.c_int32, C.c_double, PD]),
     ("dndsmi_profile_enable", C.c_int, [C.c_void_p, C.c_int32]),
     ("dndsmi_profile_query", C.c_int, [C.c_void_p, C.c_int32, PI64, PD]),
     ("dndsmi_kernel_bytes", C.c_int64, [C.c_void_p, C.c_int32]),
@@ -139,7 +141,10 @@ K_DT_FACE = 2
 K_DT_CELL = 3
 K_RHS_FACE = 4
 K_RHS_CELL = 5
-KERNEL_NAMES = ["k_jacobi", "k_wall_corr", "k_dt_face", "k_dt_cell", "k_rhs_face", "k_rhs_cell"]
+K_WBAP_FACE = 6
+K_WBAP_CELL = 7
+KERNEL_NAMES = ["k_jacobi", "k_wall_corr", "k_dt_face", "k_dt_cell", "k_rhs_face", "k_rhs_cell", "k_wbap_face",
+                "k_wbap_cell"]
 
 _lib = None
 

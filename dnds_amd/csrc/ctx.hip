@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -67,6 +68,18 @@ struct dndsmi_ctx {
     double *fctr = nullptr;    // [2][ldf]
     int32_t *bfaces = nullptr; // boundary face indices
     int64_t n_bface = 0;
+    // WBAP data (present when the setup carries limiter data)
+    bool has_limiter = false;
+    double *dg0 = nullptr;     // [180][ldf] all DiBj rows at LINE_3 point 0, both sides
+    double *fw = nullptr;      // [10][ldf] face weights
+    double *msec = nullptr;    // [162][ldf] matrixSecondary (R2L, L2R)
+    double *wsum = nullptr;    // [8][ldf] per-face indicator terms of both cells
+    double *uF = nullptr;      // [nfmax*28][ldc] WBAP per-cell face scratch
+    double *ind = nullptr;     // [ldc] indicator when the caller passes none
+    double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps
+    double tang_weight = 1.0;
+    int twd_norm = 1;
+    double wbap_scale = 1e-10, wbap_nstd = 10.0;
     // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
     double *lambda_face = nullptr;
     double *dlambda_face = nullptr;
@@ -606,6 +619,237 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
     }
 }
 
+
+// WBAP V3 step 0, face part (DNDS_FV_VR.hpp:2084-2126): for each inner face
+// the (IJI, ISI) diagonals of the (rho, E) jump functional at the first
+// LINE_3 point with the LINE_1 weight 2, for both adjacent cells.  As in the
+// reference, a cell evaluates the face's side-0 basis with its own
+// coefficients and the side-1 basis with its neighbour's.
+__global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, int64_t ldc,
+                                                      const int32_t *__restrict__ f2c,
+                                                      const double *__restrict__ dg0, const double *__restrict__ fw,
+                                                      const double *__restrict__ fn, const double *__restrict__ u,
+                                                      const double *__restrict__ x, double tang_weight, int twd_norm,
+                                                      double *__restrict__ wsum) {
+    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    const int32_t L = f2c[f], R = f2c[ldf + f];
+    if (R < 0)
+        return;
+    double W[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++)
+        W[k] = fw[k * ldf + f];
+    const double gx = fn[f], gy = fn[ldf + f];
+    const FFParams P = ff_params(W, gx, gy, tang_weight, twd_norm);
+    const double nl = sqrt(gx * gx + gy * gy);
+#pragma unroll 1
+    for (int cs = 0; cs < 2; cs++) {
+        const int32_t me = cs == 0 ? L : R, other = cs == 0 ? R : L;
+        double vL[10][2], vR[10][2];
+#pragma unroll
+        for (int r = 0; r < 10; r++)
+            vL[r][0] = vL[r][1] = vR[r][0] = vR[r][1] = 0.0;
+#pragma unroll 1
+        for (int c = 0; c < 9; c++) {
+            const double xm0 = x[c * ldc + me], xm3 = x[(c + 27) * ldc + me];
+            const double xo0 = x[c * ldc + other], xo3 = x[(c + 27) * ldc + other];
+#pragma unroll
+            for (int r = 0; r < 10; r++) {
+                const double d0 = dg0[(9 * r + c) * ldf + f], d1 = dg0[(90 + 9 * r + c) * ldf + f];
+                vL[r][0] += d0 * xm0, vL[r][1] += d0 * xm3;
+                vR[r][0] += d1 * xo0, vR[r][1] += d1 * xo3;
+            }
+        }
+        vL[0][0] += u[4 * int64_t(me)], vL[0][1] += u[4 * int64_t(me) + 3];
+        vR[0][0] += u[4 * int64_t(other)], vR[0][1] += u[4 * int64_t(other) + 3];
+        double out[4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            double val[10], jmp[10];
+#pragma unroll
+            for (int r = 0; r < 10; r++) {
+                val[r] = (vL[r][q] + vR[r][q]) * 0.5;
+                jmp[r] = (vL[r][q] - vR[r][q]) * 0.5;
+            }
+            const double isi = ff_self(P, val), iji = ff_self(P, jmp);
+            double a = iji, b = isi;
+            a *= nl, b *= nl;
+            out[2 * q] = a * 2.0;
+            out[2 * q + 1] = b * 2.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            wsum[(4 * cs + k) * ldf + f] = out[k];
+    }
+}
+
+// WBAP V3: smoothness indicator (sum over the cell's faces in cell2face
+// order, DNDS_FV_VR.hpp:2125-2137) and, for cells at or above
+// WBAP_SmoothIndicatorScale (or all cells when ifAll), the per-order
+// characteristic WBAP limiting (DNDS_FV_VR.hpp:2152-2317).  Reads x, writes
+// xo (= x for unlimited cells).  uF is per-cell scratch holding the limited
+// face values of rows 2..8 of the orders already done.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_wbap_cell(int32_t nc, int64_t ldc, int64_t ldf,
+                                                      const int32_t *__restrict__ nbr,
+                                                      const int32_t *__restrict__ sface,
+                                                      const int32_t *__restrict__ f2c,
+                                                      const double *__restrict__ wsum,
+                                                      const double *__restrict__ fnc,
+                                                      const double *__restrict__ msec, const double *__restrict__ u,
+                                                      const double *__restrict__ x, double *__restrict__ xo,
+                                                      double *__restrict__ ind, double *__restrict__ uF,
+                                                      double gamma, double scale, double nStd, double eps4,
+                                                      int if_all, int32_t *errw) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    double sum[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+        const int32_t j = nbr[s * ldc + i];
+        if (j < 0)
+            continue;
+        const int32_t f = sface[s * ldc + i];
+        const int side = f2c[f] == i ? 0 : 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            sum[k] += wsum[(4 * side + k) * ldf + f];
+    }
+    const double si0 = sum[0] / (sum[1] + kVerySmallReal);
+    const double si1 = sum[2] / (sum[3] + kVerySmallReal);
+    const double flag = sqrt(fmax(fabs(si0), fabs(si1))) * (3 * 3);
+    ind[i] = flag;
+    if (flag < scale && !if_all) {
+#pragma unroll
+        for (int k = 0; k < 36; k++)
+            xo[k * ldc + i] = x[k * ldc + i];
+        return;
+    }
+    double ui[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+    bool ok = true;
+#pragma unroll
+    for (int order = 3; order >= 1; order--) {
+        const int ls = order == 3 ? 5 : (order == 2 ? 2 : 0);
+        const int le = order == 3 ? 8 : (order == 2 ? 4 : 1);
+        constexpr int kMaxR = 4;
+        const int nr = le - ls + 1;
+        double u0[kMaxR][4], up[kMaxR][4], down[kMaxR][4];
+#pragma unroll
+        for (int r = 0; r < kMaxR; r++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                u0[r][v] = r < nr ? x[((ls + r) + 9 * v) * ldc + i] : 0.0;
+                up[r][v] = 0.0, down[r][v] = 0.0;
+            }
+#pragma unroll 1
+        for (int s = 0; s < NF; s++) {
+            const int32_t j = nbr[s * ldc + i];
+            if (j < 0)
+                continue;
+            const int32_t f = sface[s * ldc + i];
+            const int side = f2c[f] == i ? 0 : 1;
+            double un0, un1;
+            stable_normalized(fnc[f], fnc[ldf + f], un0, un1);
+            const double *ms = msec + int64_t(81 * side) * ldf + f;
+            const double *msO = msec + int64_t(81 * (1 - side)) * ldf + f;
+            double oIn[kMaxR][4];
+#pragma unroll
+            for (int r = 0; r < kMaxR; r++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    oIn[r][v] = 0.0;
+            // uOtherOther rows: neighbour's coefficients up to le, above le the
+            // limited face values of the higher orders mapped to its basis
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+                double uoo[4];
+                if (c <= le) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        uoo[v] = x[(c + 9 * v) * ldc + j];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+#pragma unroll
+                        for (int cc = le + 1; cc < 9; cc++)
+                            acc += msO[(9 * c + cc) * ldf] * uF[((s * 7 + (cc - 2)) * 4 + v) * ldc + i];
+                        uoo[v] = acc;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < kMaxR; r++)
+                    if (r < nr) {
+                        const double m = ms[(9 * (ls + r) + c) * ldf];
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            oIn[r][v] += m * uoo[v];
+                    }
+            }
+            const double *uLp = side ? u + 4 * int64_t(j) : ui;
+            const double *uRp = side ? ui : u + 4 * int64_t(j);
+            double UL[4], UR[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                UL[v] = uLp[v], UR[v] = uRp[v];
+            double ML[16], MR[16];
+            ok &= char_matrices(gamma, UL, UR, un0, un1, ML, MR);
+#pragma unroll
+            for (int r = 0; r < kMaxR; r++) {
+                if (r >= nr)
+                    continue;
+                double oC[4], tC[4], out[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    double p = 0, q = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        p += ML[4 * a + b] * oIn[r][b];
+                        q += ML[4 * a + b] * u0[r][b];
+                    }
+                    oC[a] = p, tC[a] = q;
+                }
+                // FWBAP_L2_Biway(uThisIn, uOtherIn, 1)
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const double u1 = tC[a], u2 = oC[a];
+                    const double frac = u1 / (fabs(u2) + 1e-12) * sgn(u2);
+                    const double th1 = frac * frac * frac;
+                    const double th2 = (frac * frac) * (frac * frac);
+                    out[a] = u1 * (1.0 + th1) / (1.0 + th2);
+                }
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    double xv = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        xv += MR[4 * a + b] * out[b];
+                    if (ls + r >= 2)
+                        uF[((s * 7 + (ls + r - 2)) * 4 + a) * ldc + i] = xv;
+                    // FWBAP_L2_Multiway accumulation, neighbours in cell2face order
+                    const double ti = u0[r][a] / (fabs(xv) + eps4) * sgn(xv);
+                    down[r][a] += (ti * ti) * (ti * ti);
+                    up[r][a] += ti * ti * ti;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kMaxR; r++)
+            if (r < nr)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    xo[((ls + r) + 9 * v) * ldc + i] = u0[r][v] * (up[r][v] + nStd) / (down[r][v] + nStd);
+    }
+    if (!ok)
+        atomicOr(errw, ERRBIT_NONPOSITIVE);
+}
+
 __global__ __launch_bounds__(kBlock) void k_set(int64_t n, double *__restrict__ p, double v) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
@@ -813,6 +1057,36 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             bfl.push_back(0);
         DNDSMI_TRY(c->alloc(c->bfaces, int64_t(bfl.size())));
         DNDSMI_TRY(c->upload(c->bfaces, bfl));
+        // ---- WBAP data
+        c->tang_weight = vs->tang_weight;
+        c->twd_norm = vs->tang_weight_direction == 0;
+        c->wbap_scale = vs->wbap_smooth_indicator_scale;
+        c->wbap_nstd = vs->wbap_n_std;
+        if (vr->face_dibj_g0 && vr->matrix_secondary) {
+            c->has_limiter = true;
+            std::vector<double> buf(180 * size_t(ldf), 0.0);
+            for (int64_t f = 0; f < nf; f++)
+                for (int k = 0; k < 180; k++)
+                    buf[k * ldf + f] = vr->face_dibj_g0[180 * f + k];
+            DNDSMI_TRY(c->alloc(c->dg0, 180 * ldf));
+            DNDSMI_TRY(c->upload(c->dg0, buf));
+            buf.assign(162 * size_t(ldf), 0.0);
+            for (int64_t f = 0; f < nf; f++)
+                for (int k = 0; k < 162; k++)
+                    buf[k * ldf + f] = vr->matrix_secondary[162 * f + k];
+            DNDSMI_TRY(c->alloc(c->msec, 162 * ldf));
+            DNDSMI_TRY(c->upload(c->msec, buf));
+            buf.assign(10 * size_t(ldf), 0.0);
+            for (int64_t f = 0; f < nf; f++)
+                for (int k = 0; k < 10; k++)
+                    buf[k * ldf + f] = vr->face_weight[10 * f + k];
+            DNDSMI_TRY(c->alloc(c->fw, 10 * ldf));
+            DNDSMI_TRY(c->upload(c->fw, buf));
+            DNDSMI_TRY(c->alloc(c->wsum, 8 * ldf));
+            DNDSMI_TRY(c->alloc(c->uF, int64_t(nfmax) * 28 * ldc));
+            DNDSMI_TRY(c->alloc(c->ind, ldc));
+            DNDSMI_TRY(c->alloc(c->xlim, 36 * ldc));
+        }
         // ---- scratch
         DNDSMI_TRY(c->alloc(c->lambda_face, ldf));
         DNDSMI_TRY(c->alloc(c->dlambda_face, ldf));
@@ -1017,7 +1291,47 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
     return c->launch_check("rhs kernels");
 }
 
+int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all) {
+    if (!c->has_limiter) {
+        c->err = "WBAP needs a setup built with build_limiter_data";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
+    const int nc = int(c->nc), nf = int(c->nf);
+    const double eps4 = std::pow(1e-200, 1.0 / 4); // verySmallReal_pDiP, DNDS_FV_Limiters.hpp:345
+    prof_begin(c, DNDSMI_K_WBAP_FACE);
+    hipLaunchKernelGGL(k_wbap_face, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->f2c,
+                       c->dg0, c->fw, c->fn, u, x, c->tang_weight, c->twd_norm, c->wsum);
+    prof_end(c);
+    prof_begin(c, DNDSMI_K_WBAP_CELL);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_wbap_cell<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
+                           c->nbr, c->sface, c->f2c, c->wsum, c->fnc, c->msec, u, x, xo, ind ? ind : c->ind, c->uF,
+                           c->gas.gamma, c->wbap_scale, c->wbap_nstd, eps4, if_all, c->errw);
+    else
+        hipLaunchKernelGGL(k_wbap_cell<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
+                           c->nbr, c->sface, c->f2c, c->wsum, c->fnc, c->msec, u, x, xo, ind ? ind : c->ind, c->uF,
+                           c->gas.gamma, c->wbap_scale, c->wbap_nstd, eps4, if_all, c->errw);
+    prof_end(c);
+    return c->launch_check("wbap kernels");
+}
+
 } // namespace
+
+extern "C" int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec,
+                                           dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1,
+                                           dndsmi_array *ifUseLimiter, int32_t ifAll) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNewBuf, DNDSMI_ARRAY_REC) || uRec == uRecNewBuf ||
+        (uRecNewBuf1 && !same_ctx(c, uRecNewBuf1, DNDSMI_ARRAY_REC)) ||
+        (ifUseLimiter && !same_ctx(c, ifUseLimiter, DNDSMI_ARRAY_SCALAR)))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, uRecNewBuf->d, ifUseLimiter ? ifUseLimiter->d : nullptr, ifAll));
+    if (uRecNewBuf1)
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(uRecNewBuf1->d, uRecNewBuf->d, sizeof(double) * uRecNewBuf->elems,
+                                         hipMemcpyDeviceToDevice, c->stream),
+                          "copy"));
+    return DNDSMI_OK;
+}
 
 extern "C" int dndsmi_reconstruction_jacobi_step(dndsmi_ctx *c, const dndsmi_array *u, dndsmi_array *uRec,
                                                  dndsmi_array *uRecNewBuf, double t) {
@@ -1058,11 +1372,15 @@ extern "C" int dndsmi_evaluate_rhs(dndsmi_ctx *c, dndsmi_array *rhs, const dndsm
 }
 
 extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
-                                  int32_t n_sweeps, double CFL, double MaxDt, int32_t UseLocaldt, double t,
-                                  double *dt_out) {
+                                  int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt, int32_t UseLocaldt,
+                                  double t, double *dt_out) {
     if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
         !same_ctx(c, uRecNew, DNDSMI_ARRAY_REC) || uRec == uRecNew || n_sweeps < 0)
         return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (use_limiter && !c->has_limiter) {
+        c->err = "WBAP needs a setup built with build_limiter_data";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
     // fdt: EvaluateDt; the stage update reads dt_i (local) or the device minimum
     DNDSMI_TRY(launch_dt(c, u->d, CFL, MaxDt, c->dt_tmp));
     // the global minimum sets the stage times (one host read per step)
@@ -1082,8 +1400,12 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
         }
         const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
         const double cB = stage ? 1 - Coef[stage] : 0.0;
-        DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,
-                              cB));
+        const double *xr = uRec->d;
+        if (use_limiter) { // the RHS sees the limited copy; uRec keeps the unlimited iterate
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0));
+            xr = c->xlim;
+        }
+        DNDSMI_TRY(launch_rhs(c, u->d, xr, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA, cB));
     }
     return DNDSMI_OK;
 }
@@ -1139,6 +1461,10 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return nif * (8 + 54 * 8 + 48 + 8 + 96) + nc * (36 * 8 + 32);
     case DNDSMI_K_RHS_CELL: // gather index, F + side fix per slot, volume, x read+write, xLast
         return c2f * (4 + 64) + nc * (8 + 32 * 3);
+    case DNDSMI_K_WBAP_FACE: // all-row bases of both sides, weights, normal, 8 terms out; x (rho, E) + u per cell
+        return nif * (180 * 8 + 80 + 16 + 8 + 64) + nc * (18 * 8 + 16);
+    case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out, x read, limited x out (unlimited cells)
+        return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     default:
         return 0;
     }

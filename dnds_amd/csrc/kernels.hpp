@@ -384,4 +384,119 @@ __device__ inline bool flux_face(const GasParams &gp, const double ULxy[4], cons
     return true;
 }
 
+// ----------------------------------------------------------- WBAP ----
+
+struct FFParams {
+    double W0, n0, n1, t0, t1, w2r, w3r;
+};
+
+// parameters of FFaceFunctional with USE_NORM_FUNCTIONAL (DNDS_FV_VR.hpp:1071-1098)
+__device__ inline FFParams ff_params(const double W[10], double gx, double gy, double tang_weight, int twd_norm) {
+    FFParams p;
+    const double epsR = 1e-50;
+    double w2r, w3r, length;
+    if (fabs(W[1]) >= epsR || fabs(W[2]) >= epsR) {
+        w2r = fabs(W[1]) >= fabs(W[2]) ? W[3] / (W[1] * W[1]) : W[5] / (W[2] * W[2]);
+        w3r = fabs(W[1]) >= fabs(W[2]) ? W[6] / (W[1] * W[1] * W[1]) : W[9] / (W[2] * W[2] * W[2]);
+        length = sqrt(W[1] * W[1] + W[2] * W[2]);
+    } else {
+        w2r = w3r = 0.0;
+        length = 0;
+    }
+    double n0, n1;
+    if (twd_norm) {
+        double ux, uy;
+        stable_normalized(gx, gy, ux, uy);
+        n0 = ux * length, n1 = uy * length;
+    } else {
+        n0 = W[1], n1 = W[2];
+    }
+    p.W0 = W[0];
+    p.n0 = n0, p.n1 = n1;
+    p.t0 = -n1 * (tang_weight * 1.0), p.t1 = n0 * (tang_weight * 1.0);
+    p.w2r = w2r, p.w3r = w3r;
+    return p;
+}
+
+// Conj(col, col) of FFaceFunctional for one column of a 10-row block
+__device__ inline double ff_self(const FFParams &p, const double d[10]) {
+    const double n0 = p.n0, n1 = p.n1, t0 = p.t0, t1 = p.t1, w2r = p.w2r, w3r = p.w3r;
+    double a[10];
+    a[0] = d[0];
+    a[1] = d[1] * n0 + d[2] * n1;
+    a[2] = d[1] * t0 + d[2] * t1;
+    a[3] = (d[3] * n0 * n0 + d[4] * n0 * n1 * 2 + d[5] * n1 * n1) * w2r;
+    a[4] = (d[3] * n0 * t0 + d[4] * (n0 * t1 + n1 * t0) + d[5] * n1 * t1) * w2r;
+    a[5] = (d[3] * t0 * t0 + d[4] * t0 * t1 * 2 + d[5] * t1 * t1) * w2r;
+    a[6] = (d[6] * n0 * n0 * n0 + d[7] * n0 * n0 * n1 * 3 + d[8] * n0 * n1 * n1 * 3 + d[9] * n1 * n1 * n1) * w3r;
+    a[7] = (d[6] * n0 * n0 * t0 + d[7] * (n0 * n0 * t1 + n0 * t0 * n1 + t0 * n0 * n1) +
+            d[8] * (n0 * n1 * t1 + n0 * t1 * n1 + t0 * n1 * n1) + d[9] * n1 * n1 * t1) *
+           w3r;
+    a[8] = (d[6] * n0 * t0 * t0 + d[7] * (t0 * n0 * t1 + n0 * t0 * t1 + t0 * t0 * n1) +
+            d[8] * (t0 * n1 * t1 + n0 * t1 * t1 + t0 * t1 * n1) + d[9] * n1 * t1 * t1) *
+           w3r;
+    a[9] = (d[6] * t0 * t0 * t0 + d[7] * t0 * t0 * t1 * 3 + d[8] * t0 * t1 * t1 * 3 + d[9] * t1 * t1 * t1) * w3r;
+    double c = 0;
+    c += a[0] * a[0] * p.W0 * p.W0;
+    c += a[1] * a[1];
+    c += a[2] * a[2] * 1.0;
+    c += a[3] * a[3];
+    c += a[4] * a[4] * 2 * 1.0;
+    c += a[5] * a[5] * 1.0;
+    c += a[6] * a[6];
+    c += a[7] * a[7] * 3 * 1.0;
+    c += a[8] * a[8] * 3 * 1.0;
+    c += a[9] * a[9] * 1.0;
+    return c;
+}
+
+// fML / fMR (DNDS_FV_EulerSolver.hpp:749-795): left / right eigenvector
+// matrices at the mean of UL and UR in the face frame, mapped back to the
+// global frame.  ML, MR row-major 4x4.
+__device__ inline bool char_matrices(double gamma, const double UL[4], const double UR[4], double n0, double n1,
+                                     double ML[16], double MR[16]) {
+    double UC[4];
+    for (int k = 0; k < 4; k++)
+        UC[k] = (UL[k] + UR[k]) * 0.5;
+    const double r1 = n0 * UC[1] + n1 * UC[2];
+    const double r2 = (-n1) * UC[1] + n0 * UC[2];
+    UC[1] = r1, UC[2] = r2;
+    if (!(UC[0] > 0))
+        return false;
+    const double v0 = UC[1] / UC[0], v1 = UC[2] / UC[0];
+    const double vsqr = v0 * v0 + v1 * v1;
+    double asqr, p, H;
+    ideal_gas_thermal(UC[3], UC[0], vsqr, gamma, p, asqr, H);
+    if (!(asqr >= 0))
+        return false;
+    const double a = sqrt(asqr);
+    const double gb = gamma - 1;
+    double L[16];
+    L[0] = H + a / gb * (v0 - a), L[1] = -v0 - a / gb, L[2] = -v1, L[3] = 1;
+    L[4] = -2 * H + 4 / gb * (a * a), L[5] = v0 * 2, L[6] = v1 * 2, L[7] = -2;
+    L[8] = v1 * (-2 * (a * a) / gb), L[9] = 0, L[10] = 2 * (a * a) / gb, L[11] = 0;
+    L[12] = H - a / gb * (v0 + a), L[13] = -v0 + a / gb, L[14] = -v1, L[15] = 1;
+    const double sc = gb / (2 * a * a);
+    for (int k = 0; k < 16; k++)
+        L[k] *= sc;
+    for (int r = 0; r < 4; r++) {
+        const double c1 = L[4 * r + 1] * n0 + L[4 * r + 2] * (-n1);
+        const double c2 = L[4 * r + 1] * n1 + L[4 * r + 2] * n0;
+        ML[4 * r] = L[4 * r], ML[4 * r + 1] = c1, ML[4 * r + 2] = c2, ML[4 * r + 3] = L[4 * r + 3];
+    }
+    double R[16];
+    R[0] = 1, R[1] = 1, R[2] = 0, R[3] = 1;
+    R[4] = v0 - a, R[5] = v0, R[6] = 0, R[7] = v0 + a;
+    R[8] = v1, R[9] = v1, R[10] = 1, R[11] = v1;
+    R[12] = H - v0 * a, R[13] = 0.5 * vsqr, R[14] = v1, R[15] = H + v0 * a;
+    for (int c = 0; c < 4; c++) {
+        const double a1 = n0 * R[4 + c] + (-n1) * R[8 + c];
+        const double a2 = n1 * R[4 + c] + n0 * R[8 + c];
+        MR[c] = R[c], MR[4 + c] = a1, MR[8 + c] = a2, MR[12 + c] = R[12 + c];
+    }
+    return true;
+}
+
+__device__ inline double sgn(double x) { return double((x > 0) - (x < 0)); }
+
 } // namespace dndsmi_dev

@@ -188,6 +188,17 @@ class VRFiniteVolume2D:
         d = self.device
         _check(_lib().dndsmi_reconstruction_jacobi_step(d.ctx, u.h, uRec.h, uRecNewBuf.h, t), "Jacobi step", d.ctx)
 
+    def ReconstructionWBAPLimitFacialV3(self, u: "ArrayDOFV", uRec: "ArrayRecV", uRecNewBuf: "ArrayRecV",
+                                        uRecNewBuf1: "ArrayRecV | None" = None,
+                                        ifUseLimiter: "ArrayScalar | None" = None, ifAll: bool = False):
+        """WBAP V3 with the Euler characteristic matrices; uRecNewBuf receives
+        the limited reconstruction, ifUseLimiter the smoothness indicator."""
+        d = self.device
+        _check(_lib().dndsmi_wbap_limit_facial_v3(d.ctx, u.h, uRec.h, uRecNewBuf.h,
+                                                  uRecNewBuf1.h if uRecNewBuf1 is not None else None,
+                                                  ifUseLimiter.h if ifUseLimiter is not None else None,
+                                                  int(ifAll)), "WBAP V3", d.ctx)
+
 
 class Device:
     """The device context: mesh + setup on the GPU (one per evaluator)."""
@@ -308,17 +319,19 @@ class ExplicitSSPRK3LocalDt:
     follow the reference's frhs arguments (t + dt, t + dt/4, t + 2dt/3)."""
 
     def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, MaxDt: float = 1e100,
-                 UseLocaldt: bool = False):
+                 UseLocaldt: bool = False, useLimiter: bool = False):
         self.ev = evaluator
         self.n_sweeps = nInternalRecStep
         self.CFL = CFL
         self.MaxDt = MaxDt
         self.UseLocaldt = UseLocaldt
+        self.useLimiter = useLimiter
 
     def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, t: float) -> float:
         """Advances u in place; returns the global minimum dt of the step."""
         dt = C.c_double()
         d = self.ev.device
-        _check(_lib().dndsmi_ssprk3_step(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, self.CFL, self.MaxDt,
-                                         int(self.UseLocaldt), t, C.byref(dt)), "SSP-RK3 step", d.ctx)
+        _check(_lib().dndsmi_ssprk3_step(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, int(self.useLimiter),
+                                         self.CFL, self.MaxDt, int(self.UseLocaldt), t, C.byref(dt)),
+               "SSP-RK3 step", d.ctx)
         return dt.value

@@ -233,12 +233,25 @@ int dndsmi_evaluate_dt(dndsmi_ctx *ctx, dndsmi_array *dt, const dndsmi_array *u,
 int dndsmi_evaluate_rhs(dndsmi_ctx *ctx, dndsmi_array *rhs, const dndsmi_array *u, const dndsmi_array *uRec,
                         double t);
 
-/* One ExplicitSSPRK3LocalDt::Step over frhs = n_sweeps Jacobi sweeps then
- * EvaluateRHS, fdt = EvaluateDt(CFL, MaxDt, UseLocaldt).  Stage times follow
- * DNDS_ODE.hpp:138-151 (t + dt, t + dt/4, t + 2dt/3).  `t` is advanced by the
- * global minimum dt when dt_out is not NULL (*dt_out receives it). */
+/* ReconstructionWBAPLimitFacialV3(u, uRec, uRecNewBuf, uRecNewBuf1,
+ * ifUseLimiter, ifAll, fML, fMR) (DNDS_FV_VR.hpp:2053-2329) with the Euler
+ * characteristic matrices of the solver (DNDS_FV_EulerSolver.hpp:749-795):
+ * uRecNewBuf receives the limited reconstruction (uRec for cells below
+ * WBAP_SmoothIndicatorScale), uRec is read only.  ifUseLimiter (SCALAR, may
+ * be NULL) receives the smoothness indicator; uRecNewBuf1 (may be NULL)
+ * receives a copy of the result.  Needs a setup built with build_limiter_data. */
+int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *ctx, const dndsmi_array *u, const dndsmi_array *uRec,
+                                dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1, dndsmi_array *ifUseLimiter,
+                                int32_t ifAll);
+
+/* One ExplicitSSPRK3LocalDt::Step over frhs = n_sweeps Jacobi sweeps, then
+ * (use_limiter) WBAP V3 into a context buffer, then EvaluateRHS on the
+ * limited or unlimited reconstruction; fdt = EvaluateDt(CFL, MaxDt,
+ * UseLocaldt).  Stage times follow DNDS_ODE.hpp:138-151 (t + dt, t + dt/4,
+ * t + 2dt/3) with the global minimum dt; *dt_out (may be NULL) receives it. */
 int dndsmi_ssprk3_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
-                       int32_t n_sweeps, double CFL, double MaxDt, int32_t UseLocaldt, double t, double *dt_out);
+                       int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt, int32_t UseLocaldt, double t,
+                       double *dt_out);
 
 /* ======================================================================== */
 /* Measurement                                                              */
@@ -252,7 +265,9 @@ enum {
     DNDSMI_K_DT_CELL = 3,    /* EvaluateDt, cell part + block minima          */
     DNDSMI_K_RHS_FACE = 4,   /* EvaluateRHS, inner-face fluxes                */
     DNDSMI_K_RHS_CELL = 5,   /* EvaluateRHS gather (+ SSP-RK3 stage update)   */
-    DNDSMI_K_COUNT = 6
+    DNDSMI_K_WBAP_FACE = 6,  /* WBAP smoothness-indicator face terms          */
+    DNDSMI_K_WBAP_CELL = 7,  /* WBAP indicator + characteristic limiting      */
+    DNDSMI_K_COUNT = 8
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the

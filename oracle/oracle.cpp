@@ -723,13 +723,298 @@ extern "C" int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, 
     return ERR_OK;
 }
 
+namespace {
+
+// ---------------------------------------------------------------- WBAP ----
+
+// FFaceFunctional with USE_NORM_FUNCTIONAL for 10-row blocks
+// (DNDS_FV_VR.hpp:1071-1210; tangWeightModMin = 1 so every tangModifier
+// factor is 1): returns Conj(i, i) for the two columns of D (10 x 2).
+void face_functional_diag(const oracle_setting &cfg, const double W[10], const double ng[2], const double D[10][2],
+                          double out[2]) {
+    const double epsR = 1e-50;
+    double w2r, w3r, length;
+    if (std::fabs(W[1]) >= epsR || std::fabs(W[2]) >= epsR) {
+        w2r = std::fabs(W[1]) >= std::fabs(W[2]) ? W[3] / (W[1] * W[1]) : W[5] / (W[2] * W[2]);
+        w3r = std::fabs(W[1]) >= std::fabs(W[2]) ? W[6] / (W[1] * W[1] * W[1]) : W[9] / (W[2] * W[2] * W[2]);
+        length = std::sqrt(W[1] * W[1] + W[2] * W[2]);
+    } else {
+        w2r = w3r = 0.0;
+        length = 0;
+    }
+    double n0, n1;
+    if (cfg.tang_weight_direction == 0) {
+        double ux, uy;
+        stable_normalized(ng[0], ng[1], ux, uy);
+        n0 = ux * length, n1 = uy * length;
+    } else {
+        n0 = W[1], n1 = W[2];
+    }
+    const double t0 = -n1 * (cfg.tang_weight * 1.0), t1 = n0 * (cfg.tang_weight * 1.0);
+    for (int col = 0; col < 2; col++) {
+        const double d0 = D[0][col], d1 = D[1][col], d2 = D[2][col], d3 = D[3][col], d4 = D[4][col];
+        const double d5 = D[5][col], d6 = D[6][col], d7 = D[7][col], d8 = D[8][col], d9 = D[9][col];
+        double a[10];
+        a[0] = d0;
+        a[1] = d1 * n0 + d2 * n1;
+        a[2] = d1 * t0 + d2 * t1;
+        a[3] = (d3 * n0 * n0 + d4 * n0 * n1 * 2 + d5 * n1 * n1) * w2r;
+        a[4] = (d3 * n0 * t0 + d4 * (n0 * t1 + n1 * t0) + d5 * n1 * t1) * w2r;
+        a[5] = (d3 * t0 * t0 + d4 * t0 * t1 * 2 + d5 * t1 * t1) * w2r;
+        a[6] = (d6 * n0 * n0 * n0 + d7 * n0 * n0 * n1 * 3 + d8 * n0 * n1 * n1 * 3 + d9 * n1 * n1 * n1) * w3r;
+        a[7] = (d6 * n0 * n0 * t0 + d7 * (n0 * n0 * t1 + n0 * t0 * n1 + t0 * n0 * n1) +
+                d8 * (n0 * n1 * t1 + n0 * t1 * n1 + t0 * n1 * n1) + d9 * n1 * n1 * t1) *
+               w3r;
+        a[8] = (d6 * n0 * t0 * t0 + d7 * (t0 * n0 * t1 + n0 * t0 * t1 + t0 * t0 * n1) +
+                d8 * (t0 * n1 * t1 + n0 * t1 * t1 + t0 * t1 * n1) + d9 * n1 * t1 * t1) *
+               w3r;
+        a[9] = (d6 * t0 * t0 * t0 + d7 * t0 * t0 * t1 * 3 + d8 * t0 * t1 * t1 * 3 + d9 * t1 * t1 * t1) * w3r;
+        double c = 0;
+        c += a[0] * a[0] * W[0] * W[0];
+        c += a[1] * a[1];
+        c += a[2] * a[2] * 1.0;
+        c += a[3] * a[3];
+        c += a[4] * a[4] * 2 * 1.0;
+        c += a[5] * a[5] * 1.0;
+        c += a[6] * a[6];
+        c += a[7] * a[7] * 3 * 1.0;
+        c += a[8] * a[8] * 3 * 1.0;
+        c += a[9] * a[9] * 1.0;
+        out[col] = c;
+    }
+}
+
+// fML / fMR of the solver (DNDS_FV_EulerSolver.hpp:749-795): left and right
+// eigenvector matrices at the face mean state, in the global frame.
+int char_matrices(const oracle_setting &cfg, const double UL[4], const double UR[4], const double n[2], double ML[4][4],
+                  double MR[4][4]) {
+    double UC[4];
+    for (int k = 0; k < 4; k++)
+        UC[k] = (UL[k] + UR[k]) * 0.5;
+    double r1 = n[0] * UC[1] + n[1] * UC[2];
+    double r2 = (-n[1]) * UC[1] + n[0] * UC[2];
+    UC[1] = r1, UC[2] = r2;
+    if (!(UC[0] > 0))
+        return ERR_NONPOSITIVE;
+    double velo[2] = {UC[1] / UC[0], UC[2] / UC[0]};
+    double vsqr = velo[0] * velo[0] + velo[1] * velo[1];
+    double asqr, p, H;
+    ideal_gas_thermal(UC[3], UC[0], vsqr, cfg.gamma, p, asqr, H);
+    if (!(asqr >= 0))
+        return ERR_NONPOSITIVE;
+    double a = std::sqrt(asqr);
+    double L[4][4], R[4][4];
+    left_eigen(velo, vsqr, H, a, cfg.gamma, L);
+    right_eigen(velo, vsqr, H, a, R);
+    // M(:, 1:2) *= normBase^T ; normBase = [[n0, -n1], [n1, n0]]
+    for (int r = 0; r < 4; r++) {
+        double c1 = L[r][1] * n[0] + L[r][2] * (-n[1]);
+        double c2 = L[r][1] * n[1] + L[r][2] * n[0];
+        for (int k = 0; k < 4; k++)
+            ML[r][k] = L[r][k];
+        ML[r][1] = c1, ML[r][2] = c2;
+    }
+    // M(1:2, :) = normBase * M(1:2, :)
+    for (int c = 0; c < 4; c++) {
+        double r1c = n[0] * R[1][c] + (-n[1]) * R[2][c];
+        double r2c = n[1] * R[1][c] + n[0] * R[2][c];
+        MR[0][c] = R[0][c];
+        MR[1][c] = r1c, MR[2][c] = r2c;
+        MR[3][c] = R[3][c];
+    }
+    return ERR_OK;
+}
+
+inline double sgn(double x) { return double((x > 0) - (x < 0)); }
+
+} // namespace
+
+// ReconstructionWBAPLimitFacialV3 (DNDS_FV_VR.hpp:2053-2329) with the
+// characteristic matrices fML / fMR and FWBAP_L2_Biway / FWBAP_L2_Multiway
+// (DNDS_FV_Limiters.hpp:341-449).  urec is read only; urec_new receives the
+// limited reconstruction (urec for cells below the smoothness threshold);
+// if_use_limiter receives the indicator.
+extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                              const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all) {
+    const int64_t nc = m->n_cell;
+    const int P = 3;
+    // Step 0: smoothness indicator from the (rho, E) jumps at the first LINE_3
+    // point with the LINE_1 weight 2 (DNDS_FV_VR.hpp:2075-2138)
+    for (int64_t i = 0; i < nc; i++) {
+        const int64_t b0 = m->cell_ptr[i];
+        const int nf = int(m->cell_ptr[i + 1] - b0);
+        double sum[2][2] = {{0, 0}, {0, 0}};
+        for (int lf = 0; lf < nf; lf++) {
+            const int64_t f = m->cell2face[b0 + lf];
+            const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
+            const int64_t j = L == i ? R : L;
+            double val[10][2] = {{0}}, jump[10][2] = {{0}};
+            if (j != kNoCell) {
+                const double *D0 = s->face_dibj_g0 + 180 * f;      // side 0 basis
+                const double *D1 = s->face_dibj_g0 + 180 * f + 90; // side 1 basis
+                double vL[10][2], vR[10][2];
+                const int vars[2] = {0, 3};
+                for (int r = 0; r < 10; r++)
+                    for (int q = 0; q < 2; q++) {
+                        double aL = 0, aR = 0;
+                        for (int c = 0; c < 9; c++) {
+                            aL += D0[9 * r + c] * urec[36 * i + c + 9 * vars[q]];
+                            aR += D1[9 * r + c] * urec[36 * j + c + 9 * vars[q]];
+                        }
+                        vL[r][q] = aL, vR[r][q] = aR;
+                    }
+                for (int q = 0; q < 2; q++) {
+                    vL[0][q] += u[4 * i + vars[q]];
+                    vR[0][q] += u[4 * j + vars[q]];
+                }
+                for (int r = 0; r < 10; r++)
+                    for (int q = 0; q < 2; q++) {
+                        val[r][q] = (vL[r][q] + vR[r][q]) * 0.5;
+                        jump[r][q] = (vL[r][q] - vR[r][q]) * 0.5;
+                    }
+            }
+            double ISI[2], IJI[2];
+            face_functional_diag(*cfg, s->face_weight + 10 * f, s->face_norm + 6 * f, val, ISI);
+            face_functional_diag(*cfg, s->face_weight + 10 * f, s->face_norm + 6 * f, jump, IJI);
+            const double nl = std::sqrt(s->face_norm[6 * f] * s->face_norm[6 * f] +
+                                        s->face_norm[6 * f + 1] * s->face_norm[6 * f + 1]);
+            double inc[2][2] = {{IJI[0], ISI[0]}, {IJI[1], ISI[1]}};
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) {
+                    inc[a][b] *= nl;
+                    sum[a][b] += inc[a][b] * 2.0;
+                }
+        }
+        double si0 = sum[0][0] / (sum[0][1] + kVerySmallReal);
+        double si1 = sum[1][0] / (sum[1][1] + kVerySmallReal);
+        double sImax = std::max(std::abs(si0), std::abs(si1));
+        if_use_limiter[i] = std::sqrt(sImax) * (P * P);
+    }
+    std::memcpy(urec_new, urec, sizeof(double) * 36 * nc);
+    const double eps4 = std::pow(kVerySmallReal, 1.0 / 4); // verySmallReal_pDiP
+    const double nStd = cfg->wbap_n_std;
+    std::vector<double> lim(36);
+    // Step 1: per-order limiting (DNDS_FV_VR.hpp:2152-2317)
+    for (int64_t i = 0; i < nc; i++) {
+        if (if_use_limiter[i] < cfg->wbap_smooth_indicator_scale && !if_all)
+            continue;
+        const int64_t b0 = m->cell_ptr[i];
+        const int nf = int(m->cell_ptr[i + 1] - b0);
+        double uFaces[4][36];
+        for (int k = 0; k < 4; k++)
+            for (int q = 0; q < 36; q++)
+                uFaces[k][q] = 0.0;
+        for (int order = 3; order >= 1; order--) {
+            const int ls = order == 3 ? 5 : (order == 2 ? 2 : 0);
+            const int le = order == 3 ? 8 : (order == 2 ? 4 : 1);
+            const int nr = le - ls + 1;
+            double others[7][4][4]; // [neighbour + 1][row][var]
+            int nOther = 0;
+            for (int r = 0; r < nr; r++)
+                for (int v = 0; v < 4; v++)
+                    others[0][r][v] = urec[36 * i + (ls + r) + 9 * v];
+            nOther = 1;
+            for (int lf = 0; lf < nf; lf++) {
+                const int64_t f = m->cell2face[b0 + lf];
+                const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
+                const int64_t j = L == i ? R : L;
+                if (j == kNoCell)
+                    continue;
+                const int side = L == i ? 0 : 1;
+                double un[2];
+                stable_normalized(s->face_norm_center[2 * f], s->face_norm_center[2 * f + 1], un[0], un[1]);
+                const double *ms = s->matrix_secondary + 162 * f + 81 * side;            // to this cell's basis
+                const double *msO = s->matrix_secondary + 162 * f + 81 * (1 - side);     // to the other's basis
+                double uOO[9][4];
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++)
+                        uOO[r][v] = urec[36 * j + r + 9 * v];
+                if (le < 8)
+                    for (int r = le + 1; r < 9; r++)
+                        for (int v = 0; v < 4; v++) {
+                            double acc = 0;
+                            for (int c = le + 1; c < 9; c++)
+                                acc += msO[9 * r + c] * uFaces[lf][c + 9 * v];
+                            uOO[r][v] = acc;
+                        }
+                double oIn[4][4], tIn[4][4];
+                for (int r = 0; r < nr; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += ms[9 * (ls + r) + c] * uOO[c][v];
+                        oIn[r][v] = acc;
+                        tIn[r][v] = others[0][r][v];
+                    }
+                const double *uR = side ? u + 4 * i : u + 4 * j;
+                const double *uL = side ? u + 4 * j : u + 4 * i;
+                double ML[4][4], MR[4][4];
+                int rc = char_matrices(*cfg, uL, uR, un, ML, MR);
+                if (rc)
+                    return rc;
+                double oC[4][4], tC[4][4], out[4][4];
+                for (int r = 0; r < nr; r++)
+                    for (int a = 0; a < 4; a++) {
+                        double x = 0, y = 0;
+                        for (int b = 0; b < 4; b++) {
+                            x += ML[a][b] * oIn[r][b];
+                            y += ML[a][b] * tIn[r][b];
+                        }
+                        oC[r][a] = x, tC[r][a] = y;
+                    }
+                // FWBAP_L2_Biway(uThisIn, uOtherIn, n = 1)
+                for (int r = 0; r < nr; r++)
+                    for (int a = 0; a < 4; a++) {
+                        const double u1 = tC[r][a], u2 = oC[r][a];
+                        const double frac = u1 / (std::abs(u2) + 1e-12) * sgn(u2);
+                        const double th1 = frac * frac * frac;
+                        const double th2 = (frac * frac) * (frac * frac);
+                        out[r][a] = u1 * (1.0 + th1) / (1.0 + th2);
+                    }
+                for (int r = 0; r < nr; r++)
+                    for (int a = 0; a < 4; a++) {
+                        double x = 0;
+                        for (int b = 0; b < 4; b++)
+                            x += MR[a][b] * out[r][b];
+                        others[nOther][r][a] = x;
+                        uFaces[lf][(ls + r) + 9 * a] = x;
+                    }
+                nOther++;
+            }
+            // FWBAP_L2_Multiway(uOthers, n = WBAP_nStd)
+            for (int r = 0; r < nr; r++)
+                for (int v = 0; v < 4; v++) {
+                    double up = 0, down = 0;
+                    const double u0 = others[0][r][v];
+                    for (int k = 1; k < nOther; k++) {
+                        const double uo = others[k][r][v];
+                        const double ti = u0 / (std::abs(uo) + eps4) * sgn(uo);
+                        down += (ti * ti) * (ti * ti);
+                        up += ti * ti * ti;
+                    }
+                    lim[(ls + r) + 9 * v] = u0 * (up + nStd) / (down + nStd);
+                }
+        }
+        for (int k = 0; k < 36; k++)
+            urec_new[36 * i + k] = lim[k];
+    }
+    for (int64_t k = 0; k < 36 * nc; k++)
+        if (!std::isfinite(urec_new[k]))
+            return ERR_NAN;
+    return ERR_OK;
+}
+
+namespace {
+} // namespace
+
 // ExplicitSSPRK3LocalDt::Step, DNDS_ODE.hpp:131-178; frhs as the Eikonal
 // solver composes it (DNDS_FV_EikonalCRSolver.hpp:948-999): sweeps then RHS.
 extern "C" int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
-                                   int32_t n_steps, int32_t n_sweeps, double cfl, double max_dt, int32_t use_local_dt,
-                                   double *u, double *urec, double *t) {
+                                   int32_t n_steps, int32_t n_sweeps, int32_t use_limiter, double cfl, double max_dt,
+                                   int32_t use_local_dt, double *u, double *urec, double *t) {
     const int64_t nc = m->n_cell;
     std::vector<double> dt(nc), rhs(4 * nc), xLast(4 * nc), urecNew(36 * nc), dlf(m->n_face);
+    std::vector<double> urecLim(use_limiter ? 36 * nc : 0), ifUse(use_limiter ? nc : 0);
     const double Coef[3] = {1.0, 0.25, 2.0 / 3.0};
     for (int step = 0; step < n_steps; step++) {
         double dtMin;
@@ -745,7 +1030,12 @@ extern "C" int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, 
                 if (rc)
                     return rc;
             }
-            rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, ts, rhs.data());
+            if (use_limiter) {
+                rc = oracle_wbap_v3(m, s, cfg, u, urec, urecLim.data(), ifUse.data(), 0);
+                if (rc)
+                    return rc;
+            }
+            rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, use_limiter ? urecLim.data() : urec, ts, rhs.data());
             if (rc)
                 return rc;
             for (int64_t i = 0; i < nc; i++)

@@ -72,12 +72,20 @@ int oracle_evaluate_dt(const oracle_mesh *m, const oracle_setup *s, const oracle
 int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
                         const double *delta_lambda_face, const double *u, const double *urec, double t, double *rhs);
 
+/* VRFiniteVolume2D::ReconstructionWBAPLimitFacialV3 (DNDS_FV_VR.hpp:2053-2329)
+ * with the solver's characteristic matrices (DNDS_FV_EulerSolver.hpp:749-795):
+ * urec is read only, urec_new receives the limited reconstruction,
+ * if_use_limiter the smoothness indicator. */
+int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                   const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all);
+
 /* n_steps of ExplicitSSPRK3LocalDt::Step (DNDS_ODE.hpp:131-178) with frhs =
- * n_sweeps Jacobi sweeps + EvaluateRHS and fdt = EvaluateDt.  *t advances
- * by the global minimum dt of each step. */
+ * n_sweeps Jacobi sweeps [+ WBAP V3] + EvaluateRHS (on the limited
+ * reconstruction when use_limiter) and fdt = EvaluateDt.  *t advances by the
+ * global minimum dt of each step. */
 int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, int32_t n_steps,
-                        int32_t n_sweeps, double cfl, double max_dt, int32_t use_local_dt, double *u, double *urec,
-                        double *t);
+                        int32_t n_sweeps, int32_t use_limiter, double cfl, double max_dt, int32_t use_local_dt,
+                        double *u, double *urec, double *t);
 
 /* Gas-dynamics kernels exposed for unit tests (rotated frame, 4 variables). */
 int oracle_roe_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F);

@@ -61,7 +61,8 @@ def lib() -> C.CDLL:
             "oracle_evaluate_rhs": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, C.c_double,
                                     PD],
             "oracle_ssprk3_steps": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), C.c_int32, C.c_int32,
-                                    C.c_double, C.c_double, C.c_int32, PD, PD, PD],
+                                    C.c_int32, C.c_double, C.c_double, C.c_int32, PD, PD, PD],
+            "oracle_wbap_v3": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, PD, C.c_int32],
             "oracle_roe_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_hllc_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_eigenvectors": [PD, C.c_double, PD, PD],
@@ -150,13 +151,24 @@ class Oracle:
                                                  _d(u), _d(urec), t, _d(rhs)), "evaluate_rhs")
         return rhs
 
-    def ssprk3_steps(self, u, urec, n_steps, n_sweeps, cfl, t=0.0, max_dt=1e100, use_local_dt=False):
+    def wbap_v3(self, u, urec, if_all=False):
+        """Returns (limited reconstruction, smoothness indicator)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        urec = np.ascontiguousarray(urec, dtype=np.float64)
+        out = np.empty_like(urec)
+        ind = np.empty(self.n_cell)
+        self._check(self.lib.oracle_wbap_v3(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u), _d(urec),
+                                            _d(out), _d(ind), int(if_all)), "wbap_v3")
+        return out, ind
+
+    def ssprk3_steps(self, u, urec, n_steps, n_sweeps, cfl, t=0.0, max_dt=1e100, use_local_dt=False,
+                     use_limiter=False):
         u = np.array(u, dtype=np.float64, order="C", copy=True)
         urec = np.array(urec, dtype=np.float64, order="C", copy=True)
         tt = C.c_double(t)
         self._check(self.lib.oracle_ssprk3_steps(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), n_steps,
-                                                 n_sweeps, cfl, max_dt, int(use_local_dt), _d(u), _d(urec),
-                                                 C.byref(tt)), "ssprk3_steps")
+                                                 n_sweeps, int(use_limiter), cfl, max_dt, int(use_local_dt), _d(u),
+                                                 _d(urec), C.byref(tt)), "ssprk3_steps")
         return u, urec, tt.value
 
 

@@ -32,8 +32,11 @@ def build(kind, bcname, riemann="Roe", nx=9, ny=7, jitter=0.2, seed=5):
                                seed=seed, bc=BC_SETS[bcname])
     far = (1.0, 0.3, -0.2, 2.6)
     ev = EulerEvaluator(mesh, vfv, riemannSolverType=riemann, gamma=1.4, farFieldStaticValue=far)
+    st = vfv.setting
     orc = oracle.Oracle(mesh.arrays(), vfv.arrays(), riemann_solver=1 if riemann == "Roe" else 2, gamma=1.4,
-                        far_field=far, jacobi_relax=vfv.setting.jacobi_relax)
+                        far_field=far, jacobi_relax=st.jacobi_relax, tang_weight=st.tang_weight,
+                        tang_weight_direction=st.tang_weight_direction,
+                        wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std)
     return mesh, vfv, ev, orc
 
 
@@ -143,3 +146,55 @@ def test_wall_face_in_rhs_is_an_error_on_both_sides():
     _, _, _, dlf = orc.evaluate_dt(u, 0.5)
     with pytest.raises(RuntimeError):
         orc.evaluate_rhs(u, urec, dlf)
+
+
+@pytest.mark.parametrize("kind,bcname", [(0, "periodic"), (2, "farfield"), (1, "walls"), (0, "riemann"),
+                                         (2, "mixed_bc")])
+@pytest.mark.parametrize("if_all", [False, True])
+def test_wbap_v3_matches_oracle(kind, bcname, if_all):
+    """ReconstructionWBAPLimitFacialV3: smoothness indicator and the limited
+    reconstruction (limited and unlimited cells both present)."""
+    mesh, vfv, ev, orc = build(kind, bcname)
+    # smooth means everywhere, rough reconstructions only where x > 0.5, so
+    # both limited and unlimited cells occur
+    bary = vfv.arrays(["cell_bary"])["cell_bary"].reshape(-1, 2)
+    _, urec = random_state(mesh.n_cell, 16, rec_amp=0.3)
+    rho = 1.0 + 0.1 * np.sin(3 * bary[:, 0]) * np.cos(2 * bary[:, 1])
+    vx, vy, p = 0.4 + 0.1 * bary[:, 1], -0.2 + 0.05 * bary[:, 0], 1.0 + 0.1 * bary[:, 0] * bary[:, 1]
+    u = np.stack([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx ** 2 + vy ** 2)], axis=1)
+    urec[bary[:, 0] < 0.5] = 0.0
+    du, dr, dn, ind = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec(), ev.BuildScalar()
+    vfv.ReconstructionWBAPLimitFacialV3(du, dr, dn, None, ind, ifAll=if_all)
+    ref, rind = orc.wbap_v3(u, urec, if_all)
+    assert rel_err(ind.download(), rind) <= TOL
+    got = dn.download()
+    assert rel_err(got, ref) <= TOL
+    assert rel_err(dr.download(), urec) == 0.0  # uRec is read only
+    limited = rind >= vfv.setting.wbap_smooth_indicator_scale
+    if not if_all:
+        assert 0 < limited.sum() < mesh.n_cell
+        assert np.array_equal(got[~limited], urec[~limited])
+
+
+def test_ssprk3_with_limiter_matches_oracle_on_riemann():
+    """The four-quadrant Riemann case at 24 x 24 quads: SSP-RK3 with 5 sweeps,
+    WBAP V3 and HLLC for 10 steps against the oracle."""
+    import oracle
+    from dnds_amd import cases
+    from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
+    case = cases.riemann_2d(24, nthreads=4)
+    ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType="HLLC", farFieldStaticValue=case.far)
+    st = case.vfv.setting
+    orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2, far_field=case.far,
+                        tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
+                        wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale)
+    urec0 = np.zeros((case.n_cell, 36))
+    du, dr, dn = ev.BuildMean().upload(case.u0), case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=5, CFL=0.5, useLimiter=True)
+    t = 0.0
+    for _ in range(10):
+        t += ode.Step(du, dr, dn, t)
+    ru, rr, rt = orc.ssprk3_steps(case.u0, urec0, 10, 5, 0.5, use_limiter=True)
+    assert abs(t - rt) <= TOL * rt
+    assert rel_err(du.download(), ru) <= TOL
+    assert rel_err(dr.download(), rr) <= TOL
