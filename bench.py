@@ -46,6 +46,9 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1), help="host threads for the setup")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on one step (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the timed steps, N device copies of one rec array (a plain copy of known size "
+                         "for scaling profiler byte counters)")
     ap.add_argument("--case", default="riemann", choices=["riemann", "vortex"],
                     help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale")
     args = ap.parse_args()
@@ -98,6 +101,12 @@ def main():
     lib.dndsmi_profile_enable(ev.device.ctx, 0)
     for name, p in prof.items():
         log(f"{name}: {p['launches']} launches, {p['avg_ms']:.4f} ms avg, {p['gbs']:.0f} GB/s algorithmic")
+    if args.copy_probe:
+        nbytes = 36 * ((nc + 63) // 64 * 64) * 8
+        log(f"copy probe: {args.copy_probe} device copies of {nbytes} bytes")
+        for _ in range(args.copy_probe):
+            rn.copy_from(r)
+        ev.device.sync()
     uf = u.download()
     if not np.isfinite(uf).all():
         raise RuntimeError("non-finite state after the timed steps")

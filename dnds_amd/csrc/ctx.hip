@@ -77,6 +77,8 @@ struct dndsmi_ctx {
     double *uF = nullptr;      // [nfmax*28][ldc] WBAP per-cell face scratch
     double *ind = nullptr;     // [ldc] indicator when the caller passes none
     double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps
+    int32_t *lim_list = nullptr;  // [ldc] cells to limit
+    int32_t *lim_count = nullptr; // [1]
     double tang_weight = 1.0;
     int twd_norm = 1;
     double wbap_scale = 1e-10, wbap_nstd = 10.0;
@@ -685,49 +687,77 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
     }
 }
 
-// WBAP V3: smoothness indicator (sum over the cell's faces in cell2face
-// order, DNDS_FV_VR.hpp:2125-2137) and, for cells at or above
-// WBAP_SmoothIndicatorScale (or all cells when ifAll), the per-order
-// characteristic WBAP limiting (DNDS_FV_VR.hpp:2152-2317).  Reads x, writes
-// xo (= x for unlimited cells).  uF is per-cell scratch holding the limited
-// face values of rows 2..8 of the orders already done.
+// WBAP V3 smoothness indicator: sum over the cell's faces in cell2face order
+// (DNDS_FV_VR.hpp:2125-2137).  Cells below WBAP_SmoothIndicatorScale (unless
+// ifAll) keep their reconstruction; the others are appended to `list`
+// (one atomic per wavefront; the order of the list does not affect any
+// result, each cell is limited independently).
 template <int NF>
-__global__ __launch_bounds__(kBlock) void k_wbap_cell(int32_t nc, int64_t ldc, int64_t ldf,
-                                                      const int32_t *__restrict__ nbr,
-                                                      const int32_t *__restrict__ sface,
-                                                      const int32_t *__restrict__ f2c,
-                                                      const double *__restrict__ wsum,
-                                                      const double *__restrict__ fnc,
-                                                      const double *__restrict__ msec, const double *__restrict__ u,
-                                                      const double *__restrict__ x, double *__restrict__ xo,
-                                                      double *__restrict__ ind, double *__restrict__ uF,
-                                                      double gamma, double scale, double nStd, double eps4,
-                                                      int if_all, int32_t *errw) {
+__global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, int64_t ldf,
+                                                     const int32_t *__restrict__ nbr,
+                                                     const int32_t *__restrict__ sface,
+                                                     const int32_t *__restrict__ f2c,
+                                                     const double *__restrict__ wsum, const double *__restrict__ x,
+                                                     double *__restrict__ xo, double *__restrict__ ind, double scale,
+                                                     int if_all, int32_t *__restrict__ count,
+                                                     int32_t *__restrict__ list) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc)
-        return;
-    double sum[4] = {0, 0, 0, 0};
+    bool lim = false;
+    if (i < nc) {
+        double sum[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < NF; s++) {
-        const int32_t j = nbr[s * ldc + i];
-        if (j < 0)
-            continue;
-        const int32_t f = sface[s * ldc + i];
-        const int side = f2c[f] == i ? 0 : 1;
+        for (int s = 0; s < NF; s++) {
+            const int32_t j = nbr[s * ldc + i];
+            if (j < 0)
+                continue;
+            const int32_t f = sface[s * ldc + i];
+            const int side = f2c[f] == i ? 0 : 1;
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-            sum[k] += wsum[(4 * side + k) * ldf + f];
+            for (int k = 0; k < 4; k++)
+                sum[k] += wsum[(4 * side + k) * ldf + f];
+        }
+        const double si0 = sum[0] / (sum[1] + kVerySmallReal);
+        const double si1 = sum[2] / (sum[3] + kVerySmallReal);
+        const double flag = sqrt(fmax(fabs(si0), fabs(si1))) * (3 * 3);
+        ind[i] = flag;
+        lim = !(flag < scale && !if_all);
+        if (!lim) {
+#pragma unroll
+            for (int k = 0; k < 36; k++)
+                xo[k * ldc + i] = x[k * ldc + i];
+        }
     }
-    const double si0 = sum[0] / (sum[1] + kVerySmallReal);
-    const double si1 = sum[2] / (sum[3] + kVerySmallReal);
-    const double flag = sqrt(fmax(fabs(si0), fabs(si1))) * (3 * 3);
-    ind[i] = flag;
-    if (flag < scale && !if_all) {
-#pragma unroll
-        for (int k = 0; k < 36; k++)
-            xo[k * ldc + i] = x[k * ldc + i];
+    const uint64_t mask = __ballot(lim);
+    if (mask == 0)
         return;
-    }
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((unsigned long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == first)
+        base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, first);
+    if (lim)
+        list[base + __popcll(mask & ((1ull << lane) - 1))] = i;
+}
+
+// WBAP V3 per-order characteristic limiting of the listed cells
+// (DNDS_FV_VR.hpp:2152-2317).  Reads x, writes xo.  uF is per-cell scratch
+// holding the limited face values of rows 2..8 of the orders already done.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_wbap_lim(int64_t ldc, int64_t ldf, const int32_t *__restrict__ count,
+                                                     const int32_t *__restrict__ list,
+                                                     const int32_t *__restrict__ nbr,
+                                                     const int32_t *__restrict__ sface,
+                                                     const int32_t *__restrict__ f2c,
+                                                     const double *__restrict__ fnc,
+                                                     const double *__restrict__ msec, const double *__restrict__ u,
+                                                     const double *__restrict__ x, double *__restrict__ xo,
+                                                     double *__restrict__ uF, double gamma, double nStd, double eps4,
+                                                     int32_t *errw) {
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= count[0])
+        return;
+    const int32_t i = list[tid];
     double ui[4];
 #pragma unroll
     for (int v = 0; v < 4; v++)
@@ -764,8 +794,8 @@ __global__ __launch_bounds__(kBlock) void k_wbap_cell(int32_t nc, int64_t ldc, i
 #pragma unroll
                 for (int v = 0; v < 4; v++)
                     oIn[r][v] = 0.0;
-            // uOtherOther rows: neighbour's coefficients up to le, above le the
-            // limited face values of the higher orders mapped to its basis
+            // uOtherOther rows: the neighbour's coefficients up to le, above le
+            // the limited face values of the higher orders mapped to its basis
 #pragma unroll
             for (int c = 0; c < 9; c++) {
                 double uoo[4];
@@ -848,6 +878,11 @@ __global__ __launch_bounds__(kBlock) void k_wbap_cell(int32_t nc, int64_t ldc, i
     }
     if (!ok)
         atomicOr(errw, ERRBIT_NONPOSITIVE);
+}
+
+__global__ void k_zero_int2(int32_t *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        p[0] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_set(int64_t n, double *__restrict__ p, double v) {
@@ -1086,6 +1121,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             DNDSMI_TRY(c->alloc(c->uF, int64_t(nfmax) * 28 * ldc));
             DNDSMI_TRY(c->alloc(c->ind, ldc));
             DNDSMI_TRY(c->alloc(c->xlim, 36 * ldc));
+            DNDSMI_TRY(c->alloc(c->lim_list, ldc));
+            DNDSMI_TRY(c->alloc(c->lim_count, 1));
         }
         // ---- scratch
         DNDSMI_TRY(c->alloc(c->lambda_face, ldf));
@@ -1302,15 +1339,26 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
     hipLaunchKernelGGL(k_wbap_face, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->f2c,
                        c->dg0, c->fw, c->fn, u, x, c->tang_weight, c->twd_norm, c->wsum);
     prof_end(c);
+    hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(64), 0, c->stream, c->lim_count);
     prof_begin(c, DNDSMI_K_WBAP_CELL);
     if (c->nfmax == 3)
-        hipLaunchKernelGGL(k_wbap_cell<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
-                           c->nbr, c->sface, c->f2c, c->wsum, c->fnc, c->msec, u, x, xo, ind ? ind : c->ind, c->uF,
-                           c->gas.gamma, c->wbap_scale, c->wbap_nstd, eps4, if_all, c->errw);
+        hipLaunchKernelGGL(k_wbap_ind<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->nbr,
+                           c->sface, c->f2c, c->wsum, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count,
+                           c->lim_list);
     else
-        hipLaunchKernelGGL(k_wbap_cell<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
-                           c->nbr, c->sface, c->f2c, c->wsum, c->fnc, c->msec, u, x, xo, ind ? ind : c->ind, c->uF,
-                           c->gas.gamma, c->wbap_scale, c->wbap_nstd, eps4, if_all, c->errw);
+        hipLaunchKernelGGL(k_wbap_ind<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->nbr,
+                           c->sface, c->f2c, c->wsum, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count,
+                           c->lim_list);
+    prof_end(c);
+    prof_begin(c, DNDSMI_K_WBAP_LIM);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_wbap_lim<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc, c->ldf, c->lim_count,
+                           c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x, xo, c->uF, c->gas.gamma,
+                           c->wbap_nstd, eps4, c->errw);
+    else
+        hipLaunchKernelGGL(k_wbap_lim<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc, c->ldf, c->lim_count,
+                           c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x, xo, c->uF, c->gas.gamma,
+                           c->wbap_nstd, eps4, c->errw);
     prof_end(c);
     return c->launch_check("wbap kernels");
 }
@@ -1463,8 +1511,10 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return c2f * (4 + 64) + nc * (8 + 32 * 3);
     case DNDSMI_K_WBAP_FACE: // all-row bases of both sides, weights, normal, 8 terms out; x (rho, E) + u per cell
         return nif * (180 * 8 + 80 + 16 + 8 + 64) + nc * (18 * 8 + 16);
-    case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out, x read, limited x out (unlimited cells)
+    case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out, x read and copied out
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
+    case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
+        return 0;
     default:
         return 0;
     }

@@ -141,6 +141,71 @@ SETUP_SIZES = {  # name -> (per, count) where per is "cell", "face" or "c2f"
 }
 
 
+class ExternalMesh:
+    """A mesh handed over as caller-owned arrays (keys = dndsmi_mesh_view
+    fields), e.g. read from a file; exposes the same `view` as a built mesh."""
+
+    def __init__(self, arrays: dict):
+        self._keep = {}
+
+        def d(name, dtype):
+            a = np.ascontiguousarray(arrays[name], dtype=dtype).reshape(-1)
+            self._keep[name] = a
+            return a.ctypes.data_as(C.POINTER(C.c_double if dtype == np.float64 else
+                                              (C.c_int64 if dtype == np.int64 else C.c_int32)))
+
+        self.view = A.MeshView(int(arrays["n_node"]), int(arrays["n_cell"]), int(arrays["n_face"]),
+                               d("node_xy", np.float64), d("cell_ptr", np.int64), d("cell2node", np.int64),
+                               d("cell2face", np.int64), d("face2node", np.int64), d("face2cell", np.int64),
+                               d("face2cell_ref", np.int32), d("face_bc", np.int32), d("face_shift", np.float64))
+        self._arrays = dict(arrays)
+
+    @property
+    def n_cell(self):
+        return int(self.view.n_cell)
+
+    @property
+    def n_face(self):
+        return int(self.view.n_face)
+
+    def arrays(self) -> dict:
+        return self._arrays
+
+
+class ExternalVR:
+    """A VR setup handed over as caller-owned arrays (keys = dndsmi_vr_view
+    fields) with its setting; stands in for VRFiniteVolume2D on the device."""
+
+    def __init__(self, mesh: ExternalMesh, arrays: dict, setting: A.VRSetting):
+        self.mesh = mesh
+        self.setting = setting
+        self.device = None
+        self._keep = {}
+        fields = {}
+        for name, _ in A.VRView._fields_[3:]:
+            a = arrays.get(name)
+            if a is None or np.size(a) == 0:
+                fields[name] = A.PD()
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                self._keep[name] = a
+                fields[name] = a.ctypes.data_as(A.PD)
+        m = mesh.view
+        self.view = A.VRView(m.n_cell, m.n_face, int(mesh.arrays()["cell_ptr"][-1]), **fields)
+        self._arrays = dict(arrays)
+
+    def arrays(self, names=None) -> dict:
+        return {k: v for k, v in self._arrays.items() if names is None or k in names}
+
+    BuildRec = lambda self: ArrayRecV(self.device)  # noqa: E731
+
+    def ReconstructionJacobiStep(self, u, uRec, uRecNewBuf, t=0.0):
+        VRFiniteVolume2D.ReconstructionJacobiStep(self, u, uRec, uRecNewBuf, t)
+
+    def ReconstructionWBAPLimitFacialV3(self, u, uRec, uRecNewBuf, uRecNewBuf1=None, ifUseLimiter=None, ifAll=False):
+        VRFiniteVolume2D.ReconstructionWBAPLimitFacialV3(self, u, uRec, uRecNewBuf, uRecNewBuf1, ifUseLimiter, ifAll)
+
+
 class VRFiniteVolume2D:
     """Geometry + VR setup (host FP64) and the Jacobi sweep (device)."""
 

@@ -266,8 +266,9 @@ enum {
     DNDSMI_K_RHS_FACE = 4,   /* EvaluateRHS, inner-face fluxes                */
     DNDSMI_K_RHS_CELL = 5,   /* EvaluateRHS gather (+ SSP-RK3 stage update)   */
     DNDSMI_K_WBAP_FACE = 6,  /* WBAP smoothness-indicator face terms          */
-    DNDSMI_K_WBAP_CELL = 7,  /* WBAP indicator + characteristic limiting      */
-    DNDSMI_K_COUNT = 8
+    DNDSMI_K_WBAP_CELL = 7,  /* WBAP indicator, copy of unlimited cells       */
+    DNDSMI_K_WBAP_LIM = 8,   /* WBAP characteristic limiting of flagged cells */
+    DNDSMI_K_COUNT = 9
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the
