@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
                                                    const double *__restrict__ u, const double *__restrict__ x,
                                                    double *__restrict__ xn, double relax,
                                                    const double *__restrict__ corr, int64_t ldw, int32_t *errw) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc)
         return;
     double nw[36];
@@ -326,12 +326,12 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
             double tt[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int c = 0; c < 9; c++) {
-                const double m = Ms[(r * 9 + c) * ldc];
+                const double m = __builtin_nontemporal_load(Ms + (r * 9 + c) * ldc);
 #pragma unroll
                 for (int v = 0; v < 4; v++)
                     tt[v] += m * xj[c + 9 * v];
             }
-            const double b = bmul * bs[r * ldc];
+            const double b = bmul * __builtin_nontemporal_load(bs + r * ldc);
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_dt_face(int32_t nf, int64_t ldf, con
                                                     const double *__restrict__ fnc, const double *__restrict__ u,
                                                     double gamma, double *__restrict__ lf, double *__restrict__ dlf,
                                                     int32_t *errw) {
-    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf)
         return;
     const int32_t L = f2c[f], R = f2c[ldf + f];
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_dt_cell(int32_t nc, int64_t ldc, con
                                                     const double *__restrict__ vol, double cfl, double max_dt,
                                                     double *__restrict__ dt, double *__restrict__ part) {
     __shared__ double red[kBlock];
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     double d = 3e200;
     if (i < nc) {
         double lam = 0.0;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                                                      const double *__restrict__ dlf, const double *__restrict__ u,
                                                      const double *__restrict__ x, GasParams gp, double t,
                                                      double *__restrict__ fflux, int32_t *errw) {
-    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     if (tid >= n)
         return;
     const int32_t f = BOUNDARY ? list[tid] : tid;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
             xc[v] = x[(c + 9 * v) * ldc + L];
 #pragma unroll
         for (int g = 0; g < 3; g++) {
-            const double d = fd0[(g * 18 + c) * ldf + f];
+            const double d = __builtin_nontemporal_load(fd0 + (g * 18 + c) * ldf + f);
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 incL[g][v] += d * xc[v];
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                 xc[v] = x[(c + 9 * v) * ldc + R];
 #pragma unroll
             for (int g = 0; g < 3; g++) {
-                const double d = fd0[(g * 18 + 9 + c) * ldf + f];
+                const double d = __builtin_nontemporal_load(fd0 + (g * 18 + 9 + c) * ldf + f);
 #pragma unroll
                 for (int v = 0; v < 4; v++)
                     incR[g][v] += d * xc[v];
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
                                                      double *__restrict__ xlast, const double *__restrict__ dt,
                                                      const double *__restrict__ dtmin, double coefA, double coefB,
                                                      int32_t *errw) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc)
         return;
     const double V = vol[i];
@@ -633,12 +633,37 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
                                                       const double *__restrict__ fn, const double *__restrict__ u,
                                                       const double *__restrict__ x, double tang_weight, int twd_norm,
                                                       double *__restrict__ wsum) {
-    const int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf)
         return;
     const int32_t L = f2c[f], R = f2c[ldf + f];
     if (R < 0)
         return;
+    // One pass over the Gauss-point bases: side-0 basis times the L and R
+    // coefficients, side-1 basis times R and L, for rho (q = 0) and E (q = 1).
+    double b0L[10][2], b0R[10][2], b1L[10][2], b1R[10][2];
+#pragma unroll
+    for (int r = 0; r < 10; r++)
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            b0L[r][q] = b0R[r][q] = b1L[r][q] = b1R[r][q] = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < 9; c++) {
+        const double xL[2] = {x[c * ldc + L], x[(c + 27) * ldc + L]};
+        const double xR[2] = {x[c * ldc + R], x[(c + 27) * ldc + R]};
+#pragma unroll
+        for (int r = 0; r < 10; r++) {
+            const double d0 = __builtin_nontemporal_load(dg0 + (9 * r + c) * ldf + f);
+            const double d1 = __builtin_nontemporal_load(dg0 + (90 + 9 * r + c) * ldf + f);
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                b0L[r][q] += d0 * xL[q];
+                b0R[r][q] += d0 * xR[q];
+                b1R[r][q] += d1 * xR[q];
+                b1L[r][q] += d1 * xL[q];
+            }
+        }
+    }
     double W[10];
 #pragma unroll
     for (int k = 0; k < 10; k++)
@@ -646,34 +671,26 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
     const double gx = fn[f], gy = fn[ldf + f];
     const FFParams P = ff_params(W, gx, gy, tang_weight, twd_norm);
     const double nl = sqrt(gx * gx + gy * gy);
-#pragma unroll 1
+    const double uL[2] = {u[4 * int64_t(L)], u[4 * int64_t(L) + 3]};
+    const double uR[2] = {u[4 * int64_t(R)], u[4 * int64_t(R) + 3]};
+#pragma unroll
     for (int cs = 0; cs < 2; cs++) {
-        const int32_t me = cs == 0 ? L : R, other = cs == 0 ? R : L;
-        double vL[10][2], vR[10][2];
-#pragma unroll
-        for (int r = 0; r < 10; r++)
-            vL[r][0] = vL[r][1] = vR[r][0] = vR[r][1] = 0.0;
-#pragma unroll 1
-        for (int c = 0; c < 9; c++) {
-            const double xm0 = x[c * ldc + me], xm3 = x[(c + 27) * ldc + me];
-            const double xo0 = x[c * ldc + other], xo3 = x[(c + 27) * ldc + other];
-#pragma unroll
-            for (int r = 0; r < 10; r++) {
-                const double d0 = dg0[(9 * r + c) * ldf + f], d1 = dg0[(90 + 9 * r + c) * ldf + f];
-                vL[r][0] += d0 * xm0, vL[r][1] += d0 * xm3;
-                vR[r][0] += d1 * xo0, vR[r][1] += d1 * xo3;
-            }
-        }
-        vL[0][0] += u[4 * int64_t(me)], vL[0][1] += u[4 * int64_t(me) + 3];
-        vR[0][0] += u[4 * int64_t(other)], vR[0][1] += u[4 * int64_t(other) + 3];
+        // cell side cs: its own coefficients on the side-0 basis, the
+        // neighbour's on the side-1 basis
         double out[4];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             double val[10], jmp[10];
 #pragma unroll
             for (int r = 0; r < 10; r++) {
-                val[r] = (vL[r][q] + vR[r][q]) * 0.5;
-                jmp[r] = (vL[r][q] - vR[r][q]) * 0.5;
+                double vl = cs == 0 ? b0L[r][q] : b0R[r][q];
+                double vr = cs == 0 ? b1R[r][q] : b1L[r][q];
+                if (r == 0) {
+                    vl += cs == 0 ? uL[q] : uR[q];
+                    vr += cs == 0 ? uR[q] : uL[q];
+                }
+                val[r] = (vl + vr) * 0.5;
+                jmp[r] = (vl - vr) * 0.5;
             }
             const double isi = ff_self(P, val), iji = ff_self(P, jmp);
             double a = iji, b = isi;
@@ -701,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, in
                                                      double *__restrict__ xo, double *__restrict__ ind, double scale,
                                                      int if_all, int32_t *__restrict__ count,
                                                      int32_t *__restrict__ list) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     bool lim = false;
     if (i < nc) {
         double sum[4] = {0, 0, 0, 0};

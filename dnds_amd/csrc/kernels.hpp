@@ -32,6 +32,19 @@ __device__ inline bool is_far(int32_t bc) {
 
 __device__ inline double sqr(double x) { return x * x; }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs (workgroup w runs
+// on XCD w % 8), and each XCD has its own L2.  xcd_block() renumbers the
+// workgroups so that every XCD runs one contiguous range of blocks: cells and
+// faces that gather the same neighbours then meet in the same L2.  The map is
+// a bijection of [0, gridDim.x) for any grid size.
+constexpr int32_t kNumXCD = 8;
+__device__ __forceinline__ int32_t xcd_block() {
+    const int32_t n = int32_t(gridDim.x), w = int32_t(blockIdx.x);
+    const int32_t x = w % kNumXCD, q = w / kNumXCD;
+    const int32_t base = n / kNumXCD, extra = n % kNumXCD; // XCD x < extra runs base + 1 blocks
+    return x * base + (x < extra ? x : extra) + q;
+}
+
 struct GasParams {
     double gamma;
     double far[4];

@@ -197,3 +197,16 @@ def eigenvectors(U, gamma=1.4):
     if rc:
         raise RuntimeError(f"eigenvectors failed {rc}")
     return L, R
+
+
+def boundary_value(UL, unit_norm, point, t, bc, far_field=(1.0, 0.0, 0.0, 2.5), gamma=1.4):
+    """generateBoundaryValue for one Gauss point; raises on a non-OK status."""
+    cfg = OSetting(riemann_solver=1, gamma=gamma, far_field=(C.c_double * 4)(*far_field), jacobi_relax=1.0,
+                   tang_weight=1.0, tang_weight_direction=0, wbap_smooth_indicator_scale=1e100, wbap_n_std=10.0)
+    UL = np.array(UL, dtype=np.float64)
+    UR = np.empty(4)
+    rc = lib().oracle_boundary_value(C.byref(cfg), _d(UL), _d(np.ascontiguousarray(unit_norm, float)),
+                                     _d(np.ascontiguousarray(point, float)), t, bc, 0, _d(UR))
+    if rc:
+        raise RuntimeError(f"boundary value failed {rc}")
+    return UR

@@ -129,3 +129,111 @@ def test_vortex_solver_converges():
         ex = cases.cell_means(c.mesh, vol, lambda x, y: cases.vortex_exact(x, y, t))
         errs.append(np.sqrt(np.sum((u[:, 0] - ex[:, 0]) ** 2 * vol) / vol.sum()))
     assert errs[0] / errs[1] > 8, errs
+
+
+def _cons2prim(U, gamma=1.4):
+    rho, mu, mv, E = U
+    u, v = mu / rho, mv / rho
+    return np.array([rho, u, v, (gamma - 1) * (E - 0.5 * rho * (u * u + v * v))])
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_dt_of_a_uniform_state_is_the_closed_form(kind, periodic):
+    """Uniform state: lambda_f = |v . n_f| + a on every face, so
+    dt_i = CFL V_i / sum_f (|v . n_f| + a) |f|, with V_i (shoelace), |f| and
+    n_f taken from the node coordinates, not from the setup arrays."""
+    mesh, vfv, orc = build(kind, periodic=periodic, nx=7, ny=6)
+    m = mesh.arrays()
+    U = prim(1.3, 0.6, -0.4, 0.8)
+    u = np.tile(U, (mesh.n_cell, 1))
+    cfl = 0.7
+    dt, dtmin, lf, dlf = orc.evaluate_dt(u, cfl, use_local_dt=True)
+    a = np.sqrt(1.4 * 0.8 / 1.3)
+    xy, f2n = m["node_xy"], m["face2node"]
+    e = xy[f2n[:, 1]] - xy[f2n[:, 0]]
+    length = np.linalg.norm(e, axis=1)
+    nrm = np.stack([e[:, 1], -e[:, 0]], 1) / length[:, None]
+    lam = np.abs(nrm @ np.array([0.6, -0.4])) + a
+    assert rel_err(lf, lam) <= 1e-14
+    assert np.all(dlf == 0.0)
+    cp, c2n, c2f = m["cell_ptr"], m["cell2node"], m["cell2face"]
+    for i in range(mesh.n_cell):
+        X = xy[c2n[cp[i]:cp[i + 1]]]
+        vol = 0.5 * abs(np.sum(X[:, 0] * np.roll(X[:, 1], -1) - np.roll(X[:, 0], -1) * X[:, 1]))
+        fs = c2f[cp[i]:cp[i + 1]]
+        ref = cfl * vol / np.sum(lam[fs] * length[fs])
+        assert abs(dt[i] - ref) <= 1e-13 * ref
+    assert dtmin == dt.min()
+    dtg, dtmin_g, _, _ = orc.evaluate_dt(u, cfl, use_local_dt=False)
+    assert np.all(dtg == dtmin)
+    # MaxDt caps every cell
+    dtc, _, _, _ = orc.evaluate_dt(u, cfl, max_dt=0.5 * dtmin, use_local_dt=True)
+    assert np.all(dtc == 0.5 * dtmin)
+
+
+def test_farfield_boundary_follows_the_characteristics():
+    """Farfield (DNDS_FV_EulerEvaluator.hpp:977-1016): supersonic outflow takes
+    the interior state, supersonic inflow the far state, subsonic outflow the
+    interior primitives with the far pressure, subsonic inflow the far
+    primitives with the interior pressure."""
+    from dnds_amd import _capi as A
+    far = prim(1.0, 0.2, 0.1, 1.0)
+    n = np.array([0.6, 0.8])
+    cases_ = {
+        "sup_out": prim(1.2, 3.0 * 0.6, 3.0 * 0.8, 1.0),
+        "sup_in": prim(1.2, -3.0 * 0.6, -3.0 * 0.8, 1.0),
+        "sub_out": prim(1.2, 0.3 * 0.6, 0.3 * 0.8, 0.9),
+        "sub_in": prim(1.2, -0.3 * 0.6, -0.3 * 0.8, 0.9),
+    }
+    ub = {k: oracle.boundary_value(v, n, [0.3, 0.4], 0.0, A.BC_FARFIELD, far_field=far) for k, v in cases_.items()}
+    assert np.array_equal(ub["sup_out"], cases_["sup_out"])
+    assert np.array_equal(ub["sup_in"], far)
+    pl, pf = _cons2prim(cases_["sub_out"]), _cons2prim(far)
+    assert rel_err(_cons2prim(ub["sub_out"]), np.r_[pl[:3], pf[3]]) <= 1e-14
+    pl = _cons2prim(cases_["sub_in"])
+    assert rel_err(_cons2prim(ub["sub_in"]), np.r_[pf[:3], pl[3]]) <= 1e-14
+
+
+def test_wall_boundaries():
+    """Wall_Euler removes the normal momentum and keeps the rest
+    (DNDS_FV_EulerEvaluator.hpp:1247-1251); NoSlip reverses the momentum
+    (:1252-1264); Wall is not a flux boundary."""
+    from dnds_amd import _capi as A
+    UL = prim(1.1, 0.4, -0.7, 0.9)
+    n = np.array([-0.28, 0.96])
+    ue = oracle.boundary_value(UL, n, [0, 0], 0.0, A.BC_WALL_EULER)
+    assert abs(ue[1:3] @ n) < 1e-15
+    t = np.array([n[1], -n[0]])
+    assert abs(ue[1:3] @ t - UL[1:3] @ t) < 1e-15
+    assert ue[0] == UL[0] and ue[3] == UL[3]
+    un = oracle.boundary_value(UL, n, [0, 0], 0.0, A.BC_WALL_NOSLIP)
+    assert np.array_equal(un, UL * np.array([1, -1, -1, 1]))
+    with pytest.raises(RuntimeError):
+        oracle.boundary_value(UL, n, [0, 0], 0.0, A.BC_WALL)
+
+
+def test_riemann_and_vortex_far_states():
+    """2DRiemannFar (DNDS_FV_EulerEvaluator.hpp:1126-1243): the left and
+    bottom edges carry the quadrant states of the four-quadrant problem; the
+    right and top edges mirror the interior.  IVFar (:1100-1125) is the exact
+    isentropic vortex: freestream (1, 1, 1, 1) far from the centre and the
+    closed-form temperature dip at the centre."""
+    from dnds_amd import _capi as A
+    UL = prim(0.9, 0.3, 0.2, 0.7)
+    n = np.array([-1.0, 0.0])
+    q3 = _cons2prim(oracle.boundary_value(UL, n, [0.0, 0.1], 0.0, A.BC_SPECIAL_2DRIEMANNFAR))
+    assert rel_err(q3, [0.137992831541219, 1.206045378311055, 1.206045378311055, 0.029032258064516]) <= 1e-14
+    q2 = _cons2prim(oracle.boundary_value(UL, n, [0.0, 0.95], 0.0, A.BC_SPECIAL_2DRIEMANNFAR))
+    assert rel_err(q2, [0.532258064516129, 1.206045378311055, 0.0, 0.3]) <= 1e-14
+    q4 = _cons2prim(oracle.boundary_value(UL, [0, -1.0], [0.95, 0.0], 0.0, A.BC_SPECIAL_2DRIEMANNFAR))
+    assert rel_err(q4, [0.532258064516129, 0.0, 1.206045378311055, 0.3]) <= 1e-14
+    pr = _cons2prim(oracle.boundary_value(UL, [1.0, 0], [1.0, 0.5], 0.0, A.BC_SPECIAL_2DRIEMANNFAR))
+    assert rel_err(pr, [0.9, -0.3, 0.2, 0.7]) <= 1e-14
+    far = oracle.boundary_value(UL, n, [60.0, 70.0], 0.0, A.BC_SPECIAL_IVFAR)
+    assert rel_err(far, prim(1.0, 1.0, 1.0, 1.0)) <= 1e-14
+    g, chi = 1.4, 5.0
+    T = 1 - (g - 1) / (8 * g * np.pi ** 2) * chi ** 2 * np.e
+    rho = T ** (1 / (g - 1))
+    ctr = oracle.boundary_value(UL, n, [5.5, 5.5], 0.5, A.BC_SPECIAL_IVFAR)
+    assert rel_err(_cons2prim(ctr), [rho, 1.0, 1.0, rho * T]) <= 1e-13
