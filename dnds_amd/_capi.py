@@ -130,6 +130,15 @@ SIGNATURES = [
                                               C.c_void_p, C.c_int32]),
     ("dndsmi_ssprk3_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                      C.c_double, C.c_double, C.c_int32, C.c_double, PD]),
+    ("dndsmi_lusgs_colouring", C.c_int, [C.c_void_p, PI32, PI32]),
+    ("dndsmi_lusgs_matrix_init", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double]),
+    ("dndsmi_update_lusgs_forward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p]),
+    ("dndsmi_update_lusgs_backward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]),
+    ("dndsmi_evaluate_residual", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, PD]),
+    ("dndsmi_implicit_euler_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
+                                             C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, PI32, PD]),
     ("dndsmi_profile_enable", C.c_int, [C.c_void_p, C.c_int32]),
     ("dndsmi_profile_query", C.c_int, [C.c_void_p, C.c_int32, PI64, PD]),
     ("dndsmi_kernel_bytes", C.c_int64, [C.c_void_p, C.c_int32]),
@@ -145,7 +154,7 @@ K_WBAP_FACE = 6
 K_WBAP_CELL = 7
 K_WBAP_LIM = 8
 KERNEL_NAMES = ["k_jacobi", "k_wall_corr", "k_dt_face", "k_dt_cell", "k_rhs_face", "k_rhs_cell", "k_wbap_face",
-                "k_wbap_cell", "k_wbap_lim"]
+                "k_wbap_cell", "k_wbap_lim", "k_lusgs_sweep"]
 
 _lib = None
 

@@ -143,3 +143,36 @@ def riemann_2d(n=1024, seed=2, jitter=0.2, nthreads=0, build_limiter_data=True) 
     bary = vfv.arrays(["cell_bary"])["cell_bary"].reshape(-1, 2)
     u0 = riemann_state(bary)
     return Case("riemann_2d", mesh, vfv, "HLLC", RIEMANN_FAR, u0, 200, 1, 0.5, True)
+
+
+def bump_far(mach=0.5, gamma=GAMMA):
+    """Free stream rho = 1, p = 1/gamma (sound speed 1), u = mach, v = 0."""
+    p = 1.0 / gamma
+    return (1.0, mach, 0.0, p / (gamma - 1) + 0.5 * mach * mach)
+
+
+def bump_channel(nx=48, ny=16, kind=A.MESH_TRI_SPLIT, height=0.1, mach=0.5, seed=4, nthreads=0) -> Case:
+    """Steady subsonic channel flow over a sine bump: [0,3] x [0,1] with the
+    lower wall raised by height * sin^2(pi (x - 1)) on 1 <= x <= 2 (nodes
+    moved as y' = b(x) + y (1 - b(x))), Wall_Euler top and bottom, Farfield
+    inlet and outlet at Mach `mach`.  Roe flux, no limiter; the initial state
+    is the free stream.  VR weights as the reference's implicit 2D Euler
+    configuration (data/euler2D_config.json: farWeight 1, wallWeight 0.5),
+    1 Jacobi sweep per iteration, CFL 10; the implicit LU-SGS solver drives
+    it to steady state."""
+    bcs = (A.BC_FARFIELD, A.BC_FARFIELD, A.BC_WALL_EULER, A.BC_WALL_EULER)
+    base = CompactFacedMeshSerialRW.generate(kind, nx, ny, 0.0, 3.0, 0.0, 1.0, jitter=0.0, seed=seed, bc=bcs)
+    m = base.arrays()
+    xy = m["node_xy"].copy()
+    x = xy[:, 0]
+    b = np.where((x >= 1.0) & (x <= 2.0), height * np.sin(np.pi * (x - 1.0)) ** 2, 0.0)
+    xy[:, 1] = b + xy[:, 1] * (1.0 - b)
+    bnd = m["face2cell"][:, 1] == A.NO_CELL
+    mesh = CompactFacedMeshSerialRW.from_cells(xy, m["cell_ptr"], m["cell2node"], m["face2node"][bnd],
+                                               m["face_bc"][bnd])
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**dict(RIEMANN_VR_SETTING, far_weight=1.0, wall_weight=0.5),
+                                            build_limiter_data=0))
+    vfv.Initialization(nthreads)
+    far = bump_far(mach)
+    u0 = np.tile(np.array(far), (mesh.n_cell, 1))
+    return Case("bump_channel", mesh, vfv, "Roe", far, u0, 0, 1, 10.0, False)

@@ -14,124 +14,12 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
-#include "kernels.hpp"
+#include "ctx.hpp"
 
 using namespace dndsmi_dev;
+using namespace dndsmi_host;
 
-namespace {
-
-constexpr int kBlock = 256;
-
-inline int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
-inline unsigned grid_for(int64_t n, int block = kBlock) { return unsigned((n + block - 1) / block); }
-
-} // namespace
-
-struct dndsmi_ctx {
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int device = 0;
-    std::string err;
-    int64_t nc = 0, nf = 0, ldc = 0, ldf = 0;
-    int nfmax = 3;
-    GasParams gas{};
-    double relax = 1.0;
-    int64_t bytes = 0;
-    std::vector<void *> allocs;
-
-    // cell side
-    int32_t *nbr = nullptr;    // [nfmax][ldc] neighbour cell, -1 = boundary slot, -2 = absent slot
-    int32_t *sbc = nullptr;    // [nfmax][ldc] boundary kind of the slot's face
-    int32_t *sface = nullptr;  // [nfmax][ldc] face index of the slot
-    int32_t *gat = nullptr;    // [nfmax][ldc] (face << 1 | side) sorted by face, -1 = none
-    double *M = nullptr;       // [nfmax*81][ldc]
-    double *bv = nullptr;      // [nfmax*9][ldc]
-    double *vol = nullptr;     // [ldc]
-    int32_t *wall_idx = nullptr; // [nfmax][ldc] index into the wall-face arrays, -1 = none
-    int64_t n_wall = 0;
-    double *wall_ainv = nullptr; // [81][n_wall]
-    double *wall_d0 = nullptr;   // [27][n_wall]
-    double *wall_nrm = nullptr;  // [6][n_wall]
-    double *wall_w0 = nullptr;   // [n_wall]
-    int32_t *wall_cell = nullptr; // [n_wall]
-    int32_t *wall_bc = nullptr;   // [n_wall]
-    double *wall_corr = nullptr;  // [36][n_wall]
-    int64_t n_wall_faces = 0;     // real count (n_wall >= 1 for allocation)
-    // face side
-    int32_t *f2c = nullptr;    // [2][ldf]
-    int32_t *fbc = nullptr;    // [ldf]
-    double *fd0 = nullptr;     // [54][ldf]
-    double *fn = nullptr;      // [6][ldf] normals at the LINE_3 points
-    double *fnc = nullptr;     // [2][ldf] normal at the face centre
-    double *farea = nullptr;   // [ldf]
-    double *fctr = nullptr;    // [2][ldf]
-    int32_t *bfaces = nullptr; // boundary face indices
-    int64_t n_bface = 0;
-    // WBAP data (present when the setup carries limiter data)
-    bool has_limiter = false;
-    double *dg0 = nullptr;     // [180][ldf] all DiBj rows at LINE_3 point 0, both sides
-    double *fw = nullptr;      // [10][ldf] face weights
-    double *msec = nullptr;    // [162][ldf] matrixSecondary (R2L, L2R)
-    double *wsum = nullptr;    // [8][ldf] per-face indicator terms of both cells
-    double *uF = nullptr;      // [nfmax*28][ldc] WBAP per-cell face scratch
-    double *ind = nullptr;     // [ldc] indicator when the caller passes none
-    double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps
-    int32_t *lim_list = nullptr;  // [ldc] cells to limit
-    int32_t *lim_count = nullptr; // [1]
-    double tang_weight = 1.0;
-    int twd_norm = 1;
-    double wbap_scale = 1e-10, wbap_nstd = 10.0;
-    // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
-    double *lambda_face = nullptr;
-    double *dlambda_face = nullptr;
-    double *fflux = nullptr;   // [12][ldf] F, FLfix, FRfix integrated over the face
-    double *dt_part = nullptr; // [grid] partial minima
-    double *dt_min = nullptr;  // [1]
-    int32_t *errw = nullptr;   // [1] error bits
-    double *xlast = nullptr;   // [ldc][4] SSP-RK3 xLast
-    double *dt_tmp = nullptr;  // [ldc]
-    // per-kernel HIP-event timing (dndsmi_profile_*)
-    bool prof_on = false;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-    std::vector<std::pair<int, size_t>> prof_recs; // (kernel id, index of the start event)
-    int prof_open = -1;
-    size_t prof_open_ev = 0;
-
-    template <class T>
-    int alloc(T *&p, int64_t n) {
-        size_t b = sizeof(T) * size_t(std::max<int64_t>(n, 1));
-        void *q = nullptr;
-        hipError_t e = hipMalloc(&q, b);
-        if (e != hipSuccess) {
-            err = std::string("hipMalloc: ") + hipGetErrorString(e);
-            return DNDSMI_ERR_OUT_OF_MEMORY;
-        }
-        allocs.push_back(q);
-        bytes += int64_t(b);
-        p = static_cast<T *>(q);
-        return DNDSMI_OK;
-    }
-    int hip(hipError_t e, const char *what) {
-        if (e != hipSuccess) {
-            err = std::string(what) + ": " + hipGetErrorString(e);
-            return DNDSMI_ERR_HIP;
-        }
-        return DNDSMI_OK;
-    }
-    template <class T>
-    int upload(T *dst, const std::vector<T> &src) {
-        int rc = hip(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, stream),
-                     "hipMemcpyAsync");
-        if (rc)
-            return rc;
-        return hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    }
-    int launch_check(const char *what) { return hip(hipGetLastError(), what); }
-};
-
-namespace {
+namespace dndsmi_host {
 
 hipEvent_t prof_event(dndsmi_ctx *c) {
     if (c->ev_used == c->ev_pool.size()) {
@@ -165,21 +53,7 @@ void prof_end(dndsmi_ctx *c) {
     c->prof_open = -1;
 }
 
-} // namespace
-
-struct dndsmi_array {
-    dndsmi_ctx *ctx = nullptr;
-    int32_t kind = 0;
-    double *d = nullptr;
-    int64_t elems = 0; // doubles allocated
-};
-
-#define DNDSMI_TRY(expr)                                                                                               \
-    do {                                                                                                               \
-        int rc_ = (expr);                                                                                              \
-        if (rc_ != DNDSMI_OK)                                                                                          \
-            return rc_;                                                                                                \
-    } while (0)
+} // namespace dndsmi_host
 
 // ========================================================================
 // kernels
@@ -917,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_zero_int(int32_t *p) {
 // host side
 // ========================================================================
 
-namespace {
+namespace dndsmi_host {
 
 int check_errw(dndsmi_ctx *c) {
     int32_t h = 0;
@@ -940,7 +814,7 @@ int check_errw(dndsmi_ctx *c) {
 
 bool same_ctx(const dndsmi_ctx *c, const dndsmi_array *a, int32_t kind) { return a && a->ctx == c && a->kind == kind; }
 
-} // namespace
+} // namespace dndsmi_host
 
 extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_view *vr,
                                  const dndsmi_vr_setting *vs, const dndsmi_euler_setting *es, void *stream,
@@ -1141,6 +1015,42 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             DNDSMI_TRY(c->alloc(c->lim_list, ldc));
             DNDSMI_TRY(c->alloc(c->lim_count, 1));
         }
+        // ---- LU-SGS colouring: each cell takes the smallest colour that no
+        // lower-numbered neighbour holds
+        {
+            std::vector<int32_t> col(size_t(nc), -1);
+            int32_t ncol = 0;
+            for (int64_t i = 0; i < nc; i++) {
+                uint32_t used = 0;
+                for (int s = 0; s < nfmax; s++) {
+                    const int32_t j = nbr[s * ldc + i];
+                    if (j >= 0 && j < i && col[j] < 32)
+                        used |= 1u << col[j];
+                }
+                int32_t k = 0;
+                while (used & (1u << k))
+                    k++;
+                col[i] = k;
+                ncol = std::max(ncol, k + 1);
+            }
+            c->n_colour = ncol;
+            c->col_start.assign(size_t(ncol) + 1, 0);
+            for (int64_t i = 0; i < nc; i++)
+                c->col_start[col[i] + 1]++;
+            for (int32_t k = 0; k < ncol; k++)
+                c->col_start[k + 1] += c->col_start[k];
+            std::vector<int32_t> cells(static_cast<size_t>(nc)), fill(c->col_start.begin(), c->col_start.end() - 1);
+            for (int64_t i = 0; i < nc; i++)
+                cells[fill[col[i]]++] = int32_t(i);
+            std::vector<int32_t> colp(size_t(ldc), 0);
+            std::copy(col.begin(), col.end(), colp.begin());
+            c->colour_host = col;
+            DNDSMI_TRY(c->alloc(c->colour, ldc));
+            DNDSMI_TRY(c->upload(c->colour, colp));
+            DNDSMI_TRY(c->alloc(c->col_cells, nc));
+            DNDSMI_TRY(c->upload(c->col_cells, cells));
+            DNDSMI_TRY(c->alloc(c->jinv, ldc));
+        }
         // ---- scratch
         DNDSMI_TRY(c->alloc(c->lambda_face, ldf));
         DNDSMI_TRY(c->alloc(c->dlambda_face, ldf));
@@ -1280,7 +1190,7 @@ extern "C" int dndsmi_array_set_constant(dndsmi_array *a, double value) {
     return a->ctx->launch_check("k_set");
 }
 
-namespace {
+namespace dndsmi_host {
 
 int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t) {
     const int nc = int(c->nc);
@@ -1380,7 +1290,7 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
     return c->launch_check("wbap kernels");
 }
 
-} // namespace
+} // namespace dndsmi_host
 
 extern "C" int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec,
                                            dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1,

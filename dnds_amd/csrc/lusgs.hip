@@ -1,0 +1,382 @@
+// Multicolour LU-SGS and the implicit-Euler dual-time driver (gfx950).
+//
+// The reference's LU-SGS (DNDS_FV_EulerEvaluator.cpp:995-1357) is a pair of
+// sequential sweeps over a scan order: forward uses the neighbours scanned
+// earlier, backward those scanned later.  Here the scan order is
+// colour-major: the cells are coloured so that no two neighbours share a
+// colour, and a colour's cells are scanned together.  Within one colour no
+// cell reads another of the same colour, so one launch per colour computes
+// exactly the sequential sweep in that order: the same operations in the
+// same order per cell as the oracle run with the same scan order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace dndsmi_dev;
+using namespace dndsmi_host;
+
+namespace {
+
+// fluxJacobian0_Right_Times_du (DNDS_FV_EulerEvaluator.hpp:928-956) with
+// IdealGasUIncrement and GasInviscidFluxFacialIncrement (DNDS_Gas.hpp:122-146):
+// dF = d(F(U).n)/dU dU - lambda dU
+__device__ inline void flux_jacobian_times_du(double gamma, const double U[4], double n0, double n1,
+                                              const double dU[4], double lambdaMain, double dF[4]) {
+    const double velo0 = U[1] / U[0], velo1 = U[2] / U[0];
+    double p, asqr, H;
+    ideal_gas_thermal(U[3], U[0], velo0 * velo0 + velo1 * velo1, gamma, p, asqr, H);
+    const double dVelo0 = (dU[1] - U[1] / U[0] * dU[0]) / U[0];
+    const double dVelo1 = (dU[2] - U[2] / U[0] * dU[0]) / U[0];
+    const double dp = (gamma - 1) * (dU[3] - 0.5 * ((dU[1] * velo0 + dU[2] * velo1) + (U[1] * dVelo0 + U[2] * dVelo1)));
+    const double vn = velo0 * n0 + velo1 * n1;
+    const double dvn = dVelo0 * n0 + dVelo1 * n1;
+    dF[0] = dU[1] * n0 + dU[2] * n1;
+    dF[1] = dU[1] * vn + U[1] * dvn + n0 * dp;
+    dF[2] = dU[2] * vn + U[2] * dvn + n1 * dp;
+    dF[3] = (dU[3] + dp) * vn + (U[3] + p) * dvn;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        dF[k] -= lambdaMain * dU[k];
+}
+
+// LUSGSMatrixInit (DNDS_FV_EulerEvaluator.cpp:995-1047), jacobianCode 0, zero
+// source Jacobian: jinv_i = 1 / (1/dTau_i + 1/dt + sum_f (alpha/2) S_f lambda_f / V_i)
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_lusgs_init(int32_t nc, int64_t ldc, const int32_t *__restrict__ sface,
+                                                       const double *__restrict__ farea,
+                                                       const double *__restrict__ lambda_face,
+                                                       const double *__restrict__ vol,
+                                                       const double *__restrict__ dtau, double dt, double alpha,
+                                                       double *__restrict__ jinv) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    double fp = 1.0 / dtau[i] + 1.0 / dt;
+    const double v = vol[i];
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+        const int32_t f = sface[s * ldc + i];
+        if (f < 0)
+            continue;
+        fp += (0.5 * alpha) * farea[f] * lambda_face[f] / v;
+    }
+    jinv[i] = 1.0 / (fp + alpha * 0.0);
+}
+
+// One colour of an LU-SGS sweep (UpdateLUSGSForward / UpdateLUSGSBackward,
+// DNDS_FV_EulerEvaluator.cpp:1149-1357): forward
+//   uIncNew_i = jinv_i (rhs_i - sum_{earlier j} c_f dF(u_j, uInc_j))
+// backward
+//   uIncNew_i += jinv_i (- sum_{later j} c_f dF(u_j, uInc_j)),
+// c_f = (alpha/2) S_f / V_i, the normal pointing out of i, faces in cell2face
+// order.  "Earlier" is a lower colour, "later" a higher one.
+template <int NF, bool FORWARD>
+__global__ __launch_bounds__(kBlock) void k_lusgs_sweep(int32_t n, const int32_t *__restrict__ list, int32_t col,
+                                                        int64_t ldc, int64_t ldf, const int32_t *__restrict__ nbr,
+                                                        const int32_t *__restrict__ sface,
+                                                        const int32_t *__restrict__ f2c,
+                                                        const int32_t *__restrict__ colour,
+                                                        const double *__restrict__ fnc,
+                                                        const double *__restrict__ farea,
+                                                        const double *__restrict__ vol,
+                                                        const double *__restrict__ lambda_face,
+                                                        const double *__restrict__ jinv, double gamma, double alpha,
+                                                        const double *__restrict__ rhs, const double *__restrict__ u,
+                                                        const double *uInc, double *uIncNew, int32_t *errw) {
+    const int32_t t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    const int32_t i = list[t];
+    double buf[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        buf[v] = FORWARD ? rhs[4 * int64_t(i) + v] : 0.0;
+    const double vi = vol[i];
+#pragma unroll 1
+    for (int s = 0; s < NF; s++) {
+        const int32_t j = nbr[s * ldc + i];
+        if (j < 0)
+            continue;
+        const int32_t cj = colour[j];
+        if (FORWARD ? !(cj < col) : !(cj > col))
+            continue;
+        const int32_t f = sface[s * ldc + i];
+        double n0, n1;
+        normalized(fnc[f], fnc[ldf + f], n0, n1);
+        if (f2c[f] != i) {
+            n0 *= -1.0;
+            n1 *= -1.0;
+        }
+        double uj[4], dj[4], dF[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            uj[v] = u[4 * int64_t(j) + v], dj[v] = uInc[4 * int64_t(j) + v];
+        flux_jacobian_times_du(gamma, uj, n0, n1, dj, lambda_face[f], dF);
+        const double cf = (0.5 * alpha) * farea[f] / vi;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            buf[v] -= cf * dF[v];
+    }
+    const double d = jinv[i];
+    bool finite = true;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        double o = d * buf[v];
+        if (!FORWARD)
+            o = uIncNew[4 * int64_t(i) + v] + o;
+        finite &= isfinite(o);
+        uIncNew[4 * int64_t(i) + v] = o;
+    }
+    if (!finite)
+        atomicOr(errw, ERRBIT_NAN);
+}
+
+// Dual-time right-hand side of ImplicitEulerDualTimeStep::Step
+// (DNDS_ODE.hpp:253-258) and fsolve's compression (DNDS_FV_EulerSolver.hpp:870-873):
+//   rhs = ((xLast - x) * (1/dt) + rhs); rhs = (rhs * dTau) / dTau; xinc = 0
+__global__ __launch_bounds__(kBlock) void k_dual_rhs(int32_t nc, const double *__restrict__ xlast,
+                                                     const double *__restrict__ x, double rdt,
+                                                     const double *__restrict__ dtau, double *__restrict__ rhs,
+                                                     double *__restrict__ xinc) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    const double dti = dtau[i];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int64_t k = 4 * int64_t(i) + v;
+        double r = xlast[k];
+        r -= x[k];
+        r *= rdt;
+        r = r + rhs[k];
+        rhs[k] = (r * dti) / dti;
+        xinc[k] = 0.0;
+    }
+}
+
+// EvaluateResidual partial sums (DNDS_FV_EulerEvaluator.cpp:1557-1592) of
+// one block, optionally after x += xinc: P = 1 sums |r|, P = 2 sums r^2,
+// P >= 3 takes max |r|; volWise multiplies by the volume (P < 3).
+__global__ __launch_bounds__(kBlock) void k_residual(int32_t nc, const double *__restrict__ r,
+                                                     double *__restrict__ x, const double *__restrict__ vol,
+                                                     int32_t P, int32_t vol_wise, double *__restrict__ part) {
+    __shared__ double red[4][kBlock];
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    if (i < nc) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double rv = r[4 * int64_t(i) + v];
+            if (x)
+                x[4 * int64_t(i) + v] += rv;
+            const double a = fabs(rv);
+            double q = P >= 3 ? a : (P == 2 ? a * a : a);
+            if (P < 3 && vol_wise)
+                q *= vol[i];
+            acc[v] = q;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        red[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                red[v][threadIdx.x] = P >= 3 ? fmax(red[v][threadIdx.x], red[v][threadIdx.x + st])
+                                             : red[v][threadIdx.x] + red[v][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4)
+        part[threadIdx.x * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+
+// one wavefront per variable folds the block partials in a fixed order
+__global__ __launch_bounds__(kBlock) void k_residual_final(int32_t nblk, int32_t P, const double *__restrict__ part,
+                                                           double *__restrict__ res) {
+    const int v = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (v >= 4)
+        return;
+    double a = 0.0;
+    for (int32_t b = lane; b < nblk; b += 64)
+        a = P >= 3 ? fmax(a, part[v * nblk + b]) : a + part[v * nblk + b];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(a, off, 64);
+        a = P >= 3 ? fmax(a, o) : a + o;
+    }
+    if (lane == 0)
+        res[v] = a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_scalar(int32_t n, const double *__restrict__ src,
+                                                        double *__restrict__ dst) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[0];
+}
+
+int ensure_implicit_scratch(dndsmi_ctx *c) {
+    if (c->imp_rhs)
+        return DNDSMI_OK;
+    DNDSMI_TRY(c->alloc(c->imp_rhs, 4 * c->ldc));
+    DNDSMI_TRY(c->alloc(c->imp_xinc, 4 * c->ldc));
+    DNDSMI_TRY(c->alloc(c->imp_dtau, c->ldc));
+    DNDSMI_TRY(c->alloc(c->res_part, 4 * int64_t(grid_for(c->nc))));
+    DNDSMI_TRY(c->alloc(c->res, 4));
+    return DNDSMI_OK;
+}
+
+int launch_lusgs_init(dndsmi_ctx *c, const double *dtau, double dt, double alpha) {
+    const int nc = int(c->nc);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_lusgs_init<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->sface,
+                           c->farea, c->lambda_face, c->vol, dtau, dt, alpha, c->jinv);
+    else
+        hipLaunchKernelGGL(k_lusgs_init<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->sface,
+                           c->farea, c->lambda_face, c->vol, dtau, dt, alpha, c->jinv);
+    return c->launch_check("k_lusgs_init");
+}
+
+template <bool FORWARD>
+int launch_lusgs_sweep(dndsmi_ctx *c, double alpha, const double *rhs, const double *u, const double *uInc,
+                       double *uIncNew) {
+    for (int32_t kk = 0; kk < c->n_colour; kk++) {
+        const int32_t col = FORWARD ? kk : c->n_colour - 1 - kk;
+        const int64_t b = c->col_start[col], n = c->col_start[col + 1] - b;
+        if (n == 0)
+            continue;
+        prof_begin(c, DNDSMI_K_LUSGS_SWEEP);
+        if (c->nfmax == 3)
+            hipLaunchKernelGGL((k_lusgs_sweep<3, FORWARD>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream,
+                               int32_t(n), c->col_cells + b, col, c->ldc, c->ldf, c->nbr, c->sface, c->f2c, c->colour,
+                               c->fnc, c->farea, c->vol, c->lambda_face, c->jinv, c->gas.gamma, alpha, rhs, u, uInc,
+                               uIncNew, c->errw);
+        else
+            hipLaunchKernelGGL((k_lusgs_sweep<4, FORWARD>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream,
+                               int32_t(n), c->col_cells + b, col, c->ldc, c->ldf, c->nbr, c->sface, c->f2c, c->colour,
+                               c->fnc, c->farea, c->vol, c->lambda_face, c->jinv, c->gas.gamma, alpha, rhs, u, uInc,
+                               uIncNew, c->errw);
+        prof_end(c);
+        DNDSMI_TRY(c->launch_check("k_lusgs_sweep"));
+    }
+    return DNDSMI_OK;
+}
+
+// residual into c->res (device) and then into host res[4]; x += r first when x
+int residual(dndsmi_ctx *c, const double *r, double *x, int32_t P, int32_t vol_wise, double *res_host) {
+    const unsigned g = grid_for(c->nc);
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(kBlock), 0, c->stream, int32_t(c->nc), r, x, c->vol, P, vol_wise,
+                       c->res_part);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, c->stream, int32_t(g), P, c->res_part, c->res);
+    DNDSMI_TRY(c->launch_check("residual kernels"));
+    double h[4];
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(h, c->res, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync"));
+    DNDSMI_TRY(check_errw(c));
+    for (int v = 0; v < 4; v++)
+        res_host[v] = P >= 3 ? h[v] : (P == 2 ? std::sqrt(h[v]) : h[v]);
+    return DNDSMI_OK;
+}
+
+} // namespace
+
+extern "C" int dndsmi_lusgs_colouring(const dndsmi_ctx *c, int32_t *colour, int32_t *n_colours) {
+    if (!c || !n_colours)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    *n_colours = c->n_colour;
+    if (colour)
+        std::copy(c->colour_host.begin(), c->colour_host.end(), colour);
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_lusgs_matrix_init(dndsmi_ctx *c, const dndsmi_array *dTau, double dt, double alphaDiag) {
+    if (!c || !same_ctx(c, dTau, DNDSMI_ARRAY_SCALAR))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    return launch_lusgs_init(c, dTau->d, dt, alphaDiag);
+}
+
+extern "C" int dndsmi_update_lusgs_forward(dndsmi_ctx *c, double alphaDiag, const dndsmi_array *rhs,
+                                           const dndsmi_array *u, const dndsmi_array *uInc, dndsmi_array *uIncNew) {
+    if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !same_ctx(c, u, DNDSMI_ARRAY_DOF) ||
+        !same_ctx(c, uInc, DNDSMI_ARRAY_DOF) || !same_ctx(c, uIncNew, DNDSMI_ARRAY_DOF) || uIncNew == u ||
+        uIncNew == rhs)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    return launch_lusgs_sweep<true>(c, alphaDiag, rhs->d, u->d, uInc->d, uIncNew->d);
+}
+
+extern "C" int dndsmi_update_lusgs_backward(dndsmi_ctx *c, double alphaDiag, const dndsmi_array *rhs,
+                                            const dndsmi_array *u, const dndsmi_array *uInc, dndsmi_array *uIncNew) {
+    if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !same_ctx(c, u, DNDSMI_ARRAY_DOF) ||
+        !same_ctx(c, uInc, DNDSMI_ARRAY_DOF) || !same_ctx(c, uIncNew, DNDSMI_ARRAY_DOF) || uIncNew == u ||
+        uIncNew == rhs)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    return launch_lusgs_sweep<false>(c, alphaDiag, rhs->d, u->d, uInc->d, uIncNew->d);
+}
+
+extern "C" int dndsmi_evaluate_residual(dndsmi_ctx *c, const dndsmi_array *rhs, int32_t P, int32_t volWise,
+                                        double *res) {
+    if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !res || P < 1)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(ensure_implicit_scratch(c));
+    return residual(c, rhs->d, nullptr, P, volWise, res);
+}
+
+extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                                          int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt,
+                                          int32_t max_iter, double threshold, double t, int32_t *n_iter,
+                                          double *res_hist) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNew, DNDSMI_ARRAY_REC) || uRec == uRecNew || n_sweeps < 0 || max_iter < 1 || !(dt > 0))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(ensure_implicit_scratch(c));
+    const int nc = int(c->nc);
+    const double alpha = 1.0;
+    // xLast = x
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(c->xlast, u->d, sizeof(double) * 4 * c->nc, hipMemcpyDeviceToDevice, c->stream),
+                      "copy"));
+    double base[4] = {0, 0, 0, 0};
+    if (n_iter)
+        *n_iter = 0;
+    for (int32_t iter = 1; iter <= max_iter; iter++) {
+        // fdtau: EvaluateDt(dTau, u, CFL, dtMin, 1e100, UseLocaldt), dTau /= alpha (alpha = 1)
+        DNDSMI_TRY(launch_dt(c, u->d, CFL, 1e100, c->imp_dtau));
+        if (!UseLocaldt) {
+            hipLaunchKernelGGL(k_fill_scalar, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->dt_min,
+                               c->imp_dtau);
+            DNDSMI_TRY(c->launch_check("k_fill_scalar"));
+        }
+        // frhs at t + dt: sweeps then EvaluateRHS
+        for (int k = 0; k < n_sweeps; k++) {
+            DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, t + dt));
+            std::swap(uRec->d, uRecNew->d);
+        }
+        DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, t + dt, c->imp_rhs, 0, nullptr, nullptr, 0, 0));
+        hipLaunchKernelGGL(k_dual_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->xlast, u->d, 1.0 / dt,
+                           c->imp_dtau, c->imp_rhs, c->imp_xinc);
+        DNDSMI_TRY(c->launch_check("k_dual_rhs"));
+        // fsolve: LU-SGS forward + backward, in place
+        DNDSMI_TRY(launch_lusgs_init(c, c->imp_dtau, dt, alpha));
+        DNDSMI_TRY(launch_lusgs_sweep<true>(c, alpha, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+        DNDSMI_TRY(launch_lusgs_sweep<false>(c, alpha, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+        // x += xinc, fstop on the L1 norm of xinc
+        double res[4];
+        DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
+        if (res_hist)
+            for (int v = 0; v < 4; v++)
+                res_hist[4 * (iter - 1) + v] = res[v];
+        for (int v = 0; v < 4; v++)
+            base[v] = iter == 1 ? res[v] : std::max(base[v], res[v]);
+        if (n_iter)
+            *n_iter = iter;
+        if (!std::isfinite(res[0])) {
+            c->err = "non-finite increment in the implicit iteration";
+            return DNDSMI_ERR_NAN;
+        }
+        if (res[0] / base[0] < threshold)
+            break;
+    }
+    return DNDSMI_OK;
+}

@@ -377,6 +377,41 @@ class EulerEvaluator:
     def EvaluateRHS(self, rhs: ArrayDOFV, u: ArrayDOFV, uRec: ArrayRecV, t: float):
         _check(_lib().dndsmi_evaluate_rhs(self.device.ctx, rhs.h, u.h, uRec.h, t), "EvaluateRHS", self.device.ctx)
 
+    def LUSGSColouring(self):
+        """(colour of each cell, number of colours) of the multicolour sweeps."""
+        n = C.c_int32()
+        _check(_lib().dndsmi_lusgs_colouring(self.device.ctx, None, C.byref(n)), "LU-SGS colouring", self.device.ctx)
+        col = np.empty(self.device.n_cell, dtype=np.int32)
+        _check(_lib().dndsmi_lusgs_colouring(self.device.ctx, A.i32ptr(col), C.byref(n)), "LU-SGS colouring",
+               self.device.ctx)
+        return col, n.value
+
+    def LUSGSScanOrder(self) -> np.ndarray:
+        """The colour-major scan order of the device sweeps (iScan -> iCell)."""
+        col, _ = self.LUSGSColouring()
+        return np.argsort(col, kind="stable").astype(np.int64)
+
+    def LUSGSMatrixInit(self, dTau: ArrayScalar, dt: float, alphaDiag: float):
+        """Diagonal from lambdaFace of the last EvaluateDt (jacobianCode 0)."""
+        _check(_lib().dndsmi_lusgs_matrix_init(self.device.ctx, dTau.h, dt, alphaDiag), "LUSGSMatrixInit",
+               self.device.ctx)
+
+    def UpdateLUSGSForward(self, alphaDiag: float, rhs: ArrayDOFV, u: ArrayDOFV, uInc: ArrayDOFV,
+                           uIncNew: ArrayDOFV):
+        _check(_lib().dndsmi_update_lusgs_forward(self.device.ctx, alphaDiag, rhs.h, u.h, uInc.h, uIncNew.h),
+               "UpdateLUSGSForward", self.device.ctx)
+
+    def UpdateLUSGSBackward(self, alphaDiag: float, rhs: ArrayDOFV, u: ArrayDOFV, uInc: ArrayDOFV,
+                            uIncNew: ArrayDOFV):
+        _check(_lib().dndsmi_update_lusgs_backward(self.device.ctx, alphaDiag, rhs.h, u.h, uInc.h, uIncNew.h),
+               "UpdateLUSGSBackward", self.device.ctx)
+
+    def EvaluateResidual(self, rhs: ArrayDOFV, P: int = 1, volWise: bool = False) -> np.ndarray:
+        res = np.empty(4)
+        _check(_lib().dndsmi_evaluate_residual(self.device.ctx, rhs.h, P, int(volWise), A.dptr(res)),
+               "EvaluateResidual", self.device.ctx)
+        return res
+
 
 class ExplicitSSPRK3LocalDt:
     """ODE::ExplicitSSPRK3LocalDt with frhs = nInternalRecStep Jacobi sweeps +
@@ -400,3 +435,30 @@ class ExplicitSSPRK3LocalDt:
                                          self.CFL, self.MaxDt, int(self.UseLocaldt), t, C.byref(dt)),
                "SSP-RK3 step", d.ctx)
         return dt.value
+
+
+class ImplicitEulerDualTimeStep:
+    """ODE::ImplicitEulerDualTimeStep (DNDS_ODE.hpp:201-265) with the solver's
+    frhs (nInternalRecStep sweeps + EvaluateRHS), fdtau (EvaluateDt) and
+    fsolve (LU-SGS forward + backward, jacobianTypeCode 0, gmresCode 0), and
+    fstop on the density increment (rhsThresholdInternal), all on the device."""
+
+    def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, nTimeStepInternal: int,
+                 rhsThresholdInternal: float, UseLocaldt: bool = True):
+        self.ev = evaluator
+        self.n_sweeps = nInternalRecStep
+        self.CFL = CFL
+        self.max_iter = nTimeStepInternal
+        self.threshold = rhsThresholdInternal
+        self.UseLocaldt = UseLocaldt
+
+    def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, dt: float, t: float = 0.0):
+        """One implicit-Euler step of size dt; returns the L1 increment norms
+        of the inner iterations ([iterations][4])."""
+        d = self.ev.device
+        hist = np.zeros((self.max_iter, 4))
+        n = C.c_int32()
+        _check(_lib().dndsmi_implicit_euler_step(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, self.CFL, dt,
+                                                 int(self.UseLocaldt), self.max_iter, self.threshold, t,
+                                                 C.byref(n), A.dptr(hist)), "implicit Euler step", d.ctx)
+        return hist[:n.value]

@@ -254,6 +254,51 @@ int dndsmi_ssprk3_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dnd
                        double *dt_out);
 
 /* ======================================================================== */
+/* Implicit path: LU-SGS and the implicit-Euler dual-time step              */
+/* ======================================================================== */
+
+/* The device sweeps LU-SGS in colour-major order: cells are greedily
+ * coloured in index order (no two neighbours share a colour) and scanned
+ * colour by colour, cells ascending within a colour.  This is one of the
+ * reference's sequential scan orders (SOR_iScan2iCell, DNDS_FV_VR.hpp:1376-1463),
+ * so a sweep equals the reference's sweep in that order.  colour may be NULL;
+ * otherwise it receives n_cell colours. */
+int dndsmi_lusgs_colouring(const dndsmi_ctx *ctx, int32_t *colour, int32_t *n_colours);
+
+/* EulerEvaluator::LUSGSMatrixInit(dTau, dt, alphaDiag, u, uRec, 0, t)
+ * (DNDS_FV_EulerEvaluator.cpp:995-1047; declared DNDS_FV_EulerEvaluator.hpp:1429-1433),
+ * jacobianCode 0, Euler model: the diagonal from lambdaFace of the last
+ * dndsmi_evaluate_dt on this context. */
+int dndsmi_lusgs_matrix_init(dndsmi_ctx *ctx, const dndsmi_array *dTau, double dt, double alphaDiag);
+
+/* EulerEvaluator::UpdateLUSGSForward / UpdateLUSGSBackward(alphaDiag, rhs, u,
+ * uInc, uIncNew) (DNDS_FV_EulerEvaluator.cpp:1149-1357; declared
+ * DNDS_FV_EulerEvaluator.hpp:1440-1454).  uInc and uIncNew may be the same
+ * array (the solver's in-place call); distinct arrays read the old uInc. */
+int dndsmi_update_lusgs_forward(dndsmi_ctx *ctx, double alphaDiag, const dndsmi_array *rhs, const dndsmi_array *u,
+                                const dndsmi_array *uInc, dndsmi_array *uIncNew);
+int dndsmi_update_lusgs_backward(dndsmi_ctx *ctx, double alphaDiag, const dndsmi_array *rhs, const dndsmi_array *u,
+                                 const dndsmi_array *uInc, dndsmi_array *uIncNew);
+
+/* EulerEvaluator::EvaluateResidual(res, rhs, P, volWise)
+ * (DNDS_FV_EulerEvaluator.cpp:1557-1592): per variable, (sum |r|^P)^(1/P)
+ * for P = 1, 2, max |r| for P >= 3; res receives 4 doubles (synchronises). */
+int dndsmi_evaluate_residual(dndsmi_ctx *ctx, const dndsmi_array *rhs, int32_t P, int32_t volWise, double *res);
+
+/* ODE::ImplicitEulerDualTimeStep::Step (DNDS_ODE.hpp:201-265) with the
+ * solver's lambdas (DNDS_FV_EulerSolver.hpp:713-1000) for jacobianTypeCode 0,
+ * gmresCode 0, nSGSIterationInternal 0, no limiter: per iteration
+ * EvaluateDt (pseudo time step dTau), n_sweeps Jacobi sweeps, EvaluateRHS at
+ * t + dt, the dual-time right-hand side, LUSGSMatrixInit, forward and
+ * backward LU-SGS in place, u += uInc; stops after max_iter iterations or
+ * when the L1 density increment falls below threshold times its running
+ * maximum.  *n_iter receives the iterations done; res_hist (may be NULL)
+ * the L1 increment norms, 4 per iteration. */
+int dndsmi_implicit_euler_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                               int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt, int32_t max_iter,
+                               double threshold, double t, int32_t *n_iter, double *res_hist);
+
+/* ======================================================================== */
 /* Measurement                                                              */
 /* ======================================================================== */
 
@@ -268,7 +313,8 @@ enum {
     DNDSMI_K_WBAP_FACE = 6,  /* WBAP smoothness-indicator face terms          */
     DNDSMI_K_WBAP_CELL = 7,  /* WBAP indicator, copy of unlimited cells       */
     DNDSMI_K_WBAP_LIM = 8,   /* WBAP characteristic limiting of flagged cells */
-    DNDSMI_K_COUNT = 9
+    DNDSMI_K_LUSGS_SWEEP = 9,/* one colour of an LU-SGS forward/backward sweep */
+    DNDSMI_K_COUNT = 10
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the

@@ -1054,3 +1054,184 @@ extern "C" int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, 
     }
     return ERR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// LU-SGS (DNDS_FV_EulerEvaluator.cpp:995-1357) with jacobianTypeCode 0 and the
+// Euler model (the source Jacobian is zero), and the implicit-Euler dual-time
+// driver around it (ImplicitEulerDualTimeStep, DNDS_ODE.hpp:201-265, with the
+// solver's frhs / fdtau / fsolve / fstop, DNDS_FV_EulerSolver.hpp:713-1000).
+// ---------------------------------------------------------------------------
+namespace {
+
+// fluxJacobian0_Right_Times_du, DNDS_FV_EulerEvaluator.hpp:928-956, with
+// IdealGasUIncrement and GasInviscidFluxFacialIncrement (DNDS_Gas.hpp:122-146)
+void flux_jacobian_times_du(double gamma, const double U[4], const double n[2], const double dU[4],
+                            double lambdaMain, double dF[4]) {
+    const double velo[2] = {U[1] / U[0], U[2] / U[0]};
+    double p, asqr, H;
+    ideal_gas_thermal(U[3], U[0], velo[0] * velo[0] + velo[1] * velo[1], gamma, p, asqr, H);
+    double dVelo[2];
+    for (int d = 0; d < 2; d++)
+        dVelo[d] = (dU[1 + d] - U[1 + d] / U[0] * dU[0]) / U[0];
+    const double dp = (gamma - 1) * (dU[3] - 0.5 * ((dU[1] * velo[0] + dU[2] * velo[1]) +
+                                                    (U[1] * dVelo[0] + U[2] * dVelo[1])));
+    const double vn = velo[0] * n[0] + velo[1] * n[1];
+    const double dvn = dVelo[0] * n[0] + dVelo[1] * n[1];
+    dF[0] = dU[1] * n[0] + dU[2] * n[1];
+    for (int d = 0; d < 2; d++)
+        dF[1 + d] = dU[1 + d] * vn + U[1 + d] * dvn + n[d] * dp;
+    dF[3] = (dU[3] + dp) * vn + (U[3] + p) * dvn;
+    for (int k = 0; k < 4; k++)
+        dF[k] -= lambdaMain * dU[k];
+}
+
+// one LU-SGS half sweep: forward visits the scan order ascending and uses the
+// neighbours scanned earlier, backward descending with those scanned later
+int lusgs_sweep(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const int64_t *scan2cell,
+                const double *jinv, const double *lambda_face, double alphaDiag, const double *rhs, const double *u,
+                const double *uInc, double *uIncNew, bool forward) {
+    const int64_t nc = m->n_cell;
+    std::vector<int64_t> cell2scan(nc);
+    for (int64_t k = 0; k < nc; k++)
+        cell2scan[scan2cell ? scan2cell[k] : k] = k;
+    for (int64_t kk = 0; kk < nc; kk++) {
+        const int64_t iScan = forward ? kk : nc - 1 - kk;
+        const int64_t i = scan2cell ? scan2cell[iScan] : iScan;
+        double buf[4];
+        for (int v = 0; v < 4; v++)
+            buf[v] = forward ? rhs[4 * i + v] : 0.0;
+        for (int64_t q = m->cell_ptr[i]; q < m->cell_ptr[i + 1]; q++) {
+            const int64_t f = m->cell2face[q];
+            const int side = m->face2cell[2 * f] == i ? 0 : 1;
+            const int64_t j = m->face2cell[2 * f + 1 - side];
+            if (j == kNoCell)
+                continue;
+            const int64_t jScan = cell2scan[j];
+            if (forward ? !(jScan < iScan) : !(jScan > iScan))
+                continue;
+            double n[2];
+            normalized(s->face_norm_center[2 * f], s->face_norm_center[2 * f + 1], n[0], n[1]);
+            const double sg = side ? -1.0 : 1.0;
+            n[0] *= sg, n[1] *= sg;
+            double dF[4];
+            flux_jacobian_times_du(cfg->gamma, u + 4 * j, n, uInc + 4 * j, lambda_face[f], dF);
+            const double coef = (0.5 * alphaDiag) * s->face_area[f] / s->volume[i];
+            for (int v = 0; v < 4; v++)
+                buf[v] -= coef * dF[v];
+        }
+        for (int v = 0; v < 4; v++) {
+            if (forward)
+                uIncNew[4 * i + v] = jinv[i] * buf[v];
+            else
+                uIncNew[4 * i + v] += jinv[i] * buf[v];
+        }
+        for (int v = 0; v < 4; v++)
+            if (!std::isfinite(uIncNew[4 * i + v]))
+                return ERR_NAN;
+    }
+    return ERR_OK;
+}
+
+} // namespace
+
+// LUSGSMatrixInit (DNDS_FV_EulerEvaluator.cpp:995-1047): the diagonal
+// 1/dTau + 1/dt + sum_f (alpha/2) S_f lambda_f / V, inverted
+extern "C" int oracle_lusgs_matrix_init(const oracle_mesh *m, const oracle_setup *s, const double *dTau, double dt,
+                                        double alphaDiag, const double *lambda_face, double *jinv) {
+    for (int64_t i = 0; i < m->n_cell; i++) {
+        double fp = 1.0 / dTau[i] + 1.0 / dt;
+        for (int64_t q = m->cell_ptr[i]; q < m->cell_ptr[i + 1]; q++) {
+            const int64_t f = m->cell2face[q];
+            fp += (0.5 * alphaDiag) * s->face_area[f] * lambda_face[f] / s->volume[i];
+        }
+        jinv[i] = 1.0 / (fp + alphaDiag * 0.0);
+    }
+    return ERR_OK;
+}
+
+// UpdateLUSGSForward (DNDS_FV_EulerEvaluator.cpp:1149-1276)
+extern "C" int oracle_lusgs_forward(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                    const int64_t *scan2cell, const double *jinv, const double *lambda_face,
+                                    double alphaDiag, const double *rhs, const double *u, const double *uInc,
+                                    double *uIncNew) {
+    return lusgs_sweep(m, s, cfg, scan2cell, jinv, lambda_face, alphaDiag, rhs, u, uInc, uIncNew, true);
+}
+
+// UpdateLUSGSBackward (DNDS_FV_EulerEvaluator.cpp:1285-1357)
+extern "C" int oracle_lusgs_backward(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                     const int64_t *scan2cell, const double *jinv, const double *lambda_face,
+                                     double alphaDiag, const double *rhs, const double *u, const double *uInc,
+                                     double *uIncNew) {
+    return lusgs_sweep(m, s, cfg, scan2cell, jinv, lambda_face, alphaDiag, rhs, u, uInc, uIncNew, false);
+}
+
+// Implicit-Euler dual-time iterations (DNDS_ODE.hpp:248-263) with
+// frhs = n_sweeps Jacobi sweeps + EvaluateRHS, fdtau = EvaluateDt / alpha,
+// fsolve = LU-SGS forward + backward in place (nSGSIterationInternal = 0,
+// CompressInc the identity for the Euler model) and fstop on the L1 norm of
+// the density increment relative to its running maximum
+// (DNDS_FV_EulerSolver.hpp:984-999).  res_hist receives the L1 increment
+// norms (4 per iteration); returns the iterations done in *n_done.
+extern "C" int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                     const int64_t *scan2cell, int32_t max_iter, int32_t n_sweeps, double cfl,
+                                     double dt, int32_t use_local_dt, double threshold, double t, double *u,
+                                     double *urec, double *res_hist, int32_t *n_done) {
+    const int64_t nc = m->n_cell;
+    std::vector<double> dTau(nc), rhs(4 * nc), xLast(u, u + 4 * nc), xinc(4 * nc), urecNew(36 * nc);
+    std::vector<double> lf(m->n_face), dlf(m->n_face), jinv(nc);
+    const double alphaDiag = 1.0;
+    double resBase[4] = {0, 0, 0, 0};
+    *n_done = 0;
+    for (int32_t iter = 1; iter <= max_iter; iter++) {
+        double dtMin;
+        int rc = oracle_evaluate_dt(m, s, cfg, u, cfl, 1e100, use_local_dt, dTau.data(), &dtMin, lf.data(),
+                                    dlf.data());
+        if (rc)
+            return rc;
+        for (int64_t i = 0; i < nc; i++)
+            dTau[i] /= alphaDiag;
+        for (int k = 0; k < n_sweeps; k++)
+            if ((rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), t + dt)))
+                return rc;
+        if ((rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, t + dt, rhs.data())))
+            return rc;
+        for (int64_t k = 0; k < 4 * nc; k++) {
+            double r = xLast[k];
+            r -= u[k];
+            r *= 1.0 / dt;
+            rhs[k] = r + rhs[k];
+        }
+        std::fill(xinc.begin(), xinc.end(), 0.0);
+        oracle_lusgs_matrix_init(m, s, dTau.data(), dt, alphaDiag, lf.data(), jinv.data());
+        for (int64_t i = 0; i < nc; i++)
+            for (int v = 0; v < 4; v++)
+                rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
+        if ((rc = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
+                                       xinc.data(), xinc.data())))
+            return rc;
+        if ((rc = oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
+                                        xinc.data(), xinc.data())))
+            return rc;
+        for (int64_t k = 0; k < 4 * nc; k++)
+            u[k] += xinc[k];
+        double res[4] = {0, 0, 0, 0};
+        for (int64_t i = 0; i < nc; i++)
+            for (int v = 0; v < 4; v++)
+                res[v] += std::fabs(xinc[4 * i + v]);
+        for (int v = 0; v < 4; v++) {
+            if (res_hist)
+                res_hist[4 * (iter - 1) + v] = res[v];
+            resBase[v] = iter == 1 ? res[v] : std::max(resBase[v], res[v]);
+        }
+        *n_done = iter;
+        if (res[0] / resBase[0] < threshold)
+            break;
+    }
+    return ERR_OK;
+}
+
+extern "C" int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU,
+                                             double lambda_main, double gamma, double *dF) {
+    flux_jacobian_times_du(gamma, U, unit_norm, dU, lambda_main, dF);
+    return ERR_OK;
+}

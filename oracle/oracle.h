@@ -97,8 +97,35 @@ int oracle_eigenvectors(const double *U, double gamma, double *L, double *R);
 int oracle_boundary_value(const oracle_setting *cfg, double *UL, const double *unit_norm, const double *p, double t,
                           int32_t bc, int32_t fix_ul, double *UR);
 
+/* LU-SGS, jacobianTypeCode 0, Euler model (DNDS_FV_EulerEvaluator.cpp:995-1357).
+ * scan2cell (may be NULL: identity, the reference's default order) is the
+ * order of the sequential sweeps.  jinv[i] = 1 / D_i from
+ * oracle_lusgs_matrix_init; lambda_face from oracle_evaluate_dt.  uInc and
+ * uIncNew may alias (the solver passes the same array). */
+int oracle_lusgs_matrix_init(const oracle_mesh *m, const oracle_setup *s, const double *dTau, double dt,
+                             double alphaDiag, const double *lambda_face, double *jinv);
+int oracle_lusgs_forward(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                         const int64_t *scan2cell, const double *jinv, const double *lambda_face, double alphaDiag,
+                         const double *rhs, const double *u, const double *uInc, double *uIncNew);
+int oracle_lusgs_backward(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                          const int64_t *scan2cell, const double *jinv, const double *lambda_face, double alphaDiag,
+                          const double *rhs, const double *u, const double *uInc, double *uIncNew);
+
+/* Implicit-Euler dual-time iterations with LU-SGS (DNDS_ODE.hpp:201-265;
+ * DNDS_FV_EulerSolver.hpp:713-1000): up to max_iter iterations, stopping when
+ * the L1 density increment falls below threshold x its running maximum. */
+int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                          const int64_t *scan2cell, int32_t max_iter, int32_t n_sweeps, double cfl, double dt,
+                          int32_t use_local_dt, double threshold, double t, double *u, double *urec,
+                          double *res_hist, int32_t *n_done);
+
+/* fluxJacobian0_Right_Times_du (DNDS_FV_EulerEvaluator.hpp:928-956), Euler */
+int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU, double lambda_main,
+                                  double gamma, double *dF);
+
 #ifdef __cplusplus
 }
 #endif
 
 #endif
+

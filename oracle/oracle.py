@@ -67,6 +67,15 @@ def lib() -> C.CDLL:
             "oracle_hllc_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_eigenvectors": [PD, C.c_double, PD, PD],
             "oracle_boundary_value": [C.POINTER(OSetting), PD, PD, PD, C.c_double, C.c_int32, C.c_int32, PD],
+            "oracle_flux_jacobian_times_du": [PD, PD, PD, C.c_double, C.c_double, PD],
+            "oracle_lusgs_matrix_init": [C.POINTER(OMesh), C.POINTER(OSetup), PD, C.c_double, C.c_double, PD, PD],
+            "oracle_lusgs_forward": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, PD, PD,
+                                     C.c_double, PD, PD, PD, PD],
+            "oracle_lusgs_backward": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, PD, PD,
+                                      C.c_double, PD, PD, PD, PD],
+            "oracle_implicit_euler": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32,
+                                      C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD, PD,
+                                      PD, C.POINTER(C.c_int32)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -161,6 +170,42 @@ class Oracle:
                                             _d(out), _d(ind), int(if_all)), "wbap_v3")
         return out, ind
 
+    def lusgs_matrix_init(self, dtau, dt, alpha, lambda_face):
+        jinv = np.empty(self.n_cell)
+        self._check(self.lib.oracle_lusgs_matrix_init(C.byref(self.m), C.byref(self.s), _d(np.ascontiguousarray(dtau,
+                    float)), dt, alpha, _d(np.ascontiguousarray(lambda_face, float)), _d(jinv)), "lusgs init")
+        return jinv
+
+    def lusgs_sweep(self, forward, jinv, lambda_face, alpha, rhs, u, uinc, scan2cell=None, in_place=True):
+        """One LU-SGS half sweep; returns uIncNew (a copy of uinc updated in
+        place when in_place, else a separate array as in the reference's
+        call with distinct buffers)."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        src = np.array(uinc, dtype=np.float64, order="C", copy=True)
+        dst = src if in_place else np.array(uinc, dtype=np.float64, order="C", copy=True)
+        order = None if scan2cell is None else np.ascontiguousarray(scan2cell, dtype=np.int64)
+        fn = self.lib.oracle_lusgs_forward if forward else self.lib.oracle_lusgs_backward
+        self._check(fn(C.byref(self.m), C.byref(self.s), C.byref(self.cfg),
+                       order.ctypes.data_as(PI64) if order is not None else None, _d(np.ascontiguousarray(jinv, float)),
+                       _d(np.ascontiguousarray(lambda_face, float)), alpha, _d(rhs), _d(u), _d(src), _d(dst)),
+                    "lusgs sweep")
+        return dst
+
+    def implicit_euler(self, u, urec, max_iter, n_sweeps, cfl, dt=1e100, use_local_dt=True, threshold=1e-10,
+                       t=0.0, scan2cell=None):
+        """Returns (u, urec, residual history [iters][4])."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        urec = np.array(urec, dtype=np.float64, order="C", copy=True)
+        hist = np.zeros((max_iter, 4))
+        done = C.c_int32()
+        order = None if scan2cell is None else np.ascontiguousarray(scan2cell, dtype=np.int64)
+        self._check(self.lib.oracle_implicit_euler(C.byref(self.m), C.byref(self.s), C.byref(self.cfg),
+                                                   order.ctypes.data_as(PI64) if order is not None else None,
+                                                   max_iter, n_sweeps, cfl, dt, int(use_local_dt), threshold, t,
+                                                   _d(u), _d(urec), _d(hist), C.byref(done)), "implicit euler")
+        return u, urec, hist[:done.value]
+
     def ssprk3_steps(self, u, urec, n_steps, n_sweeps, cfl, t=0.0, max_dt=1e100, use_local_dt=False,
                      use_limiter=False):
         u = np.array(u, dtype=np.float64, order="C", copy=True)
@@ -210,3 +255,10 @@ def boundary_value(UL, unit_norm, point, t, bc, far_field=(1.0, 0.0, 0.0, 2.5), 
     if rc:
         raise RuntimeError(f"boundary value failed {rc}")
     return UR
+
+
+def flux_jacobian_times_du(U, unit_norm, dU, lambda_main=0.0, gamma=1.4):
+    dF = np.empty(4)
+    lib().oracle_flux_jacobian_times_du(_d(np.ascontiguousarray(U, float)), _d(np.ascontiguousarray(unit_norm, float)),
+                                        _d(np.ascontiguousarray(dU, float)), lambda_main, gamma, _d(dF))
+    return dF
