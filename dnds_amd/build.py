@@ -49,7 +49,20 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread"], verbose)
     os.replace(tmp, LIB)
+    build_facade_run(verbose)
     return LIB
+
+
+FACADE_SRC = os.path.join(HERE, "..", "tests", "cpp", "facade_run.cpp")
+FACADE_BIN = os.path.join(HERE, "..", "tests", "cpp", "facade_run")
+
+
+def build_facade_run(verbose: bool = False) -> str:
+    """The C++ host-class example (tests/cpp/facade_run.cpp), linked to the
+    in-tree library with an rpath to it."""
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"), FACADE_SRC,
+          "-o", FACADE_BIN, "-L" + HERE, "-ldndsmi", "-Wl,-rpath,$ORIGIN/../../dnds_amd"], verbose)
+    return FACADE_BIN
 
 
 if __name__ == "__main__":
