@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed steps, N device copies of one rec array (a plain copy of known size "
                          "for scaling profiler byte counters)")
+    ap.add_argument("--mixed", type=int, default=1,
+                    help="also time the mixed-precision step (FP32 correction sweeps + FP64 refinement) "
+                         "from the same initial state and report it beside the FP64 line")
     ap.add_argument("--case", default="riemann", choices=["riemann", "vortex"],
                     help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale")
     args = ap.parse_args()
@@ -70,46 +73,64 @@ def main():
     log(f"setup done: {nc} cells, {case.mesh.n_face} faces, {time.time() - t0:.1f}s")
     ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType=case.riemann, farFieldStaticValue=case.far)
     log(f"device context: {ev.device.device_bytes() / 2**30:.2f} GiB")
-    u = ev.BuildMean().upload(case.u0)
     urec0 = np.zeros((nc, 36))
-    r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
-    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl, useLimiter=case.limiter)
-    t = 0.0
-    for _ in range(args.warmup):
-        t += ode.Step(u, r, rn, t)
-    ev.device.sync()
-    log(f"warmup {args.warmup} steps done")
     lib = A.load()
-    lib.dndsmi_profile_enable(ev.device.ctx, 1)
-    ev.device.sync()
-    ts = time.perf_counter()
-    for _ in range(args.steps):
-        t += ode.Step(u, r, rn, t)
-    ev.device.sync()
-    te = time.perf_counter()
-    elapsed = te - ts
-    log(f"timed {args.steps} steps: {elapsed:.3f}s, t = {t:.5f}")
-    prof = {}
     import ctypes as C
-    for k, name in enumerate(A.KERNEL_NAMES):
-        cnt, tot = C.c_int64(), C.c_double()
-        lib.dndsmi_profile_query(ev.device.ctx, k, C.byref(cnt), C.byref(tot))
-        if cnt.value:
-            b = int(lib.dndsmi_kernel_bytes(ev.device.ctx, k))
-            avg_ms = tot.value / cnt.value
-            prof[name] = dict(launches=cnt.value, avg_ms=avg_ms, bytes=b, gbs=b / (avg_ms * 1e-3) / 1e9)
-    lib.dndsmi_profile_enable(ev.device.ctx, 0)
-    for name, p in prof.items():
-        log(f"{name}: {p['launches']} launches, {p['avg_ms']:.4f} ms avg, {p['gbs']:.0f} GB/s algorithmic")
+
+    def timed_run(mixed: bool):
+        """warmup + timed SSP-RK3 steps from the initial state; per-kernel
+        HIP-event times over the timed steps"""
+        u = ev.BuildMean().upload(case.u0)
+        r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+        ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl, useLimiter=case.limiter,
+                                    mixedPrecision=mixed)
+        t = 0.0
+        for _ in range(args.warmup):
+            t += ode.Step(u, r, rn, t)
+        ev.device.sync()
+        tag = "mixed" if mixed else "fp64"
+        log(f"[{tag}] warmup {args.warmup} steps done")
+        lib.dndsmi_profile_enable(ev.device.ctx, 1)
+        ev.device.sync()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            t += ode.Step(u, r, rn, t)
+        ev.device.sync()
+        elapsed = time.perf_counter() - ts
+        log(f"[{tag}] timed {args.steps} steps: {elapsed:.3f}s, t = {t:.5f}")
+        prof = {}
+        for k, name in enumerate(A.KERNEL_NAMES):
+            cnt, tot = C.c_int64(), C.c_double()
+            lib.dndsmi_profile_query(ev.device.ctx, k, C.byref(cnt), C.byref(tot))
+            if cnt.value:
+                b = int(lib.dndsmi_kernel_bytes(ev.device.ctx, k))
+                avg_ms = tot.value / cnt.value
+                prof[name] = dict(launches=cnt.value, avg_ms=avg_ms, bytes=b, gbs=b / (avg_ms * 1e-3) / 1e9)
+        lib.dndsmi_profile_enable(ev.device.ctx, 0)
+        for name, p in prof.items():
+            log(f"[{tag}] {name}: {p['launches']} launches, {p['avg_ms']:.4f} ms avg, {p['gbs']:.0f} GB/s algorithmic")
+        uf = u.download()
+        if not np.isfinite(uf).all():
+            raise RuntimeError("non-finite state after the timed steps")
+        return elapsed, prof, uf, r
+
+    elapsed, prof, u_fp64, r = timed_run(False)
     if args.copy_probe:
+        rn = case.vfv.BuildRec()
         nbytes = 36 * ((nc + 63) // 64 * 64) * 8
         log(f"copy probe: {args.copy_probe} device copies of {nbytes} bytes")
         for _ in range(args.copy_probe):
             rn.copy_from(r)
         ev.device.sync()
-    uf = u.download()
-    if not np.isfinite(uf).all():
-        raise RuntimeError("non-finite state after the timed steps")
+    mixed = None
+    if args.mixed:
+        m_elapsed, m_prof, u_mixed, _ = timed_run(True)
+        diff = float(np.abs(u_mixed - u_fp64).max() / np.abs(u_fp64).max())
+        mixed = {"value": nc * 3 * args.steps / m_elapsed, "ms_per_step": m_elapsed / args.steps * 1e3,
+                 "speedup_vs_fp64": elapsed / m_elapsed, "rel_diff_u_vs_fp64": diff,
+                 "dtype": "f32 sweeps/face reconstruction + f64 refinement", "kernels": m_prof}
+        log(f"mixed: {mixed['value']:.3e} cell-updates/s, x{mixed['speedup_vs_fp64']:.2f} vs fp64, "
+            f"rel. diff of u {diff:.1e}")
 
     stages = 3 * args.steps
     value = nc * stages / elapsed
@@ -161,6 +182,7 @@ def main():
         "kernels": prof,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out), flush=True)
 

@@ -50,6 +50,7 @@ ARRAY_SCALAR = 2
 PD = C.POINTER(C.c_double)
 PI64 = C.POINTER(C.c_int64)
 PI32 = C.POINTER(C.c_int32)
+PF = C.POINTER(C.c_float)
 
 
 class MeshView(C.Structure):
@@ -130,7 +131,26 @@ SIGNATURES = [
                                               C.c_void_p, C.c_int32]),
     ("dndsmi_ssprk3_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                      C.c_double, C.c_double, C.c_int32, C.c_double, PD]),
+    ("dndsmi_ssprk3_step_mixed", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                           C.c_double, C.c_double, C.c_int32, C.c_double, PD]),
     ("dndsmi_lusgs_colouring", C.c_int, [C.c_void_p, PI32, PI32]),
+    ("dndsmi_warmstart_create", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64,
+                                          C.POINTER(C.c_void_p)]),
+    ("dndsmi_warmstart_destroy", None, [C.c_void_p]),
+    ("dndsmi_warmstart_num_params", C.c_int64, []),
+    ("dndsmi_warmstart_get_params", C.c_int, [C.c_void_p, PF]),
+    ("dndsmi_warmstart_set_params", C.c_int, [C.c_void_p, PF]),
+    ("dndsmi_warmstart_features", C.c_int, [C.c_void_p, C.c_void_p, PF, PF]),
+    ("dndsmi_warmstart_add_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("dndsmi_warmstart_num_samples", C.c_int64, [C.c_void_p]),
+    ("dndsmi_warmstart_add_rows", C.c_int, [C.c_void_p, PF, PF, C.c_int64]),
+    ("dndsmi_warmstart_targets", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, PF]),
+    ("dndsmi_warmstart_loss_grad", C.c_int, [C.c_void_p, PF, PF, C.c_int32, PF, PD]),
+    ("dndsmi_warmstart_adam_step", C.c_int, [C.c_void_p, PF, C.c_double]),
+    ("dndsmi_warmstart_train", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, PD]),
+    ("dndsmi_warmstart_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("dndsmi_reconstruct_to_tolerance", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                                  C.c_int32, C.c_double, PI32, PD]),
     ("dndsmi_lusgs_matrix_init", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double]),
     ("dndsmi_update_lusgs_forward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p]),
@@ -154,7 +174,7 @@ K_WBAP_FACE = 6
 K_WBAP_CELL = 7
 K_WBAP_LIM = 8
 KERNEL_NAMES = ["k_jacobi", "k_wall_corr", "k_dt_face", "k_dt_cell", "k_rhs_face", "k_rhs_cell", "k_wbap_face",
-                "k_wbap_cell", "k_wbap_lim", "k_lusgs_sweep"]
+                "k_wbap_cell", "k_wbap_lim", "k_lusgs_sweep", "k_delta32", "k_rhs_face32", "k_rhs_cell32"]
 
 _lib = None
 
@@ -192,6 +212,11 @@ def dptr(a: np.ndarray):
 def i64ptr(a: np.ndarray):
     assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(PI64)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PF)
 
 
 def i32ptr(a: np.ndarray):

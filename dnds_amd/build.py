@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdndsmi.so")
 
 HOST_SOURCES = ["mesh.cpp", "vr_setup.cpp"]
-HIP_SOURCES = ["ctx.hip", "lusgs.hip"]
+HIP_SOURCES = ["ctx.hip", "lusgs.hip", "warmstart.hip"]
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + os.path.join(HERE, "..", "include")]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "--offload-arch=gfx950",

@@ -134,14 +134,20 @@ __global__ __launch_bounds__(kBlock) void k_wall_corr(int32_t nw, int64_t ldc, c
 // in cell2face order, with the reference's boundary variants: far field
 // couples the cell to itself (M_if x_i), a Wall face adds b_if (0 - u_i)^T,
 // Wall_Euler / Wall_NoSlip faces add the correction of k_wall_corr.
-template <int NF>
+//
+// RESID (the FP64 step of the mixed-precision sweeps): instead of the new
+// iterate, writes the increment r = J(x) - x rounded to FP32 into r32 and d32
+// (the start of the FP32 correction sweeps, k_delta32).
+template <int NF, bool RESID = false>
 __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, const int32_t *__restrict__ nbr,
                                                    const int32_t *__restrict__ sbc,
                                                    const int32_t *__restrict__ wall_idx,
                                                    const double *__restrict__ M, const double *__restrict__ bv,
                                                    const double *__restrict__ u, const double *__restrict__ x,
                                                    double *__restrict__ xn, double relax,
-                                                   const double *__restrict__ corr, int64_t ldw, int32_t *errw) {
+                                                   const double *__restrict__ corr, int64_t ldw, int32_t *errw,
+                                                   float *__restrict__ r32 = nullptr,
+                                                   float *__restrict__ d32 = nullptr) {
     const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc)
         return;
@@ -211,9 +217,88 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
                 nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
         }
     }
+    if (RESID) {
+#pragma unroll
+        for (int k = 0; k < 36; k++) {
+            const float r = float(nw[k] - x[k * ldc + i]);
+            r32[k * ldc + i] = r;
+            d32[k * ldc + i] = r;
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 36; k++)
         xn[k * ldc + i] = nw[k];
+}
+
+// FP32 correction sweep of the mixed-precision reconstruction.  The Jacobi
+// map is affine, J(x) = G x + c with G x = (1-w) x + w sum_f M_if x_j (far
+// field faces couple the cell to itself), so with r = J(x) - x the sweeps
+//   d_0 = r,  d_{k+1} = G d_k + r
+// give x + d_k = J^{k+1}(x) exactly in real arithmetic: n - 1 FP32 sweeps
+// after one FP64 step reproduce n FP64 sweeps, with FP32 rounding confined
+// to the correction d (iterative refinement).  M is read as FP32.  LAST adds
+// the correction to the FP64 iterate in place.  Meshes with Wall_Euler /
+// Wall_NoSlip faces (whose sweep term is affine in x through the boundary
+// state) are not handled here; the host refuses them.
+template <int NF, bool LAST>
+__global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, const int32_t *__restrict__ nbr,
+                                                    const int32_t *__restrict__ sbc, const float *__restrict__ M32,
+                                                    const float *__restrict__ r32, const float *__restrict__ din,
+                                                    float *__restrict__ dout, float relax,
+                                                    double *__restrict__ x) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    float nw[36];
+#pragma unroll
+    for (int k = 0; k < 36; k++)
+        nw[k] = (1.0f - relax) * din[k * ldc + i];
+#pragma unroll 1
+    for (int s = 0; s < NF; s++) {
+        int32_t j = nbr[s * ldc + i];
+        if (j == -2)
+            continue;
+        if (j < 0) {
+            if (sbc[s * ldc + i] == BC_WALL) // constant term: in r only
+                continue;
+            j = i; // far field
+        }
+        const float *Ms = M32 + int64_t(s) * 81 * ldc + i;
+        float dj[36];
+#pragma unroll
+        for (int k = 0; k < 36; k++)
+            dj[k] = din[k * ldc + j];
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+            float tt[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+                const float m = __builtin_nontemporal_load(Ms + (r * 9 + c) * ldc);
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    tt[v] = fmaf(m, dj[c + 9 * v], tt[v]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                nw[r + 9 * v] = fmaf(relax, tt[v], nw[r + 9 * v]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 36; k++) {
+        const float d = nw[k] + r32[k * ldc + i];
+        if (LAST)
+            x[k * ldc + i] += double(d);
+        else
+            dout[k * ldc + i] = d;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ src,
+                                                   float *__restrict__ dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = float(src[i]);
 }
 
 // EvaluateDt face loop (DNDS_FV_EulerEvaluator.cpp:25-137): lambdaFace and
@@ -322,7 +407,11 @@ __global__ __launch_bounds__(kBlock) void k_fill_from(int32_t n, const double *_
 // selects the boundary-face instance (run over the list of boundary faces);
 // the inner instance skips boundary faces, so neither carries the other's
 // registers.
-template <bool BOUNDARY>
+//
+// MIXED: the face reconstruction D0 . x runs in FP32 (FP32 bases, FP32
+// accumulation) and the integrated fluxes are stored in FP32; the Riemann
+// solver and the boundary states stay FP64.
+template <bool BOUNDARY, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int64_t ldc,
                                                      const int32_t *__restrict__ list,
                                                      const int32_t *__restrict__ f2c,
@@ -330,7 +419,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                                                      const double *__restrict__ fn, const double *__restrict__ fctr,
                                                      const double *__restrict__ dlf, const double *__restrict__ u,
                                                      const double *__restrict__ x, GasParams gp, double t,
-                                                     double *__restrict__ fflux, int32_t *errw) {
+                                                     double *__restrict__ fflux, int32_t *errw,
+                                                     const float *__restrict__ fd0_32 = nullptr,
+                                                     float *__restrict__ fflux32 = nullptr) {
     const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     if (tid >= n)
         return;
@@ -344,6 +435,42 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
         uL[k] = u[4 * int64_t(L) + k];
     // reconstructed increments D0 . x at the 3 points, both sides
     double incL[3][4], incR[3][4];
+    if (MIXED) {
+        float fL[3][4], fR[3][4];
+#pragma unroll
+        for (int g = 0; g < 3; g++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                fL[g][v] = 0.f, fR[g][v] = 0.f;
+#pragma unroll 1
+        for (int c = 0; c < 9; c++) {
+            float xl[4], xr[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                xl[v] = float(x[(c + 9 * v) * ldc + L]);
+                xr[v] = BOUNDARY ? 0.f : float(x[(c + 9 * v) * ldc + R]);
+            }
+#pragma unroll
+            for (int g = 0; g < 3; g++) {
+                const float dl = __builtin_nontemporal_load(fd0_32 + (g * 18 + c) * ldf + f);
+                const float dr = BOUNDARY ? 0.f : __builtin_nontemporal_load(fd0_32 + (g * 18 + 9 + c) * ldf + f);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    fL[g][v] = fmaf(dl, xl[v], fL[g][v]);
+                    fR[g][v] = fmaf(dr, xr[v], fR[g][v]);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < 3; g++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                incL[g][v] = double(fL[g][v]), incR[g][v] = double(fR[g][v]);
+        if (!BOUNDARY)
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                uR[k] = u[4 * int64_t(R) + k];
+    } else {
 #pragma unroll
     for (int g = 0; g < 3; g++)
 #pragma unroll
@@ -381,6 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                     incR[g][v] += d * xc[v];
             }
         }
+    }
     }
     const double dl = dlf[f];
     const int32_t bc = BOUNDARY ? fbc[f] : BC_INNER;
@@ -420,6 +548,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     }
     if (!ok)
         atomicOr(errw, ERRBIT_NONPOSITIVE);
+    if (MIXED) {
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            fflux32[k * ldf + f] = float(acc[k]);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 12; k++)
         fflux[k * ldf + f] = acc[k];
@@ -431,10 +565,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
 //   stage 0: rhs = gathered value
 //   stage 1: xLast = x; x = x + rhs dt
 //   stage 2/3: x = ((x + rhs dt) c/(1-c) + xLast)(1-c), c = 1/4, 2/3
-template <int NF>
+template <int NF, typename FT = double>
 __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, int64_t ldf,
                                                      const int32_t *__restrict__ gat,
-                                                     const double *__restrict__ fflux,
+                                                     const FT *__restrict__ fflux,
                                                      const double *__restrict__ vol, double *__restrict__ rhs,
                                                      int stage, double *__restrict__ x,
                                                      double *__restrict__ xlast, const double *__restrict__ dt,
@@ -443,8 +577,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
     const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc)
         return;
-    const double V = vol[i];
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    const FT V = FT(vol[i]);
+    FT r[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < NF; s++) {
         const int32_t g = gat[s * ldc + i];
@@ -476,14 +610,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
     if (stage == 0) {
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            rhs[4 * int64_t(i) + v] = r[v];
+            rhs[4 * int64_t(i) + v] = double(r[v]);
         return;
     }
     const double d = dt ? dt[i] : dtmin[0];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const double xo = x[4 * int64_t(i) + v];
-        double xv = xo + r[v] * d;
+        double xv = xo + double(r[v]) * d;
         if (stage == 1) {
             xlast[4 * int64_t(i) + v] = xo;
         } else {
@@ -1290,6 +1424,94 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
     return c->launch_check("wbap kernels");
 }
 
+int ensure_mixed(dndsmi_ctx *c) {
+    if (c->M32)
+        return DNDSMI_OK;
+    if (c->n_wall_faces > 0) {
+        c->err = "the mixed-precision sweeps do not handle Wall_Euler / Wall_NoSlip faces";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
+    const int64_t nm = int64_t(c->nfmax) * 81 * c->ldc, nd = 54 * c->ldf;
+    DNDSMI_TRY(c->alloc(c->M32, nm));
+    DNDSMI_TRY(c->alloc(c->fd0_32, nd));
+    DNDSMI_TRY(c->alloc(c->r32, 36 * c->ldc));
+    DNDSMI_TRY(c->alloc(c->dA32, 36 * c->ldc));
+    DNDSMI_TRY(c->alloc(c->dB32, 36 * c->ldc));
+    DNDSMI_TRY(c->alloc(c->fflux32, 12 * c->ldf));
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nm)), dim3(kBlock), 0, c->stream, nm, c->M, c->M32);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nd)), dim3(kBlock), 0, c->stream, nd, c->fd0, c->fd0_32);
+    DNDSMI_TRY(c->launch_check("k_to_f32"));
+    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// one FP64 step that yields r = J(x) - x, then n - 1 FP32 correction sweeps,
+// the last adding the correction to x in place
+int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps, double t) {
+    const int nc = int(c->nc);
+    (void)t;
+    prof_begin(c, DNDSMI_K_JACOBI);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL((k_jacobi<3, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr,
+                           c->sbc, c->wall_idx, c->M, c->bv, u, x, nullptr, c->relax, c->wall_corr, c->n_wall,
+                           c->errw, c->r32, c->dA32);
+    else
+        hipLaunchKernelGGL((k_jacobi<4, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr,
+                           c->sbc, c->wall_idx, c->M, c->bv, u, x, nullptr, c->relax, c->wall_corr, c->n_wall,
+                           c->errw, c->r32, c->dA32);
+    prof_end(c);
+    DNDSMI_TRY(c->launch_check("k_jacobi (residual)"));
+    float *din = c->dA32, *dout = c->dB32;
+    const float w = float(c->relax);
+    for (int k = 1; k < n_sweeps; k++) {
+        const bool last = k == n_sweeps - 1;
+        prof_begin(c, DNDSMI_K_DELTA32);
+        if (c->nfmax == 3) {
+            if (last)
+                hipLaunchKernelGGL((k_delta32<3, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
+                                   c->nbr, c->sbc, c->M32, c->r32, din, nullptr, w, x);
+            else
+                hipLaunchKernelGGL((k_delta32<3, false>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc,
+                                   c->ldc, c->nbr, c->sbc, c->M32, c->r32, din, dout, w, nullptr);
+        } else {
+            if (last)
+                hipLaunchKernelGGL((k_delta32<4, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
+                                   c->nbr, c->sbc, c->M32, c->r32, din, nullptr, w, x);
+            else
+                hipLaunchKernelGGL((k_delta32<4, false>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc,
+                                   c->ldc, c->nbr, c->sbc, c->M32, c->r32, din, dout, w, nullptr);
+        }
+        prof_end(c);
+        DNDSMI_TRY(c->launch_check("k_delta32"));
+        std::swap(din, dout);
+    }
+    return DNDSMI_OK;
+}
+
+int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
+                     const double *dt, double coefA, double coefB) {
+    const int nc = int(c->nc), nf = int(c->nf);
+    prof_begin(c, DNDSMI_K_RHS_FACE32);
+    hipLaunchKernelGGL((k_rhs_face<false, true>), dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf,
+                       c->ldc, c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t,
+                       c->fflux, c->errw, c->fd0_32, c->fflux32);
+    prof_end(c);
+    if (c->n_bface > 0)
+        hipLaunchKernelGGL((k_rhs_face<true, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
+                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr,
+                           c->dlambda_face, u, x, c->gas, t, c->fflux, c->errw, c->fd0_32, c->fflux32);
+    prof_begin(c, DNDSMI_K_RHS_CELL32);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL((k_rhs_cell<3, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
+                           c->gat, c->fflux32, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB,
+                           c->errw);
+    else
+        hipLaunchKernelGGL((k_rhs_cell<4, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
+                           c->gat, c->fflux32, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB,
+                           c->errw);
+    prof_end(c);
+    return c->launch_check("mixed rhs kernels");
+}
+
 } // namespace dndsmi_host
 
 extern "C" int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec,
@@ -1385,6 +1607,42 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
     return DNDSMI_OK;
 }
 
+extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                                        int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt,
+                                        int32_t UseLocaldt, double t, double *dt_out) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNew, DNDSMI_ARRAY_REC) || uRec == uRecNew || n_sweeps < 2)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (use_limiter && !c->has_limiter) {
+        c->err = "WBAP needs a setup built with build_limiter_data";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
+    DNDSMI_TRY(ensure_mixed(c));
+    DNDSMI_TRY(launch_dt(c, u->d, CFL, MaxDt, c->dt_tmp));
+    double dtmin = 0;
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(&dtmin, c->dt_min, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                      "hipMemcpyAsync"));
+    DNDSMI_TRY(check_errw(c));
+    if (dt_out)
+        *dt_out = dtmin;
+    const double Coef[3] = {1.0, 0.25, 2.0 / 3.0};
+    const double ct[3] = {1, 0.25, 2. / 3.};
+    for (int stage = 0; stage < 3; stage++) {
+        const double ts = t + ct[stage] * dtmin;
+        DNDSMI_TRY(launch_jacobi_mixed(c, u->d, uRec->d, n_sweeps, ts));
+        const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
+        const double cB = stage ? 1 - Coef[stage] : 0.0;
+        const double *xr = uRec->d;
+        if (use_limiter) {
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0));
+            xr = c->xlim;
+        }
+        DNDSMI_TRY(launch_rhs_mixed(c, u->d, xr, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,
+                                    cB));
+    }
+    return DNDSMI_OK;
+}
+
 // ---- profiling ---------------------------------------------------------
 
 extern "C" int dndsmi_profile_enable(dndsmi_ctx *c, int32_t on) {
@@ -1442,6 +1700,14 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
+    case DNDSMI_K_LUSGS_SWEEP: // per colour launch; not reported per launch
+        return 0;
+    case DNDSMI_K_DELTA32: // FP32 M per slot + slot index; d read + r read + d write (FP32)
+        return c2f * (81 * 4 + 4) + nc * (36 * 4 * 3);
+    case DNDSMI_K_RHS_FACE32: // f2c, FP32 D0 rows, normals, dlambda, 12 FP32 fluxes out; x and u once per cell
+        return nif * (8 + 54 * 4 + 48 + 8 + 48) + nc * (36 * 8 + 32);
+    case DNDSMI_K_RHS_CELL32: // gather index, FP32 F + side fix per slot, volume, x read+write, xLast
+        return c2f * (4 + 32) + nc * (8 + 32 * 3);
     default:
         return 0;
     }

@@ -95,6 +95,12 @@ struct dndsmi_ctx {
     int32_t *colour = nullptr;        // [ldc]
     int32_t *col_cells = nullptr;     // [nc]
     double *jinv = nullptr;           // [ldc] 1 / LU-SGS diagonal
+    // mixed precision (allocated on first use): FP32 copies of A^-1 B and of
+    // the face bases, FP32 correction vectors, FP32 face fluxes
+    float *M32 = nullptr;             // [nfmax*81][ldc]
+    float *fd0_32 = nullptr;          // [54][ldf]
+    float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
+    float *fflux32 = nullptr;         // [12][ldf]
     // implicit dual-time scratch (allocated on first use)
     double *imp_rhs = nullptr;        // [ldc][4]
     double *imp_xinc = nullptr;       // [ldc][4]
@@ -169,5 +175,10 @@ int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double 
 int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
                const double *dt, double coefA, double coefB);
 int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all);
+// the mixed-precision pieces (ctx.hip)
+int ensure_mixed(dndsmi_ctx *c);
+int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps, double t);
+int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
+                     const double *dt, double coefA, double coefB);
 
 } // namespace dndsmi_host

@@ -1,0 +1,715 @@
+// Learned warm start of the VR reconstruction (gfx950).  No counterpart in
+// the reference: a small MLP maps a cell's stencil of cell means and its
+// geometry moments to an initial guess of its 9 x 4 rec coefficients; plain
+// Jacobi sweeps then iterate to the same tolerance, so the method stays the
+// judge and the fallback.
+//
+// Network (58 -> 128 -> 128 -> 36, GELU): inputs are the mean differences
+// u_j - u_i of 12 stencil cells (per variable divided by s_v, the largest of
+// them), the 9 base moments of the cell and 0.5 log(volume); outputs are the
+// rec coefficients divided by s_v (the reconstruction is linear in the means,
+// so this scaling makes the map scale-free).  The stencil is the first 12
+// cells met by a breadth-first walk over faces (cell2face order); missing
+// cells repeat the cell itself.
+//
+// Training and inference use hand-written kernels: one GEMM with BF16
+// operands on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation) and fused epilogues (bias, GELU, GELU backward), column
+// sums for the bias gradients, the MSE loss and Adam, all FP32 otherwise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "ctx.hpp"
+
+using namespace dndsmi_dev;
+using namespace dndsmi_host;
+
+namespace {
+
+constexpr int kIn = 58, kHid = 128, kOut = 36, kSten = 12;
+// parameter block: W1 [58][128], b1 [128], W2 [128][128], b2 [128], W3 [128][36], b3 [36]
+constexpr int64_t oW1 = 0, ob1 = oW1 + kIn * kHid, oW2 = ob1 + kHid, ob2 = oW2 + kHid * kHid, oW3 = ob2 + kHid,
+                  ob3 = oW3 + kHid * kOut, kNParam = ob3 + kOut;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs)
+__device__ inline unsigned short to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3 };
+
+// C[M][N] = op(A) op(B) with BF16 operands and FP32 accumulation.
+// A(m, k) = transA ? A[k lda + m] : A[m lda + k]; B(k, n) likewise.
+// 256 threads; a 64 x 64 tile per workgroup, a 32 x 32 quarter per wave as
+// 2 x 2 MFMA 16x16x16 blocks; K in steps of 16 staged through LDS.  Operand
+// layout of v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + 0..3]
+// and B[4 (l / 16) + 0..3][l % 16]; it receives D[4 (l / 16) + 0..3][l % 16].
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, const float *__restrict__ A,
+                                              int64_t lda, int32_t transA, const float *__restrict__ B, int64_t ldb,
+                                              int32_t transB, float *__restrict__ C, int64_t ldcm,
+                                              const float *__restrict__ bias, float *__restrict__ C2,
+                                              const float *__restrict__ Zin) {
+    __shared__ unsigned short As[64][16 + 4];
+    __shared__ unsigned short Bs[16][64 + 4];
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const int64_t m0 = int64_t(blockIdx.x) * 64, n0 = int64_t(blockIdx.y) * 64;
+    const int wm = 32 * (wave / 2), wn = 32 * (wave % 2);
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int32_t k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 256 * q;
+            {
+                const int r = e / 16, kk = e % 16;
+                const int64_t m = m0 + r, k = k0 + kk;
+                float v = 0.f;
+                if (m < M && k < K)
+                    v = transA ? A[k * lda + m] : A[m * lda + k];
+                As[r][kk] = to_bf16(v);
+            }
+            {
+                const int kk = e / 64, c = e % 64;
+                const int64_t n = n0 + c, k = k0 + kk;
+                float v = 0.f;
+                if (n < N && k < K)
+                    v = transB ? B[n * ldb + k] : B[k * ldb + n];
+                Bs[kk][c] = to_bf16(v);
+            }
+        }
+        __syncthreads();
+        bf16x4 af[2], bfr[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            const int row = wm + 16 * a + lane % 16, kb = 4 * (lane / 16);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                af[a][j] = short(As[row][kb + j]);
+        }
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int col = wn + 16 * b + lane % 16, kb = 4 * (lane / 16);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                bfr[b][j] = short(Bs[kb + j][col]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t m = m0 + wm + 16 * a + 4 * (lane / 16) + j;
+                const int64_t n = n0 + wn + 16 * b + lane % 16;
+                if (m >= M || n >= N)
+                    continue;
+                float v = acc[a][b][j];
+                const int64_t o = m * ldcm + n;
+                if (EPI == EPI_STORE) {
+                    C[o] = v;
+                } else if (EPI == EPI_BIAS) {
+                    C[o] = v + bias[n];
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = v + bias[n];
+                    C[o] = z;
+                    C2[o] = gelu(z);
+                } else {
+                    C[o] = v * gelu_grad(Zin[o]);
+                }
+            }
+}
+
+// column sums of X [rows][n] (bias gradients), one thread per column, fixed order
+__global__ __launch_bounds__(256) void k_colsum(int32_t rows, int32_t n, const float *__restrict__ X,
+                                                float *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n)
+        return;
+    float s = 0.f;
+    for (int32_t r = 0; r < rows; r++)
+        s += X[int64_t(r) * n + c];
+    out[c] = s;
+}
+
+// MSE loss and its gradient: dY = 2 (Y - T) / (rows * 36); per-block sums of (Y - T)^2
+__global__ __launch_bounds__(256) void k_mse(int64_t n, const float *__restrict__ Y, const float *__restrict__ T,
+                                             float scale, float *__restrict__ dY, float *__restrict__ part) {
+    __shared__ float red[256];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float e = 0.f;
+    if (i < n) {
+        const float d = Y[i] - T[i];
+        dY[i] = scale * d;
+        e = d * d;
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction folded into the step size
+__global__ __launch_bounds__(256) void k_adam(int64_t n, float *__restrict__ P, const float *__restrict__ G,
+                                              float *__restrict__ Mm, float *__restrict__ Vv, float lr_t, float b1,
+                                              float b2, float eps_t) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float g = G[i];
+    const float m = b1 * Mm[i] + (1.f - b1) * g;
+    const float v = b2 * Vv[i] + (1.f - b2) * g * g;
+    Mm[i] = m, Vv[i] = v;
+    P[i] -= lr_t * m / (sqrtf(v) + eps_t);
+}
+
+// features and scales of every cell (see the file comment)
+__global__ __launch_bounds__(256) void k_features(int32_t nc, int64_t ldc, const int32_t *__restrict__ sten,
+                                                  const double *__restrict__ u, const float *__restrict__ mom,
+                                                  const float *__restrict__ logsz, float *__restrict__ X,
+                                                  float *__restrict__ S) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    double ui[4], d[kSten][4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+    double s[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kSten; k++) {
+        const int32_t j = sten[k * ldc + i];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            d[k][v] = u[4 * int64_t(j) + v] - ui[v];
+            s[v] = fmax(s[v], fabs(d[k][v]));
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        s[v] = s[v] > 0 ? s[v] : 1.0;
+    float *x = X + int64_t(i) * kIn;
+#pragma unroll
+    for (int k = 0; k < kSten; k++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            x[4 * k + v] = float(d[k][v] / s[v]);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        x[48 + k] = mom[k * ldc + i];
+    x[57] = logsz[i];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        S[4 * int64_t(i) + v] = float(s[v]);
+}
+
+// targets: converged rec coefficients x [k][ldc] (k = r + 9 v) divided by s_v
+__global__ __launch_bounds__(256) void k_targets(int32_t nc, int64_t ldc, const double *__restrict__ x,
+                                                 const float *__restrict__ S, float *__restrict__ T) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+#pragma unroll
+    for (int k = 0; k < kOut; k++)
+        T[int64_t(i) * kOut + k] = float(x[k * ldc + i] / double(S[4 * int64_t(i) + k / 9]));
+}
+
+// prediction back to rec coefficients
+__global__ __launch_bounds__(256) void k_unscale(int32_t nc, int64_t ldc, const float *__restrict__ Y,
+                                                 const float *__restrict__ S, double *__restrict__ x) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+#pragma unroll
+    for (int k = 0; k < kOut; k++)
+        x[k * ldc + i] = double(Y[int64_t(i) * kOut + k]) * double(S[4 * int64_t(i) + k / 9]);
+}
+
+// batch rows by index
+__global__ __launch_bounds__(256) void k_gather_rows(int32_t rows, int32_t width, const int32_t *__restrict__ idx,
+                                                     const float *__restrict__ src, float *__restrict__ dst) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= int64_t(rows) * width)
+        return;
+    const int64_t r = e / width, c = e % width;
+    dst[e] = src[int64_t(idx[r]) * width + c];
+}
+
+// sum over all cells of |x_new - x_old|^2 (per block), for the rec residual
+__global__ __launch_bounds__(256) void k_rec_diff2(int32_t nc, int64_t ldc, const double *__restrict__ a,
+                                                   const double *__restrict__ b, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < nc)
+        for (int k = 0; k < 36; k++) {
+            const double d = a[k * ldc + i] - b[k * ldc + i];
+            s += d * d;
+        }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = red[0];
+}
+
+} // namespace
+
+struct dndsmi_warmstart {
+    dndsmi_ctx *ctx = nullptr;
+    int64_t nc = 0, ldc = 0;
+    std::vector<void *> allocs;
+    int32_t *sten = nullptr; // [12][ldc]
+    float *mom = nullptr;    // [9][ldc]
+    float *logsz = nullptr;  // [ldc]
+    float *P = nullptr, *G = nullptr, *Mm = nullptr, *Vv = nullptr; // [kNParam]
+    int64_t adam_t = 0;
+    // training set
+    float *X = nullptr, *T = nullptr;
+    int64_t n_samples = 0, capacity = 0;
+    // per-call work (rows up to max_rows)
+    int64_t max_rows = 0;
+    float *bX = nullptr, *bT = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Y = nullptr,
+          *dY = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *part = nullptr;
+    int32_t *idx = nullptr;
+    float *fX = nullptr, *fS = nullptr; // features / scales of all cells
+    double *dpart = nullptr;
+    std::mt19937_64 rng;
+
+    template <class T>
+    int alloc(T *&p, int64_t n) {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * size_t(std::max<int64_t>(n, 1))) != hipSuccess) {
+            ctx->err = "hipMalloc failed (warm start)";
+            return DNDSMI_ERR_OUT_OF_MEMORY;
+        }
+        allocs.push_back(q);
+        p = static_cast<T *>(q);
+        return DNDSMI_OK;
+    }
+    ~dndsmi_warmstart() {
+        for (void *q : allocs)
+            (void)hipFree(q);
+    }
+};
+
+namespace {
+
+int ensure_rows(dndsmi_warmstart *w, int64_t rows) {
+    if (rows <= w->max_rows)
+        return DNDSMI_OK;
+    // growth frees nothing: old buffers stay in the allocation list until destroy
+    const int64_t r = std::max<int64_t>(rows, 256);
+    DNDSMI_TRY(w->alloc(w->bX, r * kIn));
+    DNDSMI_TRY(w->alloc(w->bT, r * kOut));
+    DNDSMI_TRY(w->alloc(w->Z1, r * kHid));
+    DNDSMI_TRY(w->alloc(w->H1, r * kHid));
+    DNDSMI_TRY(w->alloc(w->Z2, r * kHid));
+    DNDSMI_TRY(w->alloc(w->H2, r * kHid));
+    DNDSMI_TRY(w->alloc(w->Y, r * kOut));
+    DNDSMI_TRY(w->alloc(w->dY, r * kOut));
+    DNDSMI_TRY(w->alloc(w->dZ2, r * kHid));
+    DNDSMI_TRY(w->alloc(w->dZ1, r * kHid));
+    DNDSMI_TRY(w->alloc(w->idx, r));
+    DNDSMI_TRY(w->alloc(w->part, (r * kOut + 255) / 256 + 1));
+    w->max_rows = r;
+    return DNDSMI_OK;
+}
+
+template <int EPI>
+void gemm(hipStream_t s, int32_t M, int32_t N, int32_t K, const float *A, int64_t lda, int ta, const float *B,
+          int64_t ldb, int tb, float *C, int64_t ldc, const float *bias = nullptr, float *C2 = nullptr,
+          const float *Zin = nullptr) {
+    dim3 grid((M + 63) / 64, (N + 63) / 64);
+    hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, M, N, K, A, lda, ta, B, ldb, tb, C, ldc, bias, C2, Zin);
+}
+
+// forward pass of `rows` rows of X into Z1, H1, Z2, H2, Y
+void forward(dndsmi_warmstart *w, const float *X, int32_t rows) {
+    hipStream_t s = w->ctx->stream;
+    const float *P = w->P;
+    gemm<EPI_BIAS_GELU>(s, rows, kHid, kIn, X, kIn, 0, P + oW1, kHid, 0, w->Z1, kHid, P + ob1, w->H1);
+    gemm<EPI_BIAS_GELU>(s, rows, kHid, kHid, w->H1, kHid, 0, P + oW2, kHid, 0, w->Z2, kHid, P + ob2, w->H2);
+    gemm<EPI_BIAS>(s, rows, kOut, kHid, w->H2, kHid, 0, P + oW3, kOut, 0, w->Y, kOut, P + ob3);
+}
+
+// loss and gradients into G for rows of (X, T); returns the mean squared error
+int loss_grad(dndsmi_warmstart *w, const float *X, const float *T, int32_t rows, double *loss) {
+    hipStream_t s = w->ctx->stream;
+    const float *P = w->P;
+    float *G = w->G;
+    forward(w, X, rows);
+    const int64_t ny = int64_t(rows) * kOut;
+    const unsigned nb = unsigned((ny + 255) / 256);
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, ny, w->Y, T, float(2.0 / double(ny)), w->dY, w->part);
+    // layer 3
+    gemm<EPI_STORE>(s, kHid, kOut, rows, w->H2, kHid, 1, w->dY, kOut, 0, G + oW3, kOut);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kOut, w->dY, G + ob3);
+    gemm<EPI_GELU_GRAD>(s, rows, kHid, kOut, w->dY, kOut, 0, P + oW3, kOut, 1, w->dZ2, kHid, nullptr, nullptr, w->Z2);
+    // layer 2
+    gemm<EPI_STORE>(s, kHid, kHid, rows, w->H1, kHid, 1, w->dZ2, kHid, 0, G + oW2, kHid);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kHid, w->dZ2, G + ob2);
+    gemm<EPI_GELU_GRAD>(s, rows, kHid, kHid, w->dZ2, kHid, 0, P + oW2, kHid, 1, w->dZ1, kHid, nullptr, nullptr, w->Z1);
+    // layer 1
+    gemm<EPI_STORE>(s, kIn, kHid, rows, X, kIn, 1, w->dZ1, kHid, 0, G + oW1, kHid);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kHid, w->dZ1, G + ob1);
+    DNDSMI_TRY(w->ctx->launch_check("warm-start backward"));
+    if (loss) {
+        std::vector<float> h(nb);
+        DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(h.data(), w->part, sizeof(float) * nb, hipMemcpyDeviceToHost, s),
+                               "hipMemcpyAsync"));
+        DNDSMI_TRY(w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize"));
+        double acc = 0;
+        for (float v : h)
+            acc += v;
+        *loss = acc / double(ny);
+    }
+    return DNDSMI_OK;
+}
+
+void adam(dndsmi_warmstart *w, double lr) {
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    w->adam_t++;
+    const double c1 = 1 - std::pow(b1, double(w->adam_t)), c2 = 1 - std::pow(b2, double(w->adam_t));
+    // step lr sqrt(c2) / c1 on m / (sqrt(v) + eps sqrt(c2)), the bias-corrected update
+    hipLaunchKernelGGL(k_adam, dim3(unsigned((kNParam + 255) / 256)), dim3(256), 0, w->ctx->stream, kNParam, w->P,
+                       w->G, w->Mm, w->Vv, float(lr * std::sqrt(c2) / c1), float(b1), float(b2),
+                       float(eps * std::sqrt(c2)));
+}
+
+int features(dndsmi_warmstart *w, const double *u) {
+    if (!w->fX) {
+        DNDSMI_TRY(w->alloc(w->fX, w->nc * kIn));
+        DNDSMI_TRY(w->alloc(w->fS, w->nc * 4));
+    }
+    hipLaunchKernelGGL(k_features, dim3(grid_for(w->nc)), dim3(256), 0, w->ctx->stream, int32_t(w->nc), w->ldc,
+                       w->sten, u, w->mom, w->logsz, w->fX, w->fS);
+    return w->ctx->launch_check("k_features");
+}
+
+} // namespace
+
+extern "C" int dndsmi_warmstart_create(dndsmi_ctx *c, const dndsmi_mesh_view *mesh, const dndsmi_vr_view *vr,
+                                       uint64_t seed, int64_t capacity, dndsmi_warmstart **out) {
+    if (!c || !mesh || !vr || !out || mesh->n_cell != c->nc || vr->n_cell != c->nc || capacity < 0)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *w = new (std::nothrow) dndsmi_warmstart();
+    if (!w)
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    w->ctx = c, w->nc = c->nc, w->ldc = c->ldc;
+    w->rng.seed(seed);
+    auto fail = [&](int rc) {
+        delete w;
+        return rc;
+    };
+    const int64_t nc = c->nc, ldc = c->ldc;
+    // stencil: breadth-first over faces, cell2face order, first 12 cells
+    std::vector<int32_t> sten(size_t(kSten) * ldc, 0);
+    std::vector<float> mom(size_t(9) * ldc, 0.f), lsz(size_t(ldc), 0.f);
+    std::vector<int64_t> ring, next;
+    std::vector<int64_t> seen;
+    for (int64_t i = 0; i < nc; i++) {
+        seen.assign(1, i);
+        ring.assign(1, i);
+        std::vector<int64_t> got;
+        while (int(got.size()) < kSten && !ring.empty()) {
+            next.clear();
+            for (int64_t a : ring)
+                for (int64_t q = mesh->cell_ptr[a]; q < mesh->cell_ptr[a + 1]; q++) {
+                    const int64_t f = mesh->cell2face[q];
+                    const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
+                    const int64_t b = L == a ? R : L;
+                    if (b == DNDSMI_NO_CELL || std::find(seen.begin(), seen.end(), b) != seen.end())
+                        continue;
+                    seen.push_back(b);
+                    next.push_back(b);
+                    if (int(got.size()) < kSten)
+                        got.push_back(b);
+                }
+            ring = next;
+        }
+        for (int k = 0; k < kSten; k++)
+            sten[k * ldc + i] = int32_t(k < int(got.size()) ? got[k] : i);
+        for (int k = 0; k < 9; k++)
+            mom[k * ldc + i] = float(vr->base_moment[10 * i + 1 + k]);
+        lsz[i] = float(0.5 * std::log(vr->volume[i]));
+    }
+    try {
+        if (w->alloc(w->sten, kSten * ldc) || w->alloc(w->mom, 9 * ldc) || w->alloc(w->logsz, ldc) ||
+            w->alloc(w->P, kNParam) || w->alloc(w->G, kNParam) || w->alloc(w->Mm, kNParam) ||
+            w->alloc(w->Vv, kNParam) || w->alloc(w->X, capacity * kIn) || w->alloc(w->T, capacity * kOut) ||
+            w->alloc(w->dpart, int64_t(grid_for(nc)) + 1))
+            return fail(DNDSMI_ERR_OUT_OF_MEMORY);
+    } catch (...) {
+        return fail(DNDSMI_ERR_OUT_OF_MEMORY);
+    }
+    w->capacity = capacity;
+    // He-style initialisation, seeded: weights U(-sqrt(6/fan_in), ...), zero biases
+    std::vector<float> P(kNParam, 0.f);
+    auto init = [&](int64_t o, int fin, int fout) {
+        std::uniform_real_distribution<float> d(-std::sqrt(6.f / fin), std::sqrt(6.f / fin));
+        for (int64_t k = 0; k < int64_t(fin) * fout; k++)
+            P[o + k] = d(w->rng);
+    };
+    init(oW1, kIn, kHid), init(oW2, kHid, kHid), init(oW3, kHid, kOut);
+    for (int64_t k = oW3; k < oW3 + kHid * kOut; k++)
+        P[k] *= 0.1f; // small initial output
+    std::vector<float> zero(kNParam, 0.f);
+    int rc = DNDSMI_OK;
+    rc = rc ? rc : c->upload(w->sten, sten);
+    rc = rc ? rc : c->upload(w->mom, mom);
+    rc = rc ? rc : c->upload(w->logsz, lsz);
+    rc = rc ? rc : c->upload(w->P, P);
+    rc = rc ? rc : c->upload(w->Mm, zero);
+    rc = rc ? rc : c->upload(w->Vv, zero);
+    if (rc)
+        return fail(rc);
+    *out = w;
+    return DNDSMI_OK;
+}
+
+extern "C" void dndsmi_warmstart_destroy(dndsmi_warmstart *w) {
+    if (w && w->ctx)
+        (void)hipStreamSynchronize(w->ctx->stream);
+    delete w;
+}
+
+extern "C" int64_t dndsmi_warmstart_num_params(void) { return kNParam; }
+
+extern "C" int dndsmi_warmstart_get_params(dndsmi_warmstart *w, float *host) {
+    if (!w || !host)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(host, w->P, sizeof(float) * kNParam, hipMemcpyDeviceToHost, w->ctx->stream),
+                           "hipMemcpyAsync"));
+    return w->ctx->hip(hipStreamSynchronize(w->ctx->stream), "hipStreamSynchronize");
+}
+
+extern "C" int dndsmi_warmstart_set_params(dndsmi_warmstart *w, const float *host) {
+    if (!w || !host)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->P, host, sizeof(float) * kNParam, hipMemcpyHostToDevice, w->ctx->stream),
+                           "hipMemcpyAsync"));
+    w->adam_t = 0;
+    std::vector<float> zero(kNParam, 0.f);
+    DNDSMI_TRY(w->ctx->upload(w->Mm, zero));
+    return w->ctx->upload(w->Vv, zero);
+}
+
+extern "C" int dndsmi_warmstart_loss_grad(dndsmi_warmstart *w, const float *X, const float *T, int32_t rows,
+                                          float *grad, double *loss) {
+    if (!w || !X || !T || rows < 1)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(ensure_rows(w, rows));
+    hipStream_t s = w->ctx->stream;
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->bX, X, sizeof(float) * rows * kIn, hipMemcpyHostToDevice, s), "copy"));
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->bT, T, sizeof(float) * rows * kOut, hipMemcpyHostToDevice, s), "copy"));
+    DNDSMI_TRY(loss_grad(w, w->bX, w->bT, rows, loss));
+    if (grad) {
+        DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(grad, w->G, sizeof(float) * kNParam, hipMemcpyDeviceToHost, s), "copy"));
+        DNDSMI_TRY(w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize"));
+    }
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_warmstart_adam_step(dndsmi_warmstart *w, const float *grad, double lr) {
+    if (!w || !grad)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->G, grad, sizeof(float) * kNParam, hipMemcpyHostToDevice, w->ctx->stream),
+                           "copy"));
+    adam(w, lr);
+    return w->ctx->launch_check("k_adam");
+}
+
+extern "C" int dndsmi_warmstart_features(dndsmi_warmstart *w, const dndsmi_array *u, float *X, float *S) {
+    if (!w || !same_ctx(w->ctx, u, DNDSMI_ARRAY_DOF))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(features(w, u->d));
+    hipStream_t s = w->ctx->stream;
+    if (X)
+        DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(X, w->fX, sizeof(float) * w->nc * kIn, hipMemcpyDeviceToHost, s), "copy"));
+    if (S)
+        DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(S, w->fS, sizeof(float) * w->nc * 4, hipMemcpyDeviceToHost, s), "copy"));
+    return w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+extern "C" int dndsmi_warmstart_add_samples(dndsmi_warmstart *w, const dndsmi_array *u, const dndsmi_array *uRec) {
+    if (!w || !same_ctx(w->ctx, u, DNDSMI_ARRAY_DOF) || !same_ctx(w->ctx, uRec, DNDSMI_ARRAY_REC))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (w->n_samples + w->nc > w->capacity) {
+        w->ctx->err = "warm-start training set is full";
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
+    DNDSMI_TRY(features(w, u->d));
+    hipStream_t s = w->ctx->stream;
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->X + w->n_samples * kIn, w->fX, sizeof(float) * w->nc * kIn,
+                                          hipMemcpyDeviceToDevice, s),
+                           "copy"));
+    hipLaunchKernelGGL(k_targets, dim3(grid_for(w->nc)), dim3(256), 0, s, int32_t(w->nc), w->ldc, uRec->d, w->fS,
+                       w->T + w->n_samples * kOut);
+    DNDSMI_TRY(w->ctx->launch_check("k_targets"));
+    w->n_samples += w->nc;
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_warmstart_add_rows(dndsmi_warmstart *w, const float *X, const float *T, int64_t rows) {
+    if (!w || !X || !T || rows < 0)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (w->n_samples + rows > w->capacity) {
+        w->ctx->err = "warm-start training set is full";
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
+    hipStream_t s = w->ctx->stream;
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->X + w->n_samples * kIn, X, sizeof(float) * rows * kIn,
+                                          hipMemcpyHostToDevice, s),
+                           "copy"));
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(w->T + w->n_samples * kOut, T, sizeof(float) * rows * kOut,
+                                          hipMemcpyHostToDevice, s),
+                           "copy"));
+    DNDSMI_TRY(w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize"));
+    w->n_samples += rows;
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_warmstart_targets(dndsmi_warmstart *w, const dndsmi_array *u, const dndsmi_array *uRec,
+                                        float *T) {
+    if (!w || !T || !same_ctx(w->ctx, u, DNDSMI_ARRAY_DOF) || !same_ctx(w->ctx, uRec, DNDSMI_ARRAY_REC))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(features(w, u->d));
+    DNDSMI_TRY(ensure_rows(w, w->nc));
+    hipStream_t s = w->ctx->stream;
+    hipLaunchKernelGGL(k_targets, dim3(grid_for(w->nc)), dim3(256), 0, s, int32_t(w->nc), w->ldc, uRec->d, w->fS,
+                       w->Y);
+    DNDSMI_TRY(w->ctx->launch_check("k_targets"));
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(T, w->Y, sizeof(float) * w->nc * kOut, hipMemcpyDeviceToHost, s), "copy"));
+    return w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+extern "C" int64_t dndsmi_warmstart_num_samples(const dndsmi_warmstart *w) { return w ? w->n_samples : -1; }
+
+extern "C" int dndsmi_warmstart_train(dndsmi_warmstart *w, int32_t epochs, int32_t batch, double lr, double lr_final,
+                                      double *loss_hist) {
+    if (!w || epochs < 1 || batch < 1 || w->n_samples < 1)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    const int64_t n = w->n_samples;
+    const int32_t b = int32_t(std::min<int64_t>(batch, n));
+    DNDSMI_TRY(ensure_rows(w, b));
+    int32_t *perm = nullptr;
+    DNDSMI_TRY(w->alloc(perm, n)); // freed with the warm start
+    std::vector<int32_t> hp(static_cast<size_t>(n));
+    std::iota(hp.begin(), hp.end(), 0);
+    hipStream_t s = w->ctx->stream;
+    const int64_t steps_per_epoch = n / b, total = steps_per_epoch * epochs;
+    int64_t step = 0;
+    for (int32_t e = 0; e < epochs; e++) {
+        std::shuffle(hp.begin(), hp.end(), w->rng);
+        DNDSMI_TRY(w->ctx->upload(perm, hp));
+        double acc = 0;
+        for (int64_t q = 0; q < steps_per_epoch; q++, step++) {
+            const int32_t *ix = perm + q * b;
+            hipLaunchKernelGGL(k_gather_rows, dim3(unsigned((int64_t(b) * kIn + 255) / 256)), dim3(256), 0, s, b, kIn,
+                               ix, w->X, w->bX);
+            hipLaunchKernelGGL(k_gather_rows, dim3(unsigned((int64_t(b) * kOut + 255) / 256)), dim3(256), 0, s, b,
+                               kOut, ix, w->T, w->bT);
+            double l = 0;
+            const bool last = q == steps_per_epoch - 1;
+            DNDSMI_TRY(loss_grad(w, w->bX, w->bT, b, last ? &l : nullptr));
+            // cosine decay from lr to lr_final
+            const double f = 0.5 * (1 + std::cos(M_PI * double(step) / double(std::max<int64_t>(total, 1))));
+            adam(w, lr_final + (lr - lr_final) * f);
+            if (last)
+                acc = l;
+        }
+        DNDSMI_TRY(w->ctx->launch_check("warm-start training"));
+        if (loss_hist)
+            loss_hist[e] = acc;
+    }
+    return w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+extern "C" int dndsmi_warmstart_predict(dndsmi_warmstart *w, const dndsmi_array *u, dndsmi_array *uRec) {
+    if (!w || !same_ctx(w->ctx, u, DNDSMI_ARRAY_DOF) || !same_ctx(w->ctx, uRec, DNDSMI_ARRAY_REC))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(features(w, u->d));
+    DNDSMI_TRY(ensure_rows(w, w->nc));
+    forward(w, w->fX, int32_t(w->nc));
+    hipLaunchKernelGGL(k_unscale, dim3(grid_for(w->nc)), dim3(256), 0, w->ctx->stream, int32_t(w->nc), w->ldc, w->Y,
+                       w->fS, uRec->d);
+    return w->ctx->launch_check("warm-start predict");
+}
+
+extern "C" int dndsmi_reconstruct_to_tolerance(dndsmi_ctx *c, const dndsmi_array *u, dndsmi_array *uRec,
+                                               dndsmi_array *uRecNewBuf, double tol, int32_t max_sweeps, double t,
+                                               int32_t *sweeps, double *res_out) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNewBuf, DNDSMI_ARRAY_REC) || uRec == uRecNewBuf || max_sweeps < 1)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    const unsigned g = grid_for(c->nc);
+    std::vector<double> part(g);
+    double *dpart = nullptr;
+    if (!c->res_part) { // the implicit path's scratch holds 4 x grid doubles
+        DNDSMI_TRY(c->alloc(c->res_part, 4 * int64_t(g)));
+        DNDSMI_TRY(c->alloc(c->res, 4));
+    }
+    dpart = c->res_part;
+    double res = 0;
+    int32_t k = 0;
+    while (k < max_sweeps) {
+        DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNewBuf->d, t));
+        hipLaunchKernelGGL(k_rec_diff2, dim3(g), dim3(256), 0, c->stream, int32_t(c->nc), c->ldc, uRec->d,
+                           uRecNewBuf->d, dpart);
+        DNDSMI_TRY(c->launch_check("k_rec_diff2"));
+        std::swap(uRec->d, uRecNewBuf->d);
+        k++;
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(part.data(), dpart, sizeof(double) * g, hipMemcpyDeviceToHost, c->stream),
+                          "hipMemcpyAsync"));
+        DNDSMI_TRY(check_errw(c));
+        res = 0;
+        for (double v : part)
+            res += v;
+        res /= double(c->nc);
+        if (res < tol)
+            break;
+    }
+    if (sweeps)
+        *sweeps = k;
+    if (res_out)
+        *res_out = res;
+    return DNDSMI_OK;
+}

@@ -265,6 +265,20 @@ class VRFiniteVolume2D:
                                                   int(ifAll)), "WBAP V3", d.ctx)
 
 
+def _reconstruct_to_tolerance(vfv, u, uRec, uRecNewBuf, tol, max_sweeps, t=0.0):
+    d = vfv.device
+    n, res = C.c_int32(), C.c_double()
+    _check(_lib().dndsmi_reconstruct_to_tolerance(d.ctx, u.h, uRec.h, uRecNewBuf.h, tol, max_sweeps, t, C.byref(n),
+                                                  C.byref(res)), "reconstruct to tolerance", d.ctx)
+    return n.value, res.value
+
+
+VRFiniteVolume2D.ReconstructToTolerance = _reconstruct_to_tolerance
+VRFiniteVolume2D.ReconstructToTolerance.__doc__ = (
+    "Jacobi sweeps until sum |uRec - uRecNew|^2 / n_cell < tol (the reference's rec increment, "
+    "DNDS_FV_VR.hpp:1684-1705); returns (sweeps, last increment).")
+
+
 class Device:
     """The device context: mesh + setup on the GPU (one per evaluator)."""
 
@@ -419,21 +433,25 @@ class ExplicitSSPRK3LocalDt:
     follow the reference's frhs arguments (t + dt, t + dt/4, t + 2dt/3)."""
 
     def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, MaxDt: float = 1e100,
-                 UseLocaldt: bool = False, useLimiter: bool = False):
+                 UseLocaldt: bool = False, useLimiter: bool = False, mixedPrecision: bool = False):
         self.ev = evaluator
         self.n_sweeps = nInternalRecStep
         self.CFL = CFL
         self.MaxDt = MaxDt
         self.UseLocaldt = UseLocaldt
         self.useLimiter = useLimiter
+        self.mixedPrecision = mixedPrecision
 
     def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, t: float) -> float:
-        """Advances u in place; returns the global minimum dt of the step."""
+        """Advances u in place; returns the global minimum dt of the step.
+        mixedPrecision: FP32 correction sweeps after one FP64 sweep per
+        stage, FP32 face reconstruction and flux storage (no reference
+        counterpart)."""
         dt = C.c_double()
         d = self.ev.device
-        _check(_lib().dndsmi_ssprk3_step(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, int(self.useLimiter),
-                                         self.CFL, self.MaxDt, int(self.UseLocaldt), t, C.byref(dt)),
-               "SSP-RK3 step", d.ctx)
+        fn = _lib().dndsmi_ssprk3_step_mixed if self.mixedPrecision else _lib().dndsmi_ssprk3_step
+        _check(fn(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, int(self.useLimiter), self.CFL, self.MaxDt,
+                  int(self.UseLocaldt), t, C.byref(dt)), "SSP-RK3 step", d.ctx)
         return dt.value
 
 
@@ -462,3 +480,79 @@ class ImplicitEulerDualTimeStep:
                                                  int(self.UseLocaldt), self.max_iter, self.threshold, t,
                                                  C.byref(n), A.dptr(hist)), "implicit Euler step", d.ctx)
         return hist[:n.value]
+
+
+class WarmStart:
+    """The learned warm start of the reconstruction (no reference
+    counterpart): features, training set, training, inference on the device."""
+
+    def __init__(self, evaluator: EulerEvaluator, seed: int = 0, capacity: int = 0):
+        self.ev = evaluator
+        h = C.c_void_p()
+        d = evaluator.device
+        _check(_lib().dndsmi_warmstart_create(d.ctx, C.byref(evaluator.mesh.view), C.byref(evaluator.vfv.view), seed,
+                                              capacity, C.byref(h)), "warm start create", d.ctx)
+        self.h = h
+        self.n_param = int(_lib().dndsmi_warmstart_num_params())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib().dndsmi_warmstart_destroy(self.h)
+            self.h = None
+
+    def _ctx(self):
+        return self.ev.device.ctx
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_param, dtype=np.float32)
+        _check(_lib().dndsmi_warmstart_get_params(self.h, A.fptr(p)), "get params", self._ctx())
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_lib().dndsmi_warmstart_set_params(self.h, A.fptr(p)), "set params", self._ctx())
+
+    def features(self, u: ArrayDOFV):
+        n = self.ev.device.n_cell
+        X = np.empty((n, 58), dtype=np.float32)
+        S = np.empty((n, 4), dtype=np.float32)
+        _check(_lib().dndsmi_warmstart_features(self.h, u.h, A.fptr(X), A.fptr(S)), "features", self._ctx())
+        return X, S
+
+    def add_samples(self, u: ArrayDOFV, uRec: ArrayRecV):
+        _check(_lib().dndsmi_warmstart_add_samples(self.h, u.h, uRec.h), "add samples", self._ctx())
+
+    def add_rows(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        _check(_lib().dndsmi_warmstart_add_rows(self.h, A.fptr(X), A.fptr(T), X.shape[0]), "add rows", self._ctx())
+
+    def targets(self, u: ArrayDOFV, uRec: ArrayRecV) -> np.ndarray:
+        T = np.empty((self.ev.device.n_cell, 36), dtype=np.float32)
+        _check(_lib().dndsmi_warmstart_targets(self.h, u.h, uRec.h, A.fptr(T)), "targets", self._ctx())
+        return T
+
+    @property
+    def n_samples(self) -> int:
+        return int(_lib().dndsmi_warmstart_num_samples(self.h))
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(self.n_param, dtype=np.float32)
+        loss = C.c_double()
+        _check(_lib().dndsmi_warmstart_loss_grad(self.h, A.fptr(X), A.fptr(T), X.shape[0], A.fptr(g),
+                                                 C.byref(loss)), "loss/grad", self._ctx())
+        return loss.value, g
+
+    def adam_step(self, grad, lr):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        _check(_lib().dndsmi_warmstart_adam_step(self.h, A.fptr(g), lr), "adam step", self._ctx())
+
+    def train(self, epochs: int, batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-5) -> np.ndarray:
+        hist = np.zeros(epochs)
+        _check(_lib().dndsmi_warmstart_train(self.h, epochs, batch, lr, lr_final, A.dptr(hist)), "train", self._ctx())
+        return hist
+
+    def predict(self, u: ArrayDOFV, uRec: ArrayRecV):
+        _check(_lib().dndsmi_warmstart_predict(self.h, u.h, uRec.h), "predict", self._ctx())

@@ -253,6 +253,21 @@ int dndsmi_ssprk3_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dnd
                        int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt, int32_t UseLocaldt, double t,
                        double *dt_out);
 
+/* Mixed precision: the same SSP-RK3 step with FP32 sweeps and flux storage.
+ * No reference counterpart (a new form of ExplicitSSPRK3LocalDt::Step).
+ * Per stage, the first Jacobi sweep runs in FP64 and yields the increment
+ * r = J(x) - x; the other n_sweeps - 1 sweeps iterate the correction
+ * d <- G d + r in FP32 with A^-1 B rounded to FP32 (G the linear part of the
+ * sweep), and x += d in FP64: in real arithmetic exactly n_sweeps FP64
+ * sweeps, with rounding confined to the correction (iterative refinement).
+ * EvaluateRHS reconstructs the face values in FP32 and stores the face
+ * fluxes in FP32; the Riemann solver, the boundary states, WBAP, the time
+ * step and the stage update stay FP64.  Meshes with Wall_Euler or
+ * Wall_NoSlip faces return DNDSMI_ERR_UNSUPPORTED; n_sweeps >= 2. */
+int dndsmi_ssprk3_step_mixed(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                             int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt, int32_t UseLocaldt,
+                             double t, double *dt_out);
+
 /* ======================================================================== */
 /* Implicit path: LU-SGS and the implicit-Euler dual-time step              */
 /* ======================================================================== */
@@ -299,6 +314,62 @@ int dndsmi_implicit_euler_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *u
                                double threshold, double t, int32_t *n_iter, double *res_hist);
 
 /* ======================================================================== */
+/* Learned warm start of the reconstruction (no counterpart in the reference) */
+/* ======================================================================== */
+
+/* An MLP (58 -> 128 -> 128 -> 36, GELU) that maps a cell's stencil of cell
+ * means and geometry moments to an initial guess of its rec coefficients,
+ * trained on converged reconstructions the method itself produced.  Inputs:
+ * the differences u_j - u_i over 12 stencil cells (breadth-first over faces),
+ * per variable divided by their largest magnitude s_v; the 9 base moments;
+ * 0.5 log(volume).  Outputs: rec coefficients [r + 9 v] divided by s_v.
+ * Training and inference run on the device with BF16 matrix-core GEMMs (FP32
+ * accumulation) and Adam.  Plain Jacobi sweeps continue from the guess to
+ * the caller's tolerance, so the method stays the judge and the fallback. */
+typedef struct dndsmi_warmstart dndsmi_warmstart;
+
+/* capacity = training samples (cells) the set can hold; seed for the weights
+ * and the batch order */
+int dndsmi_warmstart_create(dndsmi_ctx *ctx, const dndsmi_mesh_view *mesh, const dndsmi_vr_view *vr, uint64_t seed,
+                            int64_t capacity, dndsmi_warmstart **out);
+void dndsmi_warmstart_destroy(dndsmi_warmstart *ws);
+int64_t dndsmi_warmstart_num_params(void);
+/* parameters as one FP32 block: W1 [58][128], b1 [128], W2 [128][128], b2 [128], W3 [128][36], b3 [36] */
+int dndsmi_warmstart_get_params(dndsmi_warmstart *ws, float *host);
+/* sets the parameters and clears the Adam state */
+int dndsmi_warmstart_set_params(dndsmi_warmstart *ws, const float *host);
+/* features [n_cell][58] and scales [n_cell][4] of a state (either may be NULL) */
+int dndsmi_warmstart_features(dndsmi_warmstart *ws, const dndsmi_array *u, float *X, float *S);
+/* appends (features of u, converged uRec / s_v) for every cell */
+int dndsmi_warmstart_add_samples(dndsmi_warmstart *ws, const dndsmi_array *u, const dndsmi_array *uRec);
+int64_t dndsmi_warmstart_num_samples(const dndsmi_warmstart *ws);
+/* appends host rows (X [rows][58], T [rows][36]), e.g. harvested on other meshes */
+int dndsmi_warmstart_add_rows(dndsmi_warmstart *ws, const float *X, const float *T, int64_t rows);
+/* the training targets of (u, uRec): T [n_cell][36] = uRec / s_v */
+int dndsmi_warmstart_targets(dndsmi_warmstart *ws, const dndsmi_array *u, const dndsmi_array *uRec, float *T);
+/* mean-squared-error loss and its gradient (same block layout as the
+ * parameters) on host rows X [rows][58], T [rows][36]; grad may be NULL */
+int dndsmi_warmstart_loss_grad(dndsmi_warmstart *ws, const float *X, const float *T, int32_t rows, float *grad,
+                               double *loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the given gradient */
+int dndsmi_warmstart_adam_step(dndsmi_warmstart *ws, const float *grad, double lr);
+/* epochs over the set in shuffled batches, Adam with a cosine learning rate
+ * from lr to lr_final; loss_hist (may be NULL) gets the last batch's loss of
+ * each epoch */
+int dndsmi_warmstart_train(dndsmi_warmstart *ws, int32_t epochs, int32_t batch, double lr, double lr_final,
+                           double *loss_hist);
+/* uRec = the network's guess for the state u */
+int dndsmi_warmstart_predict(dndsmi_warmstart *ws, const dndsmi_array *u, dndsmi_array *uRec);
+
+/* Jacobi sweeps (as dndsmi_reconstruction_jacobi_step) until the reference's
+ * rec increment sum_i |uRec_i - uRecNew_i|^2 / n_cell (DNDS_FV_VR.hpp:1684-1705)
+ * falls below tol, or max_sweeps; *sweeps and *res receive the count and the
+ * last increment.  No direct counterpart: the reference runs a fixed count. */
+int dndsmi_reconstruct_to_tolerance(dndsmi_ctx *ctx, const dndsmi_array *u, dndsmi_array *uRec,
+                                    dndsmi_array *uRecNewBuf, double tol, int32_t max_sweeps, double t,
+                                    int32_t *sweeps, double *res);
+
+/* ======================================================================== */
 /* Measurement                                                              */
 /* ======================================================================== */
 
@@ -314,7 +385,10 @@ enum {
     DNDSMI_K_WBAP_CELL = 7,  /* WBAP indicator, copy of unlimited cells       */
     DNDSMI_K_WBAP_LIM = 8,   /* WBAP characteristic limiting of flagged cells */
     DNDSMI_K_LUSGS_SWEEP = 9,/* one colour of an LU-SGS forward/backward sweep */
-    DNDSMI_K_COUNT = 10
+    DNDSMI_K_DELTA32 = 10,   /* mixed precision: FP32 correction sweep       */
+    DNDSMI_K_RHS_FACE32 = 11,/* mixed precision: inner-face fluxes, FP32 reconstruction */
+    DNDSMI_K_RHS_CELL32 = 12,/* mixed precision: FP32 gather + stage update  */
+    DNDSMI_K_COUNT = 13
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the
