@@ -6,7 +6,8 @@
 //
 // Network (58 -> 128 -> 128 -> 36, GELU): inputs are the mean differences
 // u_j - u_i of 12 stencil cells (per variable divided by s_v, the largest of
-// them), the 9 base moments of the cell and 0.5 log(volume); outputs are the
+// them but at least 1e-3 max_v |u_i,v|), the 9 base moments of the cell and
+// 0.5 log(volume); outputs are the
 // rec coefficients divided by s_v (the reconstruction is linear in the means,
 // so this scaling makes the map scale-free).  The stencil is the first 12
 // cells met by a breadth-first walk over faces (cell2face order); missing
@@ -217,9 +218,17 @@ __global__ __launch_bounds__(256) void k_features(int32_t nc, int64_t ldc, const
             s[v] = fmax(s[v], fabs(d[k][v]));
         }
     }
+    // floor: a thousandth of the cell's largest |u_v|, so that nearly uniform
+    // stencils (whose coefficients are set by distant cells) stay bounded
+    double umax = 0;
 #pragma unroll
     for (int v = 0; v < 4; v++)
+        umax = fmax(umax, fabs(ui[v]));
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        s[v] = fmax(s[v], 1e-3 * umax);
         s[v] = s[v] > 0 ? s[v] : 1.0;
+    }
     float *x = X + int64_t(i) * kIn;
 #pragma unroll
     for (int k = 0; k < kSten; k++)

@@ -321,7 +321,8 @@ int dndsmi_implicit_euler_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *u
  * means and geometry moments to an initial guess of its rec coefficients,
  * trained on converged reconstructions the method itself produced.  Inputs:
  * the differences u_j - u_i over 12 stencil cells (breadth-first over faces),
- * per variable divided by their largest magnitude s_v; the 9 base moments;
+ * per variable divided by s_v, their largest magnitude but at least
+ * 1e-3 max_v |u_i,v|; the 9 base moments;
  * 0.5 log(volume).  Outputs: rec coefficients [r + 9 v] divided by s_v.
  * Training and inference run on the device with BF16 matrix-core GEMMs (FP32
  * accumulation) and Adam.  Plain Jacobi sweeps continue from the guess to

@@ -6,7 +6,8 @@ The network and its inputs have no counterpart in the reference; this module
 restates them from their definition:
   features: for 12 stencil cells (breadth-first over faces, cell2face order,
   missing cells repeat the cell) the differences u_j - u_i divided per
-  variable by s_v = max_j |u_j - u_i| (1 if zero); the 9 base moments
+  variable by s_v = max(max_j |u_j - u_i|, 1e-3 max_v |u_i,v|) (1 if zero);
+  the 9 base moments
   baseMoments[1..9] (DNDS_FV_VR.cpp:173-178); 0.5 log(volume).
   targets: rec coefficients [r + 9 v] / s_v.
   network: 58 -> 128 -> 128 -> 36, GELU (exact, erf), MSE loss.
@@ -119,7 +120,7 @@ def features(mesh: dict, setup: dict, u: np.ndarray):
     """(X [n][58], S [n][4])."""
     st = stencil(mesh)
     d = u[st] - u[:, None, :]                       # [n][12][4]
-    s = np.abs(d).max(1)
+    s = np.maximum(np.abs(d).max(1), 1e-3 * np.abs(u).max(1)[:, None])
     s = np.where(s > 0, s, 1.0)
     X = np.empty((u.shape[0], N_IN))
     X[:, :48] = (d / s[:, None, :]).reshape(-1, 48)

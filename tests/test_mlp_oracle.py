@@ -80,12 +80,16 @@ def test_stencil_rings_are_face_neighbours_first():
         assert len(set(real)) == len(real)
 
 
-def test_features_are_affine_invariant():
+def test_features_are_scale_invariant_and_floored():
     m, s = _small_mesh()
     rng = np.random.default_rng(2)
     u = rng.uniform(0.5, 2.0, (m["n_cell"], 4))
     X, S = M.features(m, s, u)
-    X2, S2 = M.features(m, s, 3.0 * u + 7.0)
+    X2, S2 = M.features(m, s, 3.0 * u)
     assert np.allclose(X, X2, rtol=0, atol=1e-12)
     assert np.allclose(S2, 3.0 * S, rtol=1e-13)
     assert np.abs(X[:, :48]).max() <= 1.0 + 1e-15
+    # a uniform state: differences vanish, the scale is the floor
+    U = np.tile([1.0, 0.5, -0.2, 2.5], (m["n_cell"], 1))
+    X3, S3 = M.features(m, s, U)
+    assert np.all(X3[:, :48] == 0.0) and np.allclose(S3, 2.5e-3)
