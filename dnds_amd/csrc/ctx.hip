@@ -192,7 +192,8 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
             j = i; // far field: M_if x_i, no mean-value term
             bmul = 0.0;
         }
-        const double *Ms = M + int64_t(s) * 81 * ldc + i;
+        // A^-1 B entries e = 9 r + c, two per 16-byte load (entry 81 pads)
+        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + int64_t(s) * kMPairs * ldc + i;
         double du[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
@@ -201,20 +202,25 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
 #pragma unroll
         for (int k = 0; k < 36; k++)
             xj[k] = x[k * ldc + j];
+        dbl2 mp = {0.0, 0.0};
+        double tt[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int r = 0; r < 9; r++) {
-            double tt[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int c = 0; c < 9; c++) {
-                const double m = __builtin_nontemporal_load(Ms + (r * 9 + c) * ldc);
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    tt[v] += m * xj[c + 9 * v];
-            }
-            const double b = bmul * __builtin_nontemporal_load(bs + r * ldc);
+        for (int e = 0; e < 81; e++) {
+            const int r = e / 9, c = e % 9;
+            if (e % 2 == 0)
+                mp = __builtin_nontemporal_load(Ms + (e / 2) * ldc);
+            const double m = e % 2 == 0 ? mp.x : mp.y;
+            if (c == 0)
+                tt[0] = tt[1] = tt[2] = tt[3] = 0.0;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
+                tt[v] += m * xj[c + 9 * v];
+            if (c == 8) {
+                const double b = bmul * __builtin_nontemporal_load(bs + r * ldc);
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
+            }
         }
     }
     if (RESID) {
@@ -264,24 +270,29 @@ __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, con
                 continue;
             j = i; // far field
         }
-        const float *Ms = M32 + int64_t(s) * 81 * ldc + i;
+        // FP32 A^-1 B entries, four per 16-byte load (81..83 pad)
+        const flt4 *Ms = reinterpret_cast<const flt4 *>(M32) + int64_t(s) * kMQuads * ldc + i;
         float dj[36];
 #pragma unroll
         for (int k = 0; k < 36; k++)
             dj[k] = din[k * ldc + j];
+        flt4 mq = {0.f, 0.f, 0.f, 0.f};
+        float tt[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int r = 0; r < 9; r++) {
-            float tt[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int c = 0; c < 9; c++) {
-                const float m = __builtin_nontemporal_load(Ms + (r * 9 + c) * ldc);
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    tt[v] = fmaf(m, dj[c + 9 * v], tt[v]);
-            }
+        for (int e = 0; e < 81; e++) {
+            const int r = e / 9, c = e % 9;
+            if (e % 4 == 0)
+                mq = __builtin_nontemporal_load(Ms + (e / 4) * ldc);
+            const float m = mq[e % 4];
+            if (c == 0)
+                tt[0] = tt[1] = tt[2] = tt[3] = 0.f;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                nw[r + 9 * v] = fmaf(relax, tt[v], nw[r + 9 * v]);
+                tt[v] = fmaf(m, dj[c + 9 * v], tt[v]);
+            if (c == 8)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    nw[r + 9 * v] = fmaf(relax, tt[v], nw[r + 9 * v]);
         }
     }
 #pragma unroll
@@ -291,6 +302,20 @@ __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, con
             x[k * ldc + i] += double(d);
         else
             dout[k * ldc + i] = d;
+    }
+}
+
+// A^-1 B from FP64 pairs [slot][41][ldc][2] to FP32 quads [slot][21][ldc][4]
+__global__ __launch_bounds__(kBlock) void k_m_to_f32(int32_t nfmax, int64_t ldc, const double *__restrict__ M,
+                                                     float *__restrict__ M32) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(nfmax) * kMQuads * ldc)
+        return;
+    const int64_t i = t % ldc, sq = t / ldc, s = sq / kMQuads, q = sq % kMQuads;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const int k = int(4 * q) + w;
+        M32[t * 4 + w] = k < 81 ? float(M[((s * kMPairs + k / 2) * ldc + i) * 2 + k % 2]) : 0.f;
     }
 }
 
@@ -995,7 +1020,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         // ---- cell streams
         std::vector<int32_t> nbr(size_t(nfmax) * ldc, -2), sbc(size_t(nfmax) * ldc, 0), sface(size_t(nfmax) * ldc, -1);
         std::vector<int32_t> gat(size_t(nfmax) * ldc, -1), widx(size_t(nfmax) * ldc, -1);
-        std::vector<double> M(size_t(nfmax) * 81 * ldc, 0.0), bv(size_t(nfmax) * 9 * ldc, 0.0), vol(ldc, 1.0);
+        std::vector<double> M(size_t(nfmax) * kMPairs * 2 * ldc, 0.0), bv(size_t(nfmax) * 9 * ldc, 0.0), vol(ldc, 1.0);
         std::vector<int64_t> wall_cell, wall_face, wall_c2f;
         for (int64_t i = 0; i < nc; i++) {
             const int64_t b0 = mesh->cell_ptr[i];
@@ -1011,7 +1036,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
                 sbc[s * ldc + i] = mesh->face_bc[f];
                 sface[s * ldc + i] = int32_t(f);
                 for (int k = 0; k < 81; k++)
-                    M[(size_t(s) * 81 + k) * ldc + i] = vr->ainv_b[81 * (b0 + s) + k];
+                    M[((size_t(s) * kMPairs + k / 2) * ldc + i) * 2 + k % 2] = vr->ainv_b[81 * (b0 + s) + k];
                 for (int k = 0; k < 9; k++)
                     bv[(size_t(s) * 9 + k) * ldc + i] = vr->ainv_bvec[9 * (b0 + s) + k];
                 order.push_back({f, side});
@@ -1035,7 +1060,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         DNDSMI_TRY(c->upload(c->gat, gat));
         DNDSMI_TRY(c->alloc(c->wall_idx, nfmax * ldc));
         DNDSMI_TRY(c->upload(c->wall_idx, widx));
-        DNDSMI_TRY(c->alloc(c->M, int64_t(nfmax) * 81 * ldc));
+        DNDSMI_TRY(c->alloc(c->M, int64_t(nfmax) * kMPairs * 2 * ldc));
         DNDSMI_TRY(c->upload(c->M, M));
         M.clear(), M.shrink_to_fit();
         DNDSMI_TRY(c->alloc(c->bv, int64_t(nfmax) * 9 * ldc));
@@ -1431,14 +1456,15 @@ int ensure_mixed(dndsmi_ctx *c) {
         c->err = "the mixed-precision sweeps do not handle Wall_Euler / Wall_NoSlip faces";
         return DNDSMI_ERR_UNSUPPORTED;
     }
-    const int64_t nm = int64_t(c->nfmax) * 81 * c->ldc, nd = 54 * c->ldf;
+    const int64_t nm = int64_t(c->nfmax) * kMQuads * 4 * c->ldc, nd = 54 * c->ldf;
     DNDSMI_TRY(c->alloc(c->M32, nm));
     DNDSMI_TRY(c->alloc(c->fd0_32, nd));
     DNDSMI_TRY(c->alloc(c->r32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->dA32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->dB32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->fflux32, 12 * c->ldf));
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nm)), dim3(kBlock), 0, c->stream, nm, c->M, c->M32);
+    hipLaunchKernelGGL(k_m_to_f32, dim3(grid_for(nm / 4)), dim3(kBlock), 0, c->stream, int32_t(c->nfmax), c->ldc,
+                       c->M, c->M32);
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nd)), dim3(kBlock), 0, c->stream, nd, c->fd0, c->fd0_32);
     DNDSMI_TRY(c->launch_check("k_to_f32"));
     return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");

@@ -40,7 +40,7 @@ struct dndsmi_ctx {
     int32_t *sbc = nullptr;    // [nfmax][ldc] boundary kind of the slot's face
     int32_t *sface = nullptr;  // [nfmax][ldc] face index of the slot
     int32_t *gat = nullptr;    // [nfmax][ldc] (face << 1 | side) sorted by face, -1 = none
-    double *M = nullptr;       // [nfmax*81][ldc]
+    double *M = nullptr;       // [nfmax][41][ldc][2] A^-1 B entry pairs
     double *bv = nullptr;      // [nfmax*9][ldc]
     double *vol = nullptr;     // [ldc]
     int32_t *wall_idx = nullptr; // [nfmax][ldc] index into the wall-face arrays, -1 = none
@@ -97,7 +97,7 @@ struct dndsmi_ctx {
     double *jinv = nullptr;           // [ldc] 1 / LU-SGS diagonal
     // mixed precision (allocated on first use): FP32 copies of A^-1 B and of
     // the face bases, FP32 correction vectors, FP32 face fluxes
-    float *M32 = nullptr;             // [nfmax*81][ldc]
+    float *M32 = nullptr;             // [nfmax][21][ldc][4]
     float *fd0_32 = nullptr;          // [54][ldf]
     float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
     float *fflux32 = nullptr;         // [12][ldf]

@@ -3,7 +3,7 @@
 // Layouts (all device arrays padded to ld = n rounded up to 64):
 //   DOF arrays   u[cell][4]             (AoS, 32 B per cell)
 //   REC arrays   x[k][cell], k = r + 9 v (SoA: lane-contiguous per element)
-//   A^-1 B       M[(slot*81 + r*9 + c)][cell]
+//   A^-1 B       M[slot][e/2][cell][e%2], e = r*9 + c (pairs of doubles)
 //   A^-1 b       bv[(slot*9 + r)][cell]
 //   face rows    d0[(g*18 + side*9 + c)][face]
 // One thread owns one cell (sweep, gather) or one face (fluxes); the cell
@@ -38,12 +38,19 @@ __device__ inline double sqr(double x) { return x * x; }
 // faces that gather the same neighbours then meet in the same L2.  The map is
 // a bijection of [0, gridDim.x) for any grid size.
 constexpr int32_t kNumXCD = 8;
+
 __device__ __forceinline__ int32_t xcd_block() {
     const int32_t n = int32_t(gridDim.x), w = int32_t(blockIdx.x);
     const int32_t x = w % kNumXCD, q = w / kNumXCD;
     const int32_t base = n / kNumXCD, extra = n % kNumXCD; // XCD x < extra runs base + 1 blocks
     return x * base + (x < extra ? x : extra) + q;
 }
+
+// 16-byte vectors for streaming loads of the A^-1 B blocks: 81 entries per
+// slot stored as 41 FP64 pairs (FP64 path) or 21 FP32 quads (mixed path)
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+constexpr int kMPairs = 41, kMQuads = 21;
 
 struct GasParams {
     double gamma;
