@@ -32,6 +32,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
+PROFILE = os.path.join(ROOT, "profiles", "riemann_2M_kernels_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (None if absent)."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) == 7 and cols[0] == f"`{kernel}`":
+                    return float(cols[4]) * 1e9, os.path.relpath(PROFILE, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
@@ -144,6 +161,8 @@ def main():
         roof = {"bound": "hbm", "achieved": jac["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": jac["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": "k_jacobi",
                 "bytes_per_launch": jac["bytes"], "avg_ms": jac["avg_ms"]}
+        if args.case == "riemann" and args.n == 1024:
+            roof["traffic"], roof["traffic_source"] = measured_traffic("k_jacobi<3, false>")
 
     cpu = None
     if args.cpu_baseline:

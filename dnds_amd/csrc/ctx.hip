@@ -319,6 +319,20 @@ __global__ __launch_bounds__(kBlock) void k_m_to_f32(int32_t nfmax, int64_t ldc,
     }
 }
 
+// face bases from FP64 pairs [28][ldf][2] to FP32 quads [14][ldf][4] (56 per face)
+__global__ __launch_bounds__(kBlock) void k_fd0_to_f32(int64_t ldf, const double *__restrict__ d,
+                                                       float *__restrict__ d32) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 14 * ldf)
+        return;
+    const int64_t f = t % ldf, q = t / ldf;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const int64_t e = 4 * q + w;
+        d32[t * 4 + w] = float(d[((e / 2) * ldf + f) * 2 + e % 2]);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ src,
                                                    float *__restrict__ dst) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -461,36 +475,48 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     // reconstructed increments D0 . x at the 3 points, both sides
     double incL[3][4], incR[3][4];
     if (MIXED) {
-        float fL[3][4], fR[3][4];
+        // FP32 bases per face as [side][c][g] (28 per side) quads; e = 3 c + g
+        const flt4 *dq = reinterpret_cast<const flt4 *>(fd0_32) + f;
 #pragma unroll
         for (int g = 0; g < 3; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                fL[g][v] = 0.f, fR[g][v] = 0.f;
+                incR[g][v] = 0.0;
 #pragma unroll 1
-        for (int c = 0; c < 9; c++) {
-            float xl[4], xr[4];
+        for (int sd = 0; sd < (BOUNDARY ? 1 : 2); sd++) {
+            const int32_t cell = sd ? R : L;
+            float acc[3][4];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                xl[v] = float(x[(c + 9 * v) * ldc + L]);
-                xr[v] = BOUNDARY ? 0.f : float(x[(c + 9 * v) * ldc + R]);
+            for (int g = 0; g < 3; g++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[g][v] = 0.f;
+            float xc[4];
+            flt4 q4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int e = 0; e < 27; e++) {
+                const int c = e / 3, g = e % 3;
+                if (g == 0)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        xc[v] = float(x[(c + 9 * v) * ldc + cell]);
+                if (e % 4 == 0)
+                    q4 = __builtin_nontemporal_load(dq + (sd * 7 + e / 4) * ldf);
+                const float d = q4[e % 4];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[g][v] = fmaf(d, xc[v], acc[g][v]);
             }
 #pragma unroll
-            for (int g = 0; g < 3; g++) {
-                const float dl = __builtin_nontemporal_load(fd0_32 + (g * 18 + c) * ldf + f);
-                const float dr = BOUNDARY ? 0.f : __builtin_nontemporal_load(fd0_32 + (g * 18 + 9 + c) * ldf + f);
+            for (int g = 0; g < 3; g++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    fL[g][v] = fmaf(dl, xl[v], fL[g][v]);
-                    fR[g][v] = fmaf(dr, xr[v], fR[g][v]);
+                    if (sd)
+                        incR[g][v] = double(acc[g][v]);
+                    else
+                        incL[g][v] = double(acc[g][v]);
                 }
-            }
         }
-#pragma unroll
-        for (int g = 0; g < 3; g++)
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                incL[g][v] = double(fL[g][v]), incR[g][v] = double(fR[g][v]);
         if (!BOUNDARY)
 #pragma unroll
             for (int k = 0; k < 4; k++)
@@ -501,38 +527,47 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
 #pragma unroll
         for (int v = 0; v < 4; v++)
             incL[g][v] = 0.0, incR[g][v] = 0.0;
+    // bases per face as [side][c][g] (28 per side) pairs; e = 3 c + g
+    const dbl2 *dp = reinterpret_cast<const dbl2 *>(fd0) + f;
 #pragma unroll 1
-    for (int c = 0; c < 9; c++) {
-        double xc[4];
+    for (int sd = 0; sd < (BOUNDARY ? 1 : 2); sd++) {
+        const int32_t cell = sd ? R : L;
+        double inc[3][4];
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-            xc[v] = x[(c + 9 * v) * ldc + L];
-#pragma unroll
-        for (int g = 0; g < 3; g++) {
-            const double d = __builtin_nontemporal_load(fd0 + (g * 18 + c) * ldf + f);
+        for (int g = 0; g < 3; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                incL[g][v] += d * xc[v];
+                inc[g][v] = 0.0;
+        double xc[4];
+        dbl2 pr = {0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < 27; e++) {
+            const int c = e / 3, g = e % 3;
+            if (g == 0)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    xc[v] = x[(c + 9 * v) * ldc + cell];
+            if (e % 2 == 0)
+                pr = __builtin_nontemporal_load(dp + (sd * 14 + e / 2) * ldf);
+            const double d = e % 2 ? pr.y : pr.x;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                inc[g][v] += d * xc[v];
         }
+#pragma unroll
+        for (int g = 0; g < 3; g++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                if (sd)
+                    incR[g][v] = inc[g][v];
+                else
+                    incL[g][v] = inc[g][v];
+            }
     }
     if (!BOUNDARY) {
 #pragma unroll
         for (int k = 0; k < 4; k++)
             uR[k] = u[4 * int64_t(R) + k];
-#pragma unroll 1
-        for (int c = 0; c < 9; c++) {
-            double xc[4];
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                xc[v] = x[(c + 9 * v) * ldc + R];
-#pragma unroll
-            for (int g = 0; g < 3; g++) {
-                const double d = __builtin_nontemporal_load(fd0 + (g * 18 + 9 + c) * ldf + f);
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    incR[g][v] += d * xc[v];
-            }
-        }
     }
     }
     const double dl = dlf[f];
@@ -680,20 +715,27 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
 #pragma unroll
         for (int q = 0; q < 2; q++)
             b0L[r][q] = b0R[r][q] = b1L[r][q] = b1R[r][q] = 0.0;
+    // bases stored per face as [side][c][r] pairs: 5 16-byte loads per side and c
+    const dbl2 *dp = reinterpret_cast<const dbl2 *>(dg0) + f;
 #pragma unroll 1
     for (int c = 0; c < 9; c++) {
         const double xL[2] = {x[c * ldc + L], x[(c + 27) * ldc + L]};
         const double xR[2] = {x[c * ldc + R], x[(c + 27) * ldc + R]};
 #pragma unroll
-        for (int r = 0; r < 10; r++) {
-            const double d0 = __builtin_nontemporal_load(dg0 + (9 * r + c) * ldf + f);
-            const double d1 = __builtin_nontemporal_load(dg0 + (90 + 9 * r + c) * ldf + f);
+        for (int rp = 0; rp < 5; rp++) {
+            const dbl2 p0 = __builtin_nontemporal_load(dp + (c * 5 + rp) * ldf);
+            const dbl2 p1 = __builtin_nontemporal_load(dp + (45 + c * 5 + rp) * ldf);
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                b0L[r][q] += d0 * xL[q];
-                b0R[r][q] += d0 * xR[q];
-                b1R[r][q] += d1 * xR[q];
-                b1L[r][q] += d1 * xL[q];
+            for (int h = 0; h < 2; h++) {
+                const int r = 2 * rp + h;
+                const double d0 = h ? p0.y : p0.x, d1 = h ? p1.y : p1.x;
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    b0L[r][q] += d0 * xL[q];
+                    b0R[r][q] += d0 * xR[q];
+                    b1R[r][q] += d1 * xR[q];
+                    b1L[r][q] += d1 * xL[q];
+                }
             }
         }
     }
@@ -1104,15 +1146,19 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         }
         // ---- face streams
         std::vector<int32_t> f2c(2 * size_t(ldf), -1), fbc(ldf, DNDSMI_BC_INNER);
-        std::vector<double> fd0(54 * size_t(ldf), 0.0), fn(6 * size_t(ldf), 0.0), fnc(2 * size_t(ldf), 0.0),
+        std::vector<double> fd0(56 * size_t(ldf), 0.0), fn(6 * size_t(ldf), 0.0), fnc(2 * size_t(ldf), 0.0),
             farea(ldf, 0.0), fctr(2 * size_t(ldf), 0.0);
         for (int64_t f = 0; f < nf; f++) {
             f2c[f] = int32_t(mesh->face2cell[2 * f]);
             const int64_t R = mesh->face2cell[2 * f + 1];
             f2c[ldf + f] = R == DNDSMI_NO_CELL ? -1 : int32_t(R);
             fbc[f] = mesh->face_bc[f];
-            for (int k = 0; k < 54; k++)
-                fd0[k * ldf + f] = vr->face_d0[54 * f + k];
+            for (int g = 0; g < 3; g++) // [side][c][g], 28 per side, as pairs
+                for (int sd = 0; sd < 2; sd++)
+                    for (int cc = 0; cc < 9; cc++) {
+                        const int e = sd * 28 + cc * 3 + g;
+                        fd0[(size_t(e / 2) * ldf + f) * 2 + e % 2] = vr->face_d0[54 * f + g * 18 + sd * 9 + cc];
+                    }
             for (int k = 0; k < 6; k++)
                 fn[k * ldf + f] = vr->face_norm[6 * f + k];
             fnc[f] = vr->face_norm_center[2 * f], fnc[ldf + f] = vr->face_norm_center[2 * f + 1];
@@ -1123,7 +1169,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         DNDSMI_TRY(c->upload(c->f2c, f2c));
         DNDSMI_TRY(c->alloc(c->fbc, ldf));
         DNDSMI_TRY(c->upload(c->fbc, fbc));
-        DNDSMI_TRY(c->alloc(c->fd0, 54 * ldf));
+        DNDSMI_TRY(c->alloc(c->fd0, 56 * ldf));
         DNDSMI_TRY(c->upload(c->fd0, fd0));
         DNDSMI_TRY(c->alloc(c->fn, 6 * ldf));
         DNDSMI_TRY(c->upload(c->fn, fn));
@@ -1151,8 +1197,12 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             c->has_limiter = true;
             std::vector<double> buf(180 * size_t(ldf), 0.0);
             for (int64_t f = 0; f < nf; f++)
-                for (int k = 0; k < 180; k++)
-                    buf[k * ldf + f] = vr->face_dibj_g0[180 * f + k];
+                for (int sd = 0; sd < 2; sd++)
+                    for (int r = 0; r < 10; r++)
+                        for (int cc = 0; cc < 9; cc++) {
+                            const int e = sd * 90 + cc * 10 + r; // [side][c][r]
+                            buf[(size_t(e / 2) * ldf + f) * 2 + e % 2] = vr->face_dibj_g0[180 * f + sd * 90 + r * 9 + cc];
+                        }
             DNDSMI_TRY(c->alloc(c->dg0, 180 * ldf));
             DNDSMI_TRY(c->upload(c->dg0, buf));
             buf.assign(162 * size_t(ldf), 0.0);
@@ -1456,7 +1506,7 @@ int ensure_mixed(dndsmi_ctx *c) {
         c->err = "the mixed-precision sweeps do not handle Wall_Euler / Wall_NoSlip faces";
         return DNDSMI_ERR_UNSUPPORTED;
     }
-    const int64_t nm = int64_t(c->nfmax) * kMQuads * 4 * c->ldc, nd = 54 * c->ldf;
+    const int64_t nm = int64_t(c->nfmax) * kMQuads * 4 * c->ldc, nd = 56 * c->ldf;
     DNDSMI_TRY(c->alloc(c->M32, nm));
     DNDSMI_TRY(c->alloc(c->fd0_32, nd));
     DNDSMI_TRY(c->alloc(c->r32, 36 * c->ldc));
@@ -1465,7 +1515,7 @@ int ensure_mixed(dndsmi_ctx *c) {
     DNDSMI_TRY(c->alloc(c->fflux32, 12 * c->ldf));
     hipLaunchKernelGGL(k_m_to_f32, dim3(grid_for(nm / 4)), dim3(kBlock), 0, c->stream, int32_t(c->nfmax), c->ldc,
                        c->M, c->M32);
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nd)), dim3(kBlock), 0, c->stream, nd, c->fd0, c->fd0_32);
+    hipLaunchKernelGGL(k_fd0_to_f32, dim3(grid_for(nd / 4)), dim3(kBlock), 0, c->stream, c->ldf, c->fd0, c->fd0_32);
     DNDSMI_TRY(c->launch_check("k_to_f32"));
     return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }

@@ -56,7 +56,7 @@ struct dndsmi_ctx {
     // face side
     int32_t *f2c = nullptr;    // [2][ldf]
     int32_t *fbc = nullptr;    // [ldf]
-    double *fd0 = nullptr;     // [54][ldf]
+    double *fd0 = nullptr;     // [28][ldf][2] value bases at the 3 points, per face [side][c][g], 28 per side
     double *fn = nullptr;      // [6][ldf] normals at the LINE_3 points
     double *fnc = nullptr;     // [2][ldf] normal at the face centre
     double *farea = nullptr;   // [ldf]
@@ -65,7 +65,7 @@ struct dndsmi_ctx {
     int64_t n_bface = 0;
     // WBAP data (present when the setup carries limiter data)
     bool has_limiter = false;
-    double *dg0 = nullptr;     // [180][ldf] all DiBj rows at LINE_3 point 0, both sides
+    double *dg0 = nullptr;     // [90][ldf][2] all DiBj rows at LINE_3 point 0, per face [side][c][r]
     double *fw = nullptr;      // [10][ldf] face weights
     double *msec = nullptr;    // [162][ldf] matrixSecondary (R2L, L2R)
     double *wsum = nullptr;    // [8][ldf] per-face indicator terms of both cells
@@ -98,7 +98,7 @@ struct dndsmi_ctx {
     // mixed precision (allocated on first use): FP32 copies of A^-1 B and of
     // the face bases, FP32 correction vectors, FP32 face fluxes
     float *M32 = nullptr;             // [nfmax][21][ldc][4]
-    float *fd0_32 = nullptr;          // [54][ldf]
+    float *fd0_32 = nullptr;          // [14][ldf][4]
     float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
     float *fflux32 = nullptr;         // [12][ldf]
     // implicit dual-time scratch (allocated on first use)
