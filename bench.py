@@ -167,19 +167,23 @@ def main():
     cpu = None
     if args.cpu_baseline:
         import oracle
-        log("CPU baseline: oracle (1 thread), one SSP-RK3 step on the same mesh")
         st = case.vfv.setting
         orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2 if case.riemann == "HLLC" else 1,
                             far_field=case.far, tang_weight=st.tang_weight,
                             tang_weight_direction=st.tang_weight_direction,
                             wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std)
-        c0 = time.perf_counter()
-        orc.ssprk3_steps(case.u0, urec0, 1, args.sweeps, args.cfl, use_limiter=case.limiter)
-        c1 = time.perf_counter()
-        cpu = {"value": nc * 3 / (c1 - c0), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps + RHS) on the same {nc}-cell mesh, "
-                         f"{c1 - c0:.1f}s"}
-        log(f"CPU baseline {cpu['value']:.3e} cell-updates/s ({c1 - c0:.1f}s)")
+        timed = {}
+        for threads in (args.threads, 1):
+            oracle.set_threads(threads)
+            log(f"CPU baseline: oracle on {threads} thread(s), one SSP-RK3 step on the same mesh")
+            c0 = time.perf_counter()
+            orc.ssprk3_steps(case.u0, urec0, 1, args.sweeps, args.cfl, use_limiter=case.limiter)
+            timed[threads] = time.perf_counter() - c0
+            log(f"CPU baseline {nc * 3 / timed[threads]:.3e} cell-updates/s ({timed[threads]:.1f}s)")
+        cpu = {"value": nc * 3 / timed[args.threads], "unit": "cell-updates/s", "cores": args.threads, "kind": "port",
+               "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps + WBAP + RHS) on the same {nc}-cell "
+                         f"mesh: {timed[args.threads]:.1f}s on {args.threads} threads, {timed[1]:.1f}s on 1",
+               "value_1_core": nc * 3 / timed[1]}
 
     warm = None
     if args.warmstart:

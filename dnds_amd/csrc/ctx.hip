@@ -12,6 +12,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "ctx.hpp"
@@ -611,12 +612,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     if (MIXED) {
 #pragma unroll
         for (int k = 0; k < 12; k++)
-            fflux32[k * ldf + f] = float(acc[k]);
+            fflux32[12 * int64_t(f) + k] = float(acc[k]);
         return;
     }
 #pragma unroll
     for (int k = 0; k < 12; k++)
-        fflux[k * ldf + f] = acc[k];
+        fflux[12 * int64_t(f) + k] = acc[k];
 }
 
 // EvaluateRHS scatter as a per-cell gather in ascending face order
@@ -648,17 +649,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
         if ((g & 1) == 0) {
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] += fflux[v * ldf + f] / V;
+                r[v] += fflux[12 * int64_t(f) + v] / V;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] -= fflux[(4 + v) * ldf + f] / V;
+                r[v] -= fflux[12 * int64_t(f) + 4 + v] / V;
         } else {
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] -= fflux[v * ldf + f] / V;
+                r[v] -= fflux[12 * int64_t(f) + v] / V;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] += fflux[(8 + v) * ldf + f] / V;
+                r[v] += fflux[12 * int64_t(f) + 8 + v] / V;
         }
     }
     bool finite = true;
@@ -775,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
         }
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            wsum[(4 * cs + k) * ldf + f] = out[k];
+            wsum[8 * int64_t(f) + 4 * cs + k] = out[k];
     }
 }
 
@@ -806,7 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, in
             const int side = f2c[f] == i ? 0 : 1;
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                sum[k] += wsum[(4 * side + k) * ldf + f];
+                sum[k] += wsum[8 * int64_t(f) + 4 * side + k];
         }
         const double si0 = sum[0] / (sum[1] + kVerySmallReal);
         const double si1 = sum[2] / (sum[3] + kVerySmallReal);
@@ -1059,6 +1060,38 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     c->nfmax = nfmax;
     const int64_t ldc = c->ldc, ldf = c->ldf;
     try {
+        // ---- device numbering: cells in vertical strips about 32 cells wide,
+        // bottom to top within a strip (by barycentre), faces by their first
+        // cell's device index.  A cell's neighbours are then at most about a
+        // strip row away in the order, so the sweeps and gathers find them
+        // still in L2, while consecutive cells stay next to each other.
+        std::vector<int32_t> &c2d = c->c2d, &f2d = c->f2d;
+        {
+            double xmin = 1e300, ymin = 1e300, vsum = 0;
+            for (int64_t i = 0; i < nc; i++) {
+                xmin = std::min(xmin, vr->cell_bary[2 * i]);
+                ymin = std::min(ymin, vr->cell_bary[2 * i + 1]);
+                vsum += vr->volume[i];
+            }
+            const double h = std::sqrt(vsum / double(nc)); // typical cell size
+            const double width = 32.0 * h;
+            std::vector<std::tuple<int64_t, double, double, int64_t>> key(static_cast<size_t>(nc));
+            for (int64_t i = 0; i < nc; i++) {
+                const double bx = vr->cell_bary[2 * i], by = vr->cell_bary[2 * i + 1];
+                key[i] = {int64_t((bx - xmin) / width), by, bx, i};
+            }
+            std::sort(key.begin(), key.end());
+            c2d.assign(static_cast<size_t>(nc), 0);
+            for (int64_t d = 0; d < nc; d++)
+                c2d[std::get<3>(key[d])] = int32_t(d);
+            std::vector<std::pair<int64_t, int64_t>> fk(static_cast<size_t>(nf));
+            for (int64_t f = 0; f < nf; f++)
+                fk[f] = {c2d[mesh->face2cell[2 * f]], f};
+            std::sort(fk.begin(), fk.end());
+            f2d.assign(static_cast<size_t>(nf), 0);
+            for (int64_t d = 0; d < nf; d++)
+                f2d[fk[d].second] = int32_t(d);
+        }
         // ---- cell streams
         std::vector<int32_t> nbr(size_t(nfmax) * ldc, -2), sbc(size_t(nfmax) * ldc, 0), sface(size_t(nfmax) * ldc, -1);
         std::vector<int32_t> gat(size_t(nfmax) * ldc, -1), widx(size_t(nfmax) * ldc, -1);
@@ -1067,30 +1100,33 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         for (int64_t i = 0; i < nc; i++) {
             const int64_t b0 = mesh->cell_ptr[i];
             const int n = int(mesh->cell_ptr[i + 1] - b0);
-            vol[i] = vr->volume[i];
+            const int64_t ci = c2d[i]; // device column of mesh cell i
+            vol[ci] = vr->volume[i];
             std::vector<std::pair<int64_t, int>> order;
             for (int s = 0; s < n; s++) {
                 const int64_t f = mesh->cell2face[b0 + s];
                 const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
                 const int side = L == i ? 0 : 1;
                 const int64_t j = side == 0 ? R : L;
-                nbr[s * ldc + i] = j == DNDSMI_NO_CELL ? -1 : int32_t(j);
-                sbc[s * ldc + i] = mesh->face_bc[f];
-                sface[s * ldc + i] = int32_t(f);
+                nbr[s * ldc + ci] = j == DNDSMI_NO_CELL ? -1 : c2d[j];
+                sbc[s * ldc + ci] = mesh->face_bc[f];
+                sface[s * ldc + ci] = f2d[f];
                 for (int k = 0; k < 81; k++)
-                    M[((size_t(s) * kMPairs + k / 2) * ldc + i) * 2 + k % 2] = vr->ainv_b[81 * (b0 + s) + k];
+                    M[((size_t(s) * kMPairs + k / 2) * ldc + ci) * 2 + k % 2] = vr->ainv_b[81 * (b0 + s) + k];
                 for (int k = 0; k < 9; k++)
-                    bv[(size_t(s) * 9 + k) * ldc + i] = vr->ainv_bvec[9 * (b0 + s) + k];
+                    bv[(size_t(s) * 9 + k) * ldc + ci] = vr->ainv_bvec[9 * (b0 + s) + k];
                 order.push_back({f, side});
                 if (j == DNDSMI_NO_CELL &&
                     (mesh->face_bc[f] == DNDSMI_BC_WALL_EULER || mesh->face_bc[f] == DNDSMI_BC_WALL_NOSLIP)) {
-                    widx[s * ldc + i] = int32_t(wall_cell.size());
+                    widx[s * ldc + ci] = int32_t(wall_cell.size());
                     wall_cell.push_back(i), wall_face.push_back(f), wall_c2f.push_back(b0 + s);
                 }
             }
+            // the gather visits the faces in the mesh's face order (the
+            // reference's scatter order), whatever their device numbers
             std::sort(order.begin(), order.end());
             for (int s = 0; s < n; s++)
-                gat[s * ldc + i] = int32_t((order[s].first << 1) | order[s].second);
+                gat[s * ldc + ci] = int32_t((int64_t(f2d[order[s].first]) << 1) | order[s].second);
         }
         DNDSMI_TRY(c->alloc(c->nbr, nfmax * ldc));
         DNDSMI_TRY(c->upload(c->nbr, nbr));
@@ -1136,7 +1172,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             DNDSMI_TRY(c->upload(c->wall_w0, w0));
             std::vector<int32_t> wc(size_t(c->n_wall), 0), wb(size_t(c->n_wall), 0);
             for (int64_t w = 0; w < nw; w++)
-                wc[w] = int32_t(wall_cell[w]), wb[w] = mesh->face_bc[wall_face[w]];
+                wc[w] = c2d[wall_cell[w]], wb[w] = mesh->face_bc[wall_face[w]];
             DNDSMI_TRY(c->alloc(c->wall_cell, c->n_wall));
             DNDSMI_TRY(c->upload(c->wall_cell, wc));
             DNDSMI_TRY(c->alloc(c->wall_bc, c->n_wall));
@@ -1148,22 +1184,23 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         std::vector<int32_t> f2c(2 * size_t(ldf), -1), fbc(ldf, DNDSMI_BC_INNER);
         std::vector<double> fd0(56 * size_t(ldf), 0.0), fn(6 * size_t(ldf), 0.0), fnc(2 * size_t(ldf), 0.0),
             farea(ldf, 0.0), fctr(2 * size_t(ldf), 0.0);
-        for (int64_t f = 0; f < nf; f++) {
-            f2c[f] = int32_t(mesh->face2cell[2 * f]);
-            const int64_t R = mesh->face2cell[2 * f + 1];
-            f2c[ldf + f] = R == DNDSMI_NO_CELL ? -1 : int32_t(R);
-            fbc[f] = mesh->face_bc[f];
+        for (int64_t fo = 0; fo < nf; fo++) {
+            const int64_t f = f2d[fo]; // device column of mesh face fo
+            f2c[f] = c2d[mesh->face2cell[2 * fo]];
+            const int64_t R = mesh->face2cell[2 * fo + 1];
+            f2c[ldf + f] = R == DNDSMI_NO_CELL ? -1 : c2d[R];
+            fbc[f] = mesh->face_bc[fo];
             for (int g = 0; g < 3; g++) // [side][c][g], 28 per side, as pairs
                 for (int sd = 0; sd < 2; sd++)
                     for (int cc = 0; cc < 9; cc++) {
                         const int e = sd * 28 + cc * 3 + g;
-                        fd0[(size_t(e / 2) * ldf + f) * 2 + e % 2] = vr->face_d0[54 * f + g * 18 + sd * 9 + cc];
+                        fd0[(size_t(e / 2) * ldf + f) * 2 + e % 2] = vr->face_d0[54 * fo + g * 18 + sd * 9 + cc];
                     }
             for (int k = 0; k < 6; k++)
-                fn[k * ldf + f] = vr->face_norm[6 * f + k];
-            fnc[f] = vr->face_norm_center[2 * f], fnc[ldf + f] = vr->face_norm_center[2 * f + 1];
-            farea[f] = vr->face_area[f];
-            fctr[f] = vr->face_center[2 * f], fctr[ldf + f] = vr->face_center[2 * f + 1];
+                fn[k * ldf + f] = vr->face_norm[6 * fo + k];
+            fnc[f] = vr->face_norm_center[2 * fo], fnc[ldf + f] = vr->face_norm_center[2 * fo + 1];
+            farea[f] = vr->face_area[fo];
+            fctr[f] = vr->face_center[2 * fo], fctr[ldf + f] = vr->face_center[2 * fo + 1];
         }
         DNDSMI_TRY(c->alloc(c->f2c, 2 * ldf));
         DNDSMI_TRY(c->upload(c->f2c, f2c));
@@ -1182,7 +1219,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         std::vector<int32_t> bfl;
         for (int64_t f = 0; f < nf; f++)
             if (mesh->face2cell[2 * f + 1] == DNDSMI_NO_CELL)
-                bfl.push_back(int32_t(f));
+                bfl.push_back(f2d[f]);
+        std::sort(bfl.begin(), bfl.end());
         c->n_bface = int64_t(bfl.size());
         if (bfl.empty())
             bfl.push_back(0);
@@ -1196,25 +1234,26 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         if (vr->face_dibj_g0 && vr->matrix_secondary) {
             c->has_limiter = true;
             std::vector<double> buf(180 * size_t(ldf), 0.0);
-            for (int64_t f = 0; f < nf; f++)
+            for (int64_t fo = 0; fo < nf; fo++)
                 for (int sd = 0; sd < 2; sd++)
                     for (int r = 0; r < 10; r++)
                         for (int cc = 0; cc < 9; cc++) {
                             const int e = sd * 90 + cc * 10 + r; // [side][c][r]
-                            buf[(size_t(e / 2) * ldf + f) * 2 + e % 2] = vr->face_dibj_g0[180 * f + sd * 90 + r * 9 + cc];
+                            buf[(size_t(e / 2) * ldf + f2d[fo]) * 2 + e % 2] =
+                                vr->face_dibj_g0[180 * fo + sd * 90 + r * 9 + cc];
                         }
             DNDSMI_TRY(c->alloc(c->dg0, 180 * ldf));
             DNDSMI_TRY(c->upload(c->dg0, buf));
             buf.assign(162 * size_t(ldf), 0.0);
             for (int64_t f = 0; f < nf; f++)
                 for (int k = 0; k < 162; k++)
-                    buf[k * ldf + f] = vr->matrix_secondary[162 * f + k];
+                    buf[k * ldf + f2d[f]] = vr->matrix_secondary[162 * f + k];
             DNDSMI_TRY(c->alloc(c->msec, 162 * ldf));
             DNDSMI_TRY(c->upload(c->msec, buf));
             buf.assign(10 * size_t(ldf), 0.0);
             for (int64_t f = 0; f < nf; f++)
                 for (int k = 0; k < 10; k++)
-                    buf[k * ldf + f] = vr->face_weight[10 * f + k];
+                    buf[k * ldf + f2d[f]] = vr->face_weight[10 * f + k];
             DNDSMI_TRY(c->alloc(c->fw, 10 * ldf));
             DNDSMI_TRY(c->upload(c->fw, buf));
             DNDSMI_TRY(c->alloc(c->wsum, 8 * ldf));
@@ -1229,11 +1268,13 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         {
             std::vector<int32_t> col(size_t(nc), -1);
             int32_t ncol = 0;
-            for (int64_t i = 0; i < nc; i++) {
+            for (int64_t i = 0; i < nc; i++) { // mesh numbering
                 uint32_t used = 0;
-                for (int s = 0; s < nfmax; s++) {
-                    const int32_t j = nbr[s * ldc + i];
-                    if (j >= 0 && j < i && col[j] < 32)
+                for (int64_t q = mesh->cell_ptr[i]; q < mesh->cell_ptr[i + 1]; q++) {
+                    const int64_t f = mesh->cell2face[q];
+                    const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
+                    const int64_t j = L == i ? R : L;
+                    if (j != DNDSMI_NO_CELL && j < i && col[j] < 32)
                         used |= 1u << col[j];
                 }
                 int32_t k = 0;
@@ -1249,10 +1290,11 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             for (int32_t k = 0; k < ncol; k++)
                 c->col_start[k + 1] += c->col_start[k];
             std::vector<int32_t> cells(static_cast<size_t>(nc)), fill(c->col_start.begin(), c->col_start.end() - 1);
-            for (int64_t i = 0; i < nc; i++)
-                cells[fill[col[i]]++] = int32_t(i);
+            for (int64_t i = 0; i < nc; i++) // ascending mesh index within a colour
+                cells[fill[col[i]]++] = c2d[i];
             std::vector<int32_t> colp(size_t(ldc), 0);
-            std::copy(col.begin(), col.end(), colp.begin());
+            for (int64_t i = 0; i < nc; i++)
+                colp[c2d[i]] = col[i];
             c->colour_host = col;
             DNDSMI_TRY(c->alloc(c->colour, ldc));
             DNDSMI_TRY(c->upload(c->colour, colp));
@@ -1351,16 +1393,20 @@ extern "C" int dndsmi_array_upload(dndsmi_array *a, const double *host) {
         return DNDSMI_ERR_INVALID_ARGUMENT;
     dndsmi_ctx *c = a->ctx;
     const int64_t nc = c->nc, ldc = c->ldc;
+    const std::vector<int32_t> &c2d = c->c2d;
     if (a->kind == DNDSMI_ARRAY_REC) {
         std::vector<double> buf(36 * size_t(ldc), 0.0);
         for (int64_t i = 0; i < nc; i++)
             for (int k = 0; k < 36; k++)
-                buf[k * ldc + i] = host[36 * i + k];
+                buf[k * ldc + c2d[i]] = host[36 * i + k];
         return c->upload(a->d, buf);
     }
-    const int64_t n = a->kind == DNDSMI_ARRAY_DOF ? 4 * nc : nc;
-    DNDSMI_TRY(c->hip(hipMemcpyAsync(a->d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream), "upload"));
-    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const int w = a->kind == DNDSMI_ARRAY_DOF ? 4 : 1;
+    std::vector<double> buf(size_t(w) * ldc, 0.0);
+    for (int64_t i = 0; i < nc; i++)
+        for (int v = 0; v < w; v++)
+            buf[size_t(w) * c2d[i] + v] = host[w * i + v];
+    return c->upload(a->d, buf);
 }
 
 extern "C" int dndsmi_array_download(const dndsmi_array *a, double *host) {
@@ -1376,12 +1422,18 @@ extern "C" int dndsmi_array_download(const dndsmi_array *a, double *host) {
         DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
         for (int64_t i = 0; i < nc; i++)
             for (int k = 0; k < 36; k++)
-                host[36 * i + k] = buf[k * ldc + i];
+                host[36 * i + k] = buf[k * ldc + c->c2d[i]];
         return DNDSMI_OK;
     }
-    const int64_t n = a->kind == DNDSMI_ARRAY_DOF ? 4 * nc : nc;
-    DNDSMI_TRY(c->hip(hipMemcpyAsync(host, a->d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "download"));
-    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const int w = a->kind == DNDSMI_ARRAY_DOF ? 4 : 1;
+    std::vector<double> buf(size_t(w) * ldc);
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(buf.data(), a->d, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, c->stream),
+                      "download"));
+    DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    for (int64_t i = 0; i < nc; i++)
+        for (int v = 0; v < w; v++)
+            host[w * i + v] = buf[size_t(w) * c->c2d[i] + v];
+    return DNDSMI_OK;
 }
 
 extern "C" int dndsmi_array_copy(dndsmi_array *dst, const dndsmi_array *src) {

@@ -29,6 +29,9 @@ struct dndsmi_ctx {
     int device = 0;
     std::string err;
     int64_t nc = 0, nf = 0, ldc = 0, ldf = 0;
+    // device numbering of the mesh's cells (vertical strips of the cell
+    // barycentres) and faces; host arrays keep the mesh numbering
+    std::vector<int32_t> c2d, f2d;
     int nfmax = 3;
     GasParams gas{};
     double relax = 1.0;
@@ -68,7 +71,7 @@ struct dndsmi_ctx {
     double *dg0 = nullptr;     // [90][ldf][2] all DiBj rows at LINE_3 point 0, per face [side][c][r]
     double *fw = nullptr;      // [10][ldf] face weights
     double *msec = nullptr;    // [162][ldf] matrixSecondary (R2L, L2R)
-    double *wsum = nullptr;    // [8][ldf] per-face indicator terms of both cells
+    double *wsum = nullptr;    // [ldf][8] per-face indicator terms of both cells
     double *uF = nullptr;      // [nfmax*28][ldc] WBAP per-cell face scratch
     double *ind = nullptr;     // [ldc] indicator when the caller passes none
     double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps
@@ -80,7 +83,7 @@ struct dndsmi_ctx {
     // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
     double *lambda_face = nullptr;
     double *dlambda_face = nullptr;
-    double *fflux = nullptr;   // [12][ldf] F, FLfix, FRfix integrated over the face
+    double *fflux = nullptr;   // [ldf][12] F, FLfix, FRfix integrated over the face (AoS: the gather reads 64 B per slot)
     double *dt_part = nullptr; // [grid] partial minima
     double *dt_min = nullptr;  // [1]
     int32_t *errw = nullptr;   // [1] error bits
@@ -100,7 +103,7 @@ struct dndsmi_ctx {
     float *M32 = nullptr;             // [nfmax][21][ldc][4]
     float *fd0_32 = nullptr;          // [14][ldf][4]
     float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
-    float *fflux32 = nullptr;         // [12][ldf]
+    float *fflux32 = nullptr;         // [ldf][12]
     // implicit dual-time scratch (allocated on first use)
     double *imp_rhs = nullptr;        // [ldc][4]
     double *imp_xinc = nullptr;       // [ldc][4]

@@ -61,24 +61,31 @@ enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3 };
 // 2 x 2 MFMA 16x16x16 blocks; K in steps of 16 staged through LDS.  Operand
 // layout of v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + 0..3]
 // and B[4 (l / 16) + 0..3][l % 16]; it receives D[4 (l / 16) + 0..3][l % 16].
+//
+// Split K (EPI_STORE only): block z sums k in [z kchunk, (z + 1) kchunk) and
+// writes its partial product to C + z M ldcm; k_splitk_sum adds the partials
+// in a fixed order.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, const float *__restrict__ A,
                                               int64_t lda, int32_t transA, const float *__restrict__ B, int64_t ldb,
                                               int32_t transB, float *__restrict__ C, int64_t ldcm,
                                               const float *__restrict__ bias, float *__restrict__ C2,
-                                              const float *__restrict__ Zin) {
+                                              const float *__restrict__ Zin, int32_t kchunk) {
     __shared__ unsigned short As[64][16 + 4];
     __shared__ unsigned short Bs[16][64 + 4];
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int64_t m0 = int64_t(blockIdx.x) * 64, n0 = int64_t(blockIdx.y) * 64;
     const int wm = 32 * (wave / 2), wn = 32 * (wave % 2);
+    const int32_t kbeg = int32_t(blockIdx.z) * kchunk, kend = min(K, kbeg + kchunk);
+    if (EPI == EPI_STORE)
+        C += int64_t(blockIdx.z) * M * ldcm;
     floatx4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
             acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int32_t k0 = 0; k0 < K; k0 += 16) {
+    for (int32_t k0 = kbeg; k0 < kend; k0 += 16) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int e = tid + 256 * q;
@@ -86,7 +93,7 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
                 const int r = e / 16, kk = e % 16;
                 const int64_t m = m0 + r, k = k0 + kk;
                 float v = 0.f;
-                if (m < M && k < K)
+                if (m < M && k < kend)
                     v = transA ? A[k * lda + m] : A[m * lda + k];
                 As[r][kk] = to_bf16(v);
             }
@@ -94,7 +101,7 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
                 const int kk = e / 64, c = e % 64;
                 const int64_t n = n0 + c, k = k0 + kk;
                 float v = 0.f;
-                if (n < N && k < K)
+                if (n < N && k < kend)
                     v = transB ? B[n * ldb + k] : B[k * ldb + n];
                 Bs[kk][c] = to_bf16(v);
             }
@@ -146,6 +153,18 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
                     C[o] = v * gelu_grad(Zin[o]);
                 }
             }
+}
+
+// C[i] = sum over z of part[z][i], in order
+__global__ __launch_bounds__(256) void k_splitk_sum(int64_t n, int32_t nz, const float *__restrict__ part,
+                                                    float *__restrict__ C) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float acc = 0.f;
+    for (int32_t z = 0; z < nz; z++)
+        acc += part[int64_t(z) * n + i];
+    C[i] = acc;
 }
 
 // column sums of X [rows][n] (bias gradients), one thread per column, fixed order
@@ -319,6 +338,7 @@ struct dndsmi_warmstart {
     int32_t *idx = nullptr;
     float *fX = nullptr, *fS = nullptr; // features / scales of all cells
     double *dpart = nullptr;
+    float *kpart = nullptr; // split-K partials [64][128][128]
     std::mt19937_64 rng;
 
     template <class T>
@@ -366,7 +386,23 @@ void gemm(hipStream_t s, int32_t M, int32_t N, int32_t K, const float *A, int64_
           int64_t ldb, int tb, float *C, int64_t ldc, const float *bias = nullptr, float *C2 = nullptr,
           const float *Zin = nullptr) {
     dim3 grid((M + 63) / 64, (N + 63) / 64);
-    hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, M, N, K, A, lda, ta, B, ldb, tb, C, ldc, bias, C2, Zin);
+    hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, M, N, K, A, lda, ta, B, ldb, tb, C, ldc, bias, C2, Zin,
+                       K);
+}
+
+constexpr int kSplitChunk = 256, kMaxSplit = 64;
+
+// weight gradients C[M][N] = A^T B over a long K (the batch): split K over
+// up to 64 blocks per tile, then a fixed-order sum (deterministic)
+void gemm_wgrad(hipStream_t s, int32_t M, int32_t N, int32_t K, const float *A, int64_t lda, const float *B,
+                int64_t ldb, float *C, float *part) {
+    const int32_t chunk = std::max<int32_t>(kSplitChunk, (K + kMaxSplit - 1) / kMaxSplit / 16 * 16 + 16);
+    const int32_t nz = (K + chunk - 1) / chunk;
+    dim3 grid((M + 63) / 64, (N + 63) / 64, nz);
+    hipLaunchKernelGGL(k_gemm<EPI_STORE>, grid, dim3(256), 0, s, M, N, K, A, lda, 1, B, ldb, 0, part, int64_t(N),
+                       nullptr, nullptr, nullptr, chunk);
+    const int64_t n = int64_t(M) * N;
+    hipLaunchKernelGGL(k_splitk_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, nz, part, C);
 }
 
 // forward pass of `rows` rows of X into Z1, H1, Z2, H2, Y
@@ -388,15 +424,15 @@ int loss_grad(dndsmi_warmstart *w, const float *X, const float *T, int32_t rows,
     const unsigned nb = unsigned((ny + 255) / 256);
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, ny, w->Y, T, float(2.0 / double(ny)), w->dY, w->part);
     // layer 3
-    gemm<EPI_STORE>(s, kHid, kOut, rows, w->H2, kHid, 1, w->dY, kOut, 0, G + oW3, kOut);
+    gemm_wgrad(s, kHid, kOut, rows, w->H2, kHid, w->dY, kOut, G + oW3, w->kpart);
     hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kOut, w->dY, G + ob3);
     gemm<EPI_GELU_GRAD>(s, rows, kHid, kOut, w->dY, kOut, 0, P + oW3, kOut, 1, w->dZ2, kHid, nullptr, nullptr, w->Z2);
     // layer 2
-    gemm<EPI_STORE>(s, kHid, kHid, rows, w->H1, kHid, 1, w->dZ2, kHid, 0, G + oW2, kHid);
+    gemm_wgrad(s, kHid, kHid, rows, w->H1, kHid, w->dZ2, kHid, G + oW2, w->kpart);
     hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kHid, w->dZ2, G + ob2);
     gemm<EPI_GELU_GRAD>(s, rows, kHid, kHid, w->dZ2, kHid, 0, P + oW2, kHid, 1, w->dZ1, kHid, nullptr, nullptr, w->Z1);
     // layer 1
-    gemm<EPI_STORE>(s, kIn, kHid, rows, X, kIn, 1, w->dZ1, kHid, 0, G + oW1, kHid);
+    gemm_wgrad(s, kIn, kHid, rows, X, kIn, w->dZ1, kHid, G + oW1, w->kpart);
     hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kHid, w->dZ1, G + ob1);
     DNDSMI_TRY(w->ctx->launch_check("warm-start backward"));
     if (loss) {
@@ -474,17 +510,18 @@ extern "C" int dndsmi_warmstart_create(dndsmi_ctx *c, const dndsmi_mesh_view *me
                 }
             ring = next;
         }
+        const int64_t ci = c->c2d[i]; // device numbering
         for (int k = 0; k < kSten; k++)
-            sten[k * ldc + i] = int32_t(k < int(got.size()) ? got[k] : i);
+            sten[k * ldc + ci] = c->c2d[k < int(got.size()) ? got[k] : i];
         for (int k = 0; k < 9; k++)
-            mom[k * ldc + i] = float(vr->base_moment[10 * i + 1 + k]);
-        lsz[i] = float(0.5 * std::log(vr->volume[i]));
+            mom[k * ldc + ci] = float(vr->base_moment[10 * i + 1 + k]);
+        lsz[ci] = float(0.5 * std::log(vr->volume[i]));
     }
     try {
         if (w->alloc(w->sten, kSten * ldc) || w->alloc(w->mom, 9 * ldc) || w->alloc(w->logsz, ldc) ||
             w->alloc(w->P, kNParam) || w->alloc(w->G, kNParam) || w->alloc(w->Mm, kNParam) ||
             w->alloc(w->Vv, kNParam) || w->alloc(w->X, capacity * kIn) || w->alloc(w->T, capacity * kOut) ||
-            w->alloc(w->dpart, int64_t(grid_for(nc)) + 1))
+            w->alloc(w->dpart, int64_t(grid_for(nc)) + 1) || w->alloc(w->kpart, int64_t(kMaxSplit) * kHid * kHid))
             return fail(DNDSMI_ERR_OUT_OF_MEMORY);
     } catch (...) {
         return fail(DNDSMI_ERR_OUT_OF_MEMORY);
@@ -571,11 +608,20 @@ extern "C" int dndsmi_warmstart_features(dndsmi_warmstart *w, const dndsmi_array
         return DNDSMI_ERR_INVALID_ARGUMENT;
     DNDSMI_TRY(features(w, u->d));
     hipStream_t s = w->ctx->stream;
-    if (X)
-        DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(X, w->fX, sizeof(float) * w->nc * kIn, hipMemcpyDeviceToHost, s), "copy"));
-    if (S)
-        DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(S, w->fS, sizeof(float) * w->nc * 4, hipMemcpyDeviceToHost, s), "copy"));
-    return w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    std::vector<float> hx(size_t(w->nc) * kIn), hs(size_t(w->nc) * 4);
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(hx.data(), w->fX, sizeof(float) * hx.size(), hipMemcpyDeviceToHost, s),
+                           "copy"));
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(hs.data(), w->fS, sizeof(float) * hs.size(), hipMemcpyDeviceToHost, s),
+                           "copy"));
+    DNDSMI_TRY(w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize"));
+    for (int64_t i = 0; i < w->nc; i++) { // rows back in mesh numbering
+        const int64_t d = w->ctx->c2d[i];
+        if (X)
+            std::memcpy(X + i * kIn, hx.data() + d * kIn, sizeof(float) * kIn);
+        if (S)
+            std::memcpy(S + i * 4, hs.data() + d * 4, sizeof(float) * 4);
+    }
+    return DNDSMI_OK;
 }
 
 extern "C" int dndsmi_warmstart_add_samples(dndsmi_warmstart *w, const dndsmi_array *u, const dndsmi_array *uRec) {
@@ -626,8 +672,13 @@ extern "C" int dndsmi_warmstart_targets(dndsmi_warmstart *w, const dndsmi_array 
     hipLaunchKernelGGL(k_targets, dim3(grid_for(w->nc)), dim3(256), 0, s, int32_t(w->nc), w->ldc, uRec->d, w->fS,
                        w->Y);
     DNDSMI_TRY(w->ctx->launch_check("k_targets"));
-    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(T, w->Y, sizeof(float) * w->nc * kOut, hipMemcpyDeviceToHost, s), "copy"));
-    return w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    std::vector<float> ht(size_t(w->nc) * kOut);
+    DNDSMI_TRY(w->ctx->hip(hipMemcpyAsync(ht.data(), w->Y, sizeof(float) * ht.size(), hipMemcpyDeviceToHost, s),
+                           "copy"));
+    DNDSMI_TRY(w->ctx->hip(hipStreamSynchronize(s), "hipStreamSynchronize"));
+    for (int64_t i = 0; i < w->nc; i++)
+        std::memcpy(T + i * kOut, ht.data() + int64_t(w->ctx->c2d[i]) * kOut, sizeof(float) * kOut);
+    return DNDSMI_OK;
 }
 
 extern "C" int64_t dndsmi_warmstart_num_samples(const dndsmi_warmstart *w) { return w ? w->n_samples : -1; }

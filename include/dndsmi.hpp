@@ -254,15 +254,18 @@ class EulerEvaluator {
  * + EvaluateRHS and fdt = EvaluateDt */
 class ExplicitSSPRK3LocalDt {
   public:
+    /* mixedPrecision: FP32 correction sweeps after one FP64 sweep per stage and
+     * FP32 face reconstruction (dndsmi_ssprk3_step_mixed; no reference counterpart) */
     ExplicitSSPRK3LocalDt(EulerEvaluator &ev, int nInternalRecStep, real CFL, real MaxDt = 1e100,
-                          bool UseLocaldt = false, bool useLimiter = false)
-        : ev_(ev), n_(nInternalRecStep), cfl_(CFL), maxdt_(MaxDt), local_(UseLocaldt), lim_(useLimiter) {}
+                          bool UseLocaldt = false, bool useLimiter = false, bool mixedPrecision = false)
+        : ev_(ev), n_(nInternalRecStep), cfl_(CFL), maxdt_(MaxDt), local_(UseLocaldt), lim_(useLimiter),
+          mixed_(mixedPrecision) {}
     /* advances u in place; returns the global minimum dt of the step */
     real Step(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNew, real t) {
         real dt = 0;
         dndsmi_ctx *c = ev_.device()->ctx.get();
-        detail::check(dndsmi_ssprk3_step(c, u.h(), uRec.h(), uRecNew.h(), n_, lim_, cfl_, maxdt_, local_, t, &dt),
-                      "SSP-RK3 step", c);
+        auto fn = mixed_ ? &dndsmi_ssprk3_step_mixed : &dndsmi_ssprk3_step;
+        detail::check(fn(c, u.h(), uRec.h(), uRecNew.h(), n_, lim_, cfl_, maxdt_, local_, t, &dt), "SSP-RK3 step", c);
         return dt;
     }
 
@@ -270,7 +273,7 @@ class ExplicitSSPRK3LocalDt {
     EulerEvaluator &ev_;
     int n_;
     real cfl_, maxdt_;
-    bool local_, lim_;
+    bool local_, lim_, mixed_;
 };
 
 /* ODE::ImplicitEulerDualTimeStep with LU-SGS (jacobianTypeCode 0, gmresCode 0) */

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -30,6 +31,38 @@ enum {
 enum { ERR_OK = 0, ERR_INVALID = 1, ERR_NAN = 3, ERR_NONPOSITIVE = 4, ERR_UNSUPPORTED = 5 };
 
 inline double sqr(double x) { return x * x; }
+
+// Worker threads for the independent per-cell / per-face loops (the
+// benchmark's CPU baseline uses the host's cores as the reference uses MPI
+// ranks).  Each item is computed exactly as in the serial loop; the scatters
+// stay serial, so the results do not depend on the thread count.
+int g_threads = 1;
+
+template <class F>
+int parallel_items(int64_t n, F &&body) {
+    const int T = int(std::min<int64_t>(g_threads, std::max<int64_t>(1, n / 2048)));
+    if (T <= 1) {
+        for (int64_t i = 0; i < n; i++)
+            if (int rc = body(i))
+                return rc;
+        return 0;
+    }
+    std::vector<int> rcs(T, 0);
+    std::vector<std::thread> pool;
+    const int64_t chunk = (n + T - 1) / T;
+    for (int t = 0; t < T; t++)
+        pool.emplace_back([&, t] {
+            for (int64_t i = t * chunk; i < std::min(n, (t + 1) * chunk); i++)
+                if ((rcs[t] = body(i)))
+                    return;
+        });
+    for (auto &th : pool)
+        th.join();
+    for (int rc : rcs)
+        if (rc)
+            return rc;
+    return 0;
+}
 
 bool is_far(int32_t bc) {
     return bc == BC_FARFIELD || bc == BC_DMRFAR || bc == BC_RTFAR || bc == BC_IVFAR || bc == BC_2DRIEMANNFAR;
@@ -491,7 +524,7 @@ extern "C" int oracle_jacobi_step(const oracle_mesh *m, const oracle_setup *s, c
                                   const double *u, double *urec, double *urec_new, double t) {
     (void)t;
     const double relax = cfg->jacobi_relax;
-    for (int64_t i = 0; i < m->n_cell; i++) {
+    const int rc0 = parallel_items(m->n_cell, [&](int64_t i) -> int {
         double *nw = urec_new + 36 * i;
         const double *xi = urec + 36 * i;
         for (int k = 0; k < 36; k++)
@@ -581,7 +614,10 @@ extern "C" int oracle_jacobi_step(const oracle_mesh *m, const oracle_setup *s, c
                 return ERR_UNSUPPORTED;
             }
         }
-    }
+        return ERR_OK;
+    });
+    if (rc0)
+        return rc0;
     std::memcpy(urec, urec_new, sizeof(double) * 36 * m->n_cell);
     return ERR_OK;
 }
@@ -651,7 +687,8 @@ extern "C" int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, 
     for (int64_t i = 0; i < nc; i++)
         for (int v = 0; v < 4; v++)
             rhs[4 * i + v] = 0.0;
-    for (int64_t f = 0; f < m->n_face; f++) {
+    std::vector<double> fbuf(12 * size_t(m->n_face));
+    const int rc0 = parallel_items(m->n_face, [&](int64_t f) -> int {
         const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
         double fluxEs[3][4] = {{0}};
         for (int g = 0; g < 3; g++) {
@@ -701,6 +738,15 @@ extern "C" int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, 
                 fluxEs[2][v] += c * kLine3W[g];
             }
         }
+        std::memcpy(&fbuf[12 * f], fluxEs, sizeof(fluxEs));
+        return ERR_OK;
+    });
+    if (rc0)
+        return rc0;
+    for (int64_t f = 0; f < m->n_face; f++) { // scatter in face order
+        const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
+        double fluxEs[3][4];
+        std::memcpy(fluxEs, &fbuf[12 * f], sizeof(fluxEs));
         const double VL = s->volume[L];
         for (int v = 0; v < 4; v++)
             rhs[4 * L + v] += fluxEs[0][v] / VL;
@@ -840,7 +886,7 @@ extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const
     const int P = 3;
     // Step 0: smoothness indicator from the (rho, E) jumps at the first LINE_3
     // point with the LINE_1 weight 2 (DNDS_FV_VR.hpp:2075-2138)
-    for (int64_t i = 0; i < nc; i++) {
+    parallel_items(nc, [&](int64_t i) -> int {
         const int64_t b0 = m->cell_ptr[i];
         const int nf = int(m->cell_ptr[i + 1] - b0);
         double sum[2][2] = {{0, 0}, {0, 0}};
@@ -889,15 +935,16 @@ extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const
         double si1 = sum[1][0] / (sum[1][1] + kVerySmallReal);
         double sImax = std::max(std::abs(si0), std::abs(si1));
         if_use_limiter[i] = std::sqrt(sImax) * (P * P);
-    }
+        return ERR_OK;
+    });
     std::memcpy(urec_new, urec, sizeof(double) * 36 * nc);
     const double eps4 = std::pow(kVerySmallReal, 1.0 / 4); // verySmallReal_pDiP
     const double nStd = cfg->wbap_n_std;
-    std::vector<double> lim(36);
     // Step 1: per-order limiting (DNDS_FV_VR.hpp:2152-2317)
-    for (int64_t i = 0; i < nc; i++) {
+    const int rc1 = parallel_items(nc, [&](int64_t i) -> int {
         if (if_use_limiter[i] < cfg->wbap_smooth_indicator_scale && !if_all)
-            continue;
+            return ERR_OK;
+        double lim[36];
         const int64_t b0 = m->cell_ptr[i];
         const int nf = int(m->cell_ptr[i + 1] - b0);
         double uFaces[4][36];
@@ -997,7 +1044,10 @@ extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const
         }
         for (int k = 0; k < 36; k++)
             urec_new[36 * i + k] = lim[k];
-    }
+        return ERR_OK;
+    });
+    if (rc1)
+        return rc1;
     for (int64_t k = 0; k < 36 * nc; k++)
         if (!std::isfinite(urec_new[k]))
             return ERR_NAN;
@@ -1229,6 +1279,8 @@ extern "C" int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s
     }
     return ERR_OK;
 }
+
+extern "C" void oracle_set_threads(int32_t n) { g_threads = n > 1 ? n : 1; }
 
 extern "C" int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU,
                                              double lambda_main, double gamma, double *dF) {

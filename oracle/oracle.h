@@ -123,6 +123,9 @@ int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s, const ora
 int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU, double lambda_main,
                                   double gamma, double *dF);
 
+/* worker threads for the per-cell / per-face loops (results do not depend on it) */
+void oracle_set_threads(int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
             "oracle_eigenvectors": [PD, C.c_double, PD, PD],
             "oracle_boundary_value": [C.POINTER(OSetting), PD, PD, PD, C.c_double, C.c_int32, C.c_int32, PD],
             "oracle_flux_jacobian_times_du": [PD, PD, PD, C.c_double, C.c_double, PD],
+            "oracle_set_threads": [C.c_int32],
             "oracle_lusgs_matrix_init": [C.POINTER(OMesh), C.POINTER(OSetup), PD, C.c_double, C.c_double, PD, PD],
             "oracle_lusgs_forward": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, PD, PD,
                                      C.c_double, PD, PD, PD, PD],
@@ -262,3 +263,8 @@ def flux_jacobian_times_du(U, unit_norm, dU, lambda_main=0.0, gamma=1.4):
     lib().oracle_flux_jacobian_times_du(_d(np.ascontiguousarray(U, float)), _d(np.ascontiguousarray(unit_norm, float)),
                                         _d(np.ascontiguousarray(dU, float)), lambda_main, gamma, _d(dF))
     return dF
+
+
+def set_threads(n: int):
+    """Worker threads of the oracle's per-cell / per-face loops."""
+    lib().oracle_set_threads(int(n))
