@@ -12,7 +12,6 @@
 #include <cstring>
 #include <new>
 #include <string>
-#include <tuple>
 #include <vector>
 
 #include "ctx.hpp"
@@ -612,12 +611,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     if (MIXED) {
 #pragma unroll
         for (int k = 0; k < 12; k++)
-            fflux32[12 * int64_t(f) + k] = float(acc[k]);
+            fflux32[k * ldf + f] = float(acc[k]);
         return;
     }
 #pragma unroll
     for (int k = 0; k < 12; k++)
-        fflux[12 * int64_t(f) + k] = acc[k];
+        fflux[k * ldf + f] = acc[k];
 }
 
 // EvaluateRHS scatter as a per-cell gather in ascending face order
@@ -649,17 +648,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
         if ((g & 1) == 0) {
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] += fflux[12 * int64_t(f) + v] / V;
+                r[v] += fflux[v * ldf + f] / V;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] -= fflux[12 * int64_t(f) + 4 + v] / V;
+                r[v] -= fflux[(4 + v) * ldf + f] / V;
         } else {
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] -= fflux[12 * int64_t(f) + v] / V;
+                r[v] -= fflux[v * ldf + f] / V;
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                r[v] += fflux[12 * int64_t(f) + 8 + v] / V;
+                r[v] += fflux[(8 + v) * ldf + f] / V;
         }
     }
     bool finite = true;
@@ -1060,30 +1059,17 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     c->nfmax = nfmax;
     const int64_t ldc = c->ldc, ldf = c->ldf;
     try {
-        // ---- device numbering: cells in vertical strips about 32 cells wide,
-        // bottom to top within a strip (by barycentre), faces by their first
-        // cell's device index.  A cell's neighbours are then at most about a
-        // strip row away in the order, so the sweeps and gathers find them
-        // still in L2, while consecutive cells stay next to each other.
+        // ---- device numbering.  Every device array is indexed through c2d /
+        // f2d (host arrays keep the mesh numbering).  The mesh's own order is
+        // kept: on the structured-as-unstructured meshes it is row-major, and
+        // measured faster for the sweeps than Morton or strip orders (whose
+        // neighbour loads coalesce worse); a mesh read in a scattered order
+        // would be renumbered here.  Faces follow their first cell.
         std::vector<int32_t> &c2d = c->c2d, &f2d = c->f2d;
         {
-            double xmin = 1e300, ymin = 1e300, vsum = 0;
-            for (int64_t i = 0; i < nc; i++) {
-                xmin = std::min(xmin, vr->cell_bary[2 * i]);
-                ymin = std::min(ymin, vr->cell_bary[2 * i + 1]);
-                vsum += vr->volume[i];
-            }
-            const double h = std::sqrt(vsum / double(nc)); // typical cell size
-            const double width = 32.0 * h;
-            std::vector<std::tuple<int64_t, double, double, int64_t>> key(static_cast<size_t>(nc));
-            for (int64_t i = 0; i < nc; i++) {
-                const double bx = vr->cell_bary[2 * i], by = vr->cell_bary[2 * i + 1];
-                key[i] = {int64_t((bx - xmin) / width), by, bx, i};
-            }
-            std::sort(key.begin(), key.end());
-            c2d.assign(static_cast<size_t>(nc), 0);
-            for (int64_t d = 0; d < nc; d++)
-                c2d[std::get<3>(key[d])] = int32_t(d);
+            c2d.resize(static_cast<size_t>(nc));
+            for (int64_t i = 0; i < nc; i++)
+                c2d[i] = int32_t(i);
             std::vector<std::pair<int64_t, int64_t>> fk(static_cast<size_t>(nf));
             for (int64_t f = 0; f < nf; f++)
                 fk[f] = {c2d[mesh->face2cell[2 * f]], f};

@@ -29,8 +29,8 @@ struct dndsmi_ctx {
     int device = 0;
     std::string err;
     int64_t nc = 0, nf = 0, ldc = 0, ldf = 0;
-    // device numbering of the mesh's cells (vertical strips of the cell
-    // barycentres) and faces; host arrays keep the mesh numbering
+    // device numbering of the mesh's cells and faces (host arrays keep the
+    // mesh numbering)
     std::vector<int32_t> c2d, f2d;
     int nfmax = 3;
     GasParams gas{};
@@ -83,7 +83,7 @@ struct dndsmi_ctx {
     // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
     double *lambda_face = nullptr;
     double *dlambda_face = nullptr;
-    double *fflux = nullptr;   // [ldf][12] F, FLfix, FRfix integrated over the face (AoS: the gather reads 64 B per slot)
+    double *fflux = nullptr;   // [12][ldf] F, FLfix, FRfix integrated over the face
     double *dt_part = nullptr; // [grid] partial minima
     double *dt_min = nullptr;  // [1]
     int32_t *errw = nullptr;   // [1] error bits
@@ -103,7 +103,7 @@ struct dndsmi_ctx {
     float *M32 = nullptr;             // [nfmax][21][ldc][4]
     float *fd0_32 = nullptr;          // [14][ldf][4]
     float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
-    float *fflux32 = nullptr;         // [ldf][12]
+    float *fflux32 = nullptr;         // [12][ldf]
     // implicit dual-time scratch (allocated on first use)
     double *imp_rhs = nullptr;        // [ldc][4]
     double *imp_xinc = nullptr;       // [ldc][4]
