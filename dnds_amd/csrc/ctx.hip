@@ -1308,11 +1308,25 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     return DNDSMI_OK;
 }
 
+// defined with the warm start (warmstart.hip): forgets the context
+void dndsmi_warmstart_detach(dndsmi_warmstart *w);
+
 extern "C" void dndsmi_ctx_destroy(dndsmi_ctx *c) {
     if (!c)
         return;
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
+    // arrays and warm starts may outlive their context (any destruction
+    // order is allowed): free the arrays' storage and detach them
+    for (dndsmi_array *a : c->arrays) {
+        (void)hipFree(a->d);
+        a->d = nullptr;
+        a->ctx = nullptr;
+    }
+    c->arrays.clear();
+    for (dndsmi_warmstart *w : c->warmstarts)
+        dndsmi_warmstart_detach(w);
+    c->warmstarts.clear();
     for (void *p : c->allocs)
         (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool)
@@ -1362,6 +1376,7 @@ extern "C" int dndsmi_array_create(dndsmi_ctx *c, int32_t kind, dndsmi_array **o
         delete a;
         return rc;
     }
+    c->arrays.insert(a);
     *out = a;
     return DNDSMI_OK;
 }
@@ -1369,8 +1384,11 @@ extern "C" int dndsmi_array_create(dndsmi_ctx *c, int32_t kind, dndsmi_array **o
 extern "C" void dndsmi_array_destroy(dndsmi_array *a) {
     if (!a)
         return;
-    (void)hipStreamSynchronize(a->ctx->stream);
-    (void)hipFree(a->d);
+    if (a->ctx) { // else its context was destroyed first and freed the storage
+        (void)hipStreamSynchronize(a->ctx->stream);
+        (void)hipFree(a->d);
+        a->ctx->arrays.erase(a);
+    }
     delete a;
 }
 

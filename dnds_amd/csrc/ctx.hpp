@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <string>
+#include <unordered_set>
 #include <utility>
 #include <vector>
 
@@ -37,6 +38,10 @@ struct dndsmi_ctx {
     double relax = 1.0;
     int64_t bytes = 0;
     std::vector<void *> allocs;
+    // live arrays and other objects bound to this context, detached when it
+    // is destroyed first
+    std::unordered_set<struct dndsmi_array *> arrays;
+    std::unordered_set<struct dndsmi_warmstart *> warmstarts;
 
     // cell side
     int32_t *nbr = nullptr;    // [nfmax][ldc] neighbour cell, -1 = boundary slot, -2 = absent slot

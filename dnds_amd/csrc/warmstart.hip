@@ -547,13 +547,18 @@ extern "C" int dndsmi_warmstart_create(dndsmi_ctx *c, const dndsmi_mesh_view *me
     rc = rc ? rc : c->upload(w->Vv, zero);
     if (rc)
         return fail(rc);
+    c->warmstarts.insert(w);
     *out = w;
     return DNDSMI_OK;
 }
 
+void dndsmi_warmstart_detach(dndsmi_warmstart *w) { w->ctx = nullptr; } // keeps its own allocations
+
 extern "C" void dndsmi_warmstart_destroy(dndsmi_warmstart *w) {
-    if (w && w->ctx)
+    if (w && w->ctx) {
         (void)hipStreamSynchronize(w->ctx->stream);
+        w->ctx->warmstarts.erase(w);
+    }
     delete w;
 }
 
