@@ -73,8 +73,10 @@ def main():
                     help="also run the learned warm start workload: harvest converged reconstructions of the "
                          "40x40 vortex at 8 mesh seeds (160 snapshots each, 4.1M pairs), train on the device, "
                          "report the sweeps to a 1e-10 rec increment at an unseen seed")
-    ap.add_argument("--case", default="riemann", choices=["riemann", "vortex"],
-                    help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale")
+    ap.add_argument("--case", default="riemann", choices=["riemann", "vortex", "step"],
+                    help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale; "
+                         "step: Mach-3 step on a mixed tri/quad mesh of [0,4]x[0,1] (n x n quads, --n 2048 for the "
+                         "6.3M-cell case)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -88,6 +90,8 @@ def main():
     log(f"building mesh + VR setup, {args.n}x{args.n} quads ({args.case})")
     if args.case == "riemann":
         case = cases.riemann_2d(args.n, nthreads=args.threads, build_limiter_data=True)
+    elif args.case == "step":
+        case = cases.mach3_step(args.n, nthreads=args.threads)
     else:
         case = cases.isentropic_vortex(args.n, nthreads=args.threads)
     nc = case.n_cell
@@ -216,12 +220,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded mesh, analytic initial state)",
-        "config": {"workload": (f"2D Riemann, {args.n}x{args.n} quads split to {nc} triangles, jitter 0.2h, "
-                                f"p=3 VR, HLLC, WBAP V3 limiter, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
-                                "sweeps/stage")
-                   if args.case == "riemann" else
-                   (f"periodic isentropic vortex, {args.n}x{args.n} quads split to {nc} triangles, "
-                    f"p=3 VR, Roe, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, no limiter"),
+        "config": {"workload": {
+                       "riemann": (f"2D Riemann, {args.n}x{args.n} quads split to {nc} triangles, jitter 0.2h, "
+                                   f"p=3 VR, HLLC, WBAP V3 limiter, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
+                                   "sweeps/stage"),
+                       "vortex": (f"periodic isentropic vortex, {args.n}x{args.n} quads split to {nc} triangles, "
+                                  f"p=3 VR, Roe, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, no limiter"),
+                       "step": (f"Mach-3 step, [0,4]x[0,1], {args.n}x{args.n} quads half split: {nc} mixed cells, "
+                                f"jitter 0.15h, p=3 VR, HLLC, WBAP V3, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
+                                "sweeps/stage")}[args.case],
                    "cells": nc, "sweeps_per_stage": args.sweeps,
                    "limiter": "WBAP V3" if case.limiter else "none", "precision": "fp64"},
         "kernels": prof,

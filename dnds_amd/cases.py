@@ -176,3 +176,33 @@ def bump_channel(nx=48, ny=16, kind=A.MESH_TRI_SPLIT, height=0.1, mach=0.5, seed
     far = bump_far(mach)
     u0 = np.tile(np.array(far), (mesh.n_cell, 1))
     return Case("bump_channel", mesh, vfv, "Roe", far, u0, 0, 1, 10.0, False)
+
+
+MACH3_LEFT = (3.857, 2.629, 0.0, 10.33)   # (rho, u, v, p) behind the Mach-3 shock
+MACH3_RIGHT = (1.0, 0.0, 0.0, 1.0)
+
+
+def _cons(rho, u, v, p, gamma=GAMMA):
+    return np.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)], axis=-1)
+
+
+def mach3_step(n=2048, seed=5, jitter=0.15, x_step=0.5, noise=1e-3, nthreads=0) -> Case:
+    """The combined large case: [0,4] x [0,1], n x n quads, each cut into two
+    triangles or kept (seeded, half each: a mixed mesh of about 1.5 n^2
+    cells), interior nodes jittered by U(-0.15h, 0.15h).  A Mach-3 shock
+    stands at x = 0.5: left (rho, u, v, p) = (3.857, 2.629, 0, 10.33), right
+    (1, 0, 0, 1), with seeded density noise of relative amplitude 1e-3 (u, p
+    kept).  Far-field inflow on the left (the left state, supersonic), walls
+    (Wall_Euler) elsewhere.  HLLC, WBAP V3, SSP-RK3, 100 steps."""
+    bcs = (A.BC_FARFIELD, A.BC_WALL_EULER, A.BC_WALL_EULER, A.BC_WALL_EULER)
+    mesh = CompactFacedMeshSerialRW.generate(A.MESH_MIXED, n, n, 0.0, 4.0, 0.0, 1.0, jitter=jitter, seed=seed, bc=bcs)
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**dict(RIEMANN_VR_SETTING, far_weight=1.0), build_limiter_data=1))
+    vfv.Initialization(nthreads)
+    bary = vfv.arrays(["cell_bary"])["cell_bary"].reshape(-1, 2)
+    left = bary[:, 0] < x_step
+    prim = np.where(left[:, None], np.array(MACH3_LEFT), np.array(MACH3_RIGHT))
+    rng = np.random.default_rng(seed)
+    rho = prim[:, 0] * (1 + noise * rng.uniform(-1, 1, prim.shape[0]))
+    u0 = _cons(rho, prim[:, 1], prim[:, 2], prim[:, 3])
+    far = tuple(_cons(*(np.array(v) for v in MACH3_LEFT)).tolist())
+    return Case("mach3_step", mesh, vfv, "HLLC", far, u0, 100, 5, 0.5, True)

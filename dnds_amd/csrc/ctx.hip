@@ -237,6 +237,62 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
         xn[k * ldc + i] = nw[k];
 }
 
+// Linear part of the Wall_Euler / Wall_NoSlip correction (k_wall_corr) for a
+// correction d of the reconstruction: the wall state is the reconstructed
+// value D d with the normal momentum removed (Wall_Euler) or the momentum
+// reversed (Wall_NoSlip), both linear, and the cell mean drops out.
+__global__ __launch_bounds__(kBlock) void k_wall_corr_lin(int32_t nw, int64_t ldc, const int32_t *__restrict__ wcell,
+                                                          const int32_t *__restrict__ wbc,
+                                                          const double *__restrict__ wall_ainv,
+                                                          const double *__restrict__ wall_d0,
+                                                          const double *__restrict__ wall_nrm,
+                                                          const double *__restrict__ wall_w0, int64_t ldw,
+                                                          const float *__restrict__ d, float relax,
+                                                          float *__restrict__ corr) {
+    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw)
+        return;
+    const int32_t i = wcell[w];
+    const int32_t bc = wbc[w];
+    float di[36], BCc[36];
+    for (int k = 0; k < 36; k++)
+        di[k] = d[k * ldc + i], BCc[k] = 0.f;
+    const float w0 = float(wall_w0[w]);
+    for (int g = 0; g < 3; g++) {
+        const float gw = float(g == 1 ? kLine3W1 : kLine3W0);
+        const double nx = wall_nrm[(2 * g) * ldw + w], ny = wall_nrm[(2 * g + 1) * ldw + w];
+        float uBL[4];
+        for (int v = 0; v < 4; v++) {
+            float acc = 0.f;
+            for (int c = 0; c < 9; c++)
+                acc = fmaf(float(wall_d0[(g * 9 + c) * ldw + w]), di[c + 9 * v], acc);
+            uBL[v] = acc;
+        }
+        double n0, n1;
+        stable_normalized(nx, ny, n0, n1);
+        float uBV[4] = {uBL[0], uBL[1], uBL[2], uBL[3]};
+        if (bc == BC_WALL_EULER) {
+            const float dd = float(n0) * uBL[1] + float(n1) * uBL[2];
+            uBV[1] = uBL[1] - float(n0) * dd;
+            uBV[2] = uBL[2] - float(n1) * dd;
+        } else {
+            uBV[1] = -uBL[1];
+            uBV[2] = -uBL[2];
+        }
+        const float nl = float(sqrt(nx * nx + ny * ny));
+        for (int r = 0; r < 9; r++)
+            for (int v = 0; v < 4; v++)
+                BCc[r + 9 * v] += float(wall_d0[(g * 9 + r) * ldw + w]) * w0 * w0 * uBV[v] * nl * gw;
+    }
+    for (int r = 0; r < 9; r++)
+        for (int v = 0; v < 4; v++) {
+            float acc = 0.f;
+            for (int c = 0; c < 9; c++)
+                acc = fmaf(float(wall_ainv[(r * 9 + c) * ldw + w]), BCc[c + 9 * v], acc);
+            corr[(r + 9 * v) * ldw + w] = relax * acc;
+        }
+}
+
 // FP32 correction sweep of the mixed-precision reconstruction.  The Jacobi
 // map is affine, J(x) = G x + c with G x = (1-w) x + w sum_f M_if x_j (far
 // field faces couple the cell to itself), so with r = J(x) - x the sweeps
@@ -244,15 +300,17 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
 // give x + d_k = J^{k+1}(x) exactly in real arithmetic: n - 1 FP32 sweeps
 // after one FP64 step reproduce n FP64 sweeps, with FP32 rounding confined
 // to the correction d (iterative refinement).  M is read as FP32.  LAST adds
-// the correction to the FP64 iterate in place.  Meshes with Wall_Euler /
-// Wall_NoSlip faces (whose sweep term is affine in x through the boundary
-// state) are not handled here; the host refuses them.
+// the correction to the FP64 iterate in place.  Wall_Euler / Wall_NoSlip
+// faces contribute the linear part of their boundary correction, computed
+// from d by k_wall_corr_lin (the wall state is linear in the reconstructed
+// value there).
 template <int NF, bool LAST>
 __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, const int32_t *__restrict__ nbr,
                                                     const int32_t *__restrict__ sbc, const float *__restrict__ M32,
                                                     const float *__restrict__ r32, const float *__restrict__ din,
                                                     float *__restrict__ dout, float relax,
-                                                    double *__restrict__ x) {
+                                                    double *__restrict__ x, const int32_t *__restrict__ wall_idx,
+                                                    const float *__restrict__ wcorr, int64_t ldw) {
     const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= nc)
         return;
@@ -266,8 +324,16 @@ __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, con
         if (j == -2)
             continue;
         if (j < 0) {
-            if (sbc[s * ldc + i] == BC_WALL) // constant term: in r only
+            const int32_t bc = sbc[s * ldc + i];
+            if (bc == BC_WALL) // constant term: in r only
                 continue;
+            if (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) {
+                const int64_t w = wall_idx[s * ldc + i];
+#pragma unroll
+                for (int k = 0; k < 36; k++)
+                    nw[k] += wcorr[k * ldw + w];
+                continue;
+            }
             j = i; // far field
         }
         // FP32 A^-1 B entries, four per 16-byte load (81..83 pad)
@@ -1558,10 +1624,6 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
 int ensure_mixed(dndsmi_ctx *c) {
     if (c->M32)
         return DNDSMI_OK;
-    if (c->n_wall_faces > 0) {
-        c->err = "the mixed-precision sweeps do not handle Wall_Euler / Wall_NoSlip faces";
-        return DNDSMI_ERR_UNSUPPORTED;
-    }
     const int64_t nm = int64_t(c->nfmax) * kMQuads * 4 * c->ldc, nd = 56 * c->ldf;
     DNDSMI_TRY(c->alloc(c->M32, nm));
     DNDSMI_TRY(c->alloc(c->fd0_32, nd));
@@ -1569,6 +1631,7 @@ int ensure_mixed(dndsmi_ctx *c) {
     DNDSMI_TRY(c->alloc(c->dA32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->dB32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->fflux32, 12 * c->ldf));
+    DNDSMI_TRY(c->alloc(c->wcorr32, 36 * c->n_wall));
     hipLaunchKernelGGL(k_m_to_f32, dim3(grid_for(nm / 4)), dim3(kBlock), 0, c->stream, int32_t(c->nfmax), c->ldc,
                        c->M, c->M32);
     hipLaunchKernelGGL(k_fd0_to_f32, dim3(grid_for(nd / 4)), dim3(kBlock), 0, c->stream, c->ldf, c->fd0, c->fd0_32);
@@ -1580,7 +1643,12 @@ int ensure_mixed(dndsmi_ctx *c) {
 // the last adding the correction to x in place
 int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps, double t) {
     const int nc = int(c->nc);
-    (void)t;
+    if (c->n_wall_faces > 0) {
+        hipLaunchKernelGGL(k_wall_corr, dim3(grid_for(c->n_wall_faces)), dim3(kBlock), 0, c->stream,
+                           int32_t(c->n_wall_faces), c->ldc, c->wall_cell, c->wall_bc, c->wall_ainv, c->wall_d0,
+                           c->wall_nrm, c->wall_w0, c->n_wall, u, x, c->relax, c->gas, t, c->wall_corr, c->errw);
+        DNDSMI_TRY(c->launch_check("k_wall_corr"));
+    }
     prof_begin(c, DNDSMI_K_JACOBI);
     if (c->nfmax == 3)
         hipLaunchKernelGGL((k_jacobi<3, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr,
@@ -1596,21 +1664,31 @@ int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps,
     const float w = float(c->relax);
     for (int k = 1; k < n_sweeps; k++) {
         const bool last = k == n_sweeps - 1;
+        if (c->n_wall_faces > 0) {
+            hipLaunchKernelGGL(k_wall_corr_lin, dim3(grid_for(c->n_wall_faces)), dim3(kBlock), 0, c->stream,
+                               int32_t(c->n_wall_faces), c->ldc, c->wall_cell, c->wall_bc, c->wall_ainv, c->wall_d0,
+                               c->wall_nrm, c->wall_w0, c->n_wall, din, w, c->wcorr32);
+            DNDSMI_TRY(c->launch_check("k_wall_corr_lin"));
+        }
         prof_begin(c, DNDSMI_K_DELTA32);
         if (c->nfmax == 3) {
             if (last)
                 hipLaunchKernelGGL((k_delta32<3, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
-                                   c->nbr, c->sbc, c->M32, c->r32, din, nullptr, w, x);
+                                   c->nbr, c->sbc, c->M32, c->r32, din, nullptr, w, x, c->wall_idx, c->wcorr32,
+                                   c->n_wall);
             else
                 hipLaunchKernelGGL((k_delta32<3, false>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc,
-                                   c->ldc, c->nbr, c->sbc, c->M32, c->r32, din, dout, w, nullptr);
+                                   c->ldc, c->nbr, c->sbc, c->M32, c->r32, din, dout, w, nullptr, c->wall_idx,
+                                   c->wcorr32, c->n_wall);
         } else {
             if (last)
                 hipLaunchKernelGGL((k_delta32<4, true>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
-                                   c->nbr, c->sbc, c->M32, c->r32, din, nullptr, w, x);
+                                   c->nbr, c->sbc, c->M32, c->r32, din, nullptr, w, x, c->wall_idx, c->wcorr32,
+                                   c->n_wall);
             else
                 hipLaunchKernelGGL((k_delta32<4, false>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc,
-                                   c->ldc, c->nbr, c->sbc, c->M32, c->r32, din, dout, w, nullptr);
+                                   c->ldc, c->nbr, c->sbc, c->M32, c->r32, din, dout, w, nullptr, c->wall_idx,
+                                   c->wcorr32, c->n_wall);
         }
         prof_end(c);
         DNDSMI_TRY(c->launch_check("k_delta32"));

@@ -109,6 +109,7 @@ struct dndsmi_ctx {
     float *fd0_32 = nullptr;          // [14][ldf][4]
     float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
     float *fflux32 = nullptr;         // [12][ldf]
+    float *wcorr32 = nullptr;         // [36][n_wall] linear wall correction of a correction sweep
     // implicit dual-time scratch (allocated on first use)
     double *imp_rhs = nullptr;        // [ldc][4]
     double *imp_xinc = nullptr;       // [ldc][4]

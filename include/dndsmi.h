@@ -262,8 +262,9 @@ int dndsmi_ssprk3_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dnd
  * sweeps, with rounding confined to the correction (iterative refinement).
  * EvaluateRHS reconstructs the face values in FP32 and stores the face
  * fluxes in FP32; the Riemann solver, the boundary states, WBAP, the time
- * step and the stage update stay FP64.  Meshes with Wall_Euler or
- * Wall_NoSlip faces return DNDSMI_ERR_UNSUPPORTED; n_sweeps >= 2. */
+ * step and the stage update stay FP64.  Wall_Euler / Wall_NoSlip faces
+ * enter the correction sweeps through the linear part of their boundary
+ * correction.  n_sweeps >= 2. */
 int dndsmi_ssprk3_step_mixed(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
                              int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt, int32_t UseLocaldt,
                              double t, double *dt_out);
