@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--mixed", type=int, default=1,
                     help="also time the mixed-precision step (FP32 correction sweeps + FP64 refinement) "
                          "from the same initial state and report it beside the FP64 line")
+    ap.add_argument("--implicit", type=int, default=0,
+                    help="also time N inner iterations of the implicit-Euler dual-time step with multicolour LU-SGS "
+                         "(1 sweep, CFL 10, dt = 5 explicit steps) on the same mesh and report them")
     ap.add_argument("--warmstart", type=int, default=0,
                     help="also run the learned warm start workload: harvest converged reconstructions of the "
                          "40x40 vortex at 8 mesh seeds (160 snapshots each, 4.1M pairs), train on the device, "
@@ -189,6 +192,44 @@ def main():
                          f"mesh: {timed[args.threads]:.1f}s on {args.threads} threads, {timed[1]:.1f}s on 1",
                "value_1_core": nc * 3 / timed[1]}
 
+    implicit = None
+    if args.implicit:
+        from dnds_amd.host import ImplicitEulerDualTimeStep
+        u = ev.BuildMean().upload(case.u0)
+        r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+        dts = ev.BuildScalar()
+        dt_exp = ev.EvaluateDt(dts, u, args.cfl, 1e100, False)
+        imp = ImplicitEulerDualTimeStep(ev, nInternalRecStep=1, CFL=10.0, nTimeStepInternal=args.implicit,
+                                        rhsThresholdInternal=0.0, UseLocaldt=True)
+        imp.Step(u, r, rn, dt=5 * dt_exp)  # warm-up step
+        ev.device.sync()
+        lib.dndsmi_profile_enable(ev.device.ctx, 1)
+        ts = time.perf_counter()
+        hist = imp.Step(u, r, rn, dt=5 * dt_exp)
+        ev.device.sync()
+        el = time.perf_counter() - ts
+        iprof = {}
+        for k, name in enumerate(A.KERNEL_NAMES):
+            cnt, tot = C.c_int64(), C.c_double()
+            lib.dndsmi_profile_query(ev.device.ctx, k, C.byref(cnt), C.byref(tot))
+            if cnt.value:
+                b = int(lib.dndsmi_kernel_bytes(ev.device.ctx, k))
+                if name == "k_lusgs_sweep":  # bytes are per whole sweep: 2 sweeps per iteration
+                    sweeps = 2 * len(hist)
+                    iprof[name] = dict(launches=cnt.value, sweeps=sweeps, ms_per_sweep=tot.value / sweeps, bytes=b,
+                                       gbs=b * sweeps / (tot.value * 1e-3) / 1e9)
+                else:
+                    avg = tot.value / cnt.value
+                    iprof[name] = dict(launches=cnt.value, avg_ms=avg, bytes=b, gbs=b / (avg * 1e-3) / 1e9 if b else 0)
+        lib.dndsmi_profile_enable(ev.device.ctx, 0)
+        col, ncol = ev.LUSGSColouring()
+        implicit = {"inner_iterations": len(hist), "ms_per_iteration": el / len(hist) * 1e3,
+                    "cell_iterations_per_s": nc * len(hist) / el, "colours": ncol,
+                    "increment_drop": float(hist[-1, 0] / hist[0, 0]), "kernels": iprof}
+        lus = iprof.get("k_lusgs_sweep", {})
+        log(f"implicit: {implicit['ms_per_iteration']:.3f} ms per inner iteration, {ncol} colours, "
+            f"LU-SGS {lus.get('ms_per_sweep', 0):.3f} ms per sweep, {lus.get('gbs', 0):.0f} GB/s algorithmic")
+
     warm = None
     if args.warmstart:
         from dnds_amd import learned
@@ -235,6 +276,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "implicit_lusgs": implicit,
         "learned_warm_start": warm,
     }
     print(json.dumps(out), flush=True)

@@ -1910,8 +1910,10 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
-    case DNDSMI_K_LUSGS_SWEEP: // per colour launch; not reported per launch
-        return 0;
+    case DNDSMI_K_LUSGS_SWEEP: // one whole forward (or backward) sweep, all colours together: per cell rhs,
+        // increment out, 1/D, volume, colour; per slot neighbour, colour, face, f2c, normal, area, lambda,
+        // and u, uInc of the neighbour (read by the colour that needs it: once per slot over both sweeps)
+        return nc * (32 + 32 + 8 + 8 + 4) + c2f * (4 + 4 + 4 + 4 + 16 + 8 + 8) + c2f * 64 / 2;
     case DNDSMI_K_DELTA32: // FP32 M per slot + slot index; d read + r read + d write (FP32)
         return c2f * (81 * 4 + 4) + nc * (36 * 4 * 3);
     case DNDSMI_K_RHS_FACE32: // f2c, FP32 D0 rows, normals, dlambda, 12 FP32 fluxes out; x and u once per cell

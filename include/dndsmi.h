@@ -399,7 +399,8 @@ enum {
 int dndsmi_profile_enable(dndsmi_ctx *ctx, int32_t on);
 /* Number of recorded launches of `kernel` and their summed duration (ms). */
 int dndsmi_profile_query(dndsmi_ctx *ctx, int32_t kernel, int64_t *count, double *total_ms);
-/* Algorithmic HBM bytes of one launch of `kernel` on this context's mesh. */
+/* Algorithmic HBM bytes of one launch of `kernel` on this context's mesh
+ * (DNDSMI_K_LUSGS_SWEEP: of one whole sweep, i.e. all its colour launches). */
 int64_t dndsmi_kernel_bytes(const dndsmi_ctx *ctx, int32_t kernel);
 
 #ifdef __cplusplus
