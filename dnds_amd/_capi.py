@@ -91,7 +91,7 @@ class VRView(C.Structure):
         ("volume", PD), ("face_area", PD), ("cell_bary", PD), ("base_moment", PD),
         ("face_center", PD), ("face_norm_center", PD), ("face_norm", PD), ("face_weight", PD),
         ("face_d0", PD), ("face_dibj_g0", PD), ("matrix_secondary", PD),
-        ("ainv", PD), ("ainv_b", PD), ("ainv_bvec", PD),
+        ("ainv", PD), ("ainv_b", PD), ("ainv_bvec", PD), ("face_gauss_xy", PD), ("cell_scale", PD),
     ]
 
 

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ctx.hpp"
@@ -385,20 +386,6 @@ __global__ __launch_bounds__(kBlock) void k_m_to_f32(int32_t nfmax, int64_t ldc,
     }
 }
 
-// face bases from FP64 pairs [28][ldf][2] to FP32 quads [14][ldf][4] (56 per face)
-__global__ __launch_bounds__(kBlock) void k_fd0_to_f32(int64_t ldf, const double *__restrict__ d,
-                                                       float *__restrict__ d32) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= 14 * ldf)
-        return;
-    const int64_t f = t % ldf, q = t / ldf;
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-        const int64_t e = 4 * q + w;
-        d32[t * 4 + w] = float(d[((e / 2) * ldf + f) * 2 + e % 2]);
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ src,
                                                    float *__restrict__ dst) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -516,17 +503,23 @@ __global__ __launch_bounds__(kBlock) void k_fill_from(int32_t n, const double *_
 // MIXED: the face reconstruction D0 . x runs in FP32 (FP32 bases, FP32
 // accumulation) and the integrated fluxes are stored in FP32; the Riemann
 // solver and the boundary states stay FP64.
-template <bool BOUNDARY, bool MIXED = false>
+//
+// STRAIGHT (inner faces of a mesh whose face normals are bitwise equal at the
+// three points, as on straight edges): the unit normal, |n| and the
+// flux-balance fluxes are computed at point 0 and reused, which yields the
+// same bits as recomputing them from the same inputs.
+template <bool BOUNDARY, bool MIXED = false, bool STRAIGHT = false>
 __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int64_t ldc,
                                                      const int32_t *__restrict__ list,
                                                      const int32_t *__restrict__ f2c,
-                                                     const int32_t *__restrict__ fbc, const double *__restrict__ fd0,
+                                                     const int32_t *__restrict__ fbc, const double *__restrict__ fxy,
+                                                     const double *__restrict__ cmom,
                                                      const double *__restrict__ fn, const double *__restrict__ fctr,
                                                      const double *__restrict__ dlf, const double *__restrict__ u,
                                                      const double *__restrict__ x, GasParams gp, double t,
-                                                     double *__restrict__ fflux, int32_t *errw,
-                                                     const float *__restrict__ fd0_32 = nullptr,
-                                                     float *__restrict__ fflux32 = nullptr) {
+                                                     const uint8_t *__restrict__ fpos,
+                                                     double *__restrict__ fslot, int32_t *errw,
+                                                     float *__restrict__ fslot32) {
     const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     if (tid >= n)
         return;
@@ -538,96 +531,66 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
 #pragma unroll
     for (int k = 0; k < 4; k++)
         uL[k] = u[4 * int64_t(L) + k];
-    // reconstructed increments D0 . x at the 3 points, both sides
+    // reconstructed increments D0 . x at the 3 points, both sides; the bases
+    // D0 (row 0 of the DiBj blocks) are evaluated from the points' scaled
+    // coordinates and the cell's moments, per side in c order as stored bases
+    // would be read.  MIXED: the bases are rounded to FP32 and the products
+    // accumulated in FP32.
     double incL[3][4], incR[3][4];
-    if (MIXED) {
-        // FP32 bases per face as [side][c][g] (28 per side) quads; e = 3 c + g
-        const flt4 *dq = reinterpret_cast<const flt4 *>(fd0_32) + f;
-#pragma unroll
-        for (int g = 0; g < 3; g++)
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                incR[g][v] = 0.0;
-#pragma unroll 1
-        for (int sd = 0; sd < (BOUNDARY ? 1 : 2); sd++) {
-            const int32_t cell = sd ? R : L;
-            float acc[3][4];
-#pragma unroll
-            for (int g = 0; g < 3; g++)
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    acc[g][v] = 0.f;
-            float xc[4];
-            flt4 q4 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int e = 0; e < 27; e++) {
-                const int c = e / 3, g = e % 3;
-                if (g == 0)
-#pragma unroll
-                    for (int v = 0; v < 4; v++)
-                        xc[v] = float(x[(c + 9 * v) * ldc + cell]);
-                if (e % 4 == 0)
-                    q4 = __builtin_nontemporal_load(dq + (sd * 7 + e / 4) * ldf);
-                const float d = q4[e % 4];
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    acc[g][v] = fmaf(d, xc[v], acc[g][v]);
-            }
-#pragma unroll
-            for (int g = 0; g < 3; g++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    if (sd)
-                        incR[g][v] = double(acc[g][v]);
-                    else
-                        incL[g][v] = double(acc[g][v]);
-                }
-        }
-        if (!BOUNDARY)
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                uR[k] = u[4 * int64_t(R) + k];
-    } else {
 #pragma unroll
     for (int g = 0; g < 3; g++)
 #pragma unroll
         for (int v = 0; v < 4; v++)
             incL[g][v] = 0.0, incR[g][v] = 0.0;
-    // bases per face as [side][c][g] (28 per side) pairs; e = 3 c + g
-    const dbl2 *dp = reinterpret_cast<const dbl2 *>(fd0) + f;
+    const dbl2 *xy = reinterpret_cast<const dbl2 *>(fxy) + f;
 #pragma unroll 1
     for (int sd = 0; sd < (BOUNDARY ? 1 : 2); sd++) {
         const int32_t cell = sd ? R : L;
+        double pX[3][4], pY[3][4];
+#pragma unroll
+        for (int g = 0; g < 3; g++) {
+            const dbl2 q = __builtin_nontemporal_load(xy + (sd * 3 + g) * ldf);
+            int_powers(q.x, pX[g]);
+            int_powers(q.y, pY[g]);
+        }
         double inc[3][4];
+        float acc[3][4];
 #pragma unroll
         for (int g = 0; g < 3; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                inc[g][v] = 0.0;
-        double xc[4];
-        dbl2 pr = {0.0, 0.0};
+                inc[g][v] = 0.0, acc[g][v] = 0.f;
 #pragma unroll
-        for (int e = 0; e < 27; e++) {
-            const int c = e / 3, g = e % 3;
-            if (g == 0)
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    xc[v] = x[(c + 9 * v) * ldc + cell];
-            if (e % 2 == 0)
-                pr = __builtin_nontemporal_load(dp + (sd * 14 + e / 2) * ldf);
-            const double d = e % 2 ? pr.y : pr.x;
+        for (int c = 0; c < 9; c++) {
+            const double m = cmom[c * ldc + cell];
+            double xc[4];
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                inc[g][v] += d * xc[v];
+                xc[v] = x[(c + 9 * v) * ldc + cell];
+#pragma unroll
+            for (int g = 0; g < 3; g++) {
+                const double d = pX[g][kBX[c + 1]] * pY[g][kBY[c + 1]] - m;
+                if (MIXED) {
+                    const float d32 = float(d);
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        acc[g][v] = fmaf(d32, float(xc[v]), acc[g][v]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        inc[g][v] += d * xc[v];
+                }
+            }
         }
 #pragma unroll
         for (int g = 0; g < 3; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
+                const double r = MIXED ? double(acc[g][v]) : inc[g][v];
                 if (sd)
-                    incR[g][v] = inc[g][v];
+                    incR[g][v] = r;
                 else
-                    incL[g][v] = inc[g][v];
+                    incL[g][v] = r;
             }
     }
     if (!BOUNDARY) {
@@ -635,20 +598,22 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
         for (int k = 0; k < 4; k++)
             uR[k] = u[4 * int64_t(R) + k];
     }
-    }
     const double dl = dlf[f];
     const int32_t bc = BOUNDARY ? fbc[f] : BC_INNER;
     double acc[12];
 #pragma unroll
     for (int k = 0; k < 12; k++)
         acc[k] = 0.0;
-    bool ok = true;
+    bool ok = true, okb = true;
+    double n0, n1, nl, FL[4], FR[4];
 #pragma unroll
     for (int g = 0; g < 3; g++) {
-        const double nx = fn[(2 * g) * ldf + f], ny = fn[(2 * g + 1) * ldf + f];
-        double n0, n1;
-        normalized(nx, ny, n0, n1);
         double ULxy[4], URxy[4], URm[4];
+        if (!STRAIGHT || g == 0) {
+            const double nx = fn[(2 * g) * ldf + f], ny = fn[(2 * g + 1) * ldf + f];
+            normalized(nx, ny, n0, n1);
+            nl = sqrt(nx * nx + ny * ny);
+        }
         compress_rec_part(uL, incL[g], ULxy);
         if (!BOUNDARY) {
             compress_rec_part(uR, incR[g], URxy);
@@ -660,9 +625,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
             ok &= boundary_value(gp, ULxy, n0, n1, px, py, t, bc, URxy);
             ok &= boundary_value(gp, uL, n0, n1, px, py, t, bc, URm);
         }
-        double F[4], FL[4], FR[4];
-        ok &= ok && flux_face(gp, ULxy, URxy, uL, URm, n0, n1, dl, F, FL, FR);
-        const double nl = sqrt(nx * nx + ny * ny);
+        double F[4];
+        if (!STRAIGHT || g == 0)
+            okb = ok && balance_fluxes(gp, uL, URm, n0, n1, FL, FR);
+        ok &= ok && okb && riemann_face(gp, ULxy, URxy, n0, n1, dl, F);
         const double w = g == 1 ? kLine3W1 : kLine3W0;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -674,27 +640,42 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     }
     if (!ok)
         atomicOr(errw, ERRBIT_NONPOSITIVE);
+    // each side's two terms go to its cell's slot for this face, signed as
+    // the cell applies them (negation is exact, so the cell's sums are
+    // unchanged bit for bit)
+    const int32_t pos = fpos[f];
+    const int64_t oL = int64_t(pos & 15) * 8 * ldc + L, oR = int64_t(pos >> 4) * 8 * ldc + R;
     if (MIXED) {
 #pragma unroll
-        for (int k = 0; k < 12; k++)
-            fflux32[k * ldf + f] = float(acc[k]);
+        for (int v = 0; v < 4; v++)
+            fslot32[oL + v * ldc] = float(acc[v]), fslot32[oL + (4 + v) * ldc] = -float(acc[4 + v]);
+        if (!BOUNDARY)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                fslot32[oR + v * ldc] = -float(acc[v]), fslot32[oR + (4 + v) * ldc] = float(acc[8 + v]);
         return;
     }
 #pragma unroll
-    for (int k = 0; k < 12; k++)
-        fflux[k * ldf + f] = acc[k];
+    for (int v = 0; v < 4; v++)
+        fslot[oL + v * ldc] = acc[v], fslot[oL + (4 + v) * ldc] = -acc[4 + v];
+    if (!BOUNDARY)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            fslot[oR + v * ldc] = -acc[v], fslot[oR + (4 + v) * ldc] = acc[8 + v];
 }
 
 // EvaluateRHS scatter as a per-cell gather in ascending face order
-// (DNDS_FV_EulerEvaluator.cpp:501-508), optionally fused with the SSP-RK3
-// stage update of ExplicitSSPRK3LocalDt::Step (DNDS_ODE.hpp:136-157):
+// (DNDS_FV_EulerEvaluator.cpp:501-508): the face kernel has stored each
+// face's two terms at the cell's slot for that face, slots in ascending face
+// order, signed so that the cell only adds (F/V then -FLfix/V for side 0,
+// -F/V then FRfix/V for side 1).  Optionally fused with the SSP-RK3 stage
+// update of ExplicitSSPRK3LocalDt::Step (DNDS_ODE.hpp:136-157):
 //   stage 0: rhs = gathered value
 //   stage 1: xLast = x; x = x + rhs dt
 //   stage 2/3: x = ((x + rhs dt) c/(1-c) + xLast)(1-c), c = 1/4, 2/3
 template <int NF, typename FT = double>
-__global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, int64_t ldf,
-                                                     const int32_t *__restrict__ gat,
-                                                     const FT *__restrict__ fflux,
+__global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, const int32_t *__restrict__ gat,
+                                                     const FT *__restrict__ fslot,
                                                      const double *__restrict__ vol, double *__restrict__ rhs,
                                                      int stage, double *__restrict__ x,
                                                      double *__restrict__ xlast, const double *__restrict__ dt,
@@ -704,28 +685,19 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
     if (i >= nc)
         return;
     const FT V = FT(vol[i]);
+    // a cell has 3 or 4 faces: only the 4-slot layout has an absent last slot
+    const int ns = NF == 4 && gat[3 * ldc + i] < 0 ? 3 : NF;
     FT r[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < NF; s++) {
-        const int32_t g = gat[s * ldc + i];
-        if (g < 0)
-            continue;
-        const int32_t f = g >> 1;
-        if ((g & 1) == 0) {
+        if (s >= ns)
+            break;
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                r[v] += fflux[v * ldf + f] / V;
+        for (int v = 0; v < 4; v++)
+            r[v] += fslot[(s * 8 + v) * ldc + i] / V;
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                r[v] -= fflux[(4 + v) * ldf + f] / V;
-        } else {
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                r[v] -= fflux[v * ldf + f] / V;
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                r[v] += fflux[(8 + v) * ldf + f] / V;
-        }
+        for (int v = 0; v < 4; v++)
+            r[v] += fslot[(s * 8 + 4 + v) * ldc + i] / V;
     }
     bool finite = true;
 #pragma unroll
@@ -763,7 +735,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cell(int32_t nc, int64_t ldc, in
 // coefficients and the side-1 basis with its neighbour's.
 __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, int64_t ldc,
                                                       const int32_t *__restrict__ f2c,
-                                                      const double *__restrict__ dg0, const double *__restrict__ fw,
+                                                      const double *__restrict__ fxy, const double *__restrict__ cmom,
+                                                      const double *__restrict__ cscale,
+                                                      const double *__restrict__ fw,
                                                       const double *__restrict__ fn, const double *__restrict__ u,
                                                       const double *__restrict__ x, double tang_weight, int twd_norm,
                                                       double *__restrict__ wsum) {
@@ -773,38 +747,21 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
     const int32_t L = f2c[f], R = f2c[ldf + f];
     if (R < 0)
         return;
-    // One pass over the Gauss-point bases: side-0 basis times the L and R
-    // coefficients, side-1 basis times R and L, for rho (q = 0) and E (q = 1).
-    double b0L[10][2], b0R[10][2], b1L[10][2], b1R[10][2];
-#pragma unroll
-    for (int r = 0; r < 10; r++)
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-            b0L[r][q] = b0R[r][q] = b1L[r][q] = b1R[r][q] = 0.0;
-    // bases stored per face as [side][c][r] pairs: 5 16-byte loads per side and c
-    const dbl2 *dp = reinterpret_cast<const dbl2 *>(dg0) + f;
-#pragma unroll 1
-    for (int c = 0; c < 9; c++) {
-        const double xL[2] = {x[c * ldc + L], x[(c + 27) * ldc + L]};
-        const double xR[2] = {x[c * ldc + R], x[(c + 27) * ldc + R]};
-#pragma unroll
-        for (int rp = 0; rp < 5; rp++) {
-            const dbl2 p0 = __builtin_nontemporal_load(dp + (c * 5 + rp) * ldf);
-            const dbl2 p1 = __builtin_nontemporal_load(dp + (45 + c * 5 + rp) * ldf);
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int r = 2 * rp + h;
-                const double d0 = h ? p0.y : p0.x, d1 = h ? p1.y : p1.x;
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    b0L[r][q] += d0 * xL[q];
-                    b0R[r][q] += d0 * xR[q];
-                    b1R[r][q] += d1 * xR[q];
-                    b1L[r][q] += d1 * xL[q];
-                }
-            }
-        }
-    }
+    // The DiBj blocks of both sides at point 0 (side 0: cell L's basis,
+    // side 1: cell R's), evaluated from the point's scaled coordinates and
+    // the cells' scales and moments; zero entries are skipped (adding them
+    // would not change any sum).  Cell side cs sees its own coefficients on
+    // the side-0 basis and the neighbour's on the side-1 basis.  The rows are
+    // taken in the functional's groups {0}, {1, 2}, {3, 4, 5}, {6..9}: one
+    // pass over the coefficients per group, whose terms then join the
+    // indicator sums in their order.
+    const dbl2 *xy = reinterpret_cast<const dbl2 *>(fxy) + f;
+    const dbl2 q0 = __builtin_nontemporal_load(xy), q1 = __builtin_nontemporal_load(xy + 3 * ldf);
+    const dbl2 *sc = reinterpret_cast<const dbl2 *>(cscale);
+    const dbl2 s0 = sc[L], s1 = sc[R];
+    double pX0[4], pY0[4], pX1[4], pY1[4], sX0[4], sY0[4], sX1[4], sY1[4];
+    int_powers(q0.x, pX0), int_powers(q0.y, pY0), int_powers(q1.x, pX1), int_powers(q1.y, pY1);
+    int_powers(s0.x, sX0), int_powers(s0.y, sY0), int_powers(s1.x, sX1), int_powers(s1.y, sY1);
     double W[10];
 #pragma unroll
     for (int k = 0; k < 10; k++)
@@ -814,27 +771,79 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
     const double nl = sqrt(gx * gx + gy * gy);
     const double uL[2] = {u[4 * int64_t(L)], u[4 * int64_t(L) + 3]};
     const double uR[2] = {u[4 * int64_t(R)], u[4 * int64_t(R) + 3]};
+    double sums[2][2][2]; // [cell side][rho, E][IJI, ISI]
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        (&sums[0][0][0])[k] = 0.0;
+    auto pass = [&](auto group) {
+        constexpr int G = decltype(group)::value;
+        constexpr int r0 = G == 0 ? 0 : G == 1 ? 1 : G == 2 ? 3 : 6, nr = G == 0 ? 1 : G == 1 ? 2 : G == 2 ? 3 : 4;
+        // each pass reloads the coefficients (cache hits) rather than keeping
+        // all of them in registers across the passes
+        __asm__ volatile("" ::: "memory");
+        // b[0]: side-0 basis x L, b[1]: side-0 x R, b[2]: side-1 x R, b[3]: side-1 x L
+        double b[4][nr][2];
+#pragma unroll
+        for (int k = 0; k < 4 * nr * 2; k++)
+            (&b[0][0][0])[k] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const double xL[2] = {x[c * ldc + L], x[(c + 27) * ldc + L]};
+            const double xR[2] = {x[c * ldc + R], x[(c + 27) * ldc + R]};
+            const double m0 = G == 0 ? cmom[c * ldc + L] : 0.0, m1 = G == 0 ? cmom[c * ldc + R] : 0.0;
+#pragma unroll
+            for (int rr = 0; rr < nr; rr++) {
+                const int r = r0 + rr;
+                const int coef = dibj_coef(r, c + 1);
+                if (coef == 0)
+                    continue;
+                const int ex = kBX[c + 1] - kBX[r], ey = kBY[c + 1] - kBY[r];
+                double d0 = double(coef) * pX0[ex] * pY0[ey] * 1.0;
+                double d1 = double(coef) * pX1[ex] * pY1[ey] * 1.0;
+                if (r == 0) {
+                    d0 -= m0, d1 -= m1;
+                } else {
+                    d0 = d0 / (sX0[kBX[r]] * sY0[kBY[r]] * 1.0);
+                    d1 = d1 / (sX1[kBX[r]] * sY1[kBY[r]] * 1.0);
+                }
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    b[0][rr][q] += d0 * xL[q];
+                    b[1][rr][q] += d0 * xR[q];
+                    b[2][rr][q] += d1 * xR[q];
+                    b[3][rr][q] += d1 * xL[q];
+                }
+            }
+        }
+#pragma unroll
+        for (int cs = 0; cs < 2; cs++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                double val[nr], jmp[nr];
+#pragma unroll
+                for (int rr = 0; rr < nr; rr++) {
+                    double vl = b[cs == 0 ? 0 : 1][rr][q], vr = b[cs == 0 ? 2 : 3][rr][q];
+                    if (r0 + rr == 0) {
+                        vl += cs == 0 ? uL[q] : uR[q];
+                        vr += cs == 0 ? uR[q] : uL[q];
+                    }
+                    val[rr] = (vl + vr) * 0.5;
+                    jmp[rr] = (vl - vr) * 0.5;
+                }
+                ff_group<G>(P, jmp, sums[cs][q][0]);
+                ff_group<G>(P, val, sums[cs][q][1]);
+            }
+    };
+    pass(std::integral_constant<int, 0>{});
+    pass(std::integral_constant<int, 1>{});
+    pass(std::integral_constant<int, 2>{});
+    pass(std::integral_constant<int, 3>{});
 #pragma unroll
     for (int cs = 0; cs < 2; cs++) {
-        // cell side cs: its own coefficients on the side-0 basis, the
-        // neighbour's on the side-1 basis
         double out[4];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            double val[10], jmp[10];
-#pragma unroll
-            for (int r = 0; r < 10; r++) {
-                double vl = cs == 0 ? b0L[r][q] : b0R[r][q];
-                double vr = cs == 0 ? b1R[r][q] : b1L[r][q];
-                if (r == 0) {
-                    vl += cs == 0 ? uL[q] : uR[q];
-                    vr += cs == 0 ? uR[q] : uL[q];
-                }
-                val[r] = (vl + vr) * 0.5;
-                jmp[r] = (vl - vr) * 0.5;
-            }
-            const double isi = ff_self(P, val), iji = ff_self(P, jmp);
-            double a = iji, b = isi;
+            double a = sums[cs][q][0], b = sums[cs][q][1];
             a *= nl, b *= nl;
             out[2 * q] = a * 2.0;
             out[2 * q + 1] = b * 2.0;
@@ -1092,6 +1101,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     if (vr->n_cell != mesh->n_cell || vr->n_face != mesh->n_face || mesh->n_cell < 1 ||
         mesh->n_cell > (int64_t(1) << 30) || mesh->n_face > (int64_t(1) << 30))
         return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (!vr->face_gauss_xy || !vr->cell_scale || !vr->base_moment || !vr->face_d0)
+        return DNDSMI_ERR_INVALID_ARGUMENT; // the device evaluates the face bases from these
     if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC)
         return DNDSMI_ERR_UNSUPPORTED;
     dndsmi_ctx *c = new (std::nothrow) dndsmi_ctx();
@@ -1148,12 +1159,16 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         std::vector<int32_t> nbr(size_t(nfmax) * ldc, -2), sbc(size_t(nfmax) * ldc, 0), sface(size_t(nfmax) * ldc, -1);
         std::vector<int32_t> gat(size_t(nfmax) * ldc, -1), widx(size_t(nfmax) * ldc, -1);
         std::vector<double> M(size_t(nfmax) * kMPairs * 2 * ldc, 0.0), bv(size_t(nfmax) * 9 * ldc, 0.0), vol(ldc, 1.0);
+        std::vector<double> cmom(9 * size_t(ldc), 0.0), cscale(2 * size_t(ldc), 1.0);
         std::vector<int64_t> wall_cell, wall_face, wall_c2f;
         for (int64_t i = 0; i < nc; i++) {
             const int64_t b0 = mesh->cell_ptr[i];
             const int n = int(mesh->cell_ptr[i + 1] - b0);
             const int64_t ci = c2d[i]; // device column of mesh cell i
             vol[ci] = vr->volume[i];
+            for (int k = 0; k < 9; k++)
+                cmom[k * ldc + ci] = vr->base_moment[10 * i + 1 + k];
+            cscale[2 * ci] = vr->cell_scale[2 * i], cscale[2 * ci + 1] = vr->cell_scale[2 * i + 1];
             std::vector<std::pair<int64_t, int>> order;
             for (int s = 0; s < n; s++) {
                 const int64_t f = mesh->cell2face[b0 + s];
@@ -1188,6 +1203,17 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         DNDSMI_TRY(c->upload(c->sface, sface));
         DNDSMI_TRY(c->alloc(c->gat, nfmax * ldc));
         DNDSMI_TRY(c->upload(c->gat, gat));
+        {
+            std::vector<uint8_t> fpos(ldf, 0);
+            for (int s = 0; s < nfmax; s++)
+                for (int64_t ci = 0; ci < nc; ci++) {
+                    const int32_t g = gat[s * ldc + ci];
+                    if (g >= 0)
+                        fpos[g >> 1] |= uint8_t(s << (4 * (g & 1)));
+                }
+            DNDSMI_TRY(c->alloc(c->fpos, ldf));
+            DNDSMI_TRY(c->upload(c->fpos, fpos));
+        }
         DNDSMI_TRY(c->alloc(c->wall_idx, nfmax * ldc));
         DNDSMI_TRY(c->upload(c->wall_idx, widx));
         DNDSMI_TRY(c->alloc(c->M, int64_t(nfmax) * kMPairs * 2 * ldc));
@@ -1197,6 +1223,10 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         DNDSMI_TRY(c->upload(c->bv, bv));
         DNDSMI_TRY(c->alloc(c->vol, ldc));
         DNDSMI_TRY(c->upload(c->vol, vol));
+        DNDSMI_TRY(c->alloc(c->cmom, 9 * ldc));
+        DNDSMI_TRY(c->upload(c->cmom, cmom));
+        DNDSMI_TRY(c->alloc(c->cscale, 2 * ldc));
+        DNDSMI_TRY(c->upload(c->cscale, cscale));
         // wall faces (Wall_Euler / Wall_NoSlip): compact arrays
         const int64_t nw = int64_t(wall_cell.size());
         c->n_wall = std::max<int64_t>(nw, 1);
@@ -1234,7 +1264,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         }
         // ---- face streams
         std::vector<int32_t> f2c(2 * size_t(ldf), -1), fbc(ldf, DNDSMI_BC_INNER);
-        std::vector<double> fd0(56 * size_t(ldf), 0.0), fn(6 * size_t(ldf), 0.0), fnc(2 * size_t(ldf), 0.0),
+        bool straight = true;
+        std::vector<double> fxy(12 * size_t(ldf), 0.0), fn(6 * size_t(ldf), 0.0), fnc(2 * size_t(ldf), 0.0),
             farea(ldf, 0.0), fctr(2 * size_t(ldf), 0.0);
         for (int64_t fo = 0; fo < nf; fo++) {
             const int64_t f = f2d[fo]; // device column of mesh face fo
@@ -1242,14 +1273,14 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             const int64_t R = mesh->face2cell[2 * fo + 1];
             f2c[ldf + f] = R == DNDSMI_NO_CELL ? -1 : c2d[R];
             fbc[f] = mesh->face_bc[fo];
-            for (int g = 0; g < 3; g++) // [side][c][g], 28 per side, as pairs
+            for (int g = 0; g < 3; g++) // rows [side][g] of (X, Y) pairs
                 for (int sd = 0; sd < 2; sd++)
-                    for (int cc = 0; cc < 9; cc++) {
-                        const int e = sd * 28 + cc * 3 + g;
-                        fd0[(size_t(e / 2) * ldf + f) * 2 + e % 2] = vr->face_d0[54 * fo + g * 18 + sd * 9 + cc];
-                    }
+                    for (int k = 0; k < 2; k++)
+                        fxy[(size_t(sd * 3 + g) * ldf + f) * 2 + k] = vr->face_gauss_xy[12 * fo + 4 * g + 2 * sd + k];
             for (int k = 0; k < 6; k++)
                 fn[k * ldf + f] = vr->face_norm[6 * fo + k];
+            for (int k = 2; k < 6; k++)
+                straight &= std::memcmp(&vr->face_norm[6 * fo + k], &vr->face_norm[6 * fo + k % 2], 8) == 0;
             fnc[f] = vr->face_norm_center[2 * fo], fnc[ldf + f] = vr->face_norm_center[2 * fo + 1];
             farea[f] = vr->face_area[fo];
             fctr[f] = vr->face_center[2 * fo], fctr[ldf + f] = vr->face_center[2 * fo + 1];
@@ -1258,10 +1289,11 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         DNDSMI_TRY(c->upload(c->f2c, f2c));
         DNDSMI_TRY(c->alloc(c->fbc, ldf));
         DNDSMI_TRY(c->upload(c->fbc, fbc));
-        DNDSMI_TRY(c->alloc(c->fd0, 56 * ldf));
-        DNDSMI_TRY(c->upload(c->fd0, fd0));
+        DNDSMI_TRY(c->alloc(c->fxy, 12 * ldf));
+        DNDSMI_TRY(c->upload(c->fxy, fxy));
         DNDSMI_TRY(c->alloc(c->fn, 6 * ldf));
         DNDSMI_TRY(c->upload(c->fn, fn));
+        c->straight_faces = straight;
         DNDSMI_TRY(c->alloc(c->fnc, 2 * ldf));
         DNDSMI_TRY(c->upload(c->fnc, fnc));
         DNDSMI_TRY(c->alloc(c->farea, ldf));
@@ -1283,19 +1315,9 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         c->twd_norm = vs->tang_weight_direction == 0;
         c->wbap_scale = vs->wbap_smooth_indicator_scale;
         c->wbap_nstd = vs->wbap_n_std;
-        if (vr->face_dibj_g0 && vr->matrix_secondary) {
+        if (vr->matrix_secondary) {
             c->has_limiter = true;
-            std::vector<double> buf(180 * size_t(ldf), 0.0);
-            for (int64_t fo = 0; fo < nf; fo++)
-                for (int sd = 0; sd < 2; sd++)
-                    for (int r = 0; r < 10; r++)
-                        for (int cc = 0; cc < 9; cc++) {
-                            const int e = sd * 90 + cc * 10 + r; // [side][c][r]
-                            buf[(size_t(e / 2) * ldf + f2d[fo]) * 2 + e % 2] =
-                                vr->face_dibj_g0[180 * fo + sd * 90 + r * 9 + cc];
-                        }
-            DNDSMI_TRY(c->alloc(c->dg0, 180 * ldf));
-            DNDSMI_TRY(c->upload(c->dg0, buf));
+            std::vector<double> buf;
             buf.assign(162 * size_t(ldf), 0.0);
             for (int64_t f = 0; f < nf; f++)
                 for (int k = 0; k < 162; k++)
@@ -1357,7 +1379,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         // ---- scratch
         DNDSMI_TRY(c->alloc(c->lambda_face, ldf));
         DNDSMI_TRY(c->alloc(c->dlambda_face, ldf));
-        DNDSMI_TRY(c->alloc(c->fflux, 12 * ldf));
+        DNDSMI_TRY(c->alloc(c->fslot, int64_t(nfmax) * 8 * ldc));
         DNDSMI_TRY(c->alloc(c->dt_part, int64_t(grid_for(nc)) + 1));
         DNDSMI_TRY(c->alloc(c->dt_min, 1));
         DNDSMI_TRY(c->alloc(c->errw, 1));
@@ -1567,21 +1589,22 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
                const double *dt, double coefA, double coefB) {
     const int nc = int(c->nc), nf = int(c->nf);
     prof_begin(c, DNDSMI_K_RHS_FACE);
-    hipLaunchKernelGGL(k_rhs_face<false>, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc,
-                       c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fflux,
-                       c->errw);
+    hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, false, true> : k_rhs_face<false, false, false>),
+                       dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc,
+                       c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw,
+                       nullptr);
     prof_end(c);
     if (c->n_bface > 0)
         hipLaunchKernelGGL(k_rhs_face<true>, dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
-                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr,
-                           c->dlambda_face, u, x, c->gas, t, c->fflux, c->errw);
+                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
+                           c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw, nullptr);
     prof_begin(c, DNDSMI_K_RHS_CELL);
     if (c->nfmax == 3)
-        hipLaunchKernelGGL(k_rhs_cell<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->gat,
-                           c->fflux, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB, c->errw);
+        hipLaunchKernelGGL(k_rhs_cell<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->gat,
+                           c->fslot, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB, c->errw);
     else
-        hipLaunchKernelGGL(k_rhs_cell<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->gat,
-                           c->fflux, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB, c->errw);
+        hipLaunchKernelGGL(k_rhs_cell<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->gat,
+                           c->fslot, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB, c->errw);
     prof_end(c);
     return c->launch_check("rhs kernels");
 }
@@ -1595,7 +1618,7 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
     const double eps4 = std::pow(1e-200, 1.0 / 4); // verySmallReal_pDiP, DNDS_FV_Limiters.hpp:345
     prof_begin(c, DNDSMI_K_WBAP_FACE);
     hipLaunchKernelGGL(k_wbap_face, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->f2c,
-                       c->dg0, c->fw, c->fn, u, x, c->tang_weight, c->twd_norm, c->wsum);
+                       c->fxy, c->cmom, c->cscale, c->fw, c->fn, u, x, c->tang_weight, c->twd_norm, c->wsum);
     prof_end(c);
     hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(64), 0, c->stream, c->lim_count);
     prof_begin(c, DNDSMI_K_WBAP_CELL);
@@ -1624,17 +1647,15 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
 int ensure_mixed(dndsmi_ctx *c) {
     if (c->M32)
         return DNDSMI_OK;
-    const int64_t nm = int64_t(c->nfmax) * kMQuads * 4 * c->ldc, nd = 56 * c->ldf;
+    const int64_t nm = int64_t(c->nfmax) * kMQuads * 4 * c->ldc;
     DNDSMI_TRY(c->alloc(c->M32, nm));
-    DNDSMI_TRY(c->alloc(c->fd0_32, nd));
     DNDSMI_TRY(c->alloc(c->r32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->dA32, 36 * c->ldc));
     DNDSMI_TRY(c->alloc(c->dB32, 36 * c->ldc));
-    DNDSMI_TRY(c->alloc(c->fflux32, 12 * c->ldf));
+    DNDSMI_TRY(c->alloc(c->fslot32, int64_t(c->nfmax) * 8 * c->ldc));
     DNDSMI_TRY(c->alloc(c->wcorr32, 36 * c->n_wall));
     hipLaunchKernelGGL(k_m_to_f32, dim3(grid_for(nm / 4)), dim3(kBlock), 0, c->stream, int32_t(c->nfmax), c->ldc,
                        c->M, c->M32);
-    hipLaunchKernelGGL(k_fd0_to_f32, dim3(grid_for(nd / 4)), dim3(kBlock), 0, c->stream, c->ldf, c->fd0, c->fd0_32);
     DNDSMI_TRY(c->launch_check("k_to_f32"));
     return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }
@@ -1701,22 +1722,23 @@ int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, 
                      const double *dt, double coefA, double coefB) {
     const int nc = int(c->nc), nf = int(c->nf);
     prof_begin(c, DNDSMI_K_RHS_FACE32);
-    hipLaunchKernelGGL((k_rhs_face<false, true>), dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf,
-                       c->ldc, c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t,
-                       c->fflux, c->errw, c->fd0_32, c->fflux32);
+    hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, true, true> : k_rhs_face<false, true, false>),
+                       dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc,
+                       c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw,
+                       c->fslot32);
     prof_end(c);
     if (c->n_bface > 0)
         hipLaunchKernelGGL((k_rhs_face<true, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
-                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fd0, c->fn, c->fctr,
-                           c->dlambda_face, u, x, c->gas, t, c->fflux, c->errw, c->fd0_32, c->fflux32);
+                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
+                           c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw, c->fslot32);
     prof_begin(c, DNDSMI_K_RHS_CELL32);
     if (c->nfmax == 3)
-        hipLaunchKernelGGL((k_rhs_cell<3, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
-                           c->gat, c->fflux32, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB,
+        hipLaunchKernelGGL((k_rhs_cell<3, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
+                           c->gat, c->fslot32, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB,
                            c->errw);
     else
-        hipLaunchKernelGGL((k_rhs_cell<4, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
-                           c->gat, c->fflux32, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB,
+        hipLaunchKernelGGL((k_rhs_cell<4, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
+                           c->gat, c->fslot32, c->vol, rhs, stage, xs, c->xlast, dt, c->dt_min, coefA, coefB,
                            c->errw);
     prof_end(c);
     return c->launch_check("mixed rhs kernels");
@@ -1900,12 +1922,14 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return nf * (8 + 16 + 16) + nc * 32;
     case DNDSMI_K_DT_CELL: // gather index, lambda + area per slot, volume, dt out
         return c2f * (4 + 16) + nc * (8 + 8);
-    case DNDSMI_K_RHS_FACE: // f2c, D0 rows, normals, dlambda, 12 flux values out; x and u once per cell
-        return nif * (8 + 54 * 8 + 48 + 8 + 96) + nc * (36 * 8 + 32);
-    case DNDSMI_K_RHS_CELL: // gather index, F + side fix per slot, volume, x read+write, xLast
-        return c2f * (4 + 64) + nc * (8 + 32 * 3);
-    case DNDSMI_K_WBAP_FACE: // all-row bases of both sides, weights, normal, 8 terms out; x (rho, E) + u per cell
-        return nif * (180 * 8 + 80 + 16 + 8 + 64) + nc * (18 * 8 + 16);
+    case DNDSMI_K_RHS_FACE: // f2c, slot positions, point coordinates, normals, dlambda, 2 x 8 terms out;
+        // x, u and the moments once per cell
+        return nif * (8 + 1 + 96 + (c->straight_faces ? 16 : 48) + 8 + 128) + nc * (36 * 8 + 32 + 72);
+    case DNDSMI_K_RHS_CELL: // 8 terms per slot, volume, x read+write, xLast (+ last-slot index for 4 slots)
+        return c2f * 64 + nc * (8 + 32 * 3 + (c->nfmax == 4 ? 4 : 0));
+    case DNDSMI_K_WBAP_FACE: // point-0 coordinates of both sides, f2c, weights, normal, 8 terms out;
+        // x (rho, E), u (rho, E), moments and scales once per cell
+        return nif * (32 + 8 + 80 + 16 + 64) + nc * (18 * 8 + 16 + 72 + 16);
     case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out, x read and copied out
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
@@ -1916,10 +1940,10 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return nc * (32 + 32 + 8 + 8 + 4) + c2f * (4 + 4 + 4 + 4 + 16 + 8 + 8) + c2f * 64 / 2;
     case DNDSMI_K_DELTA32: // FP32 M per slot + slot index; d read + r read + d write (FP32)
         return c2f * (81 * 4 + 4) + nc * (36 * 4 * 3);
-    case DNDSMI_K_RHS_FACE32: // f2c, FP32 D0 rows, normals, dlambda, 12 FP32 fluxes out; x and u once per cell
-        return nif * (8 + 54 * 4 + 48 + 8 + 48) + nc * (36 * 8 + 32);
-    case DNDSMI_K_RHS_CELL32: // gather index, FP32 F + side fix per slot, volume, x read+write, xLast
-        return c2f * (4 + 32) + nc * (8 + 32 * 3);
+    case DNDSMI_K_RHS_FACE32: // as DNDSMI_K_RHS_FACE with 2 x 8 FP32 terms out
+        return nif * (8 + 1 + 96 + (c->straight_faces ? 16 : 48) + 8 + 64) + nc * (36 * 8 + 32 + 72);
+    case DNDSMI_K_RHS_CELL32: // 8 FP32 terms per slot, volume, x read+write, xLast (+ last-slot index)
+        return c2f * 32 + nc * (8 + 32 * 3 + (c->nfmax == 4 ? 4 : 0));
     default:
         return 0;
     }

@@ -64,7 +64,9 @@ struct dndsmi_ctx {
     // face side
     int32_t *f2c = nullptr;    // [2][ldf]
     int32_t *fbc = nullptr;    // [ldf]
-    double *fd0 = nullptr;     // [28][ldf][2] value bases at the 3 points, per face [side][c][g], 28 per side
+    double *fxy = nullptr;     // [6][ldf][2] (X, Y) of LINE_3 point g seen from side s, row 3 s + g
+    double *cmom = nullptr;    // [9][ldc] base moments 1..9 of the cell
+    double *cscale = nullptr;  // [ldc][2] (sx, sy) of the cell's scaled monomials
     double *fn = nullptr;      // [6][ldf] normals at the LINE_3 points
     double *fnc = nullptr;     // [2][ldf] normal at the face centre
     double *farea = nullptr;   // [ldf]
@@ -73,7 +75,6 @@ struct dndsmi_ctx {
     int64_t n_bface = 0;
     // WBAP data (present when the setup carries limiter data)
     bool has_limiter = false;
-    double *dg0 = nullptr;     // [90][ldf][2] all DiBj rows at LINE_3 point 0, per face [side][c][r]
     double *fw = nullptr;      // [10][ldf] face weights
     double *msec = nullptr;    // [162][ldf] matrixSecondary (R2L, L2R)
     double *wsum = nullptr;    // [ldf][8] per-face indicator terms of both cells
@@ -88,7 +89,12 @@ struct dndsmi_ctx {
     // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
     double *lambda_face = nullptr;
     double *dlambda_face = nullptr;
-    double *fflux = nullptr;   // [12][ldf] F, FLfix, FRfix integrated over the face
+    // face fluxes integrated over the face, stored at the slot of each side's
+    // cell: [nfmax][8][ldc], per slot F then the side's flux-balance term,
+    // signed as the cell adds them (side 0: F, -FLfix; side 1: -F, FRfix)
+    double *fslot = nullptr;
+    uint8_t *fpos = nullptr;   // [ldf] slot of the face in cell L (low 4 bits) and in cell R (high 4 bits)
+    bool straight_faces = false; // every face's normal is bitwise the same at its three LINE_3 points
     double *dt_part = nullptr; // [grid] partial minima
     double *dt_min = nullptr;  // [1]
     int32_t *errw = nullptr;   // [1] error bits
@@ -103,12 +109,11 @@ struct dndsmi_ctx {
     int32_t *colour = nullptr;        // [ldc]
     int32_t *col_cells = nullptr;     // [nc]
     double *jinv = nullptr;           // [ldc] 1 / LU-SGS diagonal
-    // mixed precision (allocated on first use): FP32 copies of A^-1 B and of
-    // the face bases, FP32 correction vectors, FP32 face fluxes
+    // mixed precision (allocated on first use): FP32 copy of A^-1 B, FP32
+    // correction vectors, FP32 face fluxes
     float *M32 = nullptr;             // [nfmax][21][ldc][4]
-    float *fd0_32 = nullptr;          // [14][ldf][4]
     float *r32 = nullptr, *dA32 = nullptr, *dB32 = nullptr; // [36][ldc]
-    float *fflux32 = nullptr;         // [12][ldf]
+    float *fslot32 = nullptr;         // [nfmax][8][ldc]
     float *wcorr32 = nullptr;         // [36][n_wall] linear wall correction of a correction sweep
     // implicit dual-time scratch (allocated on first use)
     double *imp_rhs = nullptr;        // [ldc][4]

@@ -361,47 +361,69 @@ __device__ inline void stable_normalized(double x, double y, double &ox, double 
         ox = x, oy = y;
 }
 
-// fluxFace (DNDS_FV_EulerEvaluator.hpp:371-591) for muGas = 0: the Riemann
-// flux in the face frame, rotated back and negated, and the flux-balance
-// fluxes of the two cell means.
-__device__ inline bool flux_face(const GasParams &gp, const double ULxy[4], const double URxy[4],
-                                 const double ULm[4], const double URm[4], double n0, double n1, double dLambda,
-                                 double out[4], double FLfix[4], double FRfix[4]) {
-    auto rotT = [&](const double in[4], double o[4]) {
-        o[0] = in[0];
-        o[1] = n0 * in[1] + n1 * in[2];
-        o[2] = (-n1) * in[1] + n0 * in[2];
-        o[3] = in[3];
-    };
-    auto rot = [&](double v[4]) {
-        const double a = n0 * v[1] + (-n1) * v[2];
-        const double b = n1 * v[1] + n0 * v[2];
-        v[1] = a, v[2] = b;
-    };
+// fluxFace (DNDS_FV_EulerEvaluator.hpp:371-591) for muGas = 0, in two parts.
+// balance_fluxes: the flux-balance fluxes FLfix, FRfix of the two cell means
+// in the face frame, rotated back (:487-506).
+__device__ inline bool balance_fluxes(const GasParams &gp, const double ULm[4], const double URm[4], double n0,
+                                      double n1, double FLfix[4], double FRfix[4]) {
+    double Lm[4], Rm[4], wL[4], wR[4];
+    Lm[0] = ULm[0], Lm[1] = n0 * ULm[1] + n1 * ULm[2], Lm[2] = (-n1) * ULm[1] + n0 * ULm[2], Lm[3] = ULm[3];
+    Rm[0] = URm[0], Rm[1] = n0 * URm[1] + n1 * URm[2], Rm[2] = (-n1) * URm[1] + n0 * URm[2], Rm[3] = URm[3];
+    if (!(Lm[0] > 0) || !(Rm[0] > 0))
+        return false;
+    cons2prim(Lm, wL, gp.gamma);
+    cons2prim(Rm, wR, gp.gamma);
+    inviscid_flux(Lm, wL[1], wL[3], FLfix);
+    inviscid_flux(Rm, wR[1], wR[3], FRfix);
+    double a = n0 * FLfix[1] + (-n1) * FLfix[2], b = n1 * FLfix[1] + n0 * FLfix[2];
+    FLfix[1] = a, FLfix[2] = b;
+    a = n0 * FRfix[1] + (-n1) * FRfix[2], b = n1 * FRfix[1] + n0 * FRfix[2];
+    FRfix[1] = a, FRfix[2] = b;
+    return true;
+}
+
+// riemann_face: the Riemann flux in the face frame, rotated back and negated
+// (:519-591).
+__device__ inline bool riemann_face(const GasParams &gp, const double ULxy[4], const double URxy[4], double n0,
+                                    double n1, double dLambda, double out[4]) {
     double UL[4], UR[4];
-    rotT(ULxy, UL);
-    rotT(URxy, UR);
-    {
-        double Lm[4], Rm[4], wL[4], wR[4];
-        rotT(ULm, Lm);
-        rotT(URm, Rm);
-        if (!(Lm[0] > 0) || !(Rm[0] > 0))
-            return false;
-        cons2prim(Lm, wL, gp.gamma);
-        cons2prim(Rm, wR, gp.gamma);
-        inviscid_flux(Lm, wL[1], wL[3], FLfix);
-        inviscid_flux(Rm, wR[1], wR[3], FRfix);
-        rot(FLfix);
-        rot(FRfix);
-    }
+    UL[0] = ULxy[0], UL[1] = n0 * ULxy[1] + n1 * ULxy[2], UL[2] = (-n1) * ULxy[1] + n0 * ULxy[2], UL[3] = ULxy[3];
+    UR[0] = URxy[0], UR[1] = n0 * URxy[1] + n1 * URxy[2], UR[2] = (-n1) * URxy[1] + n0 * URxy[2], UR[3] = URxy[3];
     double f[4];
     const bool ok = gp.rs == 2 ? hllc_flux(UL, UR, gp.gamma, f) : roe_flux(UL, UR, gp.gamma, dLambda, f);
     if (!ok)
         return false;
-    rot(f);
-    for (int k = 0; k < 4; k++)
-        out[k] = -f[k];
+    const double a = n0 * f[1] + (-n1) * f[2];
+    const double b = n1 * f[1] + n0 * f[2];
+    out[0] = -f[0], out[1] = -a, out[2] = -b, out[3] = -f[3];
     return true;
+}
+
+__device__ inline bool flux_face(const GasParams &gp, const double ULxy[4], const double URxy[4],
+                                 const double ULm[4], const double URm[4], double n0, double n1, double dLambda,
+                                 double out[4], double FLfix[4], double FRfix[4]) {
+    return balance_fluxes(gp, ULm, URm, n0, n1, FLfix, FRfix) && riemann_face(gp, ULxy, URxy, n0, n1, dLambda, out);
+}
+
+// ------------------------------------------------------------ bases ----
+// The scaled cubic monomials of FDiffBaseValue in global coordinates
+// (DNDS_FV_VR.hpp:300-356), evaluated at a face point from its scaled
+// coordinates (X, Y) = ((px - cx) / sx, (py - cy) / sy) instead of being
+// stored per face: entry (r, ib) of the 10 x 10 DiBj block is
+//   ((coef(r, ib) X^(bx - rx)) Y^(by - ry)) / (sx^rx sy^ry)   (r > 0)
+//   X^bx Y^by - moment[ib]                                    (r = 0)
+// with (bx, by), (rx, ry) the exponents of base ib and derivative r. Integer
+// powers are formed by repeated multiplication and every operation is the
+// host setup's, in its order, so each value equals the setup's bit for bit.
+constexpr int kBX[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0}, kBY[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+constexpr int kDFac[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 2, 0}, {1, 3, 6, 6}};
+
+__host__ __device__ constexpr int dibj_coef(int r, int ib) {
+    return (kBX[r] <= kBX[ib] ? kDFac[kBX[ib]][kBX[r]] : 0) * (kBY[r] <= kBY[ib] ? kDFac[kBY[ib]][kBY[r]] : 0);
+}
+
+__device__ __forceinline__ void int_powers(double a, double p[4]) {
+    p[0] = 1.0, p[1] = a, p[2] = a * a, p[3] = p[2] * a;
 }
 
 // ----------------------------------------------------------- WBAP ----
@@ -439,34 +461,53 @@ __device__ inline FFParams ff_params(const double W[10], double gx, double gy, d
 }
 
 // Conj(col, col) of FFaceFunctional for one column of a 10-row block
-__device__ inline double ff_self(const FFParams &p, const double d[10]) {
+// FFaceFunctional's self product of one 10-row derivative vector d, split by
+// the groups of rows that its terms mix -- {0}, {1, 2}, {3, 4, 5}, {6..9}:
+// ff_group<G> adds group G's terms to c (d points at the group's first row),
+// in the order of the full sum, so that calling it for G = 0..3 on the same
+// c gives ff_self's value bit for bit.
+template <int G>
+__device__ inline void ff_group(const FFParams &p, const double *dg, double &c) {
     const double n0 = p.n0, n1 = p.n1, t0 = p.t0, t1 = p.t1, w2r = p.w2r, w3r = p.w3r;
-    double a[10];
-    a[0] = d[0];
-    a[1] = d[1] * n0 + d[2] * n1;
-    a[2] = d[1] * t0 + d[2] * t1;
-    a[3] = (d[3] * n0 * n0 + d[4] * n0 * n1 * 2 + d[5] * n1 * n1) * w2r;
-    a[4] = (d[3] * n0 * t0 + d[4] * (n0 * t1 + n1 * t0) + d[5] * n1 * t1) * w2r;
-    a[5] = (d[3] * t0 * t0 + d[4] * t0 * t1 * 2 + d[5] * t1 * t1) * w2r;
-    a[6] = (d[6] * n0 * n0 * n0 + d[7] * n0 * n0 * n1 * 3 + d[8] * n0 * n1 * n1 * 3 + d[9] * n1 * n1 * n1) * w3r;
-    a[7] = (d[6] * n0 * n0 * t0 + d[7] * (n0 * n0 * t1 + n0 * t0 * n1 + t0 * n0 * n1) +
-            d[8] * (n0 * n1 * t1 + n0 * t1 * n1 + t0 * n1 * n1) + d[9] * n1 * n1 * t1) *
-           w3r;
-    a[8] = (d[6] * n0 * t0 * t0 + d[7] * (t0 * n0 * t1 + n0 * t0 * t1 + t0 * t0 * n1) +
-            d[8] * (t0 * n1 * t1 + n0 * t1 * t1 + t0 * t1 * n1) + d[9] * n1 * t1 * t1) *
-           w3r;
-    a[9] = (d[6] * t0 * t0 * t0 + d[7] * t0 * t0 * t1 * 3 + d[8] * t0 * t1 * t1 * 3 + d[9] * t1 * t1 * t1) * w3r;
+    if (G == 0) {
+        const double a0 = dg[0];
+        c += a0 * a0 * p.W0 * p.W0;
+    } else if (G == 1) {
+        const double a1 = dg[0] * n0 + dg[1] * n1;
+        const double a2 = dg[0] * t0 + dg[1] * t1;
+        c += a1 * a1;
+        c += a2 * a2 * 1.0;
+    } else if (G == 2) {
+        const double d3 = dg[0], d4 = dg[1], d5 = dg[2];
+        const double a3 = (d3 * n0 * n0 + d4 * n0 * n1 * 2 + d5 * n1 * n1) * w2r;
+        const double a4 = (d3 * n0 * t0 + d4 * (n0 * t1 + n1 * t0) + d5 * n1 * t1) * w2r;
+        const double a5 = (d3 * t0 * t0 + d4 * t0 * t1 * 2 + d5 * t1 * t1) * w2r;
+        c += a3 * a3;
+        c += a4 * a4 * 2 * 1.0;
+        c += a5 * a5 * 1.0;
+    } else {
+        const double d6 = dg[0], d7 = dg[1], d8 = dg[2], d9 = dg[3];
+        const double a6 = (d6 * n0 * n0 * n0 + d7 * n0 * n0 * n1 * 3 + d8 * n0 * n1 * n1 * 3 + d9 * n1 * n1 * n1) * w3r;
+        const double a7 = (d6 * n0 * n0 * t0 + d7 * (n0 * n0 * t1 + n0 * t0 * n1 + t0 * n0 * n1) +
+                           d8 * (n0 * n1 * t1 + n0 * t1 * n1 + t0 * n1 * n1) + d9 * n1 * n1 * t1) *
+                          w3r;
+        const double a8 = (d6 * n0 * t0 * t0 + d7 * (t0 * n0 * t1 + n0 * t0 * t1 + t0 * t0 * n1) +
+                           d8 * (t0 * n1 * t1 + n0 * t1 * t1 + t0 * t1 * n1) + d9 * n1 * t1 * t1) *
+                          w3r;
+        const double a9 = (d6 * t0 * t0 * t0 + d7 * t0 * t0 * t1 * 3 + d8 * t0 * t1 * t1 * 3 + d9 * t1 * t1 * t1) * w3r;
+        c += a6 * a6;
+        c += a7 * a7 * 3 * 1.0;
+        c += a8 * a8 * 3 * 1.0;
+        c += a9 * a9 * 1.0;
+    }
+}
+
+__device__ inline double ff_self(const FFParams &p, const double d[10]) {
     double c = 0;
-    c += a[0] * a[0] * p.W0 * p.W0;
-    c += a[1] * a[1];
-    c += a[2] * a[2] * 1.0;
-    c += a[3] * a[3];
-    c += a[4] * a[4] * 2 * 1.0;
-    c += a[5] * a[5] * 1.0;
-    c += a[6] * a[6];
-    c += a[7] * a[7] * 3 * 1.0;
-    c += a[8] * a[8] * 3 * 1.0;
-    c += a[9] * a[9] * 1.0;
+    ff_group<0>(p, d, c);
+    ff_group<1>(p, d + 1, c);
+    ff_group<2>(p, d + 3, c);
+    ff_group<3>(p, d + 6, c);
     return c;
 }
 

@@ -177,22 +177,29 @@ Cell load_cell(const dndsmi_mesh_view &m, int64_t ic, double shx = 0, double shy
     return c;
 }
 
+// Integer powers 0..3 by repeated multiplication: p[2] = a a, p[3] = (a a) a.
+// The reference takes std::pow (FPolynomial3D, DNDS_FV_VR.hpp:254-259); the
+// two agree to an ulp, and this form is the one the device kernels repeat
+// when they evaluate the bases at face points instead of storing them.
+inline void int_powers(double a, double p[4]) {
+    p[0] = 1.0, p[1] = a, p[2] = a * a, p[3] = p[2] * a;
+}
+
 // FDiffBaseValue, global coordinates (DNDS_FV_VR.hpp:300-356): D[idiff][ibase],
 // 10x10, derivatives of the scaled cubic monomials about (cx, cy), row 0
-// minus the base moments.
-void diff_base(double px, double py, double cx, double cy, double sx, double sy, const double *moment, double D[10][10]) {
+// minus the base moments.  XY (optional) receives the scaled coordinates.
+void diff_base(double px, double py, double cx, double cy, double sx, double sy, const double *moment, double D[10][10],
+               double *XY = nullptr) {
     double X = (px - cx) / sx;
     double Y = (py - cy) / sy;
+    if (XY)
+        XY[0] = X, XY[1] = Y;
     double powX[4], powY[4];
-    for (int k = 0; k < 4; k++) {
-        powX[k] = std::pow(X, double(k));
-        powY[k] = std::pow(Y, double(k));
-    }
+    int_powers(X, powX);
+    int_powers(Y, powY);
     double sPowX[4], sPowY[4];
-    for (int k = 0; k < 4; k++) {
-        sPowX[k] = std::pow(sx, double(k));
-        sPowY[k] = std::pow(sy, double(k));
-    }
+    int_powers(sx, sPowX);
+    int_powers(sy, sPowY);
     for (int id = 0; id < 10; id++) {
         const int ndx = kDiffX[id], ndy = kDiffY[id];
         const double den = sPowX[ndx] * sPowY[ndy] * 1.0; // pow(1, ndz) = 1
@@ -260,8 +267,9 @@ bool face_inverted(const Ctx &C, int64_t f, int side) {
     return !(d0 < d1);
 }
 
-// 10x10 DiBj of side `side` of face f at face coordinate xi.
-void face_dibj(const Ctx &C, int64_t f, int side, double xi, double D[10][10]) {
+// 10x10 DiBj of side `side` of face f at face coordinate xi (and, optionally,
+// the scaled coordinates of the point seen from that side).
+void face_dibj(const Ctx &C, int64_t f, int side, double xi, double D[10][10], double *XY = nullptr) {
     const dndsmi_mesh_view &m = *C.m;
     FaceSide fs = face_side(C, f, side);
     double pxi, peta;
@@ -269,7 +277,7 @@ void face_dibj(const Ctx &C, int64_t f, int side, double xi, double D[10][10]) {
     double px, py, detJ;
     map_point(fs.c, pxi, peta, px, py, detJ);
     diff_base(px, py, fs.cx, fs.cy, C.scale[2 * fs.cell], C.scale[2 * fs.cell + 1],
-              &C.out->base_moment[10 * fs.cell], D);
+              &C.out->base_moment[10 * fs.cell], D, XY);
 }
 
 inline bool is_far(int32_t bc) {
@@ -504,6 +512,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
         out->face_norm.assign(6 * nf, 0.0);
         out->face_weight.assign(10 * nf, 0.0);
         out->face_d0.assign(size_t(54) * nf, 0.0);
+        out->face_gauss_xy.assign(size_t(12) * nf, 0.0);
         out->ainv.assign(size_t(81) * nc, 0.0);
         out->ainv_b.assign(size_t(81) * nc2f, 0.0);
         out->ainv_bvec.assign(size_t(9) * nc2f, 0.0);
@@ -560,6 +569,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
         });
         if (failed)
             throw SetupError("non-positive cell Jacobian (orientation wrong or distorted)");
+        out->cell_scale = C.scale;
 
         // ---- pass 2: faces -- area (LINE_5), centre and normals, weights,
         // row 0 of the Gauss-point bases, limiter data (DNDS_FV_VR.hpp:75-101,
@@ -656,7 +666,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
             double D[10][10];
             for (int g = 0; g < 3; g++)
                 for (int side = 0; side < (R != DNDSMI_NO_CELL ? 2 : 1); side++) {
-                    face_dibj(C, f, side, kLine3X[g], D);
+                    face_dibj(C, f, side, kLine3X[g], D, &out->face_gauss_xy[12 * f + 4 * g + 2 * side]);
                     for (int c = 0; c < 9; c++)
                         out->face_d0[54 * f + 18 * g + 9 * side + c] = D[0][c + 1];
                     if (setting->build_limiter_data && g == 0)
@@ -807,6 +817,8 @@ extern "C" int dndsmi_vr_get_view(const dndsmi_vr *vr, dndsmi_vr_view *v) {
     v->ainv = vr->ainv.data();
     v->ainv_b = vr->ainv_b.data();
     v->ainv_bvec = vr->ainv_bvec.data();
+    v->face_gauss_xy = vr->face_gauss_xy.data();
+    v->cell_scale = vr->cell_scale.data();
     return DNDSMI_OK;
 }
 

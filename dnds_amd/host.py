@@ -138,6 +138,7 @@ SETUP_SIZES = {  # name -> (per, count) where per is "cell", "face" or "c2f"
     "face_center": ("face", 2), "face_norm_center": ("face", 2), "face_norm": ("face", 6),
     "face_weight": ("face", 10), "face_d0": ("face", 54), "face_dibj_g0": ("face", 180),
     "matrix_secondary": ("face", 162), "ainv": ("cell", 81), "ainv_b": ("c2f", 81), "ainv_bvec": ("c2f", 9),
+    "face_gauss_xy": ("face", 12), "cell_scale": ("cell", 2),
 }
 
 

@@ -165,6 +165,12 @@ typedef struct dndsmi_vr_view {
     const double *ainv;               /* [n_cell][9][9] matrixBatch.m(0)          */
     const double *ainv_b;             /* [n_c2f][9][9] matrixBatch.m(ic2f+1)      */
     const double *ainv_bvec;          /* [n_c2f][9] vectorBatch rows              */
+    /* [n_face][3][2][2] (X, Y) = ((px - cx) / sx, (py - cy) / sy) of LINE_3
+     * point g seen from side s: the arguments of the scaled monomials whose
+     * values (minus the cell's moments) and derivatives form face_d0 and
+     * face_dibj_g0; the device evaluates the bases from these. */
+    const double *face_gauss_xy;
+    const double *cell_scale;         /* [n_cell][2] (sx, sy) of FDiffBaseValue (CoordMinMaxScale) */
 } dndsmi_vr_view;
 
 typedef struct dndsmi_vr dndsmi_vr;

@@ -127,7 +127,8 @@ def build(mesh: dict, setting: dict) -> dict:
                      for a, b, w in rule_b) / vol[i]
         mom[i, 0] = 1.0
 
-    def side_basis(f, side, xi):
+    def side_point(f, side, xi):
+        """(point, base centre) of face point xi seen from side `side`"""
         i = f2c[f, side]
         sh = shift[f] if side == 1 else np.zeros(2)
         X = coords[i] + sh
@@ -138,7 +139,12 @@ def build(mesh: dict, setting: dict) -> dict:
         inv = not np.allclose(start, fx0, atol=1e-12 * (1 + np.abs(fx0).max()))
         a, b = face_param(nv, lf, inv, xi)
         p, _ = point_det(X, a, b)
-        return basis(p, bary[i] + sh, scale[i], mom[i])
+        return p, bary[i] + sh
+
+    def side_basis(f, side, xi):
+        p, c = side_point(f, side, xi)
+        i = f2c[f, side]
+        return basis(p, c, scale[i], mom[i])
 
     fn = np.zeros((nf, 2))
     area = np.zeros(nf)
@@ -175,6 +181,7 @@ def build(mesh: dict, setting: dict) -> dict:
         W[f] = Wf
 
     d0 = np.zeros((nf, 3, 2, 9))
+    gxy = np.zeros((nf, 3, 2, 2))
     dg0 = np.zeros((nf, 2, 10, 9))
     ms = np.zeros((nf, 2, 9, 9))
     for f in range(nf):
@@ -182,6 +189,8 @@ def build(mesh: dict, setting: dict) -> dict:
             for g in range(3):
                 Db = side_basis(f, side, G3[g])
                 d0[f, g, side] = Db[0, 1:]
+                p, c = side_point(f, side, G3[g])
+                gxy[f, g, side] = (p - c) / scale[f2c[f, side]]
                 if g == 0:
                     dg0[f, side] = Db[:, 1:]
         if f2c[f, 1] != NO_CELL:
@@ -222,4 +231,5 @@ def build(mesh: dict, setting: dict) -> dict:
             ainv_bvec[cp[i] + k] = Ai @ row
     return dict(volume=vol, face_area=area, cell_bary=bary.ravel(), base_moment=mom.ravel(), face_center=fc.ravel(),
                 face_norm_center=fn.ravel(), face_weight=W.ravel(), face_d0=d0.ravel(), face_dibj_g0=dg0.ravel(),
-                matrix_secondary=ms.ravel(), ainv=ainv.ravel(), ainv_b=ainv_b.ravel(), ainv_bvec=ainv_bvec.ravel())
+                matrix_secondary=ms.ravel(), ainv=ainv.ravel(), ainv_b=ainv_b.ravel(), ainv_bvec=ainv_bvec.ravel(),
+                face_gauss_xy=gxy.ravel(), cell_scale=np.asarray(scale, dtype=float).ravel())

@@ -32,7 +32,7 @@ def test_setup_matches_numpy_restatement(kind, sname, bcset):
     got = vfv.arrays()
     ref = setup_np.build(mesh.arrays(), s)
     for k in ("volume", "face_area", "cell_bary", "base_moment", "face_center", "face_norm_center", "face_weight",
-              "face_d0", "face_dibj_g0"):
+              "face_d0", "face_dibj_g0", "face_gauss_xy", "cell_scale"):
         assert rel_err(got[k], ref[k]) <= 1e-13, k
     # A_ii^-1 and its products carry the conditioning of A_ii
     for k in ("ainv", "ainv_b", "ainv_bvec", "matrix_secondary"):
