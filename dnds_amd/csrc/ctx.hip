@@ -1934,10 +1934,10 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
-    case DNDSMI_K_LUSGS_SWEEP: // one whole forward (or backward) sweep, all colours together: per cell rhs,
-        // increment out, 1/D, volume, colour; per slot neighbour, colour, face, f2c, normal, area, lambda,
-        // and u, uInc of the neighbour (read by the colour that needs it: once per slot over both sweeps)
-        return nc * (32 + 32 + 8 + 8 + 4) + c2f * (4 + 4 + 4 + 4 + 16 + 8 + 8) + c2f * 64 / 2;
+    case DNDSMI_K_LUSGS_SWEEP: // one whole forward (or backward) sweep, all colours together: per scan position
+        // cell, volume, 1/D, rhs (or the increment) in, increment out; per slot the neighbour index, and per used
+        // slot (half of them) face, normal, area, u and the increment of the neighbour, lambda
+        return nc * (4 + 8 + 8 + 32 + 32) + c2f * 4 + c2f / 2 * (4 + 16 + 8 + 32 + 32 + 8);
     case DNDSMI_K_DELTA32: // FP32 M per slot + slot index; d read + r read + d write (FP32)
         return c2f * (81 * 4 + 4) + nc * (36 * 4 * 3);
     case DNDSMI_K_RHS_FACE32: // as DNDSMI_K_RHS_FACE with 2 x 8 FP32 terms out

@@ -108,7 +108,13 @@ struct dndsmi_ctx {
     std::vector<int32_t> colour_host; // [nc]
     int32_t *colour = nullptr;        // [ldc]
     int32_t *col_cells = nullptr;     // [nc]
-    double *jinv = nullptr;           // [ldc] 1 / LU-SGS diagonal
+    double *jinv = nullptr;           // [ldc] 1 / LU-SGS diagonal, scan order
+    // per scan position p (col_cells order), built on first LU-SGS use:
+    // per slot [s][ldc] the neighbour used by the forward / backward sweep
+    // (-1 if none), the face (-1 if no slot), the unit normal out of the
+    // cell and the face area; per position the volume
+    int32_t *sc_jf = nullptr, *sc_jb = nullptr, *sc_face = nullptr;
+    double *sc_n0 = nullptr, *sc_n1 = nullptr, *sc_area = nullptr, *sc_vol = nullptr;
     // mixed precision (allocated on first use): FP32 copy of A^-1 B, FP32
     // correction vectors, FP32 face fluxes
     float *M32 = nullptr;             // [nfmax][21][ldc][4]

@@ -42,46 +42,88 @@ __device__ inline void flux_jacobian_times_du(double gamma, const double U[4], d
         dF[k] -= lambdaMain * dU[k];
 }
 
-// LUSGSMatrixInit (DNDS_FV_EulerEvaluator.cpp:995-1047), jacobianCode 0, zero
-// source Jacobian: jinv_i = 1 / (1/dTau_i + 1/dt + sum_f (alpha/2) S_f lambda_f / V_i)
+// Scan-order geometry of the sweeps, one thread per scan position: which
+// neighbours each sweep uses (a lower colour is earlier, a higher one later),
+// the face, its unit normal pointed out of the cell, its area, the volume.
 template <int NF>
-__global__ __launch_bounds__(kBlock) void k_lusgs_init(int32_t nc, int64_t ldc, const int32_t *__restrict__ sface,
+__global__ __launch_bounds__(kBlock) void k_lusgs_scan(int32_t nc, int64_t ldc, int64_t ldf,
+                                                       const int32_t *__restrict__ cells,
+                                                       const int32_t *__restrict__ nbr,
+                                                       const int32_t *__restrict__ sface,
+                                                       const int32_t *__restrict__ f2c,
+                                                       const int32_t *__restrict__ colour,
+                                                       const double *__restrict__ fnc,
                                                        const double *__restrict__ farea,
-                                                       const double *__restrict__ lambda_face,
-                                                       const double *__restrict__ vol,
-                                                       const double *__restrict__ dtau, double dt, double alpha,
-                                                       double *__restrict__ jinv) {
-    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= nc)
+                                                       const double *__restrict__ vol, int32_t *__restrict__ jf,
+                                                       int32_t *__restrict__ jb, int32_t *__restrict__ sf,
+                                                       double *__restrict__ sn0, double *__restrict__ sn1,
+                                                       double *__restrict__ sS, double *__restrict__ sV) {
+    const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nc)
         return;
-    double fp = 1.0 / dtau[i] + 1.0 / dt;
-    const double v = vol[i];
+    const int32_t i = cells[p], col = colour[i];
+    sV[p] = vol[i];
 #pragma unroll
     for (int s = 0; s < NF; s++) {
-        const int32_t f = sface[s * ldc + i];
+        const int32_t j = nbr[s * ldc + i], f = sface[s * ldc + i];
+        const int32_t cj = j >= 0 ? colour[j] : col;
+        jf[s * ldc + p] = j >= 0 && cj < col ? j : -1;
+        jb[s * ldc + p] = j >= 0 && cj > col ? j : -1;
+        sf[s * ldc + p] = f;
+        double n0 = 0.0, n1 = 0.0, S = 0.0;
+        if (f >= 0) {
+            normalized(fnc[f], fnc[ldf + f], n0, n1);
+            if (f2c[f] != i) {
+                n0 *= -1.0;
+                n1 *= -1.0;
+            }
+            S = farea[f];
+        }
+        sn0[s * ldc + p] = n0, sn1[s * ldc + p] = n1, sS[s * ldc + p] = S;
+    }
+}
+
+// LUSGSMatrixInit (DNDS_FV_EulerEvaluator.cpp:995-1047), jacobianCode 0, zero
+// source Jacobian, per scan position:
+// jinv = 1 / (1/dTau_i + 1/dt + sum_f (alpha/2) S_f lambda_f / V_i)
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_lusgs_init(int32_t nc, int64_t ldc, const int32_t *__restrict__ cells,
+                                                       const int32_t *__restrict__ sf, const double *__restrict__ sS,
+                                                       const double *__restrict__ sV,
+                                                       const double *__restrict__ lambda_face,
+                                                       const double *__restrict__ dtau, double dt, double alpha,
+                                                       double *__restrict__ jinv) {
+    const int32_t p = xcd_block() * blockDim.x + threadIdx.x;
+    if (p >= nc)
+        return;
+    double fp = 1.0 / dtau[cells[p]] + 1.0 / dt;
+    const double v = sV[p];
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+        const int32_t f = sf[s * ldc + p];
         if (f < 0)
             continue;
-        fp += (0.5 * alpha) * farea[f] * lambda_face[f] / v;
+        fp += (0.5 * alpha) * sS[s * ldc + p] * lambda_face[f] / v;
     }
-    jinv[i] = 1.0 / (fp + alpha * 0.0);
+    jinv[p] = 1.0 / (fp + alpha * 0.0);
 }
 
 // One colour of an LU-SGS sweep (UpdateLUSGSForward / UpdateLUSGSBackward,
-// DNDS_FV_EulerEvaluator.cpp:1149-1357): forward
+// DNDS_FV_EulerEvaluator.cpp:1149-1357), scan positions [b, b + n): forward
 //   uIncNew_i = jinv_i (rhs_i - sum_{earlier j} c_f dF(u_j, uInc_j))
 // backward
 //   uIncNew_i += jinv_i (- sum_{later j} c_f dF(u_j, uInc_j)),
 // c_f = (alpha/2) S_f / V_i, the normal pointing out of i, faces in cell2face
 // order.  "Earlier" is a lower colour, "later" a higher one.
 template <int NF, bool FORWARD>
-__global__ __launch_bounds__(kBlock) void k_lusgs_sweep(int32_t n, const int32_t *__restrict__ list, int32_t col,
-                                                        int64_t ldc, int64_t ldf, const int32_t *__restrict__ nbr,
-                                                        const int32_t *__restrict__ sface,
-                                                        const int32_t *__restrict__ f2c,
-                                                        const int32_t *__restrict__ colour,
-                                                        const double *__restrict__ fnc,
-                                                        const double *__restrict__ farea,
-                                                        const double *__restrict__ vol,
+__global__ __launch_bounds__(kBlock) void k_lusgs_sweep(int32_t n, int32_t b, int64_t ldc,
+                                                        const int32_t *__restrict__ cells,
+                                                        const int32_t *__restrict__ sj,
+                                                        const int32_t *__restrict__ sf,
+                                                        const double *__restrict__ sn0,
+                                                        const double *__restrict__ sn1,
+                                                        const double *__restrict__ sS,
+                                                        const double *__restrict__ sV,
                                                         const double *__restrict__ lambda_face,
                                                         const double *__restrict__ jinv, double gamma, double alpha,
                                                         const double *__restrict__ rhs, const double *__restrict__ u,
@@ -89,38 +131,30 @@ __global__ __launch_bounds__(kBlock) void k_lusgs_sweep(int32_t n, const int32_t
     const int32_t t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= n)
         return;
-    const int32_t i = list[t];
+    const int32_t p = b + t, i = cells[p];
     double buf[4];
 #pragma unroll
     for (int v = 0; v < 4; v++)
         buf[v] = FORWARD ? rhs[4 * int64_t(i) + v] : 0.0;
-    const double vi = vol[i];
-#pragma unroll 1
+    const double vi = sV[p];
+#pragma unroll
     for (int s = 0; s < NF; s++) {
-        const int32_t j = nbr[s * ldc + i];
+        const int32_t j = sj[s * ldc + p];
         if (j < 0)
             continue;
-        const int32_t cj = colour[j];
-        if (FORWARD ? !(cj < col) : !(cj > col))
-            continue;
-        const int32_t f = sface[s * ldc + i];
-        double n0, n1;
-        normalized(fnc[f], fnc[ldf + f], n0, n1);
-        if (f2c[f] != i) {
-            n0 *= -1.0;
-            n1 *= -1.0;
-        }
+        const int32_t f = sf[s * ldc + p];
+        const double n0 = sn0[s * ldc + p], n1 = sn1[s * ldc + p];
         double uj[4], dj[4], dF[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
             uj[v] = u[4 * int64_t(j) + v], dj[v] = uInc[4 * int64_t(j) + v];
         flux_jacobian_times_du(gamma, uj, n0, n1, dj, lambda_face[f], dF);
-        const double cf = (0.5 * alpha) * farea[f] / vi;
+        const double cf = (0.5 * alpha) * sS[s * ldc + p] / vi;
 #pragma unroll
         for (int v = 0; v < 4; v++)
             buf[v] -= cf * dF[v];
     }
-    const double d = jinv[i];
+    const double d = jinv[p];
     bool finite = true;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -230,20 +264,46 @@ int ensure_implicit_scratch(dndsmi_ctx *c) {
     return DNDSMI_OK;
 }
 
-int launch_lusgs_init(dndsmi_ctx *c, const double *dtau, double dt, double alpha) {
+int ensure_lusgs_scan(dndsmi_ctx *c) {
+    if (c->sc_jf)
+        return DNDSMI_OK;
+    const int64_t n = int64_t(c->nfmax) * c->ldc;
+    DNDSMI_TRY(c->alloc(c->sc_jf, n));
+    DNDSMI_TRY(c->alloc(c->sc_jb, n));
+    DNDSMI_TRY(c->alloc(c->sc_face, n));
+    DNDSMI_TRY(c->alloc(c->sc_n0, n));
+    DNDSMI_TRY(c->alloc(c->sc_n1, n));
+    DNDSMI_TRY(c->alloc(c->sc_area, n));
+    DNDSMI_TRY(c->alloc(c->sc_vol, c->ldc));
     const int nc = int(c->nc);
     if (c->nfmax == 3)
-        hipLaunchKernelGGL(k_lusgs_init<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->sface,
-                           c->farea, c->lambda_face, c->vol, dtau, dt, alpha, c->jinv);
+        hipLaunchKernelGGL(k_lusgs_scan<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
+                           c->col_cells, c->nbr, c->sface, c->f2c, c->colour, c->fnc, c->farea, c->vol, c->sc_jf,
+                           c->sc_jb, c->sc_face, c->sc_n0, c->sc_n1, c->sc_area, c->sc_vol);
     else
-        hipLaunchKernelGGL(k_lusgs_init<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->sface,
-                           c->farea, c->lambda_face, c->vol, dtau, dt, alpha, c->jinv);
+        hipLaunchKernelGGL(k_lusgs_scan<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf,
+                           c->col_cells, c->nbr, c->sface, c->f2c, c->colour, c->fnc, c->farea, c->vol, c->sc_jf,
+                           c->sc_jb, c->sc_face, c->sc_n0, c->sc_n1, c->sc_area, c->sc_vol);
+    return c->launch_check("k_lusgs_scan");
+}
+
+int launch_lusgs_init(dndsmi_ctx *c, const double *dtau, double dt, double alpha) {
+    DNDSMI_TRY(ensure_lusgs_scan(c));
+    const int nc = int(c->nc);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_lusgs_init<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
+                           c->sc_face, c->sc_area, c->sc_vol, c->lambda_face, dtau, dt, alpha, c->jinv);
+    else
+        hipLaunchKernelGGL(k_lusgs_init<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
+                           c->sc_face, c->sc_area, c->sc_vol, c->lambda_face, dtau, dt, alpha, c->jinv);
     return c->launch_check("k_lusgs_init");
 }
 
 template <bool FORWARD>
 int launch_lusgs_sweep(dndsmi_ctx *c, double alpha, const double *rhs, const double *u, const double *uInc,
                        double *uIncNew) {
+    DNDSMI_TRY(ensure_lusgs_scan(c));
+    const int32_t *sj = FORWARD ? c->sc_jf : c->sc_jb;
     for (int32_t kk = 0; kk < c->n_colour; kk++) {
         const int32_t col = FORWARD ? kk : c->n_colour - 1 - kk;
         const int64_t b = c->col_start[col], n = c->col_start[col + 1] - b;
@@ -252,13 +312,13 @@ int launch_lusgs_sweep(dndsmi_ctx *c, double alpha, const double *rhs, const dou
         prof_begin(c, DNDSMI_K_LUSGS_SWEEP);
         if (c->nfmax == 3)
             hipLaunchKernelGGL((k_lusgs_sweep<3, FORWARD>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream,
-                               int32_t(n), c->col_cells + b, col, c->ldc, c->ldf, c->nbr, c->sface, c->f2c, c->colour,
-                               c->fnc, c->farea, c->vol, c->lambda_face, c->jinv, c->gas.gamma, alpha, rhs, u, uInc,
+                               int32_t(n), int32_t(b), c->ldc, c->col_cells, sj, c->sc_face, c->sc_n0, c->sc_n1,
+                               c->sc_area, c->sc_vol, c->lambda_face, c->jinv, c->gas.gamma, alpha, rhs, u, uInc,
                                uIncNew, c->errw);
         else
             hipLaunchKernelGGL((k_lusgs_sweep<4, FORWARD>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream,
-                               int32_t(n), c->col_cells + b, col, c->ldc, c->ldf, c->nbr, c->sface, c->f2c, c->colour,
-                               c->fnc, c->farea, c->vol, c->lambda_face, c->jinv, c->gas.gamma, alpha, rhs, u, uInc,
+                               int32_t(n), int32_t(b), c->ldc, c->col_cells, sj, c->sc_face, c->sc_n0, c->sc_n1,
+                               c->sc_area, c->sc_vol, c->lambda_face, c->jinv, c->gas.gamma, alpha, rhs, u, uInc,
                                uIncNew, c->errw);
         prof_end(c);
         DNDSMI_TRY(c->launch_check("k_lusgs_sweep"));
