@@ -85,3 +85,34 @@ def test_device_reproduces_golden_fixture_from_stored_arrays(fx):
                 assert rel_err(r2.download(), d[f"{key}_urec"]) <= 1e-12
     assert rel_err(results["Roe"], d["rhs_roe"]) <= 1e-12
     assert rel_err(results["HLLC"], d["rhs_hllc"]) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rs", ["Roe", "HLLC"])
+def test_device_rhs_with_distinct_gauss_point_normals(fx, rs):
+    """Face normals that differ between the three LINE_3 points (as on curved
+    faces; here the stored normals of points 1 and 2 are rescaled) select the
+    face kernel that recomputes the unit normal, |n| and the balance fluxes
+    at every point; it must match the oracle fed the same arrays."""
+    from dnds_amd.host import EulerEvaluator, ExternalMesh, ExternalVR, vr_setting
+    d, mesh, setup = fx
+    tw, twd, scale, nstd, relax, gamma = d["setting"]
+    setup = dict(setup)
+    fn = np.array(setup["face_norm"], dtype=np.float64).reshape(-1, 3, 2)
+    fn[:, 1] *= 1.0 + 2e-3
+    fn[:, 2] *= 1.0 - 1e-3
+    setup["face_norm"] = fn.ravel()
+    st = vr_setting(tang_weight=tw, tang_weight_direction=int(twd), wbap_smooth_indicator_scale=scale,
+                    wbap_n_std=nstd, jacobi_relax=relax, build_limiter_data=1)
+    em = ExternalMesh(mesh)
+    evr = ExternalVR(em, setup, st)
+    ev = EulerEvaluator(em, evr, riemannSolverType=rs, gamma=gamma, farFieldStaticValue=tuple(d["far"]))
+    u, r, rhs, dt = ev.BuildMean().upload(d["u"]), evr.BuildRec().upload(d["urec"]), ev.BuildMean(), ev.BuildScalar()
+    ev.EvaluateDt(dt, u, 0.5)
+    ev.EvaluateRHS(rhs, u, r, 0.1)
+    orc = make_oracle(d, mesh, setup, 1 if rs == "Roe" else 2)
+    _, _, _, dlf = orc.evaluate_dt(d["u"], 0.5)
+    ref = orc.evaluate_rhs(d["u"], d["urec"], dlf, t=0.1)
+    assert rel_err(rhs.download(), ref) <= 1e-12
+    # the rescaled normals change the result: the per-point normals were used
+    assert rel_err(ref, d["rhs_roe" if rs == "Roe" else "rhs_hllc"]) > 1e-6
