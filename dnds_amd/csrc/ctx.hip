@@ -1396,8 +1396,10 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     return DNDSMI_OK;
 }
 
-// defined with the warm start (warmstart.hip): forgets the context
+// defined with the warm start (warmstart.hip) and the BDF integrator
+// (lusgs.hip): forget the context
 void dndsmi_warmstart_detach(dndsmi_warmstart *w);
+void dndsmi_bdf_detach(dndsmi_bdf *b);
 
 extern "C" void dndsmi_ctx_destroy(dndsmi_ctx *c) {
     if (!c)
@@ -1415,6 +1417,9 @@ extern "C" void dndsmi_ctx_destroy(dndsmi_ctx *c) {
     for (dndsmi_warmstart *w : c->warmstarts)
         dndsmi_warmstart_detach(w);
     c->warmstarts.clear();
+    for (dndsmi_bdf *b : c->bdfs)
+        dndsmi_bdf_detach(b);
+    c->bdfs.clear();
     for (void *p : c->allocs)
         (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool)

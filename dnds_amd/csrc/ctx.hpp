@@ -42,6 +42,7 @@ struct dndsmi_ctx {
     // is destroyed first
     std::unordered_set<struct dndsmi_array *> arrays;
     std::unordered_set<struct dndsmi_warmstart *> warmstarts;
+    std::unordered_set<struct dndsmi_bdf *> bdfs;
 
     // cell side
     int32_t *nbr = nullptr;    // [nfmax][ldc] neighbour cell, -1 = boundary slot, -2 = absent slot

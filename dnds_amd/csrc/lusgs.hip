@@ -246,6 +246,46 @@ __global__ __launch_bounds__(kBlock) void k_residual_final(int32_t nblk, int32_t
         res[v] = a;
 }
 
+// fdtau's division by alphaDiag (DNDS_FV_EulerSolver.hpp:846-847)
+__global__ __launch_bounds__(kBlock) void k_div_scalar(int32_t n, double *__restrict__ p, double a) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] /= a;
+}
+
+// BDF dual-time right-hand side (DNDS_ODE.hpp:459-471), in the reference's
+// order: rhs *= alpha0; rhs += x * (-1/dt); rhs += xLast * (c1/dt);
+// rhs += xPrev_j * (c_j/dt) for the previous states in the history's order;
+// then fsolve's compression rhs = (rhs dTau) / dTau and xinc = 0.
+__global__ __launch_bounds__(kBlock) void k_bdf_rhs(int32_t nc, double alpha0, const double *__restrict__ x,
+                                                    double mdt, const double *__restrict__ xlast, double c1,
+                                                    int32_t nprev, const double *__restrict__ p0, double c2,
+                                                    const double *__restrict__ p1, double c3,
+                                                    const double *__restrict__ p2, double c4,
+                                                    const double *__restrict__ dtau, double *__restrict__ rhs,
+                                                    double *__restrict__ xinc) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    const double dti = dtau[i];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int64_t k = 4 * int64_t(i) + v;
+        double r = rhs[k];
+        r *= alpha0;
+        r += x[k] * mdt;
+        r += xlast[k] * c1;
+        if (nprev > 0)
+            r += p0[k] * c2;
+        if (nprev > 1)
+            r += p1[k] * c3;
+        if (nprev > 2)
+            r += p2[k] * c4;
+        rhs[k] = (r * dti) / dti;
+        xinc[k] = 0.0;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_scalar(int32_t n, const double *__restrict__ src,
                                                         double *__restrict__ dst) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -437,6 +477,142 @@ extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi
         }
         if (res[0] / base[0] < threshold)
             break;
+    }
+    return DNDSMI_OK;
+}
+
+// ------------------------------------------------------------------ BDF ----
+
+struct dndsmi_bdf {
+    dndsmi_ctx *ctx = nullptr;
+    int32_t k = 2;
+    double *prev = nullptr; // [k - 1][ldc][4] previous steps' initial states
+    double dt_prev[3] = {0, 0, 0};
+    int32_t cn_prev = 0, prev_start = 0;
+    ~dndsmi_bdf() {
+        if (prev)
+            (void)hipFree(prev);
+    }
+};
+
+void dndsmi_bdf_detach(dndsmi_bdf *b) { b->ctx = nullptr; } // keeps its own allocation
+
+namespace {
+// ImplicitBDFDualTimeStep<TDATA>::BDFCoefs (DNDS_ODE.hpp:506-511)
+constexpr double kBDFCoefs[4][5] = {{1. / 1., 1. / 1., 0., 0., 0.},
+                                    {2. / 3., 4. / 3., -1. / 3., 0., 0.},
+                                    {6. / 11., 18. / 11., -9. / 11., 2. / 11., 0.},
+                                    {12. / 25., 48. / 25., -36. / 25., 16. / 25., -3. / 25.}};
+inline int32_t pos_mod(int32_t a, int32_t m) { return ((a % m) + m) % m; }
+} // namespace
+
+extern "C" int dndsmi_bdf_create(dndsmi_ctx *c, int32_t k, dndsmi_bdf **out) {
+    if (!c || !out || k < 1 || k > 4)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto *b = new (std::nothrow) dndsmi_bdf();
+    if (!b)
+        return DNDSMI_ERR_OUT_OF_MEMORY;
+    b->ctx = c, b->k = k, b->prev_start = k - 2 >= 0 ? k - 2 : 0;
+    if (k > 1) {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(double) * 4 * size_t(c->ldc) * size_t(k - 1)) != hipSuccess) {
+            delete b;
+            c->err = "hipMalloc failed for the BDF history";
+            return DNDSMI_ERR_OUT_OF_MEMORY;
+        }
+        b->prev = static_cast<double *>(q);
+    }
+    c->bdfs.insert(b);
+    *out = b;
+    return DNDSMI_OK;
+}
+
+extern "C" void dndsmi_bdf_destroy(dndsmi_bdf *b) {
+    if (b && b->ctx) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        b->ctx->bdfs.erase(b);
+    }
+    delete b;
+}
+
+extern "C" int dndsmi_bdf_step(dndsmi_bdf *b, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                               int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt, int32_t max_iter,
+                               double threshold, double t, int32_t *n_iter, double *res_hist) {
+    if (!b || !b->ctx)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    dndsmi_ctx *c = b->ctx;
+    if (!same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNew, DNDSMI_ARRAY_REC) || uRec == uRecNew || n_sweeps < 0 || max_iter < 1 || !(dt > 0))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    const int32_t prev_siz = b->k - 1, k_cur = b->cn_prev + 1;
+    for (int32_t ip = 0; ip < b->cn_prev; ip++)
+        if (!(std::abs(b->dt_prev[pos_mod(ip + b->prev_start, prev_siz)] - dt) < dt * 1e-8)) {
+            c->err = "BDF step: dt differs from the previous steps' dt";
+            return DNDSMI_ERR_INVALID_ARGUMENT;
+        }
+    DNDSMI_TRY(ensure_implicit_scratch(c));
+    const int nc = int(c->nc);
+    const double *co = kBDFCoefs[k_cur - 1];
+    const double alpha0 = co[0];
+    const double *ph[3] = {nullptr, nullptr, nullptr};
+    double pc[3] = {0, 0, 0};
+    for (int32_t ip = 0; ip < b->cn_prev; ip++) {
+        ph[ip] = b->prev + 4 * c->ldc * pos_mod(ip + b->prev_start, prev_siz);
+        pc[ip] = co[2 + ip] / dt;
+    }
+    // xLast = x
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(c->xlast, u->d, sizeof(double) * 4 * c->nc, hipMemcpyDeviceToDevice, c->stream),
+                      "copy"));
+    double base[4] = {0, 0, 0, 0};
+    if (n_iter)
+        *n_iter = 0;
+    for (int32_t iter = 1; iter <= max_iter; iter++) {
+        // fdtau(dTau, alpha0)
+        DNDSMI_TRY(launch_dt(c, u->d, CFL, 1e100, c->imp_dtau));
+        if (!UseLocaldt)
+            hipLaunchKernelGGL(k_fill_scalar, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->dt_min,
+                               c->imp_dtau);
+        hipLaunchKernelGGL(k_div_scalar, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->imp_dtau, alpha0);
+        DNDSMI_TRY(c->launch_check("fdtau"));
+        // frhs at t + dt
+        for (int q = 0; q < n_sweeps; q++) {
+            DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, t + dt));
+            std::swap(uRec->d, uRecNew->d);
+        }
+        DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, t + dt, c->imp_rhs, 0, nullptr, nullptr, 0, 0));
+        hipLaunchKernelGGL(k_bdf_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, alpha0, u->d, -1. / dt,
+                           c->xlast, co[1] / dt, b->cn_prev, ph[0], pc[0], ph[1], pc[1], ph[2], pc[2], c->imp_dtau,
+                           c->imp_rhs, c->imp_xinc);
+        DNDSMI_TRY(c->launch_check("k_bdf_rhs"));
+        // fsolve with alphaDiag = alpha0
+        DNDSMI_TRY(launch_lusgs_init(c, c->imp_dtau, dt, alpha0));
+        DNDSMI_TRY(launch_lusgs_sweep<true>(c, alpha0, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+        DNDSMI_TRY(launch_lusgs_sweep<false>(c, alpha0, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+        double res[4];
+        DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
+        if (res_hist)
+            for (int v = 0; v < 4; v++)
+                res_hist[4 * (iter - 1) + v] = res[v];
+        for (int v = 0; v < 4; v++)
+            base[v] = iter == 1 ? res[v] : std::max(base[v], res[v]);
+        if (n_iter)
+            *n_iter = iter;
+        if (!std::isfinite(res[0])) {
+            c->err = "non-finite increment in the BDF iteration";
+            return DNDSMI_ERR_NAN;
+        }
+        if (res[0] / base[0] < threshold)
+            break;
+    }
+    // the step's initial state joins the history (DNDS_ODE.hpp:488-495)
+    if (prev_siz) {
+        b->prev_start = pos_mod(b->prev_start - 1, prev_siz);
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(b->prev + 4 * c->ldc * b->prev_start, c->xlast, sizeof(double) * 4 * c->nc,
+                                         hipMemcpyDeviceToDevice, c->stream),
+                          "copy"));
+        b->dt_prev[b->prev_start] = dt;
+        b->cn_prev = std::min(b->cn_prev + 1, prev_siz);
     }
     return DNDSMI_OK;
 }

@@ -483,6 +483,40 @@ class ImplicitEulerDualTimeStep:
         return hist[:n.value]
 
 
+class ImplicitBDFDualTimeStep:
+    """ODE::ImplicitBDFDualTimeStep (DNDS_ODE.hpp:392-512) of order k with the
+    same device frhs / fdtau / fsolve / fstop as ImplicitEulerDualTimeStep;
+    the k-1 previous states live on the device."""
+
+    def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, nTimeStepInternal: int,
+                 rhsThresholdInternal: float, UseLocaldt: bool = True, k: int = 2):
+        self.ev = evaluator
+        self.n_sweeps = nInternalRecStep
+        self.CFL = CFL
+        self.max_iter = nTimeStepInternal
+        self.threshold = rhsThresholdInternal
+        self.UseLocaldt = UseLocaldt
+        h = C.c_void_p()
+        _check(_lib().dndsmi_bdf_create(evaluator.device.ctx, k, C.byref(h)), "BDF create", evaluator.device.ctx)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib().dndsmi_bdf_destroy(self.h)
+            self.h = None
+
+    def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, dt: float, t: float = 0.0):
+        """One BDF step of size dt; returns the L1 increment norms of the
+        inner iterations ([iterations][4])."""
+        d = self.ev.device
+        hist = np.zeros((self.max_iter, 4))
+        n = C.c_int32()
+        _check(_lib().dndsmi_bdf_step(self.h, u.h, uRec.h, uRecNew.h, self.n_sweeps, self.CFL, dt,
+                                      int(self.UseLocaldt), self.max_iter, self.threshold, t, C.byref(n),
+                                      A.dptr(hist)), "BDF step", d.ctx)
+        return hist[:n.value]
+
+
 class WarmStart:
     """The learned warm start of the reconstruction (no reference
     counterpart): features, training set, training, inference on the device."""

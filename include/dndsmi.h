@@ -320,6 +320,25 @@ int dndsmi_implicit_euler_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *u
                                int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt, int32_t max_iter,
                                double threshold, double t, int32_t *n_iter, double *res_hist);
 
+/* ODE::ImplicitBDFDualTimeStep (DNDS_ODE.hpp:392-512): a BDF integrator of
+ * order k (1..4) holding the previous k-1 states on the device.  Step i uses
+ * order min(i, k) (the reference's start-up).  A handle outlives nothing:
+ * after dndsmi_ctx_destroy its steps fail and only destroy is valid. */
+typedef struct dndsmi_bdf dndsmi_bdf;
+int dndsmi_bdf_create(dndsmi_ctx *ctx, int32_t k, dndsmi_bdf **out);
+void dndsmi_bdf_destroy(dndsmi_bdf *bdf);
+
+/* ImplicitBDFDualTimeStep::Step (DNDS_ODE.hpp:441-500) with the same
+ * frhs / fdtau / fsolve / fstop as dndsmi_implicit_euler_step: per iteration
+ * dTau = EvaluateDt / alpha0, the sweeps and EvaluateRHS at t + dt,
+ * rhs = alpha0 rhs - x / dt + (c1 / dt) xLast + sum_j (c_j / dt) xPrev_j,
+ * LUSGSMatrixInit and LU-SGS with alphaDiag = alpha0, u += uInc; then the
+ * step's initial state joins the history.  dt must equal the previous
+ * steps' dt (the reference asserts it). */
+int dndsmi_bdf_step(dndsmi_bdf *bdf, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew, int32_t n_sweeps,
+                    double CFL, double dt, int32_t UseLocaldt, int32_t max_iter, double threshold, double t,
+                    int32_t *n_iter, double *res_hist);
+
 /* ======================================================================== */
 /* Learned warm start of the reconstruction (no counterpart in the reference) */
 /* ======================================================================== */

@@ -304,6 +304,43 @@ class ImplicitEulerDualTimeStep {
     bool local_;
 };
 
+/* ODE::ImplicitBDFDualTimeStep (DNDS_ODE.hpp:392-512): order k, the k-1
+ * previous states on the device; frhs / fdtau / fsolve / fstop as
+ * ImplicitEulerDualTimeStep.  Must not outlive the evaluator's device. */
+class ImplicitBDFDualTimeStep {
+  public:
+    ImplicitBDFDualTimeStep(EulerEvaluator &ev, int nInternalRecStep, real CFL, int nTimeStepInternal,
+                            real rhsThresholdInternal, bool UseLocaldt = true, int k = 2)
+        : ev_(ev), n_(nInternalRecStep), cfl_(CFL), maxit_(nTimeStepInternal), thr_(rhsThresholdInternal),
+          local_(UseLocaldt) {
+        dndsmi_ctx *c = ev_.device()->ctx.get();
+        detail::check(dndsmi_bdf_create(c, k, &bdf_), "BDF create", c);
+    }
+    ~ImplicitBDFDualTimeStep() { dndsmi_bdf_destroy(bdf_); }
+    ImplicitBDFDualTimeStep(const ImplicitBDFDualTimeStep &) = delete;
+    ImplicitBDFDualTimeStep &operator=(const ImplicitBDFDualTimeStep &) = delete;
+    /* one step of size dt; returns the inner L1 increment norms (4 per iteration) */
+    std::vector<real> Step(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNew, real dt, real t = 0) {
+        std::vector<real> hist(size_t(maxit_) * 4, 0.0);
+        int32_t n = 0;
+        dndsmi_ctx *c = ev_.device()->ctx.get();
+        detail::check(dndsmi_bdf_step(bdf_, u.h(), uRec.h(), uRecNew.h(), n_, cfl_, dt, local_, maxit_, thr_, t, &n,
+                                      hist.data()),
+                      "BDF step", c);
+        hist.resize(size_t(n) * 4);
+        return hist;
+    }
+
+  private:
+    EulerEvaluator &ev_;
+    int n_;
+    real cfl_;
+    int maxit_;
+    real thr_;
+    bool local_;
+    dndsmi_bdf *bdf_ = nullptr;
+};
+
 } // namespace dndsmi
 
 #endif

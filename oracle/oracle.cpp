@@ -1280,6 +1280,89 @@ extern "C" int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s
     return ERR_OK;
 }
 
+// ImplicitBDFDualTimeStep::Step (DNDS_ODE.hpp:441-500), n_steps steps of
+// size dt from a fresh integrator of order k (step i uses order min(i, k)),
+// with the same frhs / fdtau / fsolve / fstop as oracle_implicit_euler and
+// alphaDiag = BDFCoefs(kCurrent - 1, 0) (DNDS_ODE.hpp:506-511).  n_iter
+// receives the iterations done per step.
+extern "C" int oracle_bdf_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                const int64_t *scan2cell, int32_t k, int32_t n_steps, int32_t max_iter,
+                                int32_t n_sweeps, double cfl, double dt, int32_t use_local_dt, double threshold,
+                                double t, double *u, double *urec, int32_t *n_iter) {
+    static const double coefs[4][5] = {{1. / 1., 1. / 1., 0. / 0., 0. / 0., 0. / 0.},
+                                       {2. / 3., 4. / 3., -1. / 3., 0. / 0., 0. / 0.},
+                                       {6. / 11., 18. / 11., -9. / 11., 2. / 11., 0. / 0.},
+                                       {12. / 25., 48. / 25., -36. / 25., 16. / 25., -3. / 25.}};
+    if (k < 1 || k > 4)
+        return ERR_INVALID;
+    const int64_t nc = m->n_cell;
+    const int32_t prevSiz = k - 1;
+    std::vector<std::vector<double>> xPrevs(size_t(prevSiz), std::vector<double>(4 * nc));
+    int32_t cnPrev = 0, prevStart = k - 2;
+    auto mod = [](int32_t a, int32_t b) { return ((a % b) + b) % b; };
+    std::vector<double> dTau(nc), rhs(4 * nc), xLast(4 * nc), xinc(4 * nc), urecNew(36 * nc);
+    std::vector<double> lf(m->n_face), dlf(m->n_face), jinv(nc);
+    for (int32_t step = 0; step < n_steps; step++) {
+        const int32_t kCurrent = cnPrev + 1;
+        const double alphaDiag = coefs[kCurrent - 1][0];
+        std::copy(u, u + 4 * nc, xLast.begin());
+        double resBase[4] = {0, 0, 0, 0};
+        n_iter[step] = 0;
+        for (int32_t iter = 1; iter <= max_iter; iter++) {
+            double dtMin;
+            int rc = oracle_evaluate_dt(m, s, cfg, u, cfl, 1e100, use_local_dt, dTau.data(), &dtMin, lf.data(),
+                                        dlf.data());
+            if (rc)
+                return rc;
+            for (int64_t i = 0; i < nc; i++)
+                dTau[i] /= alphaDiag;
+            for (int q = 0; q < n_sweeps; q++)
+                if ((rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), t + dt)))
+                    return rc;
+            if ((rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, t + dt, rhs.data())))
+                return rc;
+            for (int64_t q = 0; q < 4 * nc; q++) {
+                double r = rhs[q];
+                r *= alphaDiag;
+                r += u[q] * (-1. / dt);
+                r += xLast[q] * (coefs[kCurrent - 1][1] / dt);
+                for (int32_t ip = 0; ip < cnPrev; ip++)
+                    r += xPrevs[size_t(mod(ip + prevStart, prevSiz))][q] * (coefs[kCurrent - 1][2 + ip] / dt);
+                rhs[q] = r;
+            }
+            std::fill(xinc.begin(), xinc.end(), 0.0);
+            oracle_lusgs_matrix_init(m, s, dTau.data(), dt, alphaDiag, lf.data(), jinv.data());
+            for (int64_t i = 0; i < nc; i++)
+                for (int v = 0; v < 4; v++)
+                    rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
+            if ((rc = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
+                                           xinc.data(), xinc.data())))
+                return rc;
+            if ((rc = oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
+                                            xinc.data(), xinc.data())))
+                return rc;
+            for (int64_t q = 0; q < 4 * nc; q++)
+                u[q] += xinc[q] * 1.0;
+            double res[4] = {0, 0, 0, 0};
+            for (int64_t i = 0; i < nc; i++)
+                for (int v = 0; v < 4; v++)
+                    res[v] += std::fabs(xinc[4 * i + v]);
+            for (int v = 0; v < 4; v++)
+                resBase[v] = iter == 1 ? res[v] : std::max(resBase[v], res[v]);
+            n_iter[step] = iter;
+            if (res[0] / resBase[0] < threshold)
+                break;
+        }
+        if (prevSiz) {
+            prevStart = mod(prevStart - 1, prevSiz);
+            xPrevs[size_t(prevStart)] = xLast;
+            cnPrev = std::min(cnPrev + 1, prevSiz);
+        }
+        t += dt;
+    }
+    return ERR_OK;
+}
+
 extern "C" void oracle_set_threads(int32_t n) { g_threads = n > 1 ? n : 1; }
 
 extern "C" int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU,

@@ -77,6 +77,9 @@ def lib() -> C.CDLL:
             "oracle_implicit_euler": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32,
                                       C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD, PD,
                                       PD, C.POINTER(C.c_int32)],
+            "oracle_bdf_steps": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32, C.c_int32,
+                                 C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD,
+                                 PD, C.POINTER(C.c_int32)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -206,6 +209,20 @@ class Oracle:
                                                    max_iter, n_sweeps, cfl, dt, int(use_local_dt), threshold, t,
                                                    _d(u), _d(urec), _d(hist), C.byref(done)), "implicit euler")
         return u, urec, hist[:done.value]
+
+    def bdf_steps(self, u, urec, k, n_steps, max_iter, n_sweeps, cfl, dt, use_local_dt=True, threshold=1e-10,
+                  t=0.0, scan2cell=None):
+        """n_steps BDF steps of order k (DNDS_ODE.hpp:392-512) from a fresh
+        integrator; returns (u, urec, inner iterations per step)."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        urec = np.array(urec, dtype=np.float64, order="C", copy=True)
+        its = np.zeros(n_steps, dtype=np.int32)
+        order = None if scan2cell is None else np.ascontiguousarray(scan2cell, dtype=np.int64)
+        self._check(self.lib.oracle_bdf_steps(C.byref(self.m), C.byref(self.s), C.byref(self.cfg),
+                                              order.ctypes.data_as(PI64) if order is not None else None, k, n_steps,
+                                              max_iter, n_sweeps, cfl, dt, int(use_local_dt), threshold, t, _d(u),
+                                              _d(urec), its.ctypes.data_as(C.POINTER(C.c_int32))), "bdf steps")
+        return u, urec, its
 
     def ssprk3_steps(self, u, urec, n_steps, n_sweeps, cfl, t=0.0, max_dt=1e100, use_local_dt=False,
                      use_limiter=False):

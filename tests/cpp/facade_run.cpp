@@ -48,6 +48,10 @@ int main(int argc, char **argv) {
             t += rk.Step(u, uRec, uRecNew, t);
         dndsmi::ImplicitEulerDualTimeStep imp(ev, 1, 5.0, 4, 0.0, true);
         auto hist = imp.Step(u, uRec, uRecNew, 0.05, t);
+        t += 0.05;
+        dndsmi::ImplicitBDFDualTimeStep bdf(ev, 1, 5.0, 3, 0.0, true, 2);
+        for (int step = 0; step < 2; step++, t += 0.05)
+            bdf.Step(u, uRec, uRecNew, 0.05, t);
         std::array<double, 4> res{};
         auto rhs = ev.BuildMean();
         ev.EvaluateRHS(rhs, u, uRec, t);

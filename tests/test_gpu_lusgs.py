@@ -111,3 +111,36 @@ def test_converged_implicit_step_matches_reference_order():
     gu = u.download()
     assert np.linalg.norm(gu - ou) / np.linalg.norm(ou) <= 1e-8
     assert np.abs(gu - c.u0).max() > 1e-3
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_bdf_steps_match_oracle_in_colour_order(k):
+    """ImplicitBDFDualTimeStep: four steps of a fresh order-k integrator (the
+    first steps run at orders 1, 2, ... as in the reference), a fixed number
+    of inner iterations each, against the oracle in the same scan order."""
+    from dnds_amd.host import ImplicitBDFDualTimeStep
+    c, ev, orc = _vortex(12)
+    bdf = ImplicitBDFDualTimeStep(ev, nInternalRecStep=2, CFL=10.0, nTimeStepInternal=5, rhsThresholdInternal=0.0,
+                                  k=k)
+    u, r, rn = ev.BuildMean().upload(c.u0), c.vfv.BuildRec().setConstant(0.0), c.vfv.BuildRec()
+    dt, t = 0.2, 0.0
+    for step in range(4):
+        hist = bdf.Step(u, r, rn, dt=dt, t=t)
+        assert hist.shape == (5, 4)
+        t += dt
+    ou, orr, its = orc.bdf_steps(c.u0, np.zeros((c.n_cell, 36)), k, 4, 5, 2, 10.0, dt, threshold=0.0,
+                                 scan2cell=ev.LUSGSScanOrder())
+    assert list(its) == [5, 5, 5, 5]
+    assert rel_err(u.download(), ou) <= TOL
+    assert rel_err(r.download(), orr) <= TOL
+
+
+def test_bdf_rejects_a_changed_dt_and_outlives_nothing():
+    from dnds_amd.host import ImplicitBDFDualTimeStep
+    c, ev, orc = _vortex(8)
+    bdf = ImplicitBDFDualTimeStep(ev, nInternalRecStep=1, CFL=10.0, nTimeStepInternal=2, rhsThresholdInternal=0.0)
+    u, r, rn = ev.BuildMean().upload(c.u0), c.vfv.BuildRec().setConstant(0.0), c.vfv.BuildRec()
+    bdf.Step(u, r, rn, dt=0.2)
+    with pytest.raises(Exception):
+        bdf.Step(u, r, rn, dt=0.3)  # the reference asserts a constant dt
+    bdf.Step(u, r, rn, dt=0.2)

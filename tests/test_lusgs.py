@@ -129,3 +129,36 @@ def test_converged_implicit_step_does_not_depend_on_the_order():
     assert np.linalg.norm(a - b) / np.linalg.norm(a) <= 1e-10
     # and it is a genuine step: the state moved
     assert np.abs(a - c.u0).max() > 1e-3
+
+
+def _bdf_vortex(n):
+    from dnds_amd import cases
+    c = cases.isentropic_vortex(n, seed=3, nthreads=2)
+    st = c.vfv.setting
+    orc = oracle.Oracle(c.mesh.arrays(), c.vfv.arrays(), riemann_solver=1, far_field=c.far,
+                        tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction)
+    return c, orc
+
+
+def test_bdf1_is_implicit_euler():
+    """Order 1 of ImplicitBDFDualTimeStep (BDFCoefs row 0, DNDS_ODE.hpp:506-511)
+    is the implicit-Euler dual-time step, up to the order of the rounding."""
+    c, orc = _bdf_vortex(10)
+    z = np.zeros((c.n_cell, 36))
+    u1, r1, _ = orc.bdf_steps(c.u0, z, 1, 1, 20, 2, 5.0, 0.05, threshold=0.0)
+    u2, r2, _ = orc.implicit_euler(c.u0, z, 20, 2, 5.0, dt=0.05, threshold=0.0)
+    assert np.abs(u1 - u2).max() <= 1e-14 * np.abs(u2).max()
+    assert np.abs(u1 - c.u0).max() > 1e-4
+
+
+def test_bdf_orders_of_accuracy_in_time():
+    """Converged inner iterations: against a 32-step solution, halving dt
+    divides the error by about 2 at order 1 and by about 4 at order 2."""
+    c, orc = _bdf_vortex(10)
+    z = np.zeros((c.n_cell, 36))
+    T = 0.4
+    ref, _, _ = orc.bdf_steps(c.u0, z, 2, 32, 150, 2, 5.0, T / 32, threshold=1e-11)
+    for k, lo, hi in ((1, 0.8, 1.2), (2, 1.7, 2.3)):
+        e = [np.abs(orc.bdf_steps(c.u0, z, k, ns, 150, 2, 5.0, T / ns, threshold=1e-11)[0] - ref).max()
+             for ns in (4, 8)]
+        assert lo <= np.log2(e[0] / e[1]) <= hi, (k, e)
