@@ -126,6 +126,7 @@ struct dndsmi_ctx {
     double *imp_rhs = nullptr;        // [ldc][4]
     double *imp_xinc = nullptr;       // [ldc][4]
     double *imp_dtau = nullptr;       // [ldc]
+    double *imp_stage = nullptr;      // [3][ldc][4] SDIRK4 stage right-hand sides (first use)
     double *res_part = nullptr;       // [4][grid]
     double *res = nullptr;            // [4]
     // per-kernel HIP-event timing (dndsmi_profile_*)

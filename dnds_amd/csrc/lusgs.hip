@@ -481,6 +481,128 @@ extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi
     return DNDSMI_OK;
 }
 
+// ---------------------------------------------------------------- SDIRK4 ----
+
+namespace {
+// ImplicitSDIRK4DualTimeStep butcherA / butcherC / butcherB (DNDS_ODE.hpp:374-390)
+constexpr double kZeta = 0.128886400515;
+constexpr double kSA[3][3] = {{kZeta, 0, 0}, {0.5 - kZeta, kZeta, 0}, {2 * kZeta, 1 - 4 * kZeta, kZeta}};
+constexpr double kSC[3] = {kZeta, 0.5, 1 - kZeta};
+constexpr double kSB[3] = {1. / (6 * ((2 * kZeta - 1) * (2 * kZeta - 1))),
+                           (4 * (kZeta * kZeta) - 4 * kZeta + 2. / 3.) / ((2 * kZeta - 1) * (2 * kZeta - 1)),
+                           1. / (6 * ((2 * kZeta - 1) * (2 * kZeta - 1)))};
+
+// stage right-hand side (DNDS_ODE.hpp:333-337): rhs = xLast; rhs -= x;
+// rhs *= 1/dt; rhs += rhs_j a_ij for j <= i; then fsolve's compression and
+// xinc = 0
+__global__ __launch_bounds__(kBlock) void k_sdirk_rhs(int32_t nc, const double *__restrict__ xlast,
+                                                      const double *__restrict__ x, double rdt,
+                                                      const double *__restrict__ stage, int64_t lds, int32_t ns,
+                                                      double a0, double a1, double a2,
+                                                      const double *__restrict__ dtau, double *__restrict__ rhs,
+                                                      double *__restrict__ xinc) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    const double dti = dtau[i];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int64_t k = 4 * int64_t(i) + v;
+        double r = xlast[k];
+        r -= x[k];
+        r *= rdt;
+        r += stage[k] * a0;
+        if (ns > 1)
+            r += stage[lds + k] * a1;
+        if (ns > 2)
+            r += stage[2 * lds + k] * a2;
+        rhs[k] = (r * dti) / dti;
+        xinc[k] = 0.0;
+    }
+}
+
+// the step's result (DNDS_ODE.hpp:368-370): x = xLast; x += rhs_j (b_j dt)
+__global__ __launch_bounds__(kBlock) void k_sdirk_final(int32_t nc, const double *__restrict__ xlast,
+                                                        const double *__restrict__ stage, int64_t lds, double w0,
+                                                        double w1, double w2, double *__restrict__ x) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int64_t k = 4 * int64_t(i) + v;
+        double r = xlast[k];
+        r += stage[k] * w0;
+        r += stage[lds + k] * w1;
+        r += stage[2 * lds + k] * w2;
+        x[k] = r;
+    }
+}
+} // namespace
+
+extern "C" int dndsmi_sdirk4_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
+                                  int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt, int32_t max_iter,
+                                  double threshold, double t, int32_t *n_iter, double *res_hist) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
+        !same_ctx(c, uRecNew, DNDSMI_ARRAY_REC) || uRec == uRecNew || n_sweeps < 0 || max_iter < 1 || !(dt > 0))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    DNDSMI_TRY(ensure_implicit_scratch(c));
+    if (!c->imp_stage)
+        DNDSMI_TRY(c->alloc(c->imp_stage, 3 * 4 * c->ldc));
+    const int nc = int(c->nc);
+    const int64_t lds = 4 * c->ldc;
+    const size_t nbytes = sizeof(double) * 4 * c->nc;
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(c->xlast, u->d, nbytes, hipMemcpyDeviceToDevice, c->stream), "copy"));
+    for (int ib = 0; ib < 3; ib++) {
+        double *stage_rhs = c->imp_stage + ib * lds;
+        const double aii = kSA[ib][ib];
+        // x = xLast
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(u->d, c->xlast, nbytes, hipMemcpyDeviceToDevice, c->stream), "copy"));
+        double base[4] = {0, 0, 0, 0};
+        if (n_iter)
+            n_iter[ib] = 0;
+        for (int32_t iter = 1; iter <= max_iter; iter++) {
+            DNDSMI_TRY(launch_dt(c, u->d, CFL, 1e100, c->imp_dtau));
+            if (!UseLocaldt)
+                hipLaunchKernelGGL(k_fill_scalar, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->dt_min,
+                                   c->imp_dtau);
+            hipLaunchKernelGGL(k_div_scalar, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->imp_dtau, aii);
+            DNDSMI_TRY(c->launch_check("fdtau"));
+            const double ts = t + kSC[ib] * dt;
+            for (int q = 0; q < n_sweeps; q++) {
+                DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, ts));
+                std::swap(uRec->d, uRecNew->d);
+            }
+            DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, ts, stage_rhs, 0, nullptr, nullptr, 0, 0));
+            hipLaunchKernelGGL(k_sdirk_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->xlast, u->d,
+                               1.0 / dt, c->imp_stage, lds, ib + 1, kSA[ib][0], kSA[ib][1], kSA[ib][2], c->imp_dtau,
+                               c->imp_rhs, c->imp_xinc);
+            DNDSMI_TRY(c->launch_check("k_sdirk_rhs"));
+            DNDSMI_TRY(launch_lusgs_init(c, c->imp_dtau, dt, aii));
+            DNDSMI_TRY(launch_lusgs_sweep<true>(c, aii, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+            DNDSMI_TRY(launch_lusgs_sweep<false>(c, aii, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+            double res[4];
+            DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
+            if (res_hist)
+                for (int v = 0; v < 4; v++)
+                    res_hist[(int64_t(ib) * max_iter + (iter - 1)) * 4 + v] = res[v];
+            for (int v = 0; v < 4; v++)
+                base[v] = iter == 1 ? res[v] : std::max(base[v], res[v]);
+            if (n_iter)
+                n_iter[ib] = iter;
+            if (!std::isfinite(res[0])) {
+                c->err = "non-finite increment in the SDIRK4 iteration";
+                return DNDSMI_ERR_NAN;
+            }
+            if (res[0] / base[0] < threshold)
+                break;
+        }
+    }
+    hipLaunchKernelGGL(k_sdirk_final, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->xlast, c->imp_stage, lds,
+                       kSB[0] * dt, kSB[1] * dt, kSB[2] * dt, u->d);
+    return c->launch_check("k_sdirk_final");
+}
+
 // ------------------------------------------------------------------ BDF ----
 
 struct dndsmi_bdf {

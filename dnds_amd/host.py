@@ -483,6 +483,32 @@ class ImplicitEulerDualTimeStep:
         return hist[:n.value]
 
 
+class ImplicitSDIRK4DualTimeStep:
+    """ODE::ImplicitSDIRK4DualTimeStep (DNDS_ODE.hpp:270-372), three stages
+    with the same device frhs / fdtau / fsolve / fstop as
+    ImplicitEulerDualTimeStep."""
+
+    def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, nTimeStepInternal: int,
+                 rhsThresholdInternal: float, UseLocaldt: bool = True):
+        self.ev = evaluator
+        self.n_sweeps = nInternalRecStep
+        self.CFL = CFL
+        self.max_iter = nTimeStepInternal
+        self.threshold = rhsThresholdInternal
+        self.UseLocaldt = UseLocaldt
+
+    def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, dt: float, t: float = 0.0):
+        """One step of size dt; returns (iterations per stage [3], L1
+        increment norms [3][max_iter][4], zero past each stage's end)."""
+        d = self.ev.device
+        hist = np.zeros((3, self.max_iter, 4))
+        its = np.zeros(3, dtype=np.int32)
+        _check(_lib().dndsmi_sdirk4_step(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, self.CFL, dt,
+                                         int(self.UseLocaldt), self.max_iter, self.threshold, t,
+                                         its.ctypes.data_as(A.PI32), A.dptr(hist)), "SDIRK4 step", d.ctx)
+        return its, hist
+
+
 class ImplicitBDFDualTimeStep:
     """ODE::ImplicitBDFDualTimeStep (DNDS_ODE.hpp:392-512) of order k with the
     same device frhs / fdtau / fsolve / fstop as ImplicitEulerDualTimeStep;

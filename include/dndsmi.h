@@ -320,6 +320,17 @@ int dndsmi_implicit_euler_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *u
                                int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt, int32_t max_iter,
                                double threshold, double t, int32_t *n_iter, double *res_hist);
 
+/* ODE::ImplicitSDIRK4DualTimeStep::Step (DNDS_ODE.hpp:270-372): three
+ * stages i, each started from the step's initial state and iterated with the
+ * same frhs / fdtau / fsolve / fstop as dndsmi_implicit_euler_step, with
+ * alphaDiag = a_ii, the boundary time t + c_i dt and the stage right-hand side
+ * (xLast - x) / dt + sum_{j<=i} a_ij rhs_j; then u = xLast + sum_j b_j dt rhs_j.
+ * n_iter (may be NULL) receives the iterations of each stage (3 values);
+ * res_hist (may be NULL) the L1 increment norms, max_iter x 4 per stage. */
+int dndsmi_sdirk4_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew, int32_t n_sweeps,
+                       double CFL, double dt, int32_t UseLocaldt, int32_t max_iter, double threshold, double t,
+                       int32_t *n_iter, double *res_hist);
+
 /* ODE::ImplicitBDFDualTimeStep (DNDS_ODE.hpp:392-512): a BDF integrator of
  * order k (1..4) holding the previous k-1 states on the device.  Step i uses
  * order min(i, k) (the reference's start-up).  A handle outlives nothing:

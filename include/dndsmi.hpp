@@ -304,6 +304,33 @@ class ImplicitEulerDualTimeStep {
     bool local_;
 };
 
+/* ODE::ImplicitSDIRK4DualTimeStep (DNDS_ODE.hpp:270-372): three stages with
+ * the frhs / fdtau / fsolve / fstop of ImplicitEulerDualTimeStep. */
+class ImplicitSDIRK4DualTimeStep {
+  public:
+    ImplicitSDIRK4DualTimeStep(EulerEvaluator &ev, int nInternalRecStep, real CFL, int nTimeStepInternal,
+                               real rhsThresholdInternal, bool UseLocaldt = true)
+        : ev_(ev), n_(nInternalRecStep), cfl_(CFL), maxit_(nTimeStepInternal), thr_(rhsThresholdInternal),
+          local_(UseLocaldt) {}
+    /* one step of size dt; returns the iterations of the three stages */
+    std::array<int32_t, 3> Step(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNew, real dt, real t = 0) {
+        std::array<int32_t, 3> its{};
+        dndsmi_ctx *c = ev_.device()->ctx.get();
+        detail::check(dndsmi_sdirk4_step(c, u.h(), uRec.h(), uRecNew.h(), n_, cfl_, dt, local_, maxit_, thr_, t,
+                                         its.data(), nullptr),
+                      "SDIRK4 step", c);
+        return its;
+    }
+
+  private:
+    EulerEvaluator &ev_;
+    int n_;
+    real cfl_;
+    int maxit_;
+    real thr_;
+    bool local_;
+};
+
 /* ODE::ImplicitBDFDualTimeStep (DNDS_ODE.hpp:392-512): order k, the k-1
  * previous states on the device; frhs / fdtau / fsolve / fstop as
  * ImplicitEulerDualTimeStep.  Must not outlive the evaluator's device. */

@@ -1363,6 +1363,82 @@ extern "C" int oracle_bdf_steps(const oracle_mesh *m, const oracle_setup *s, con
     return ERR_OK;
 }
 
+// ImplicitSDIRK4DualTimeStep::Step (DNDS_ODE.hpp:316-371) with the same
+// frhs / fdtau / fsolve / fstop as oracle_implicit_euler; frhs evaluates the
+// boundary states at t + c_i dt.  n_iter receives the iterations of the 3
+// stages.
+extern "C" int oracle_sdirk4_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                  const int64_t *scan2cell, int32_t max_iter, int32_t n_sweeps, double cfl, double dt,
+                                  int32_t use_local_dt, double threshold, double t, double *u, double *urec,
+                                  int32_t *n_iter) {
+    const double zeta = 0.128886400515;
+    const double A[3][3] = {{zeta, 0, 0}, {0.5 - zeta, zeta, 0}, {2 * zeta, 1 - 4 * zeta, zeta}};
+    const double Cc[3] = {zeta, 0.5, 1 - zeta};
+    auto sqr = [](double x) { return x * x; };
+    const double B[3] = {1. / (6 * sqr(2 * zeta - 1)), (4 * sqr(zeta) - 4 * zeta + 2. / 3.) / sqr(2 * zeta - 1),
+                         1. / (6 * sqr(2 * zeta - 1))};
+    const int64_t nc = m->n_cell;
+    std::vector<double> dTau(nc), rhs(4 * nc), xLast(u, u + 4 * nc), xinc(4 * nc), urecNew(36 * nc);
+    std::vector<std::vector<double>> rhsbuf(3, std::vector<double>(4 * nc));
+    std::vector<double> lf(m->n_face), dlf(m->n_face), jinv(nc);
+    for (int iB = 0; iB < 3; iB++) {
+        std::copy(xLast.begin(), xLast.end(), u);
+        double resBase[4] = {0, 0, 0, 0};
+        const double alphaDiag = A[iB][iB];
+        n_iter[iB] = 0;
+        for (int32_t iter = 1; iter <= max_iter; iter++) {
+            double dtMin;
+            int rc = oracle_evaluate_dt(m, s, cfg, u, cfl, 1e100, use_local_dt, dTau.data(), &dtMin, lf.data(),
+                                        dlf.data());
+            if (rc)
+                return rc;
+            for (int64_t i = 0; i < nc; i++)
+                dTau[i] /= alphaDiag;
+            const double ts = t + Cc[iB] * dt;
+            for (int q = 0; q < n_sweeps; q++)
+                if ((rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), ts)))
+                    return rc;
+            if ((rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, ts, rhsbuf[size_t(iB)].data())))
+                return rc;
+            for (int64_t q = 0; q < 4 * nc; q++) {
+                double r = xLast[q];
+                r -= u[q];
+                r *= 1.0 / dt;
+                for (int jB = 0; jB <= iB; jB++)
+                    r += rhsbuf[size_t(jB)][q] * A[iB][jB];
+                rhs[q] = r;
+            }
+            std::fill(xinc.begin(), xinc.end(), 0.0);
+            oracle_lusgs_matrix_init(m, s, dTau.data(), dt, alphaDiag, lf.data(), jinv.data());
+            for (int64_t i = 0; i < nc; i++)
+                for (int v = 0; v < 4; v++)
+                    rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
+            if ((rc = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
+                                           xinc.data(), xinc.data())))
+                return rc;
+            if ((rc = oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
+                                            xinc.data(), xinc.data())))
+                return rc;
+            for (int64_t q = 0; q < 4 * nc; q++)
+                u[q] += xinc[q] * 1.0;
+            double res[4] = {0, 0, 0, 0};
+            for (int64_t i = 0; i < nc; i++)
+                for (int v = 0; v < 4; v++)
+                    res[v] += std::fabs(xinc[4 * i + v]);
+            for (int v = 0; v < 4; v++)
+                resBase[v] = iter == 1 ? res[v] : std::max(resBase[v], res[v]);
+            n_iter[iB] = iter;
+            if (res[0] / resBase[0] < threshold)
+                break;
+        }
+    }
+    std::copy(xLast.begin(), xLast.end(), u);
+    for (int jB = 0; jB < 3; jB++)
+        for (int64_t q = 0; q < 4 * nc; q++)
+            u[q] += rhsbuf[size_t(jB)][q] * (B[jB] * dt);
+    return ERR_OK;
+}
+
 extern "C" void oracle_set_threads(int32_t n) { g_threads = n > 1 ? n : 1; }
 
 extern "C" int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU,

@@ -121,6 +121,9 @@ int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s, const ora
 int oracle_bdf_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const int64_t *scan2cell,
                      int32_t k, int32_t n_steps, int32_t max_iter, int32_t n_sweeps, double cfl, double dt,
                      int32_t use_local_dt, double threshold, double t, double *u, double *urec, int32_t *n_iter);
+int oracle_sdirk4_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const int64_t *scan2cell,
+                       int32_t max_iter, int32_t n_sweeps, double cfl, double dt, int32_t use_local_dt,
+                       double threshold, double t, double *u, double *urec, int32_t *n_iter);
 
 /* fluxJacobian0_Right_Times_du (DNDS_FV_EulerEvaluator.hpp:928-956), Euler */
 int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU, double lambda_main,

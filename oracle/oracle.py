@@ -80,6 +80,9 @@ def lib() -> C.CDLL:
             "oracle_bdf_steps": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD,
                                  PD, C.POINTER(C.c_int32)],
+            "oracle_sdirk4_step": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32,
+                                   C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD, PD,
+                                   C.POINTER(C.c_int32)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -222,6 +225,20 @@ class Oracle:
                                               order.ctypes.data_as(PI64) if order is not None else None, k, n_steps,
                                               max_iter, n_sweeps, cfl, dt, int(use_local_dt), threshold, t, _d(u),
                                               _d(urec), its.ctypes.data_as(C.POINTER(C.c_int32))), "bdf steps")
+        return u, urec, its
+
+    def sdirk4_step(self, u, urec, max_iter, n_sweeps, cfl, dt, use_local_dt=True, threshold=1e-10, t=0.0,
+                    scan2cell=None):
+        """One SDIRK4 dual-time step (DNDS_ODE.hpp:270-372); returns (u, urec,
+        inner iterations of the 3 stages)."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        urec = np.array(urec, dtype=np.float64, order="C", copy=True)
+        its = np.zeros(3, dtype=np.int32)
+        order = None if scan2cell is None else np.ascontiguousarray(scan2cell, dtype=np.int64)
+        self._check(self.lib.oracle_sdirk4_step(C.byref(self.m), C.byref(self.s), C.byref(self.cfg),
+                                                order.ctypes.data_as(PI64) if order is not None else None, max_iter,
+                                                n_sweeps, cfl, dt, int(use_local_dt), threshold, t, _d(u), _d(urec),
+                                                its.ctypes.data_as(C.POINTER(C.c_int32))), "sdirk4 step")
         return u, urec, its
 
     def ssprk3_steps(self, u, urec, n_steps, n_sweeps, cfl, t=0.0, max_dt=1e100, use_local_dt=False,

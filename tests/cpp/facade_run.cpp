@@ -52,6 +52,9 @@ int main(int argc, char **argv) {
         dndsmi::ImplicitBDFDualTimeStep bdf(ev, 1, 5.0, 3, 0.0, true, 2);
         for (int step = 0; step < 2; step++, t += 0.05)
             bdf.Step(u, uRec, uRecNew, 0.05, t);
+        dndsmi::ImplicitSDIRK4DualTimeStep sdirk(ev, 1, 5.0, 2, 0.0, true);
+        sdirk.Step(u, uRec, uRecNew, 0.05, t);
+        t += 0.05;
         std::array<double, 4> res{};
         auto rhs = ev.BuildMean();
         ev.EvaluateRHS(rhs, u, uRec, t);

@@ -30,7 +30,8 @@ def test_example_program_is_built_and_links():
 def _python_run():
     from dnds_amd import _capi as A
     from dnds_amd.host import (CompactFacedMeshSerialRW, EulerEvaluator, ExplicitSSPRK3LocalDt,
-                               ImplicitBDFDualTimeStep, ImplicitEulerDualTimeStep, VRFiniteVolume2D, vr_setting)
+                               ImplicitBDFDualTimeStep, ImplicitEulerDualTimeStep, ImplicitSDIRK4DualTimeStep,
+                               VRFiniteVolume2D, vr_setting)
     mesh = CompactFacedMeshSerialRW.generate(A.MESH_TRI_SPLIT, 10, 8, 0.0, 1.0, 0.0, 1.0, jitter=0.2, seed=5,
                                              bc=(A.BC_FARFIELD,) * 4)
     vfv = VRFiniteVolume2D(mesh, vr_setting(build_limiter_data=1, far_weight=1.0))
@@ -56,6 +57,9 @@ def _python_run():
     for _ in range(2):
         bdf.Step(u, r, rn, dt=0.05, t=t)
         t += 0.05
+    ImplicitSDIRK4DualTimeStep(ev, nInternalRecStep=1, CFL=5.0, nTimeStepInternal=2, rhsThresholdInternal=0.0,
+                               UseLocaldt=True).Step(u, r, rn, dt=0.05, t=t)
+    t += 0.05
     rhs = ev.BuildMean()
     ev.EvaluateRHS(rhs, u, r, t)
     res = ev.EvaluateResidual(rhs, 1, False)

@@ -144,3 +144,22 @@ def test_bdf_rejects_a_changed_dt_and_outlives_nothing():
     with pytest.raises(Exception):
         bdf.Step(u, r, rn, dt=0.3)  # the reference asserts a constant dt
     bdf.Step(u, r, rn, dt=0.2)
+
+
+def test_sdirk4_step_matches_oracle_in_colour_order():
+    """Two SDIRK4 steps, a fixed number of inner iterations per stage."""
+    from dnds_amd.host import ImplicitSDIRK4DualTimeStep
+    c, ev, orc = _vortex(12)
+    ode = ImplicitSDIRK4DualTimeStep(ev, nInternalRecStep=2, CFL=10.0, nTimeStepInternal=4, rhsThresholdInternal=0.0)
+    u, r, rn = ev.BuildMean().upload(c.u0), c.vfv.BuildRec().setConstant(0.0), c.vfv.BuildRec()
+    ou, orr = c.u0.copy(), np.zeros((c.n_cell, 36))
+    dt = 0.2
+    for step in range(2):
+        its, hist = ode.Step(u, r, rn, dt=dt, t=step * dt)
+        assert list(its) == [4, 4, 4] and np.all(hist[:, :, 0] > 0)
+        ou, orr, oits = orc.sdirk4_step(ou, orr, 4, 2, 10.0, dt, threshold=0.0, t=step * dt,
+                                        scan2cell=ev.LUSGSScanOrder())
+        assert list(oits) == [4, 4, 4]
+    assert rel_err(u.download(), ou) <= TOL
+    assert rel_err(r.download(), orr) <= TOL
+    assert np.abs(ou - c.u0).max() > 1e-4

@@ -162,3 +162,21 @@ def test_bdf_orders_of_accuracy_in_time():
         e = [np.abs(orc.bdf_steps(c.u0, z, k, ns, 150, 2, 5.0, T / ns, threshold=1e-11)[0] - ref).max()
              for ns in (4, 8)]
         assert lo <= np.log2(e[0] / e[1]) <= hi, (k, e)
+
+
+def test_sdirk4_order_of_accuracy_in_time():
+    """ImplicitSDIRK4DualTimeStep with converged stages: against a 16-step
+    solution, halving dt divides the error by about 2^4."""
+    c, orc = _bdf_vortex(10)
+    z = np.zeros((c.n_cell, 36))
+    T = 0.4
+
+    def run(ns):
+        u, r = c.u0.copy(), z.copy()
+        for k in range(ns):
+            u, r, _ = orc.sdirk4_step(u, r, 200, 2, 5.0, T / ns, threshold=1e-12, t=k * T / ns)
+        return u
+
+    ref = run(16)
+    e = [np.abs(run(ns) - ref).max() for ns in (1, 2, 4)]
+    assert 3.5 <= np.log2(e[0] / e[1]) <= 4.6 and 3.5 <= np.log2(e[1] / e[2]) <= 4.6, e
