@@ -159,6 +159,8 @@ SIGNATURES = [
     ("dndsmi_evaluate_residual", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, PD]),
     ("dndsmi_implicit_euler_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                                              C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, PI32, PD]),
+    ("dndsmi_lusgs_matrix_vec", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("dndsmi_set_linear_solver", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     ("dndsmi_sdirk4_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                                      C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, PI32, PD]),
     ("dndsmi_bdf_create", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),

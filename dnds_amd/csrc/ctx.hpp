@@ -110,6 +110,15 @@ struct dndsmi_ctx {
     int32_t *colour = nullptr;        // [ldc]
     int32_t *col_cells = nullptr;     // [nc]
     double *jinv = nullptr;           // [ldc] 1 / LU-SGS diagonal, scan order
+    double *jdiag = nullptr;          // [ldc] the LU-SGS diagonal, scan order (first use)
+    // fsolve's linear solver: 0 LU-SGS, 1 GMRES, 2 LU-SGS then GMRES
+    // (left-preconditioned by LU-SGS), with its subspace size and restarts
+    int32_t gmres_code = 0, n_gmres_space = 10, n_gmres_iter = 2;
+    double *gm_v = nullptr;           // [n_gmres_space + 3][ldc][4] Krylov vectors, A v, ML b (first use)
+    double *gm_h = nullptr;           // [(n_gmres_space + 1) * n_gmres_space] Hessenberg matrix
+    double *gm_part = nullptr;        // [n_gmres_space + 1][grid] dot partial sums
+    double *gm_y = nullptr;           // [n_gmres_space] least-squares solution
+    int32_t gm_alloc_space = 0;
     // per scan position p (col_cells order), built on first LU-SGS use:
     // per slot [s][ldc] the neighbour used by the forward / backward sweep
     // (-1 if none), the face (-1 if no slot), the unit normal out of the

@@ -10,6 +10,7 @@
 // same order per cell as the oracle run with the same scan order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_lusgs_init(int32_t nc, int64_t ldc, 
                                                        const double *__restrict__ sV,
                                                        const double *__restrict__ lambda_face,
                                                        const double *__restrict__ dtau, double dt, double alpha,
-                                                       double *__restrict__ jinv) {
+                                                       double *__restrict__ jinv, double *__restrict__ jdiag) {
     const int32_t p = xcd_block() * blockDim.x + threadIdx.x;
     if (p >= nc)
         return;
@@ -105,7 +106,136 @@ __global__ __launch_bounds__(kBlock) void k_lusgs_init(int32_t nc, int64_t ldc, 
             continue;
         fp += (0.5 * alpha) * sS[s * ldc + p] * lambda_face[f] / v;
     }
+    jdiag[p] = fp + alpha * 0.0; // jacobianCell = fpDivisor I + alphaDiag 0 (source Jacobian)
     jinv[p] = 1.0 / (fp + alpha * 0.0);
+}
+
+// LUSGSMatrixVec (DNDS_FV_EulerEvaluator.cpp:1062-1145), per scan position:
+// Ax_i = D_i x_i - sum_f (alpha/2) S_f / V_i dF(u_j, x_j) over every
+// neighbour (the earlier ones and the later ones), faces in cell2face order
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_lusgs_matvec(int32_t nc, int64_t ldc, const int32_t *__restrict__ cells,
+                                                         const int32_t *__restrict__ jf,
+                                                         const int32_t *__restrict__ jb,
+                                                         const int32_t *__restrict__ sf,
+                                                         const double *__restrict__ sn0,
+                                                         const double *__restrict__ sn1,
+                                                         const double *__restrict__ sS,
+                                                         const double *__restrict__ sV,
+                                                         const double *__restrict__ lambda_face,
+                                                         const double *__restrict__ jdiag, double gamma, double alpha,
+                                                         const double *__restrict__ u, const double *__restrict__ x,
+                                                         double *__restrict__ Ax) {
+    const int32_t p = xcd_block() * blockDim.x + threadIdx.x;
+    if (p >= nc)
+        return;
+    const int32_t i = cells[p];
+    const double vi = sV[p];
+    double buf[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < NF; s++) {
+        const int32_t a = jf[s * ldc + p], b = jb[s * ldc + p];
+        const int32_t j = a >= 0 ? a : b;
+        if (j < 0)
+            continue;
+        const int32_t f = sf[s * ldc + p];
+        double uj[4], xj[4], dF[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            uj[v] = u[4 * int64_t(j) + v], xj[v] = x[4 * int64_t(j) + v];
+        flux_jacobian_times_du(gamma, uj, sn0[s * ldc + p], sn1[s * ldc + p], xj, lambda_face[f], dF);
+        const double cf = (0.5 * alpha) * sS[s * ldc + p] / vi;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            buf[v] -= cf * dF[v];
+    }
+    const double d = jdiag[p];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        Ax[4 * int64_t(i) + v] = d * x[4 * int64_t(i) + v] - buf[v];
+}
+
+// ------------------------------------------------------- GMRES vectors ----
+// Per-block partial sums of nq dot products w . V_q (V_q = V + q ldv), each
+// cell's 4 products summed in order, then a fixed tree over the block.
+constexpr int kMaxDots = 33;
+__global__ __launch_bounds__(kBlock) void k_gm_dots(int32_t nc, int32_t nq, const double *__restrict__ w,
+                                                    const double *__restrict__ V, int64_t ldv,
+                                                    double *__restrict__ part) {
+    __shared__ double red[kBlock];
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double wi[4] = {0, 0, 0, 0};
+    if (i < nc)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            wi[v] = w[4 * int64_t(i) + v];
+    for (int32_t q = 0; q < nq; q++) {
+        double a = 0.0;
+        if (i < nc)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                a += wi[v] * V[q * ldv + 4 * int64_t(i) + v];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        for (int st = kBlock / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st)
+                red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            part[int64_t(q) * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+// folds the partials of each dot product in a fixed order; out[q * ostride]
+// = the sum, or its square root (norms)
+__global__ __launch_bounds__(64) void k_gm_dots_final(int32_t nblk, int32_t nq, const double *__restrict__ part,
+                                                      int32_t take_sqrt, double *__restrict__ out, int32_t ostride) {
+    const int lane = threadIdx.x;
+    for (int32_t q = 0; q < nq; q++) {
+        double a = 0.0;
+        for (int32_t b = lane; b < nblk; b += 64)
+            a += part[int64_t(q) * nblk + b];
+        for (int off = 32; off > 0; off >>= 1)
+            a += __shfl_down(a, off, 64);
+        if (lane == 0)
+            out[q * ostride] = take_sqrt ? sqrt(a) : a;
+    }
+}
+
+// y += x * a (ArrayDOFV::addTo, DNDS_DerivedArray.hpp:372-377)
+__global__ __launch_bounds__(kBlock) void k_gm_axpy(int64_t n, double *__restrict__ y, const double *__restrict__ x,
+                                                    double a) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n)
+        y[k] += x[k] * a;
+}
+
+// v *= a, with a given, or a = 1 / h[0] read from the device
+__global__ __launch_bounds__(kBlock) void k_gm_scale(int64_t n, double *__restrict__ v, double a,
+                                                     const double *__restrict__ h) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const double s = h ? 1.0 / h[0] : a;
+    v[k] *= s;
+}
+
+// classical Gram-Schmidt subtraction: v += V_i * (-h_i) for i = 0 .. nq-1 in
+// turn, h_i = h[i * hstride] on the device; or, with sign +1, x += V_j y_j
+__global__ __launch_bounds__(kBlock) void k_gm_combine(int64_t n, int32_t nq, double *__restrict__ v,
+                                                       const double *__restrict__ V, int64_t ldv,
+                                                       const double *__restrict__ h, int32_t hstride, int32_t negate) {
+    const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    double a = v[k];
+    for (int32_t q = 0; q < nq; q++) {
+        const double c = h[q * hstride];
+        a += V[q * ldv + k] * (negate ? -c : c);
+    }
+    v[k] = a;
 }
 
 // One colour of an LU-SGS sweep (UpdateLUSGSForward / UpdateLUSGSBackward,
@@ -329,13 +459,15 @@ int ensure_lusgs_scan(dndsmi_ctx *c) {
 
 int launch_lusgs_init(dndsmi_ctx *c, const double *dtau, double dt, double alpha) {
     DNDSMI_TRY(ensure_lusgs_scan(c));
+    if (!c->jdiag)
+        DNDSMI_TRY(c->alloc(c->jdiag, c->ldc));
     const int nc = int(c->nc);
     if (c->nfmax == 3)
         hipLaunchKernelGGL(k_lusgs_init<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
-                           c->sc_face, c->sc_area, c->sc_vol, c->lambda_face, dtau, dt, alpha, c->jinv);
+                           c->sc_face, c->sc_area, c->sc_vol, c->lambda_face, dtau, dt, alpha, c->jinv, c->jdiag);
     else
         hipLaunchKernelGGL(k_lusgs_init<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
-                           c->sc_face, c->sc_area, c->sc_vol, c->lambda_face, dtau, dt, alpha, c->jinv);
+                           c->sc_face, c->sc_area, c->sc_vol, c->lambda_face, dtau, dt, alpha, c->jinv, c->jdiag);
     return c->launch_check("k_lusgs_init");
 }
 
@@ -363,6 +495,166 @@ int launch_lusgs_sweep(dndsmi_ctx *c, double alpha, const double *rhs, const dou
         prof_end(c);
         DNDSMI_TRY(c->launch_check("k_lusgs_sweep"));
     }
+    return DNDSMI_OK;
+}
+
+int launch_lusgs_matvec(dndsmi_ctx *c, double alpha, const double *u, const double *x, double *Ax) {
+    DNDSMI_TRY(ensure_lusgs_scan(c));
+    const int nc = int(c->nc);
+    if (c->nfmax == 3)
+        hipLaunchKernelGGL(k_lusgs_matvec<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
+                           c->sc_jf, c->sc_jb, c->sc_face, c->sc_n0, c->sc_n1, c->sc_area, c->sc_vol, c->lambda_face,
+                           c->jdiag, c->gas.gamma, alpha, u, x, Ax);
+    else
+        hipLaunchKernelGGL(k_lusgs_matvec<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
+                           c->sc_jf, c->sc_jb, c->sc_face, c->sc_n0, c->sc_n1, c->sc_area, c->sc_vol, c->lambda_face,
+                           c->jdiag, c->gas.gamma, alpha, u, x, Ax);
+    return c->launch_check("k_lusgs_matvec");
+}
+
+// least squares min |H y - beta e1| for the (k+1) x k Hessenberg H
+// (row-major) by Householder reflections
+std::vector<double> hessenberg_lsq_householder(std::vector<double> H, int k, double beta) {
+    const int m = k + 1;
+    std::vector<double> g(static_cast<size_t>(m), 0.0), w(static_cast<size_t>(m), 0.0);
+    g[0] = beta;
+    for (int j = 0; j < k; j++) {
+        double nrm = 0.0;
+        for (int r = j; r < m; r++)
+            nrm += H[size_t(r) * k + j] * H[size_t(r) * k + j];
+        nrm = std::sqrt(nrm);
+        if (nrm == 0.0)
+            continue;
+        const double a = H[size_t(j) * k + j], alpha = a > 0 ? -nrm : nrm;
+        for (int r = 0; r < m; r++)
+            w[size_t(r)] = r < j ? 0.0 : H[size_t(r) * k + j];
+        w[size_t(j)] -= alpha;
+        double ww = 0.0;
+        for (int r = j; r < m; r++)
+            ww += w[size_t(r)] * w[size_t(r)];
+        if (ww == 0.0)
+            continue;
+        for (int col = j; col < k; col++) {
+            double d = 0.0;
+            for (int r = j; r < m; r++)
+                d += w[size_t(r)] * H[size_t(r) * k + col];
+            const double f = 2.0 * d / ww;
+            for (int r = j; r < m; r++)
+                H[size_t(r) * k + col] -= f * w[size_t(r)];
+        }
+        double d = 0.0;
+        for (int r = j; r < m; r++)
+            d += w[size_t(r)] * g[size_t(r)];
+        const double f = 2.0 * d / ww;
+        for (int r = j; r < m; r++)
+            g[size_t(r)] -= f * w[size_t(r)];
+    }
+    std::vector<double> y(size_t(k), 0.0);
+    for (int j = k - 1; j >= 0; j--) {
+        double acc = g[size_t(j)];
+        for (int col = j + 1; col < k; col++)
+            acc -= H[size_t(j) * k + col] * y[size_t(col)];
+        y[size_t(j)] = acc / H[size_t(j) * k + j];
+    }
+    return y;
+}
+
+int ensure_gmres(dndsmi_ctx *c) {
+    const int k = c->n_gmres_space;
+    if (c->gm_v && c->gm_alloc_space >= k)
+        return DNDSMI_OK;
+    auto release = [&](double *&p) {
+        if (p) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(p);
+            c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), static_cast<void *>(p)), c->allocs.end());
+            p = nullptr;
+        }
+    };
+    release(c->gm_v), release(c->gm_h), release(c->gm_part), release(c->gm_y);
+    DNDSMI_TRY(c->alloc(c->gm_v, int64_t(k + 3) * 4 * c->ldc));
+    DNDSMI_TRY(c->alloc(c->gm_h, int64_t(k + 1) * k));
+    DNDSMI_TRY(c->alloc(c->gm_part, int64_t(k + 1) * grid_for(c->nc)));
+    DNDSMI_TRY(c->alloc(c->gm_y, k));
+    c->gm_alloc_space = k;
+    return DNDSMI_OK;
+}
+
+// GMRES_LeftPreconditioned::solve (DNDS_Linear.hpp:48-104) with FA =
+// LUSGSMatrixVec, FML = LU-SGS forward + backward, FStop never true (the
+// solver's, DNDS_FV_EulerSolver.hpp:959-969); x is updated in place.
+int gmres_solve(dndsmi_ctx *c, double alpha, const double *b, const double *u, double *x) {
+    DNDSMI_TRY(ensure_gmres(c));
+    const int k = c->n_gmres_space;
+    const int64_t ldv = 4 * c->ldc, n = 4 * c->nc;
+    const int nc = int(c->nc);
+    const unsigned gn = grid_for(n), gc = grid_for(nc);
+    double *V = c->gm_v, *Vt = V + int64_t(k + 1) * ldv, *MLb = V + int64_t(k + 2) * ldv;
+    auto FML = [&](const double *in, double *out) -> int {
+        DNDSMI_TRY(launch_lusgs_sweep<true>(c, alpha, in, u, out, out));
+        return launch_lusgs_sweep<false>(c, alpha, in, u, out, out);
+    };
+    auto dots = [&](const double *w, const double *Vq, int nq, int take_sqrt, double *out, int ostride) -> int {
+        hipLaunchKernelGGL(k_gm_dots, dim3(gc), dim3(kBlock), 0, c->stream, nc, nq, w, Vq, ldv, c->gm_part);
+        hipLaunchKernelGGL(k_gm_dots_final, dim3(1), dim3(64), 0, c->stream, int32_t(gc), nq, c->gm_part, take_sqrt,
+                           out, ostride);
+        return c->launch_check("GMRES dot products");
+    };
+    DNDSMI_TRY(FML(b, MLb));
+    std::vector<double> H(size_t(k + 1) * size_t(k));
+    for (int32_t iRestart = 0; iRestart <= c->n_gmres_iter; iRestart++) {
+        DNDSMI_TRY(launch_lusgs_matvec(c, alpha, u, x, Vt));
+        DNDSMI_TRY(FML(Vt, V));
+        hipLaunchKernelGGL(k_gm_axpy, dim3(gn), dim3(kBlock), 0, c->stream, n, V, MLb, -1.0);
+        DNDSMI_TRY(dots(V, V, 1, 1, c->gm_y, 1));
+        double beta = 0.0;
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(&beta, c->gm_y, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy"));
+        DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+        if (std::fabs(beta) < 1e-200 || iRestart == c->n_gmres_iter)
+            break;
+        hipLaunchKernelGGL(k_gm_scale, dim3(gn), dim3(kBlock), 0, c->stream, n, V, -1.0 / beta, nullptr);
+        for (int j = 0; j < k; j++) {
+            double *Vj = V + int64_t(j) * ldv, *Vn = Vj + ldv;
+            DNDSMI_TRY(launch_lusgs_matvec(c, alpha, u, Vj, Vt));
+            DNDSMI_TRY(FML(Vt, Vn));
+            DNDSMI_TRY(dots(Vn, V, j + 1, 0, c->gm_h + j, k)); // h(i, j), i <= j
+            hipLaunchKernelGGL(k_gm_combine, dim3(gn), dim3(kBlock), 0, c->stream, n, j + 1, Vn, V, ldv, c->gm_h + j,
+                               k, 1);
+            DNDSMI_TRY(dots(Vn, Vn, 1, 1, c->gm_h + int64_t(j + 1) * k + j, 1)); // h(j + 1, j)
+            hipLaunchKernelGGL(k_gm_scale, dim3(gn), dim3(kBlock), 0, c->stream, n, Vn, 0.0,
+                               c->gm_h + int64_t(j + 1) * k + j);
+            DNDSMI_TRY(c->launch_check("GMRES Arnoldi"));
+        }
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(H.data(), c->gm_h, sizeof(double) * H.size(), hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "copy"));
+        DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+        for (int i = 0; i <= k; i++) // below the subdiagonal: zero (never written on the device)
+            for (int j = 0; j < i - 1; j++)
+                H[size_t(i) * k + j] = 0.0;
+        const std::vector<double> y = hessenberg_lsq_householder(H, k, beta);
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(c->gm_y, y.data(), sizeof(double) * size_t(k), hipMemcpyHostToDevice,
+                                         c->stream),
+                          "copy"));
+        hipLaunchKernelGGL(k_gm_combine, dim3(gn), dim3(kBlock), 0, c->stream, n, k, x, V, ldv, c->gm_y, 1, 0);
+        DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize")); // y lives on the host stack
+        DNDSMI_TRY(c->launch_check("GMRES update"));
+    }
+    return DNDSMI_OK;
+}
+
+// the solver's fsolve after the right-hand side and its compression (the
+// drivers' rhs kernels also zero xinc): LUSGSMatrixInit, then LU-SGS and / or
+// GMRES as c->gmres_code says (DNDS_FV_EulerSolver.hpp:852-975)
+int fsolve(dndsmi_ctx *c, const double *dtau, double dt, double alpha, const double *rhs, const double *u,
+           double *xinc) {
+    DNDSMI_TRY(launch_lusgs_init(c, dtau, dt, alpha));
+    if (c->gmres_code == 0 || c->gmres_code == 2) {
+        DNDSMI_TRY(launch_lusgs_sweep<true>(c, alpha, rhs, u, xinc, xinc));
+        DNDSMI_TRY(launch_lusgs_sweep<false>(c, alpha, rhs, u, xinc, xinc));
+    }
+    if (c->gmres_code != 0)
+        DNDSMI_TRY(gmres_solve(c, alpha, rhs, u, xinc));
     return DNDSMI_OK;
 }
 
@@ -416,6 +708,25 @@ extern "C" int dndsmi_update_lusgs_backward(dndsmi_ctx *c, double alphaDiag, con
     return launch_lusgs_sweep<false>(c, alphaDiag, rhs->d, u->d, uInc->d, uIncNew->d);
 }
 
+extern "C" int dndsmi_lusgs_matrix_vec(dndsmi_ctx *c, double alphaDiag, const dndsmi_array *u,
+                                       const dndsmi_array *uInc, dndsmi_array *AuInc) {
+    if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uInc, DNDSMI_ARRAY_DOF) ||
+        !same_ctx(c, AuInc, DNDSMI_ARRAY_DOF) || AuInc == uInc || AuInc == u)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (!c->jdiag) {
+        c->err = "LUSGSMatrixVec needs a LUSGSMatrixInit first";
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
+    return launch_lusgs_matvec(c, alphaDiag, u->d, uInc->d, AuInc->d);
+}
+
+extern "C" int dndsmi_set_linear_solver(dndsmi_ctx *c, int32_t gmres_code, int32_t n_space, int32_t n_iter) {
+    if (!c || gmres_code < 0 || gmres_code > 2 || n_space < 1 || n_space > kMaxDots - 1 || n_iter < 0)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    c->gmres_code = gmres_code, c->n_gmres_space = n_space, c->n_gmres_iter = n_iter;
+    return DNDSMI_OK;
+}
+
 extern "C" int dndsmi_evaluate_residual(dndsmi_ctx *c, const dndsmi_array *rhs, int32_t P, int32_t volWise,
                                         double *res) {
     if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !res || P < 1)
@@ -457,10 +768,8 @@ extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi
         hipLaunchKernelGGL(k_dual_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->xlast, u->d, 1.0 / dt,
                            c->imp_dtau, c->imp_rhs, c->imp_xinc);
         DNDSMI_TRY(c->launch_check("k_dual_rhs"));
-        // fsolve: LU-SGS forward + backward, in place
-        DNDSMI_TRY(launch_lusgs_init(c, c->imp_dtau, dt, alpha));
-        DNDSMI_TRY(launch_lusgs_sweep<true>(c, alpha, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
-        DNDSMI_TRY(launch_lusgs_sweep<false>(c, alpha, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+        // fsolve
+        DNDSMI_TRY(fsolve(c, c->imp_dtau, dt, alpha, c->imp_rhs, u->d, c->imp_xinc));
         // x += xinc, fstop on the L1 norm of xinc
         double res[4];
         DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
@@ -578,9 +887,7 @@ extern "C" int dndsmi_sdirk4_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
                                1.0 / dt, c->imp_stage, lds, ib + 1, kSA[ib][0], kSA[ib][1], kSA[ib][2], c->imp_dtau,
                                c->imp_rhs, c->imp_xinc);
             DNDSMI_TRY(c->launch_check("k_sdirk_rhs"));
-            DNDSMI_TRY(launch_lusgs_init(c, c->imp_dtau, dt, aii));
-            DNDSMI_TRY(launch_lusgs_sweep<true>(c, aii, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
-            DNDSMI_TRY(launch_lusgs_sweep<false>(c, aii, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+            DNDSMI_TRY(fsolve(c, c->imp_dtau, dt, aii, c->imp_rhs, u->d, c->imp_xinc));
             double res[4];
             DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
             if (res_hist)
@@ -708,9 +1015,7 @@ extern "C" int dndsmi_bdf_step(dndsmi_bdf *b, dndsmi_array *u, dndsmi_array *uRe
                            c->imp_rhs, c->imp_xinc);
         DNDSMI_TRY(c->launch_check("k_bdf_rhs"));
         // fsolve with alphaDiag = alpha0
-        DNDSMI_TRY(launch_lusgs_init(c, c->imp_dtau, dt, alpha0));
-        DNDSMI_TRY(launch_lusgs_sweep<true>(c, alpha0, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
-        DNDSMI_TRY(launch_lusgs_sweep<false>(c, alpha0, c->imp_rhs, u->d, c->imp_xinc, c->imp_xinc));
+        DNDSMI_TRY(fsolve(c, c->imp_dtau, dt, alpha0, c->imp_rhs, u->d, c->imp_xinc));
         double res[4];
         DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
         if (res_hist)

@@ -421,6 +421,19 @@ class EulerEvaluator:
         _check(_lib().dndsmi_update_lusgs_backward(self.device.ctx, alphaDiag, rhs.h, u.h, uInc.h, uIncNew.h),
                "UpdateLUSGSBackward", self.device.ctx)
 
+    def LUSGSMatrixVec(self, alphaDiag: float, u: ArrayDOFV, uInc: ArrayDOFV, AuInc: ArrayDOFV):
+        """EulerEvaluator::LUSGSMatrixVec (DNDS_FV_EulerEvaluator.cpp:1062-1145),
+        with the diagonal of the last LUSGSMatrixInit."""
+        _check(_lib().dndsmi_lusgs_matrix_vec(self.device.ctx, alphaDiag, u.h, uInc.h, AuInc.h), "LUSGSMatrixVec",
+               self.device.ctx)
+
+    def SetLinearSolver(self, gmresCode: int = 0, nGmresSpace: int = 10, nGmresIter: int = 2):
+        """The dual-time steps' fsolve: 0 LU-SGS, 1 GMRES preconditioned by
+        LU-SGS, 2 LU-SGS then GMRES (the solver's gmresCode, nGmresSpace,
+        nGmresIter, DNDS_FV_EulerSolver.hpp:117-119)."""
+        _check(_lib().dndsmi_set_linear_solver(self.device.ctx, gmresCode, nGmresSpace, nGmresIter),
+               "linear solver", self.device.ctx)
+
     def EvaluateResidual(self, rhs: ArrayDOFV, P: int = 1, volWise: bool = False) -> np.ndarray:
         res = np.empty(4)
         _check(_lib().dndsmi_evaluate_residual(self.device.ctx, rhs.h, P, int(volWise), A.dptr(res)),

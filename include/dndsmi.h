@@ -320,6 +320,19 @@ int dndsmi_implicit_euler_step(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *u
                                int32_t n_sweeps, double CFL, double dt, int32_t UseLocaldt, int32_t max_iter,
                                double threshold, double t, int32_t *n_iter, double *res_hist);
 
+/* EulerEvaluator::LUSGSMatrixVec (DNDS_FV_EulerEvaluator.cpp:1062-1145):
+ * AuInc_i = D_i uInc_i - sum_f (alphaDiag/2) S_f / V_i dF(u_j, uInc_j) over
+ * every neighbour, D from the last LUSGSMatrixInit. */
+int dndsmi_lusgs_matrix_vec(dndsmi_ctx *ctx, double alphaDiag, const dndsmi_array *u, const dndsmi_array *uInc,
+                            dndsmi_array *AuInc);
+
+/* The linear solver of the dual-time steps' fsolve (the solver's gmresCode,
+ * nGmresSpace, nGmresIter; DNDS_FV_EulerSolver.hpp:117-119, 852-975):
+ * 0 LU-SGS forward + backward, 1 GMRES_LeftPreconditioned (DNDS_Linear.hpp:
+ * 13-108) with LU-SGS as the preconditioner from a zero start, 2 LU-SGS then
+ * GMRES from its result.  n_space >= 1, n_iter >= 0 restarts. */
+int dndsmi_set_linear_solver(dndsmi_ctx *ctx, int32_t gmres_code, int32_t n_space, int32_t n_iter);
+
 /* ODE::ImplicitSDIRK4DualTimeStep::Step (DNDS_ODE.hpp:270-372): three
  * stages i, each started from the step's initial state and iterated with the
  * same frhs / fdtau / fsolve / fstop as dndsmi_implicit_euler_step, with

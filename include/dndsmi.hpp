@@ -240,6 +240,13 @@ class EulerEvaluator {
         detail::check(dndsmi_update_lusgs_backward(ctx(), alphaDiag, rhs.h(), u.h(), uInc.h(), uIncNew.h()),
                       "UpdateLUSGSBackward", ctx());
     }
+    void LUSGSMatrixVec(real alphaDiag, ArrayDOFV &u, ArrayDOFV &uInc, ArrayDOFV &AuInc) {
+        detail::check(dndsmi_lusgs_matrix_vec(ctx(), alphaDiag, u.h(), uInc.h(), AuInc.h()), "LUSGSMatrixVec", ctx());
+    }
+    /* the dual-time steps' fsolve: gmresCode 0 LU-SGS, 1 GMRES, 2 LU-SGS then GMRES */
+    void SetLinearSolver(int gmresCode, int nGmresSpace = 10, int nGmresIter = 2) {
+        detail::check(dndsmi_set_linear_solver(ctx(), gmresCode, nGmresSpace, nGmresIter), "SetLinearSolver", ctx());
+    }
     void EvaluateResidual(std::array<real, 4> &res, ArrayDOFV &rhs, index P = 1, bool volWise = false) {
         detail::check(dndsmi_evaluate_residual(ctx(), rhs.h(), int32_t(P), volWise, res.data()), "EvaluateResidual",
                       ctx());

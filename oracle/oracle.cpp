@@ -1215,6 +1215,156 @@ extern "C" int oracle_lusgs_backward(const oracle_mesh *m, const oracle_setup *s
     return lusgs_sweep(m, s, cfg, scan2cell, jinv, lambda_face, alphaDiag, rhs, u, uInc, uIncNew, false);
 }
 
+// LUSGSMatrixVec (DNDS_FV_EulerEvaluator.cpp:1062-1145), jacobianTypeCode 0:
+// Ax_i = D_i x_i - sum_f (alpha/2) S_f / V_i dF(u_j, x_j) over every
+// neighbour, faces in cell2face order
+extern "C" int oracle_lusgs_matrix_vec(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                                       const double *diag, const double *lambda_face, double alphaDiag,
+                                       const double *u, const double *x, double *Ax) {
+    for (int64_t i = 0; i < m->n_cell; i++) {
+        double buf[4] = {0, 0, 0, 0};
+        for (int64_t q = m->cell_ptr[i]; q < m->cell_ptr[i + 1]; q++) {
+            const int64_t f = m->cell2face[q];
+            const int side = m->face2cell[2 * f] == i ? 0 : 1;
+            const int64_t j = m->face2cell[2 * f + 1 - side];
+            if (j == kNoCell)
+                continue;
+            double n[2];
+            normalized(s->face_norm_center[2 * f], s->face_norm_center[2 * f + 1], n[0], n[1]);
+            const double sg = side ? -1.0 : 1.0;
+            n[0] *= sg, n[1] *= sg;
+            double dF[4];
+            flux_jacobian_times_du(cfg->gamma, u + 4 * j, n, x + 4 * j, lambda_face[f], dF);
+            const double coef = (0.5 * alphaDiag) * s->face_area[f] / s->volume[i];
+            for (int v = 0; v < 4; v++)
+                buf[v] -= coef * dF[v];
+        }
+        for (int v = 0; v < 4; v++)
+            Ax[4 * i + v] = diag[i] * x[4 * i + v] - buf[v];
+    }
+    return ERR_OK;
+}
+
+namespace {
+
+double norm2(const std::vector<double> &a) {
+    double q = 0;
+    for (double v : a)
+        q += v * v;
+    return std::sqrt(q);
+}
+
+double dot(const std::vector<double> &a, const std::vector<double> &b) {
+    double q = 0;
+    for (size_t k = 0; k < a.size(); k++)
+        q += a[k] * b[k];
+    return q;
+}
+
+// least squares min |H y - beta e1| for the (k+1) x k Hessenberg H (row-major),
+// by Givens rotations
+std::vector<double> hessenberg_lsq(std::vector<double> H, int k, double beta) {
+    std::vector<double> g(size_t(k) + 1, 0.0);
+    g[0] = beta;
+    for (int j = 0; j < k; j++) {
+        const double a = H[size_t(j) * k + j], b = H[size_t(j + 1) * k + j];
+        const double r = std::hypot(a, b), c = r > 0 ? a / r : 1.0, sn = r > 0 ? b / r : 0.0;
+        for (int col = j; col < k; col++) {
+            const double x = H[size_t(j) * k + col], y = H[size_t(j + 1) * k + col];
+            H[size_t(j) * k + col] = c * x + sn * y;
+            H[size_t(j + 1) * k + col] = -sn * x + c * y;
+        }
+        const double x = g[size_t(j)], y = g[size_t(j) + 1];
+        g[size_t(j)] = c * x + sn * y, g[size_t(j) + 1] = -sn * x + c * y;
+    }
+    std::vector<double> yv(size_t(k), 0.0);
+    for (int j = k - 1; j >= 0; j--) {
+        double acc = g[size_t(j)];
+        for (int col = j + 1; col < k; col++)
+            acc -= H[size_t(j) * k + col] * yv[size_t(col)];
+        yv[size_t(j)] = acc / H[size_t(j) * k + j];
+    }
+    return yv;
+}
+
+} // namespace
+
+// the solver's fsolve (DNDS_FV_EulerSolver.hpp:852-975), jacobianTypeCode 0,
+// nSGSIterationInternal 0, CompressInc the identity for the Euler model
+extern "C" int oracle_fsolve(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                             const int64_t *scan2cell, const double *dTau, double dt, double alphaDiag,
+                             const double *lambda_face, const double *u, double *rhs, double *xinc) {
+    const int64_t nc = m->n_cell, n = 4 * nc;
+    std::vector<double> jinv(nc), diag(nc);
+    std::fill(xinc, xinc + n, 0.0);
+    oracle_lusgs_matrix_init(m, s, dTau, dt, alphaDiag, lambda_face, jinv.data());
+    for (int64_t i = 0; i < nc; i++) { // jacobianCell = fpDivisor I + alphaDiag 0
+        double fp = 1.0 / dTau[i] + 1.0 / dt;
+        for (int64_t q = m->cell_ptr[i]; q < m->cell_ptr[i + 1]; q++) {
+            const int64_t f = m->cell2face[q];
+            fp += (0.5 * alphaDiag) * s->face_area[f] * lambda_face[f] / s->volume[i];
+        }
+        diag[i] = fp + alphaDiag * 0.0;
+    }
+    for (int64_t i = 0; i < nc; i++)
+        for (int v = 0; v < 4; v++)
+            rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
+    int rc;
+    auto FML = [&](const double *b, double *out) -> int { // LU-SGS forward + backward, in place
+        int r = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lambda_face, alphaDiag, b, u, out, out);
+        return r ? r : oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lambda_face, alphaDiag, b, u, out, out);
+    };
+    if (cfg->gmres_code == 0 || cfg->gmres_code == 2)
+        if ((rc = FML(rhs, xinc)))
+            return rc;
+    if (cfg->gmres_code != 0) {
+        // GMRES_LeftPreconditioned::solve (DNDS_Linear.hpp:48-104), FStop never true
+        const int k = cfg->n_gmres_space;
+        const uint32_t nRestart = uint32_t(cfg->n_gmres_iter);
+        std::vector<std::vector<double>> Vs(static_cast<size_t>(k) + 1, std::vector<double>(static_cast<size_t>(n)));
+        std::vector<double> Vt(static_cast<size_t>(n)), MLb(static_cast<size_t>(n), 0.0),
+            H(static_cast<size_t>(k + 1) * static_cast<size_t>(k), 0.0);
+        if ((rc = FML(rhs, MLb.data())))
+            return rc;
+        for (uint32_t iRestart = 0; iRestart <= nRestart; iRestart++) {
+            oracle_lusgs_matrix_vec(m, s, cfg, diag.data(), lambda_face, alphaDiag, u, xinc, Vt.data());
+            std::fill(Vs[0].begin(), Vs[0].end(), 0.0);
+            if ((rc = FML(Vt.data(), Vs[0].data())))
+                return rc;
+            for (int64_t q = 0; q < n; q++)
+                Vs[0][size_t(q)] += MLb[size_t(q)] * -1.0;
+            const double beta = norm2(Vs[0]);
+            if (std::fabs(beta) < 1e-200 || iRestart == nRestart)
+                break;
+            for (double &v : Vs[0])
+                v *= -1.0 / beta;
+            for (int j = 0; j < k; j++) {
+                oracle_lusgs_matrix_vec(m, s, cfg, diag.data(), lambda_face, alphaDiag, u, Vs[size_t(j)].data(),
+                                        Vt.data());
+                std::fill(Vs[size_t(j) + 1].begin(), Vs[size_t(j) + 1].end(), 0.0);
+                if ((rc = FML(Vt.data(), Vs[size_t(j) + 1].data())))
+                    return rc;
+                for (int i = 0; i <= j; i++)
+                    H[size_t(i) * k + j] = dot(Vs[size_t(j) + 1], Vs[size_t(i)]);
+                for (int i = 0; i <= j; i++)
+                    for (int64_t q = 0; q < n; q++)
+                        Vs[size_t(j) + 1][size_t(q)] += Vs[size_t(i)][size_t(q)] * -H[size_t(i) * k + j];
+                H[size_t(j + 1) * k + j] = norm2(Vs[size_t(j) + 1]);
+                for (double &v : Vs[size_t(j) + 1])
+                    v *= 1.0 / H[size_t(j + 1) * k + j];
+            }
+            const std::vector<double> y = hessenberg_lsq(H, k, beta);
+            for (int j = 0; j < k; j++)
+                for (int64_t q = 0; q < n; q++)
+                    xinc[q] += Vs[size_t(j)][size_t(q)] * y[size_t(j)];
+        }
+    }
+    for (int64_t q = 0; q < n; q++)
+        if (!std::isfinite(xinc[q]))
+            return ERR_NAN;
+    return ERR_OK;
+}
+
 // Implicit-Euler dual-time iterations (DNDS_ODE.hpp:248-263) with
 // frhs = n_sweeps Jacobi sweeps + EvaluateRHS, fdtau = EvaluateDt / alpha,
 // fsolve = LU-SGS forward + backward in place (nSGSIterationInternal = 0,
@@ -1251,16 +1401,8 @@ extern "C" int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s
             r *= 1.0 / dt;
             rhs[k] = r + rhs[k];
         }
-        std::fill(xinc.begin(), xinc.end(), 0.0);
-        oracle_lusgs_matrix_init(m, s, dTau.data(), dt, alphaDiag, lf.data(), jinv.data());
-        for (int64_t i = 0; i < nc; i++)
-            for (int v = 0; v < 4; v++)
-                rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
-        if ((rc = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
-                                       xinc.data(), xinc.data())))
-            return rc;
-        if ((rc = oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
-                                        xinc.data(), xinc.data())))
+        if ((rc = oracle_fsolve(m, s, cfg, scan2cell, dTau.data(), dt, alphaDiag, lf.data(), u, rhs.data(),
+                                    xinc.data())))
             return rc;
         for (int64_t k = 0; k < 4 * nc; k++)
             u[k] += xinc[k];
@@ -1330,17 +1472,9 @@ extern "C" int oracle_bdf_steps(const oracle_mesh *m, const oracle_setup *s, con
                     r += xPrevs[size_t(mod(ip + prevStart, prevSiz))][q] * (coefs[kCurrent - 1][2 + ip] / dt);
                 rhs[q] = r;
             }
-            std::fill(xinc.begin(), xinc.end(), 0.0);
-            oracle_lusgs_matrix_init(m, s, dTau.data(), dt, alphaDiag, lf.data(), jinv.data());
-            for (int64_t i = 0; i < nc; i++)
-                for (int v = 0; v < 4; v++)
-                    rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
-            if ((rc = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
-                                           xinc.data(), xinc.data())))
-                return rc;
-            if ((rc = oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
-                                            xinc.data(), xinc.data())))
-                return rc;
+            if ((rc = oracle_fsolve(m, s, cfg, scan2cell, dTau.data(), dt, alphaDiag, lf.data(), u, rhs.data(),
+                                    xinc.data())))
+            return rc;
             for (int64_t q = 0; q < 4 * nc; q++)
                 u[q] += xinc[q] * 1.0;
             double res[4] = {0, 0, 0, 0};
@@ -1408,17 +1542,9 @@ extern "C" int oracle_sdirk4_step(const oracle_mesh *m, const oracle_setup *s, c
                     r += rhsbuf[size_t(jB)][q] * A[iB][jB];
                 rhs[q] = r;
             }
-            std::fill(xinc.begin(), xinc.end(), 0.0);
-            oracle_lusgs_matrix_init(m, s, dTau.data(), dt, alphaDiag, lf.data(), jinv.data());
-            for (int64_t i = 0; i < nc; i++)
-                for (int v = 0; v < 4; v++)
-                    rhs[4 * i + v] = (rhs[4 * i + v] * dTau[i]) / dTau[i];
-            if ((rc = oracle_lusgs_forward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
-                                           xinc.data(), xinc.data())))
-                return rc;
-            if ((rc = oracle_lusgs_backward(m, s, cfg, scan2cell, jinv.data(), lf.data(), alphaDiag, rhs.data(), u,
-                                            xinc.data(), xinc.data())))
-                return rc;
+            if ((rc = oracle_fsolve(m, s, cfg, scan2cell, dTau.data(), dt, alphaDiag, lf.data(), u, rhs.data(),
+                                    xinc.data())))
+            return rc;
             for (int64_t q = 0; q < 4 * nc; q++)
                 u[q] += xinc[q] * 1.0;
             double res[4] = {0, 0, 0, 0};

@@ -53,6 +53,9 @@ typedef struct oracle_setting {
     int32_t tang_weight_direction; /* 0 Norm, 1 Bary          */
     double wbap_smooth_indicator_scale;
     double wbap_n_std;
+    int32_t gmres_code;            /* fsolve: 0 LU-SGS, 1 GMRES, 2 LU-SGS then GMRES */
+    int32_t n_gmres_space;         /* Krylov subspace size      */
+    int32_t n_gmres_iter;          /* restarts                  */
 } oracle_setting;
 
 /* VRFiniteVolume2D::ReconstructionJacobiStep, SOR_Instead = false
@@ -124,6 +127,17 @@ int oracle_bdf_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_s
 int oracle_sdirk4_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const int64_t *scan2cell,
                        int32_t max_iter, int32_t n_sweeps, double cfl, double dt, int32_t use_local_dt,
                        double threshold, double t, double *u, double *urec, int32_t *n_iter);
+
+/* LUSGSMatrixVec (DNDS_FV_EulerEvaluator.cpp:1062-1145): Ax_i = D_i x_i -
+ * sum_f (alpha/2) S_f / V_i dF(u_j, x_j) over all neighbours; diag = D. */
+int oracle_lusgs_matrix_vec(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                            const double *diag, const double *lambda_face, double alphaDiag, const double *u,
+                            const double *x, double *Ax);
+/* the solver's fsolve (DNDS_FV_EulerSolver.hpp:852-975) for cfg->gmres_code:
+ * LUSGSMatrixInit, the compression of rhs, LU-SGS and / or GMRES. xinc out. */
+int oracle_fsolve(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const int64_t *scan2cell,
+                  const double *dTau, double dt, double alphaDiag, const double *lambda_face, const double *u,
+                  double *rhs, double *xinc);
 
 /* fluxJacobian0_Right_Times_du (DNDS_FV_EulerEvaluator.hpp:928-956), Euler */
 int oracle_flux_jacobian_times_du(const double *U, const double *unit_norm, const double *dU, double lambda_main,

@@ -34,7 +34,8 @@ class OSetup(C.Structure):
 class OSetting(C.Structure):
     _fields_ = [("riemann_solver", C.c_int32), ("gamma", C.c_double), ("far_field", C.c_double * 4),
                 ("jacobi_relax", C.c_double), ("tang_weight", C.c_double), ("tang_weight_direction", C.c_int32),
-                ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double)]
+                ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double),
+                ("gmres_code", C.c_int32), ("n_gmres_space", C.c_int32), ("n_gmres_iter", C.c_int32)]
 
 
 def build(quiet: bool = True) -> str:
@@ -80,6 +81,10 @@ def lib() -> C.CDLL:
             "oracle_bdf_steps": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32, C.c_int32,
                                  C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD,
                                  PD, C.POINTER(C.c_int32)],
+            "oracle_lusgs_matrix_vec": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, C.c_double,
+                                        PD, PD, PD],
+            "oracle_fsolve": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, PD, C.c_double,
+                              C.c_double, PD, PD, PD, PD],
             "oracle_sdirk4_step": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PI64, C.c_int32,
                                    C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_double, PD, PD,
                                    C.POINTER(C.c_int32)],
@@ -110,7 +115,7 @@ class Oracle:
     def __init__(self, mesh: dict, setup: dict, riemann_solver: int = 1, gamma: float = 1.4,
                  far_field=(1.0, 0.0, 0.0, 2.5), jacobi_relax: float = 1.0, tang_weight: float = 1.0,
                  tang_weight_direction: int = 0, wbap_smooth_indicator_scale: float = 1e-10,
-                 wbap_n_std: float = 10.0):
+                 wbap_n_std: float = 10.0, gmres_code: int = 0, n_gmres_space: int = 10, n_gmres_iter: int = 2):
         self._keep = {}
 
         def keep(name, a, dtype):
@@ -132,7 +137,7 @@ class Oracle:
         self.s = OSetup(**fields)
         ff = (C.c_double * 4)(*far_field)
         self.cfg = OSetting(riemann_solver, gamma, ff, jacobi_relax, tang_weight, tang_weight_direction,
-                            wbap_smooth_indicator_scale, wbap_n_std)
+                            wbap_smooth_indicator_scale, wbap_n_std, gmres_code, n_gmres_space, n_gmres_iter)
         self.lib = lib()
 
     def _check(self, rc, what):
@@ -226,6 +231,29 @@ class Oracle:
                                               max_iter, n_sweeps, cfl, dt, int(use_local_dt), threshold, t, _d(u),
                                               _d(urec), its.ctypes.data_as(C.POINTER(C.c_int32))), "bdf steps")
         return u, urec, its
+
+    def lusgs_matrix_vec(self, diag, lambda_face, alpha, u, x):
+        """LUSGSMatrixVec: the implicit operator times x (diag = D per cell)."""
+        out = np.empty((self.n_cell, 4))
+        self._check(self.lib.oracle_lusgs_matrix_vec(C.byref(self.m), C.byref(self.s), C.byref(self.cfg),
+                                                     _d(np.ascontiguousarray(diag, float)),
+                                                     _d(np.ascontiguousarray(lambda_face, float)), alpha,
+                                                     _d(np.ascontiguousarray(u, float)),
+                                                     _d(np.ascontiguousarray(x, float)), _d(out)), "matrix vec")
+        return out
+
+    def fsolve(self, dtau, dt, alpha, lambda_face, u, rhs, scan2cell=None):
+        """The solver's fsolve for this oracle's gmres_code: returns
+        (xinc, the compressed rhs)."""
+        rhs = np.array(rhs, dtype=np.float64, order="C", copy=True)
+        xinc = np.empty((self.n_cell, 4))
+        order = None if scan2cell is None else np.ascontiguousarray(scan2cell, dtype=np.int64)
+        self._check(self.lib.oracle_fsolve(C.byref(self.m), C.byref(self.s), C.byref(self.cfg),
+                                           order.ctypes.data_as(PI64) if order is not None else None,
+                                           _d(np.ascontiguousarray(dtau, float)), dt, alpha,
+                                           _d(np.ascontiguousarray(lambda_face, float)),
+                                           _d(np.ascontiguousarray(u, float)), _d(rhs), _d(xinc)), "fsolve")
+        return xinc, rhs
 
     def sdirk4_step(self, u, urec, max_iter, n_sweeps, cfl, dt, use_local_dt=True, threshold=1e-10, t=0.0,
                     scan2cell=None):

@@ -163,3 +163,47 @@ def test_sdirk4_step_matches_oracle_in_colour_order():
     assert rel_err(u.download(), ou) <= TOL
     assert rel_err(r.download(), orr) <= TOL
     assert np.abs(ou - c.u0).max() > 1e-4
+
+
+def test_lusgs_matrix_vec_matches_oracle():
+    c, ev, orc = _vortex(12)
+    n = c.n_cell
+    u, _ = random_state(n, seed=21)
+    alpha, dt = 0.9, 0.4
+    du, dtau = ev.BuildMean().upload(u), ev.BuildScalar()
+    ev.EvaluateDt(dtau, du, 5.0, 1e100, True)
+    ev.LUSGSMatrixInit(dtau, dt, alpha)
+    x = np.random.default_rng(22).standard_normal((n, 4))
+    dx, dax = ev.BuildMean().upload(x), ev.BuildMean()
+    ev.LUSGSMatrixVec(alpha, du, dx, dax)
+    dto, _, lam, _ = orc.evaluate_dt(u, 5.0, use_local_dt=True)
+    jinv = orc.lusgs_matrix_init(dto, dt, alpha, lam)
+    ref = orc.lusgs_matrix_vec(1.0 / jinv, lam, alpha, u, x)
+    assert rel_err(dax.download(), ref) <= 1e-13
+
+
+@pytest.mark.parametrize("code", [1, 2])
+def test_gmres_implicit_iterations_match_oracle(code):
+    """Implicit-Euler iterations whose fsolve is GMRES (gmresCode 1: from zero,
+    2: from the LU-SGS increment), LU-SGS-preconditioned, 2 restarts of a
+    6-vector subspace; the device's dot products and least squares round
+    differently from the oracle's, hence 1e-9."""
+    from dnds_amd.host import ImplicitEulerDualTimeStep
+    c, ev, orc = _vortex(12)
+    ev.SetLinearSolver(code, 6, 2)
+    ode = ImplicitEulerDualTimeStep(ev, nInternalRecStep=2, CFL=10.0, nTimeStepInternal=4, rhsThresholdInternal=0.0)
+    u, r, rn = ev.BuildMean().upload(c.u0), c.vfv.BuildRec().setConstant(0.0), c.vfv.BuildRec()
+    ode.Step(u, r, rn, dt=0.3)
+    import oracle
+    st = c.vfv.setting
+    orc_g = oracle.Oracle(c.mesh.arrays(), c.vfv.arrays(), riemann_solver=1, far_field=c.far,
+                          tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
+                          gmres_code=code, n_gmres_space=6, n_gmres_iter=2)
+    ou, orr, _ = orc_g.implicit_euler(c.u0, np.zeros((c.n_cell, 36)), 4, 2, 10.0, dt=0.3, threshold=0.0,
+                                      scan2cell=ev.LUSGSScanOrder())
+    assert rel_err(u.download(), ou) <= 1e-9
+    # GMRES changes the iterate: not the LU-SGS-only result
+    ol, _, _ = orc.implicit_euler(c.u0, np.zeros((c.n_cell, 36)), 4, 2, 10.0, dt=0.3, threshold=0.0,
+                                  scan2cell=ev.LUSGSScanOrder())
+    assert rel_err(ou, ol) > 1e-8
+    ev.SetLinearSolver(0)

@@ -180,3 +180,64 @@ def test_sdirk4_order_of_accuracy_in_time():
     ref = run(16)
     e = [np.abs(run(ns) - ref).max() for ns in (1, 2, 4)]
     assert 3.5 <= np.log2(e[0] / e[1]) <= 4.6 and 3.5 <= np.log2(e[1] / e[2]) <= 4.6, e
+
+
+def _small_operator(n=4, gmres_code=0, n_space=10, n_iter=2):
+    from dnds_amd import cases
+    c = cases.isentropic_vortex(n, seed=5, nthreads=1)
+    st = c.vfv.setting
+    orc = oracle.Oracle(c.mesh.arrays(), c.vfv.arrays(), riemann_solver=1, far_field=c.far,
+                        tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
+                        gmres_code=gmres_code, n_gmres_space=n_space, n_gmres_iter=n_iter)
+    rng = np.random.default_rng(4)
+    u = c.u0 * (1 + 0.05 * rng.uniform(-1, 1, (c.n_cell, 1)))
+    dtau, _, lam, _ = orc.evaluate_dt(u, 5.0, use_local_dt=True)
+    return c, orc, u, dtau, lam
+
+
+def test_lusgs_sweeps_invert_the_factored_matrix_vec_operator():
+    """With A = L + D + U the operator of LUSGSMatrixVec split by the scan
+    order, the forward + backward sweeps apply ((L + D) D^-1 (D + U))^-1 =
+    (A + L D^-1 U)^-1: the matrix-vector product and the sweeps use the same
+    off-diagonal terms with the same signs."""
+    c, orc, u, dtau, lam = _small_operator()
+    n = 4 * c.n_cell
+    alpha, dt = 0.8, 0.5
+    jinv = orc.lusgs_matrix_init(dtau, dt, alpha, lam)
+    diag = 1.0 / jinv
+    A = np.zeros((n, n))
+    Minv = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        A[:, k] = orc.lusgs_matrix_vec(diag, lam, alpha, u, e.reshape(-1, 4)).ravel()
+        y = orc.lusgs_sweep(True, jinv, lam, alpha, e.reshape(-1, 4), u, np.zeros((c.n_cell, 4)))
+        Minv[:, k] = orc.lusgs_sweep(False, jinv, lam, alpha, e.reshape(-1, 4), u, y).ravel()
+    cell = np.arange(n) // 4
+    L = np.where(cell[:, None] > cell[None, :], A, 0.0)
+    U = np.where(cell[:, None] < cell[None, :], A, 0.0)
+    D = np.diag(np.repeat(diag, 4))
+    assert np.allclose(np.where(cell[:, None] == cell[None, :], A, 0.0), D, rtol=1e-14, atol=0)
+    M = A + L @ np.diag(np.repeat(jinv, 4)) @ U
+    assert np.abs(np.linalg.inv(Minv) - M).max() <= 1e-11 * np.abs(M).max()
+    assert np.abs(L).max() > 0 and np.abs(U).max() > 0
+
+
+@pytest.mark.parametrize("code", [1, 2])
+def test_gmres_with_a_full_subspace_solves_the_system(code):
+    """GMRES_LeftPreconditioned (DNDS_Linear.hpp:48-104) with a subspace as
+    large as the system is exact: fsolve returns A^-1 rhs, A the operator of
+    LUSGSMatrixVec (gmres_code 1 from zero, 2 from the LU-SGS increment)."""
+    c, orc, u, dtau, lam = _small_operator(gmres_code=code, n_space=4 * 16, n_iter=1)
+    n = 4 * c.n_cell
+    alpha, dt = 1.0, 0.5
+    rhs = np.random.default_rng(9).standard_normal((c.n_cell, 4))
+    x, crhs = orc.fsolve(dtau, dt, alpha, lam, u, rhs)
+    jinv = orc.lusgs_matrix_init(dtau, dt, alpha, lam)
+    A = np.stack([orc.lusgs_matrix_vec(1.0 / jinv, lam, alpha, u, e.reshape(-1, 4)).ravel() for e in np.eye(n)], 1)
+    ref = np.linalg.solve(A, crhs.ravel())
+    assert np.abs(x.ravel() - ref).max() <= 1e-9 * np.abs(ref).max()
+    # LU-SGS alone is only an approximation of that solve
+    orc0 = oracle.Oracle(c.mesh.arrays(), c.vfv.arrays(), riemann_solver=1, far_field=c.far)
+    x0, _ = orc0.fsolve(dtau, dt, alpha, lam, u, rhs)
+    assert np.abs(x0.ravel() - ref).max() > 1e-6 * np.abs(ref).max()
