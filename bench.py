@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--implicit", type=int, default=0,
                     help="also time N inner iterations of the implicit-Euler dual-time step with multicolour LU-SGS "
                          "(1 sweep, CFL 10, dt = 5 explicit steps) on the same mesh and report them")
+    ap.add_argument("--gmres", type=int, default=0,
+                    help="fsolve of the implicit timing: 0 LU-SGS, 1 GMRES(10) x 2 restarts preconditioned by "
+                         "LU-SGS, 2 LU-SGS then GMRES")
     ap.add_argument("--warmstart", type=int, default=0,
                     help="also run the learned warm start workload: harvest converged reconstructions of the "
                          "40x40 vortex at 8 mesh seeds (160 snapshots each, 4.1M pairs), train on the device, "
@@ -210,6 +213,7 @@ def main():
         r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
         dts = ev.BuildScalar()
         dt_exp = ev.EvaluateDt(dts, u, args.cfl, 1e100, False)
+        ev.SetLinearSolver(args.gmres, 10, 2)
         imp = ImplicitEulerDualTimeStep(ev, nInternalRecStep=1, CFL=10.0, nTimeStepInternal=args.implicit,
                                         rhsThresholdInternal=0.0, UseLocaldt=True)
         imp.Step(u, r, rn, dt=5 * dt_exp)  # warm-up step
@@ -225,8 +229,8 @@ def main():
             lib.dndsmi_profile_query(ev.device.ctx, k, C.byref(cnt), C.byref(tot))
             if cnt.value:
                 b = int(lib.dndsmi_kernel_bytes(ev.device.ctx, k))
-                if name == "k_lusgs_sweep":  # bytes are per whole sweep: 2 sweeps per iteration
-                    sweeps = 2 * len(hist)
+                if name == "k_lusgs_sweep":  # bytes are per whole sweep (one launch per colour)
+                    sweeps = cnt.value // max(ev.LUSGSColouring()[1], 1)
                     iprof[name] = dict(launches=cnt.value, sweeps=sweeps, ms_per_sweep=tot.value / sweeps, bytes=b,
                                        gbs=b * sweeps / (tot.value * 1e-3) / 1e9)
                 else:
@@ -234,7 +238,9 @@ def main():
                     iprof[name] = dict(launches=cnt.value, avg_ms=avg, bytes=b, gbs=b / (avg * 1e-3) / 1e9 if b else 0)
         lib.dndsmi_profile_enable(ev.device.ctx, 0)
         col, ncol = ev.LUSGSColouring()
-        implicit = {"inner_iterations": len(hist), "ms_per_iteration": el / len(hist) * 1e3,
+        implicit = {"linear_solver": ["LU-SGS", "GMRES(10) x 2 restarts, LU-SGS preconditioned",
+                                      "LU-SGS then GMRES(10) x 2 restarts"][args.gmres],
+                    "inner_iterations": len(hist), "ms_per_iteration": el / len(hist) * 1e3,
                     "cell_iterations_per_s": nc * len(hist) / el, "colours": ncol,
                     "increment_drop": float(hist[-1, 0] / hist[0, 0]), "kernels": iprof}
         lus = iprof.get("k_lusgs_sweep", {})

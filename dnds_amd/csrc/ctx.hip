@@ -1939,6 +1939,9 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
+    case DNDSMI_K_LUSGS_MATVEC: // per scan position cell, D, volume, x own in, Ax out; per slot both
+        // neighbour indices, face, normal, area, u and x of the neighbour, lambda
+        return nc * (4 + 8 + 8 + 32 + 32) + c2f * (8 + 4 + 16 + 8 + 32 + 32 + 8);
     case DNDSMI_K_LUSGS_SWEEP: // one whole forward (or backward) sweep, all colours together: per scan position
         // cell, volume, 1/D, rhs (or the increment) in, increment out; per slot the neighbour index, and per used
         // slot (half of them) face, normal, area, u and the increment of the neighbour, lambda

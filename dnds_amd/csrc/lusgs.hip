@@ -501,6 +501,7 @@ int launch_lusgs_sweep(dndsmi_ctx *c, double alpha, const double *rhs, const dou
 int launch_lusgs_matvec(dndsmi_ctx *c, double alpha, const double *u, const double *x, double *Ax) {
     DNDSMI_TRY(ensure_lusgs_scan(c));
     const int nc = int(c->nc);
+    prof_begin(c, DNDSMI_K_LUSGS_MATVEC);
     if (c->nfmax == 3)
         hipLaunchKernelGGL(k_lusgs_matvec<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
                            c->sc_jf, c->sc_jb, c->sc_face, c->sc_n0, c->sc_n1, c->sc_area, c->sc_vol, c->lambda_face,
@@ -509,6 +510,7 @@ int launch_lusgs_matvec(dndsmi_ctx *c, double alpha, const double *u, const doub
         hipLaunchKernelGGL(k_lusgs_matvec<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->col_cells,
                            c->sc_jf, c->sc_jb, c->sc_face, c->sc_n0, c->sc_n1, c->sc_area, c->sc_vol, c->lambda_face,
                            c->jdiag, c->gas.gamma, alpha, u, x, Ax);
+    prof_end(c);
     return c->launch_check("k_lusgs_matvec");
 }
 

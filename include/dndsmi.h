@@ -439,7 +439,8 @@ enum {
     DNDSMI_K_DELTA32 = 10,   /* mixed precision: FP32 correction sweep       */
     DNDSMI_K_RHS_FACE32 = 11,/* mixed precision: inner-face fluxes, FP32 reconstruction */
     DNDSMI_K_RHS_CELL32 = 12,/* mixed precision: FP32 gather + stage update  */
-    DNDSMI_K_COUNT = 13
+    DNDSMI_K_LUSGS_MATVEC = 13,/* LUSGSMatrixVec (GMRES operator)            */
+    DNDSMI_K_COUNT = 14
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the
