@@ -2,7 +2,7 @@
 
 Workload (default): the four-quadrant 2D Riemann problem on [0,1]^2,
 1024 x 1024 quads cut along seeded diagonals (2,097,152 triangles), interior
-nodes jittered by U(-0.2h, 0.2h); p = 3 VR, HLLC, WBAP V3 limiter, SSP-RK3 at
+nodes jittered by U(-0.2h, 0.2h); p = 3 VR, HLLC, WBAP limiter, SSP-RK3 at
 CFL 0.5 with a global time step, 5 Jacobi sweeps per stage, FP64.  One
 "step" is one SSP-RK3 step: EvaluateDt once, then three stages of (5 sweeps +
 WBAP + EvaluateRHS + stage update), all on the device with the state resident
@@ -14,7 +14,8 @@ Printed: one JSON line with the metric, the roofline of the dominant kernel
 (the Jacobi sweep: algorithmic bytes per launch / mean launch time from HIP
 events recorded by the library on its stream) and a CPU baseline (the FP64
 oracle, single thread, one SSP-RK3 step on the same mesh).  The default
-workload carries the WBAP V3 limiter (limiterProcedure 1) in every stage.
+workload carries the WBAP limiter in every stage: V2, the solver's default
+limiterProcedure 0, or V3 with --limiter-procedure 1.
 """
 from __future__ import annotations
 
@@ -79,6 +80,8 @@ def main():
                     help="also run the learned warm start workload: harvest converged reconstructions of the "
                          "40x40 vortex at 8 mesh seeds (160 snapshots each, 4.1M pairs), train on the device, "
                          "report the sweeps to a 1e-10 rec increment at an unseen seed")
+    ap.add_argument("--limiter-procedure", type=int, default=0, choices=[0, 1],
+                    help="WBAP of the limited cases: 0 V2 (the reference solver's default), 1 V3")
     ap.add_argument("--case", default="riemann", choices=["riemann", "vortex", "step"],
                     help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale; "
                          "step: Mach-3 step on a mixed tri/quad mesh of [0,4]x[0,1] (n x n quads, --n 2048 for the "
@@ -114,7 +117,7 @@ def main():
         u = ev.BuildMean().upload(case.u0)
         r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
         ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl, useLimiter=case.limiter,
-                                    mixedPrecision=mixed)
+                                    mixedPrecision=mixed, limiterProcedure=args.limiter_procedure)
         t = 0.0
         for _ in range(args.warmup):
             t += ode.Step(u, r, rn, t)
@@ -192,7 +195,8 @@ def main():
         orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2 if case.riemann == "HLLC" else 1,
                             far_field=case.far, tang_weight=st.tang_weight,
                             tang_weight_direction=st.tang_weight_direction,
-                            wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std)
+                            wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
+                            limiter_procedure=args.limiter_procedure)
         timed = {}
         for threads in (args.threads, 1):
             oracle.set_threads(threads)
@@ -265,6 +269,7 @@ def main():
                 "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
                 "eval_seed": 9, "tolerance": 1e-10, **ev_ws}
 
+    wbap = "V2" if args.limiter_procedure == 0 else "V3"
     out = {
         "metric": "p=3 VR+RHS cell-updates/s",
         "value": value,
@@ -280,15 +285,15 @@ def main():
         "data": "synthetic (seeded mesh, analytic initial state)",
         "config": {"workload": {
                        "riemann": (f"2D Riemann, {args.n}x{args.n} quads split to {nc} triangles, jitter 0.2h, "
-                                   f"p=3 VR, HLLC, WBAP V3 limiter, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
+                                   f"p=3 VR, HLLC, WBAP {wbap} limiter, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
                                    "sweeps/stage"),
                        "vortex": (f"periodic isentropic vortex, {args.n}x{args.n} quads split to {nc} triangles, "
                                   f"p=3 VR, Roe, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, no limiter"),
                        "step": (f"Mach-3 step, [0,4]x[0,1], {args.n}x{args.n} quads half split: {nc} mixed cells, "
-                                f"jitter 0.15h, p=3 VR, HLLC, WBAP V3, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
+                                f"jitter 0.15h, p=3 VR, HLLC, WBAP {wbap}, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
                                 "sweeps/stage")}[args.case],
                    "cells": nc, "sweeps_per_stage": args.sweeps,
-                   "limiter": "WBAP V3" if case.limiter else "none", "precision": "fp64"},
+                   "limiter": f"WBAP {wbap}" if case.limiter else "none", "precision": "fp64"},
         "kernels": prof,
         "roofline": roof,
         "evaluate_rhs": rhs_obj,

@@ -129,6 +129,9 @@ SIGNATURES = [
     ("dndsmi_evaluate_rhs", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     ("dndsmi_wbap_limit_facial_v3", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_int32]),
+    ("dndsmi_wbap_limit_facial_v2", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_int32]),
+    ("dndsmi_set_limiter_procedure", C.c_int, [C.c_void_p, C.c_int32]),
     ("dndsmi_ssprk3_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                      C.c_double, C.c_double, C.c_int32, C.c_double, PD]),
     ("dndsmi_ssprk3_step_mixed", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,

@@ -1047,6 +1047,135 @@ __global__ __launch_bounds__(kBlock) void k_wbap_lim(int64_t ldc, int64_t ldf, c
         atomicOr(errw, ERRBIT_NONPOSITIVE);
 }
 
+// WBAP V2 limiting of the listed cells for one order (DNDS_FV_VR.hpp:1843-2040,
+// rows ls..le of the order).  xo is the reference's uRecNewBuf: before this
+// launch it holds x for the unlimited cells and, for the limited ones, the rows
+// of the higher orders as limited by the earlier launches; rows ls..le and
+// below are still x everywhere.  Each neighbour's full 9-row buffer is mapped
+// to this cell's basis (rows <= le read from x, rows > le from xo), so the
+// launch reads xo only above le and writes it only at ls..le: no cell reads
+// what another writes.  FWBAP_L2_Biway with n = nStd per neighbour, then
+// FWBAP_L2_Multiway centred on the first neighbour's result with n1 = 1.
+template <int NF, int ORDER>
+__global__ __launch_bounds__(kBlock) void k_wbap_lim_v2(int64_t ldc, int64_t ldf, const int32_t *__restrict__ count,
+                                                        const int32_t *__restrict__ list,
+                                                        const int32_t *__restrict__ nbr,
+                                                        const int32_t *__restrict__ sface,
+                                                        const int32_t *__restrict__ f2c,
+                                                        const double *__restrict__ fnc,
+                                                        const double *__restrict__ msec,
+                                                        const double *__restrict__ u, const double *__restrict__ x,
+                                                        double *__restrict__ xo, double gamma, double nStd,
+                                                        double eps4, int32_t *errw) {
+    constexpr int ls = ORDER == 3 ? 5 : (ORDER == 2 ? 2 : 0);
+    constexpr int le = ORDER == 3 ? 8 : (ORDER == 2 ? 4 : 1);
+    constexpr int nr = le - ls + 1;
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= count[0])
+        return;
+    const int32_t i = list[tid];
+    double ui[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+    double uC[nr][4], first[nr][4], up[nr][4], down[nr][4];
+#pragma unroll
+    for (int r = 0; r < nr; r++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            uC[r][v] = x[((ls + r) + 9 * v) * ldc + i];
+            first[r][v] = 0.0, up[r][v] = 0.0, down[r][v] = 0.0;
+        }
+    bool ok = true;
+    int nOther = 0;
+#pragma unroll 1
+    for (int s = 0; s < NF; s++) {
+        const int32_t j = nbr[s * ldc + i];
+        if (j < 0)
+            continue;
+        const int32_t f = sface[s * ldc + i];
+        const int side = f2c[f] == i ? 0 : 1;
+        double un0, un1;
+        stable_normalized(fnc[f], fnc[ldf + f], un0, un1);
+        const double *ms = msec + int64_t(81 * side) * ldf + f;
+        double oIn[nr][4];
+#pragma unroll
+        for (int r = 0; r < nr; r++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                oIn[r][v] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const double *src = c <= le ? x : xo;
+            double uj[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                uj[v] = src[(c + 9 * v) * ldc + j];
+#pragma unroll
+            for (int r = 0; r < nr; r++) {
+                const double m = ms[(9 * (ls + r) + c) * ldf];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    oIn[r][v] += m * uj[v];
+            }
+        }
+        const double *uLp = side ? u + 4 * int64_t(j) : ui;
+        const double *uRp = side ? ui : u + 4 * int64_t(j);
+        double UL[4], UR[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            UL[v] = uLp[v], UR[v] = uRp[v];
+        double ML[16], MR[16];
+        ok &= char_matrices(gamma, UL, UR, un0, un1, ML, MR);
+#pragma unroll
+        for (int r = 0; r < nr; r++) {
+            double oC[4], tC[4], out[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                double p = 0, q = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    p += ML[4 * a + b] * oIn[r][b];
+                    q += ML[4 * a + b] * uC[r][b];
+                }
+                oC[a] = p, tC[a] = q;
+            }
+            // FWBAP_L2_Biway(uThisIn, uOtherIn, nStd)
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const double u1 = tC[a], u2 = oC[a];
+                const double frac = u1 / (fabs(u2) + 1e-12) * sgn(u2);
+                const double th1 = frac * frac * frac;
+                const double th2 = (frac * frac) * (frac * frac);
+                out[a] = u1 * (nStd + th1) / (nStd + th2);
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                double xv = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    xv += MR[4 * a + b] * out[b];
+                if (nOther == 0) {
+                    first[r][a] = xv;
+                } else {
+                    const double ti = first[r][a] / (fabs(xv) + eps4) * sgn(xv);
+                    down[r][a] += (ti * ti) * (ti * ti);
+                    up[r][a] += ti * ti * ti;
+                }
+            }
+        }
+        nOther++;
+    }
+#pragma unroll
+    for (int r = 0; r < nr; r++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            xo[((ls + r) + 9 * v) * ldc + i] =
+                nOther ? first[r][v] * (up[r][v] + 1.0) / (down[r][v] + 1.0) : uC[r][v];
+    if (!ok)
+        atomicOr(errw, ERRBIT_NONPOSITIVE);
+}
+
 __global__ void k_zero_int2(int32_t *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0)
         p[0] = 0;
@@ -1614,7 +1743,8 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
     return c->launch_check("rhs kernels");
 }
 
-int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all) {
+int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all,
+                int procedure) {
     if (!c->has_limiter) {
         c->err = "WBAP needs a setup built with build_limiter_data";
         return DNDSMI_ERR_UNSUPPORTED;
@@ -1637,14 +1767,34 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
                            c->lim_list);
     prof_end(c);
     prof_begin(c, DNDSMI_K_WBAP_LIM);
-    if (c->nfmax == 3)
-        hipLaunchKernelGGL(k_wbap_lim<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc, c->ldf, c->lim_count,
-                           c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x, xo, c->uF, c->gas.gamma,
-                           c->wbap_nstd, eps4, c->errw);
-    else
-        hipLaunchKernelGGL(k_wbap_lim<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc, c->ldf, c->lim_count,
-                           c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x, xo, c->uF, c->gas.gamma,
-                           c->wbap_nstd, eps4, c->errw);
+    if (procedure == 1) {
+        if (c->nfmax == 3)
+            hipLaunchKernelGGL(k_wbap_lim<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc, c->ldf,
+                               c->lim_count, c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x, xo, c->uF,
+                               c->gas.gamma, c->wbap_nstd, eps4, c->errw);
+        else
+            hipLaunchKernelGGL(k_wbap_lim<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc, c->ldf,
+                               c->lim_count, c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x, xo, c->uF,
+                               c->gas.gamma, c->wbap_nstd, eps4, c->errw);
+    } else {
+        // orders 3, 2, 1 in stream order: each launch reads the rows the
+        // previous ones wrote
+        auto order = [&](auto nf_c, auto order_c) {
+            constexpr int NF = decltype(nf_c)::value, O = decltype(order_c)::value;
+            hipLaunchKernelGGL((k_wbap_lim_v2<NF, O>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc,
+                               c->ldf, c->lim_count, c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x,
+                               xo, c->gas.gamma, c->wbap_nstd, eps4, c->errw);
+        };
+        using I3 = std::integral_constant<int, 3>;
+        using I4 = std::integral_constant<int, 4>;
+        using O3 = std::integral_constant<int, 3>;
+        using O2 = std::integral_constant<int, 2>;
+        using O1 = std::integral_constant<int, 1>;
+        if (c->nfmax == 3)
+            order(I3{}, O3{}), order(I3{}, O2{}), order(I3{}, O1{});
+        else
+            order(I4{}, O3{}), order(I4{}, O2{}), order(I4{}, O1{});
+    }
     prof_end(c);
     return c->launch_check("wbap kernels");
 }
@@ -1751,19 +1901,43 @@ int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, 
 
 } // namespace dndsmi_host
 
-extern "C" int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec,
-                                           dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1,
-                                           dndsmi_array *ifUseLimiter, int32_t ifAll) {
+namespace {
+
+int wbap_entry(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec, dndsmi_array *uRecNewBuf,
+               dndsmi_array *uRecNewBuf1, dndsmi_array *ifUseLimiter, int32_t ifAll, int procedure) {
     if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
         !same_ctx(c, uRecNewBuf, DNDSMI_ARRAY_REC) || uRec == uRecNewBuf ||
-        (uRecNewBuf1 && !same_ctx(c, uRecNewBuf1, DNDSMI_ARRAY_REC)) ||
+        (uRecNewBuf1 && (!same_ctx(c, uRecNewBuf1, DNDSMI_ARRAY_REC) || uRecNewBuf1 == uRec ||
+                         uRecNewBuf1 == uRecNewBuf)) ||
         (ifUseLimiter && !same_ctx(c, ifUseLimiter, DNDSMI_ARRAY_SCALAR)))
         return DNDSMI_ERR_INVALID_ARGUMENT;
-    DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, uRecNewBuf->d, ifUseLimiter ? ifUseLimiter->d : nullptr, ifAll));
+    DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, uRecNewBuf->d, ifUseLimiter ? ifUseLimiter->d : nullptr, ifAll,
+                           procedure));
     if (uRecNewBuf1)
         DNDSMI_TRY(c->hip(hipMemcpyAsync(uRecNewBuf1->d, uRecNewBuf->d, sizeof(double) * uRecNewBuf->elems,
                                          hipMemcpyDeviceToDevice, c->stream),
                           "copy"));
+    return DNDSMI_OK;
+}
+
+} // namespace
+
+extern "C" int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec,
+                                           dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1,
+                                           dndsmi_array *ifUseLimiter, int32_t ifAll) {
+    return wbap_entry(c, u, uRec, uRecNewBuf, uRecNewBuf1, ifUseLimiter, ifAll, 1);
+}
+
+extern "C" int dndsmi_wbap_limit_facial_v2(dndsmi_ctx *c, const dndsmi_array *u, const dndsmi_array *uRec,
+                                           dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1,
+                                           dndsmi_array *ifUseLimiter, int32_t ifAll) {
+    return wbap_entry(c, u, uRec, uRecNewBuf, uRecNewBuf1, ifUseLimiter, ifAll, 0);
+}
+
+extern "C" int dndsmi_set_limiter_procedure(dndsmi_ctx *c, int32_t limiterProcedure) {
+    if (!c || (limiterProcedure != 0 && limiterProcedure != 1))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    c->limiter_procedure = limiterProcedure;
     return DNDSMI_OK;
 }
 
@@ -1836,7 +2010,7 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
         const double cB = stage ? 1 - Coef[stage] : 0.0;
         const double *xr = uRec->d;
         if (use_limiter) { // the RHS sees the limited copy; uRec keeps the unlimited iterate
-            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0));
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure));
             xr = c->xlim;
         }
         DNDSMI_TRY(launch_rhs(c, u->d, xr, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA, cB));
@@ -1871,7 +2045,7 @@ extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_a
         const double cB = stage ? 1 - Coef[stage] : 0.0;
         const double *xr = uRec->d;
         if (use_limiter) {
-            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0));
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure));
             xr = c->xlim;
         }
         DNDSMI_TRY(launch_rhs_mixed(c, u->d, xr, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,

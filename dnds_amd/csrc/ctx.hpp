@@ -87,6 +87,7 @@ struct dndsmi_ctx {
     double tang_weight = 1.0;
     int twd_norm = 1;
     double wbap_scale = 1e-10, wbap_nstd = 10.0;
+    int32_t limiter_procedure = 0; // the steps' limiter: 0 WBAP V2, 1 WBAP V3
     // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
     double *lambda_face = nullptr;
     double *dlambda_face = nullptr;
@@ -205,7 +206,8 @@ int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, d
 int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double *dt_out);
 int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
                const double *dt, double coefA, double coefB);
-int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all);
+int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all,
+                int procedure);
 // the mixed-precision pieces (ctx.hip)
 int ensure_mixed(dndsmi_ctx *c);
 int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps, double t);

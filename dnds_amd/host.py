@@ -206,6 +206,9 @@ class ExternalVR:
     def ReconstructionWBAPLimitFacialV3(self, u, uRec, uRecNewBuf, uRecNewBuf1=None, ifUseLimiter=None, ifAll=False):
         VRFiniteVolume2D.ReconstructionWBAPLimitFacialV3(self, u, uRec, uRecNewBuf, uRecNewBuf1, ifUseLimiter, ifAll)
 
+    def ReconstructionWBAPLimitFacialV2(self, u, uRec, uRecNewBuf, uRecNewBuf1=None, ifUseLimiter=None, ifAll=False):
+        VRFiniteVolume2D.ReconstructionWBAPLimitFacialV2(self, u, uRec, uRecNewBuf, uRecNewBuf1, ifUseLimiter, ifAll)
+
 
 class VRFiniteVolume2D:
     """Geometry + VR setup (host FP64) and the Jacobi sweep (device)."""
@@ -264,6 +267,17 @@ class VRFiniteVolume2D:
                                                   uRecNewBuf1.h if uRecNewBuf1 is not None else None,
                                                   ifUseLimiter.h if ifUseLimiter is not None else None,
                                                   int(ifAll)), "WBAP V3", d.ctx)
+
+    def ReconstructionWBAPLimitFacialV2(self, u: "ArrayDOFV", uRec: "ArrayRecV", uRecNewBuf: "ArrayRecV",
+                                        uRecNewBuf1: "ArrayRecV | None" = None,
+                                        ifUseLimiter: "ArrayScalar | None" = None, ifAll: bool = False):
+        """WBAP V2 (the solver's default limiterProcedure 0) with the Euler
+        characteristic matrices; arguments as for V3."""
+        d = self.device
+        _check(_lib().dndsmi_wbap_limit_facial_v2(d.ctx, u.h, uRec.h, uRecNewBuf.h,
+                                                  uRecNewBuf1.h if uRecNewBuf1 is not None else None,
+                                                  ifUseLimiter.h if ifUseLimiter is not None else None,
+                                                  int(ifAll)), "WBAP V2", d.ctx)
 
 
 def _reconstruct_to_tolerance(vfv, u, uRec, uRecNewBuf, tol, max_sweeps, t=0.0):
@@ -442,12 +456,14 @@ class EulerEvaluator:
 
 
 class ExplicitSSPRK3LocalDt:
-    """ODE::ExplicitSSPRK3LocalDt with frhs = nInternalRecStep Jacobi sweeps +
+    """ODE::ExplicitSSPRK3LocalDt with frhs = nInternalRecStep Jacobi sweeps
+    (+ WBAP V2 or V3 by limiterProcedure, DNDS_FV_EulerSolver.hpp:796-806) +
     EvaluateRHS and fdt = EvaluateDt (DNDS_ODE.hpp:131-178); the stage times
     follow the reference's frhs arguments (t + dt, t + dt/4, t + 2dt/3)."""
 
     def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, MaxDt: float = 1e100,
-                 UseLocaldt: bool = False, useLimiter: bool = False, mixedPrecision: bool = False):
+                 UseLocaldt: bool = False, useLimiter: bool = False, mixedPrecision: bool = False,
+                 limiterProcedure: int = 0):
         self.ev = evaluator
         self.n_sweeps = nInternalRecStep
         self.CFL = CFL
@@ -455,6 +471,7 @@ class ExplicitSSPRK3LocalDt:
         self.UseLocaldt = UseLocaldt
         self.useLimiter = useLimiter
         self.mixedPrecision = mixedPrecision
+        self.limiterProcedure = limiterProcedure
 
     def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, t: float) -> float:
         """Advances u in place; returns the global minimum dt of the step.
@@ -464,6 +481,7 @@ class ExplicitSSPRK3LocalDt:
         dt = C.c_double()
         d = self.ev.device
         fn = _lib().dndsmi_ssprk3_step_mixed if self.mixedPrecision else _lib().dndsmi_ssprk3_step
+        _check(_lib().dndsmi_set_limiter_procedure(d.ctx, self.limiterProcedure), "limiterProcedure", d.ctx)
         _check(fn(d.ctx, u.h, uRec.h, uRecNew.h, self.n_sweeps, int(self.useLimiter), self.CFL, self.MaxDt,
                   int(self.UseLocaldt), t, C.byref(dt)), "SSP-RK3 step", d.ctx)
         return dt.value

@@ -250,8 +250,24 @@ int dndsmi_wbap_limit_facial_v3(dndsmi_ctx *ctx, const dndsmi_array *u, const dn
                                 dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1, dndsmi_array *ifUseLimiter,
                                 int32_t ifAll);
 
+/* ReconstructionWBAPLimitFacialV2(u, uRec, uRecNewBuf, uRecNewBuf1,
+ * ifUseLimiter, ifAll, fML, fMR) (DNDS_FV_VR.hpp:1749-2047; the solver's
+ * limiterProcedure 0, its default, DNDS_FV_EulerSolver.hpp:108 and 796-806):
+ * the same indicator as V3; per order 3, 2, 1 each limited cell limits its
+ * rows against every neighbour's buffer (higher orders already limited)
+ * mapped to its basis.  Arguments and outputs as for V3. */
+int dndsmi_wbap_limit_facial_v2(dndsmi_ctx *ctx, const dndsmi_array *u, const dndsmi_array *uRec,
+                                dndsmi_array *uRecNewBuf, dndsmi_array *uRecNewBuf1, dndsmi_array *ifUseLimiter,
+                                int32_t ifAll);
+
+/* The solver's limiterProcedure (DNDS_FV_EulerSolver.hpp:108): the WBAP the
+ * steps below apply when use_limiter is set, 0 for V2 (the default) or 1 for
+ * V3.  Any other value is DNDSMI_ERR_INVALID_ARGUMENT. */
+int dndsmi_set_limiter_procedure(dndsmi_ctx *ctx, int32_t limiterProcedure);
+
 /* One ExplicitSSPRK3LocalDt::Step over frhs = n_sweeps Jacobi sweeps, then
- * (use_limiter) WBAP V3 into a context buffer, then EvaluateRHS on the
+ * (use_limiter) WBAP V2 or V3 (dndsmi_set_limiter_procedure) into a context
+ * buffer, then EvaluateRHS on the
  * limited or unlimited reconstruction; fdt = EvaluateDt(CFL, MaxDt,
  * UseLocaldt).  Stage times follow DNDS_ODE.hpp:138-151 (t + dt, t + dt/4,
  * t + 2dt/3) with the global minimum dt; *dt_out (may be NULL) receives it. */

@@ -156,6 +156,8 @@ class VRFiniteVolume2D {
     void ReconstructionJacobiStep(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNewBuf, real t = 0);
     void ReconstructionWBAPLimitFacialV3(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNewBuf,
                                          ArrayRecV *uRecNewBuf1, ArrayScalar *ifUseLimiter, bool ifAll);
+    void ReconstructionWBAPLimitFacialV2(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNewBuf,
+                                         ArrayRecV *uRecNewBuf1, ArrayScalar *ifUseLimiter, bool ifAll);
     ArrayRecV BuildRec() const;
 
   private:
@@ -199,6 +201,14 @@ inline void VRFiniteVolume2D::ReconstructionWBAPLimitFacialV3(ArrayDOFV &u, Arra
                                               uRecNewBuf1 ? uRecNewBuf1->h() : nullptr,
                                               ifUseLimiter ? ifUseLimiter->h() : nullptr, ifAll),
                   "ReconstructionWBAPLimitFacialV3", dev_->ctx.get());
+}
+inline void VRFiniteVolume2D::ReconstructionWBAPLimitFacialV2(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNewBuf,
+                                                              ArrayRecV *uRecNewBuf1, ArrayScalar *ifUseLimiter,
+                                                              bool ifAll) {
+    detail::check(dndsmi_wbap_limit_facial_v2(dev_->ctx.get(), u.h(), uRec.h(), uRecNewBuf.h(),
+                                              uRecNewBuf1 ? uRecNewBuf1->h() : nullptr,
+                                              ifUseLimiter ? ifUseLimiter->h() : nullptr, ifAll),
+                  "ReconstructionWBAPLimitFacialV2", dev_->ctx.get());
 }
 inline ArrayRecV VRFiniteVolume2D::BuildRec() const { return ArrayRecV(dev_); }
 
@@ -257,20 +267,22 @@ class EulerEvaluator {
     std::shared_ptr<Device> dev_;
 };
 
-/* ODE::ExplicitSSPRK3LocalDt with frhs = nInternalRecStep sweeps (+ WBAP V3)
- * + EvaluateRHS and fdt = EvaluateDt */
+/* ODE::ExplicitSSPRK3LocalDt with frhs = nInternalRecStep sweeps (+ WBAP V2
+ * or V3 by limiterProcedure) + EvaluateRHS and fdt = EvaluateDt */
 class ExplicitSSPRK3LocalDt {
   public:
     /* mixedPrecision: FP32 correction sweeps after one FP64 sweep per stage and
      * FP32 face reconstruction (dndsmi_ssprk3_step_mixed; no reference counterpart) */
     ExplicitSSPRK3LocalDt(EulerEvaluator &ev, int nInternalRecStep, real CFL, real MaxDt = 1e100,
-                          bool UseLocaldt = false, bool useLimiter = false, bool mixedPrecision = false)
+                          bool UseLocaldt = false, bool useLimiter = false, bool mixedPrecision = false,
+                          int limiterProcedure = 0)
         : ev_(ev), n_(nInternalRecStep), cfl_(CFL), maxdt_(MaxDt), local_(UseLocaldt), lim_(useLimiter),
-          mixed_(mixedPrecision) {}
+          mixed_(mixedPrecision), proc_(limiterProcedure) {}
     /* advances u in place; returns the global minimum dt of the step */
     real Step(ArrayDOFV &u, ArrayRecV &uRec, ArrayRecV &uRecNew, real t) {
         real dt = 0;
         dndsmi_ctx *c = ev_.device()->ctx.get();
+        detail::check(dndsmi_set_limiter_procedure(c, proc_), "limiterProcedure", c);
         auto fn = mixed_ ? &dndsmi_ssprk3_step_mixed : &dndsmi_ssprk3_step;
         detail::check(fn(c, u.h(), uRec.h(), uRecNew.h(), n_, lim_, cfl_, maxdt_, local_, t, &dt), "SSP-RK3 step", c);
         return dt;
@@ -281,6 +293,7 @@ class ExplicitSSPRK3LocalDt {
     int n_;
     real cfl_, maxdt_;
     bool local_, lim_, mixed_;
+    int proc_;
 };
 
 /* ODE::ImplicitEulerDualTimeStep with LU-SGS (jacobianTypeCode 0, gmresCode 0) */

@@ -875,17 +875,15 @@ inline double sgn(double x) { return double((x > 0) - (x < 0)); }
 
 } // namespace
 
-// ReconstructionWBAPLimitFacialV3 (DNDS_FV_VR.hpp:2053-2329) with the
-// characteristic matrices fML / fMR and FWBAP_L2_Biway / FWBAP_L2_Multiway
-// (DNDS_FV_Limiters.hpp:341-449).  urec is read only; urec_new receives the
-// limited reconstruction (urec for cells below the smoothness threshold);
-// if_use_limiter receives the indicator.
-extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
-                              const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all) {
+namespace {
+
+// Step 0 of both WBAP procedures (V2: DNDS_FV_VR.hpp:1770-1831, V3:
+// DNDS_FV_VR.hpp:2075-2138, the same code): the smoothness indicator from the
+// (rho, E) jumps at the first LINE_3 point with the LINE_1 weight 2.
+void wbap_indicator(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                    const double *urec, double *if_use_limiter) {
     const int64_t nc = m->n_cell;
     const int P = 3;
-    // Step 0: smoothness indicator from the (rho, E) jumps at the first LINE_3
-    // point with the LINE_1 weight 2 (DNDS_FV_VR.hpp:2075-2138)
     parallel_items(nc, [&](int64_t i) -> int {
         const int64_t b0 = m->cell_ptr[i];
         const int nf = int(m->cell_ptr[i + 1] - b0);
@@ -937,6 +935,19 @@ extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const
         if_use_limiter[i] = std::sqrt(sImax) * (P * P);
         return ERR_OK;
     });
+}
+
+} // namespace
+
+// ReconstructionWBAPLimitFacialV3 (DNDS_FV_VR.hpp:2053-2329) with the
+// characteristic matrices fML / fMR and FWBAP_L2_Biway / FWBAP_L2_Multiway
+// (DNDS_FV_Limiters.hpp:341-449).  urec is read only; urec_new receives the
+// limited reconstruction (urec for cells below the smoothness threshold);
+// if_use_limiter receives the indicator.
+extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                              const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all) {
+    const int64_t nc = m->n_cell;
+    wbap_indicator(m, s, cfg, u, urec, if_use_limiter);
     std::memcpy(urec_new, urec, sizeof(double) * 36 * nc);
     const double eps4 = std::pow(kVerySmallReal, 1.0 / 4); // verySmallReal_pDiP
     const double nStd = cfg->wbap_n_std;
@@ -1054,8 +1065,121 @@ extern "C" int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const
     return ERR_OK;
 }
 
-namespace {
-} // namespace
+// ReconstructionWBAPLimitFacialV2 (DNDS_FV_VR.hpp:1749-2047, limiterProcedure
+// 0) with fML / fMR and FWBAP_L2_Biway / FWBAP_L2_Multiway
+// (DNDS_FV_Limiters.hpp:341-449).  Per order 3, 2, 1 every limited cell maps
+// each neighbour's current buffer (rows of the higher orders already limited)
+// through the full matrixSecondary to its own basis, limits its own rows
+// against it in characteristic variables (biway, n = WBAP_nStd), and combines
+// the per-neighbour results by the multiway centred on the first neighbour's
+// (n1 = 1); the orders' results are copied back between orders.  A cell
+// without neighbours keeps its rows (the reference indexes an empty list
+// there).  urec is read only; urec_new receives the result.
+extern "C" int oracle_wbap_v2(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                              const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all) {
+    const int64_t nc = m->n_cell;
+    wbap_indicator(m, s, cfg, u, urec, if_use_limiter);
+    double *buf = urec_new; // uRecNewBuf
+    std::memcpy(buf, urec, sizeof(double) * 36 * nc);
+    std::vector<double> buf1(36 * nc, 0.0); // uRecNewBuf1
+    const double eps4 = std::pow(kVerySmallReal, 1.0 / 4);
+    const double nStd = cfg->wbap_n_std;
+    auto limited = [&](int64_t i) { return !(if_use_limiter[i] < cfg->wbap_smooth_indicator_scale && !if_all); };
+    for (int order = 3; order >= 1; order--) {
+        const int ls = order == 3 ? 5 : (order == 2 ? 2 : 0);
+        const int le = order == 3 ? 8 : (order == 2 ? 4 : 1);
+        const int nr = le - ls + 1;
+        const int rc = parallel_items(nc, [&](int64_t i) -> int {
+            if (!limited(i))
+                return ERR_OK;
+            const int64_t b0 = m->cell_ptr[i];
+            const int nf = int(m->cell_ptr[i + 1] - b0);
+            double uC[4][4];
+            for (int r = 0; r < nr; r++)
+                for (int v = 0; v < 4; v++)
+                    uC[r][v] = buf[36 * i + (ls + r) + 9 * v];
+            double first[4][4], up[4][4], down[4][4];
+            for (int r = 0; r < 4; r++)
+                for (int v = 0; v < 4; v++)
+                    first[r][v] = up[r][v] = down[r][v] = 0.0;
+            int nOther = 0;
+            for (int lf = 0; lf < nf; lf++) {
+                const int64_t f = m->cell2face[b0 + lf];
+                const int64_t L = m->face2cell[2 * f], R = m->face2cell[2 * f + 1];
+                const int64_t j = L == i ? R : L;
+                if (j == kNoCell)
+                    continue;
+                const int side = L == i ? 0 : 1;
+                double un[2];
+                stable_normalized(s->face_norm_center[2 * f], s->face_norm_center[2 * f + 1], un[0], un[1]);
+                const double *ms = s->matrix_secondary + 162 * f + 81 * side;
+                double oIn[4][4];
+                for (int r = 0; r < nr; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += ms[9 * (ls + r) + c] * buf[36 * j + c + 9 * v];
+                        oIn[r][v] = acc;
+                    }
+                const double *uR = side ? u + 4 * i : u + 4 * j;
+                const double *uL = side ? u + 4 * j : u + 4 * i;
+                double ML[4][4], MR[4][4];
+                int rc2 = char_matrices(*cfg, uL, uR, un, ML, MR);
+                if (rc2)
+                    return rc2;
+                for (int r = 0; r < nr; r++) {
+                    double oC[4], tC[4], out[4];
+                    for (int a = 0; a < 4; a++) {
+                        double x = 0, y = 0;
+                        for (int b = 0; b < 4; b++) {
+                            x += ML[a][b] * oIn[r][b];
+                            y += ML[a][b] * uC[r][b];
+                        }
+                        oC[a] = x, tC[a] = y;
+                    }
+                    // FWBAP_L2_Biway(uThisIn, uOtherIn, n = WBAP_nStd)
+                    for (int a = 0; a < 4; a++) {
+                        const double u1 = tC[a], u2 = oC[a];
+                        const double frac = u1 / (std::abs(u2) + 1e-12) * sgn(u2);
+                        const double th1 = frac * frac * frac;
+                        const double th2 = (frac * frac) * (frac * frac);
+                        out[a] = u1 * (nStd + th1) / (nStd + th2);
+                    }
+                    for (int a = 0; a < 4; a++) {
+                        double x = 0;
+                        for (int b = 0; b < 4; b++)
+                            x += MR[a][b] * out[b];
+                        // FWBAP_L2_Multiway(uOthers, n1 = 1): the first entry is the centre
+                        if (nOther == 0) {
+                            first[r][a] = x;
+                        } else {
+                            const double ti = first[r][a] / (std::abs(x) + eps4) * sgn(x);
+                            down[r][a] += (ti * ti) * (ti * ti);
+                            up[r][a] += ti * ti * ti;
+                        }
+                    }
+                }
+                nOther++;
+            }
+            for (int r = 0; r < nr; r++)
+                for (int v = 0; v < 4; v++)
+                    buf1[36 * i + (ls + r) + 9 * v] =
+                        nOther ? first[r][v] * (up[r][v] + 1.0) / (down[r][v] + 1.0) : uC[r][v];
+            return ERR_OK;
+        });
+        if (rc)
+            return rc;
+        for (int64_t i = 0; i < nc; i++)
+            if (limited(i))
+                for (int r = ls; r <= le; r++)
+                    for (int v = 0; v < 4; v++)
+                        buf[36 * i + r + 9 * v] = buf1[36 * i + r + 9 * v];
+    }
+    for (int64_t k = 0; k < 36 * nc; k++)
+        if (!std::isfinite(urec_new[k]))
+            return ERR_NAN;
+    return ERR_OK;
+}
 
 // ExplicitSSPRK3LocalDt::Step, DNDS_ODE.hpp:131-178; frhs as the Eikonal
 // solver composes it (DNDS_FV_EikonalCRSolver.hpp:948-999): sweeps then RHS.
@@ -1081,7 +1205,9 @@ extern "C" int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, 
                     return rc;
             }
             if (use_limiter) {
-                rc = oracle_wbap_v3(m, s, cfg, u, urec, urecLim.data(), ifUse.data(), 0);
+                rc = cfg->limiter_procedure == 1
+                         ? oracle_wbap_v3(m, s, cfg, u, urec, urecLim.data(), ifUse.data(), 0)
+                         : oracle_wbap_v2(m, s, cfg, u, urec, urecLim.data(), ifUse.data(), 0);
                 if (rc)
                     return rc;
             }

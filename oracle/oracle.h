@@ -56,6 +56,7 @@ typedef struct oracle_setting {
     int32_t gmres_code;            /* fsolve: 0 LU-SGS, 1 GMRES, 2 LU-SGS then GMRES */
     int32_t n_gmres_space;         /* Krylov subspace size      */
     int32_t n_gmres_iter;          /* restarts                  */
+    int32_t limiter_procedure;     /* 0 WBAP V2, 1 WBAP V3 (limiterProcedure) */
 } oracle_setting;
 
 /* VRFiniteVolume2D::ReconstructionJacobiStep, SOR_Instead = false
@@ -82,8 +83,13 @@ int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, const oracl
 int oracle_wbap_v3(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
                    const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all);
 
+/* VRFiniteVolume2D::ReconstructionWBAPLimitFacialV2 (DNDS_FV_VR.hpp:1749-2047),
+ * same arguments as oracle_wbap_v3. */
+int oracle_wbap_v2(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                   const double *urec, double *urec_new, double *if_use_limiter, int32_t if_all);
+
 /* n_steps of ExplicitSSPRK3LocalDt::Step (DNDS_ODE.hpp:131-178) with frhs =
- * n_sweeps Jacobi sweeps [+ WBAP V3] + EvaluateRHS (on the limited
+ * n_sweeps Jacobi sweeps [+ WBAP V2 or V3 by limiter_procedure] + EvaluateRHS (on the limited
  * reconstruction when use_limiter) and fdt = EvaluateDt.  *t advances by the
  * global minimum dt of each step. */
 int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, int32_t n_steps,

@@ -35,7 +35,8 @@ class OSetting(C.Structure):
     _fields_ = [("riemann_solver", C.c_int32), ("gamma", C.c_double), ("far_field", C.c_double * 4),
                 ("jacobi_relax", C.c_double), ("tang_weight", C.c_double), ("tang_weight_direction", C.c_int32),
                 ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double),
-                ("gmres_code", C.c_int32), ("n_gmres_space", C.c_int32), ("n_gmres_iter", C.c_int32)]
+                ("gmres_code", C.c_int32), ("n_gmres_space", C.c_int32), ("n_gmres_iter", C.c_int32),
+                ("limiter_procedure", C.c_int32)]
 
 
 def build(quiet: bool = True) -> str:
@@ -64,6 +65,7 @@ def lib() -> C.CDLL:
             "oracle_ssprk3_steps": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), C.c_int32, C.c_int32,
                                     C.c_int32, C.c_double, C.c_double, C.c_int32, PD, PD, PD],
             "oracle_wbap_v3": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, PD, C.c_int32],
+            "oracle_wbap_v2": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, PD, C.c_int32],
             "oracle_roe_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_hllc_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_eigenvectors": [PD, C.c_double, PD, PD],
@@ -115,7 +117,8 @@ class Oracle:
     def __init__(self, mesh: dict, setup: dict, riemann_solver: int = 1, gamma: float = 1.4,
                  far_field=(1.0, 0.0, 0.0, 2.5), jacobi_relax: float = 1.0, tang_weight: float = 1.0,
                  tang_weight_direction: int = 0, wbap_smooth_indicator_scale: float = 1e-10,
-                 wbap_n_std: float = 10.0, gmres_code: int = 0, n_gmres_space: int = 10, n_gmres_iter: int = 2):
+                 wbap_n_std: float = 10.0, gmres_code: int = 0, n_gmres_space: int = 10, n_gmres_iter: int = 2,
+                 limiter_procedure: int = 0):
         self._keep = {}
 
         def keep(name, a, dtype):
@@ -137,7 +140,8 @@ class Oracle:
         self.s = OSetup(**fields)
         ff = (C.c_double * 4)(*far_field)
         self.cfg = OSetting(riemann_solver, gamma, ff, jacobi_relax, tang_weight, tang_weight_direction,
-                            wbap_smooth_indicator_scale, wbap_n_std, gmres_code, n_gmres_space, n_gmres_iter)
+                            wbap_smooth_indicator_scale, wbap_n_std, gmres_code, n_gmres_space, n_gmres_iter,
+                            limiter_procedure)
         self.lib = lib()
 
     def _check(self, rc, what):
@@ -172,15 +176,22 @@ class Oracle:
                                                  _d(u), _d(urec), t, _d(rhs)), "evaluate_rhs")
         return rhs
 
-    def wbap_v3(self, u, urec, if_all=False):
-        """Returns (limited reconstruction, smoothness indicator)."""
+    def _wbap(self, fn, u, urec, if_all):
         u = np.ascontiguousarray(u, dtype=np.float64)
         urec = np.ascontiguousarray(urec, dtype=np.float64)
         out = np.empty_like(urec)
         ind = np.empty(self.n_cell)
-        self._check(self.lib.oracle_wbap_v3(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u), _d(urec),
-                                            _d(out), _d(ind), int(if_all)), "wbap_v3")
+        self._check(getattr(self.lib, "oracle_" + fn)(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u),
+                                                      _d(urec), _d(out), _d(ind), int(if_all)), fn)
         return out, ind
+
+    def wbap_v3(self, u, urec, if_all=False):
+        """Returns (limited reconstruction, smoothness indicator)."""
+        return self._wbap("wbap_v3", u, urec, if_all)
+
+    def wbap_v2(self, u, urec, if_all=False):
+        """Returns (limited reconstruction, smoothness indicator)."""
+        return self._wbap("wbap_v2", u, urec, if_all)
 
     def lusgs_matrix_init(self, dtau, dt, alpha, lambda_face):
         jinv = np.empty(self.n_cell)

@@ -1,8 +1,8 @@
 """Writes tests/golden/vortex8.npz: a seeded 8 x 8 periodic split-quad mesh
 (128 cells) of the vortex case, its VR setup, seeded states, and the
 oracle's outputs for each hot-path call (Jacobi sweep, EvaluateDt,
-EvaluateRHS with Roe and HLLC, WBAP V3, two SSP-RK3 steps with and without
-the limiter).  Inputs and outputs are stored, so the fixture pins the oracle
+EvaluateRHS with Roe and HLLC, WBAP V3 and V2, two SSP-RK3 steps without the
+limiter and with each).  Inputs and outputs are stored, so the fixture pins the oracle
 (regression) and the device (parity) independently of the host setup code.
 
 Run from the repository root: python tests/golden/make_golden.py
@@ -32,7 +32,7 @@ def main():
     for rs, name in ((1, "roe"), (2, "hllc")):
         orc = oracle.Oracle(m, s, riemann_solver=rs, far_field=case.far, tang_weight=st.tang_weight,
                             tang_weight_direction=st.tang_weight_direction,
-                            wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale)
+                            wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, limiter_procedure=1)
         dt, dtmin, lf, dlf = orc.evaluate_dt(u, 0.5)
         out[f"rhs_{name}"] = orc.evaluate_rhs(u, urec, dlf, t=0.1)
         if rs == 1:
@@ -43,6 +43,12 @@ def main():
             out["rk3_u"], out["rk3_urec"], out["rk3_t"] = ru, rr, np.array([rt])
             ru, rr, rt = orc.ssprk3_steps(case.u0, np.zeros((n, 36)), 2, 3, 0.5, use_limiter=True)
             out["rk3lim_u"], out["rk3lim_urec"], out["rk3lim_t"] = ru, rr, np.array([rt])
+            out["wbap2_lim"], _ = orc.wbap_v2(u, urec)
+            orc2 = oracle.Oracle(m, s, riemann_solver=rs, far_field=case.far, tang_weight=st.tang_weight,
+                                 tang_weight_direction=st.tang_weight_direction,
+                                 wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, limiter_procedure=0)
+            ru, rr, rt = orc2.ssprk3_steps(case.u0, np.zeros((n, 36)), 2, 3, 0.5, use_limiter=True)
+            out["rk3lim2_u"], out["rk3lim2_urec"], out["rk3lim2_t"] = ru, rr, np.array([rt])
     setting = np.array([st.tang_weight, st.tang_weight_direction, st.wbap_smooth_indicator_scale, st.wbap_n_std,
                         st.jacobi_relax, 1.4])
     np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "vortex8.npz"),

@@ -24,11 +24,11 @@ def fx():
     return d, mesh, setup
 
 
-def make_oracle(d, mesh, setup, rs):
+def make_oracle(d, mesh, setup, rs, limiter_procedure=1):
     tw, twd, scale, nstd, relax, gamma = d["setting"]
     return oracle.Oracle(mesh, setup, riemann_solver=rs, gamma=gamma, far_field=tuple(d["far"]), jacobi_relax=relax,
                          tang_weight=tw, tang_weight_direction=int(twd), wbap_smooth_indicator_scale=scale,
-                         wbap_n_std=nstd)
+                         wbap_n_std=nstd, limiter_procedure=limiter_procedure)
 
 
 def test_oracle_reproduces_golden_fixture(fx):
@@ -46,6 +46,11 @@ def test_oracle_reproduces_golden_fixture(fx):
     assert np.array_equal(ru, d["rk3_u"]) and np.array_equal(rr, d["rk3_urec"]) and rt == d["rk3_t"][0]
     ru, rr, rt = orc.ssprk3_steps(d["u0"], np.zeros_like(urec), 2, 3, 0.5, use_limiter=True)
     assert np.array_equal(ru, d["rk3lim_u"]) and np.array_equal(rr, d["rk3lim_urec"]) and rt == d["rk3lim_t"][0]
+    lim, ind = orc.wbap_v2(u, urec)
+    assert np.array_equal(lim, d["wbap2_lim"]) and np.array_equal(ind, d["wbap_ind"])
+    orc2 = make_oracle(d, mesh, setup, 1, limiter_procedure=0)
+    ru, rr, rt = orc2.ssprk3_steps(d["u0"], np.zeros_like(urec), 2, 3, 0.5, use_limiter=True)
+    assert np.array_equal(ru, d["rk3lim2_u"]) and np.array_equal(rr, d["rk3lim2_urec"]) and rt == d["rk3lim2_t"][0]
 
 
 @pytest.mark.gpu
@@ -72,11 +77,14 @@ def test_device_reproduces_golden_fixture_from_stored_arrays(fx):
             evr.ReconstructionWBAPLimitFacialV3(u, r, rn, None, ind)
             assert rel_err(rn.download(), d["wbap_lim"]) <= 1e-12
             assert rel_err(ind.download(), d["wbap_ind"]) <= 1e-12
+            evr.ReconstructionWBAPLimitFacialV2(u, r, rn, None, ind)
+            assert rel_err(rn.download(), d["wbap2_lim"]) <= 1e-12
+            assert rel_err(ind.download(), d["wbap_ind"]) <= 1e-12
             evr.ReconstructionJacobiStep(u, r, rn)
             assert rel_err(r.download(), d["jacobi1"]) <= 1e-12
-            for lim, key in ((False, "rk3"), (True, "rk3lim")):
+            for lim, proc, key in ((False, 0, "rk3"), (True, 1, "rk3lim"), (True, 0, "rk3lim2")):
                 u2, r2, rn2 = ev.BuildMean().upload(d["u0"]), evr.BuildRec(), evr.BuildRec()
-                ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=3, CFL=0.5, useLimiter=lim)
+                ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=3, CFL=0.5, useLimiter=lim, limiterProcedure=proc)
                 t = 0.0
                 for _ in range(2):
                     t += ode.Step(u2, r2, rn2, t)

@@ -151,9 +151,10 @@ def test_wall_face_in_rhs_is_an_error_on_both_sides():
 @pytest.mark.parametrize("kind,bcname", [(0, "periodic"), (2, "farfield"), (1, "walls"), (0, "riemann"),
                                          (2, "mixed_bc")])
 @pytest.mark.parametrize("if_all", [False, True])
-def test_wbap_v3_matches_oracle(kind, bcname, if_all):
-    """ReconstructionWBAPLimitFacialV3: smoothness indicator and the limited
-    reconstruction (limited and unlimited cells both present)."""
+@pytest.mark.parametrize("version", ["V3", "V2"])
+def test_wbap_matches_oracle(kind, bcname, if_all, version):
+    """ReconstructionWBAPLimitFacialV3 / V2: smoothness indicator and the
+    limited reconstruction (limited and unlimited cells both present)."""
     mesh, vfv, ev, orc = build(kind, bcname)
     # smooth means everywhere, rough reconstructions only where x > 0.5, so
     # both limited and unlimited cells occur
@@ -164,8 +165,8 @@ def test_wbap_v3_matches_oracle(kind, bcname, if_all):
     u = np.stack([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx ** 2 + vy ** 2)], axis=1)
     urec[bary[:, 0] < 0.5] = 0.0
     du, dr, dn, ind = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec(), ev.BuildScalar()
-    vfv.ReconstructionWBAPLimitFacialV3(du, dr, dn, None, ind, ifAll=if_all)
-    ref, rind = orc.wbap_v3(u, urec, if_all)
+    getattr(vfv, "ReconstructionWBAPLimitFacial" + version)(du, dr, dn, None, ind, ifAll=if_all)
+    ref, rind = getattr(orc, "wbap_" + version.lower())(u, urec, if_all)
     assert rel_err(ind.download(), rind) <= TOL
     got = dn.download()
     assert rel_err(got, ref) <= TOL
@@ -176,9 +177,11 @@ def test_wbap_v3_matches_oracle(kind, bcname, if_all):
         assert np.array_equal(got[~limited], urec[~limited])
 
 
-def test_ssprk3_with_limiter_matches_oracle_on_riemann():
+@pytest.mark.parametrize("procedure", [0, 1])
+def test_ssprk3_with_limiter_matches_oracle_on_riemann(procedure):
     """The four-quadrant Riemann case at 24 x 24 quads: SSP-RK3 with 5 sweeps,
-    WBAP V3 and HLLC for 10 steps against the oracle."""
+    WBAP (limiterProcedure 0: V2, 1: V3) and HLLC for 10 steps against the
+    oracle."""
     import oracle
     from dnds_amd import cases
     from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
@@ -187,10 +190,10 @@ def test_ssprk3_with_limiter_matches_oracle_on_riemann():
     st = case.vfv.setting
     orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2, far_field=case.far,
                         tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
-                        wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale)
+                        wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, limiter_procedure=procedure)
     urec0 = np.zeros((case.n_cell, 36))
     du, dr, dn = ev.BuildMean().upload(case.u0), case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
-    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=5, CFL=0.5, useLimiter=True)
+    ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=5, CFL=0.5, useLimiter=True, limiterProcedure=procedure)
     t = 0.0
     for _ in range(10):
         t += ode.Step(du, dr, dn, t)
