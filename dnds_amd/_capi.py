@@ -23,6 +23,7 @@ ERR_NONPOSITIVE_STATE = 4
 ERR_UNSUPPORTED = 5
 ERR_OUT_OF_MEMORY = 6
 ERR_SETUP = 7
+ERR_IO = 8
 
 BC_INNER = -1
 BC_WALL = 1
@@ -106,6 +107,9 @@ SIGNATURES = [
     ("dndsmi_mesh_generate", C.c_int, [C.POINTER(MeshSpec), C.POINTER(C.c_void_p)]),
     ("dndsmi_mesh_from_cells", C.c_int, [C.c_int64, PD, C.c_int64, PI64, PI64, C.c_int64, PI64, PI32,
                                          C.POINTER(C.c_void_p)]),
+    ("dndsmi_mesh_read_gmsh", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
+    ("dndsmi_mesh_write_plt", C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), PD, C.c_double,
+                                        C.c_char_p, C.c_size_t]),
     ("dndsmi_mesh_get_view", C.c_int, [C.c_void_p, C.POINTER(MeshView)]),
     ("dndsmi_mesh_destroy", None, [C.c_void_p]),
     ("dndsmi_vr_setting_default", None, [C.POINTER(VRSetting)]),

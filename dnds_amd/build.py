@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdndsmi.so")
 
-HOST_SOURCES = ["mesh.cpp", "vr_setup.cpp"]
+HOST_SOURCES = ["mesh.cpp", "vr_setup.cpp", "io.cpp"]
 HIP_SOURCES = ["ctx.hip", "lusgs.hip", "warmstart.hip"]
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + os.path.join(HERE, "..", "include")]

@@ -21,6 +21,11 @@ struct dndsmi_mesh {
     std::vector<double> face_shift;   // [n_face][2]
 };
 
+// dndsmi_mesh_from_cells with the failure's description in err.
+int mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const int64_t *cell_ptr,
+                    const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node, const int32_t *bnd_bc,
+                    dndsmi_mesh **out, std::string &err);
+
 struct dndsmi_vr {
     int64_t n_cell = 0, n_face = 0, n_c2f = 0;
     std::vector<double> volume, face_area, cell_bary, base_moment;

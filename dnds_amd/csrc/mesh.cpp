@@ -254,12 +254,14 @@ extern "C" int dndsmi_mesh_generate(const dndsmi_mesh_spec *spec, dndsmi_mesh **
     return DNDSMI_OK;
 }
 
-extern "C" int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const int64_t *cell_ptr,
-                                      const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node,
-                                      const int32_t *bnd_bc, dndsmi_mesh **out) {
+int mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const int64_t *cell_ptr,
+                    const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node, const int32_t *bnd_bc,
+                    dndsmi_mesh **out, std::string &err) {
     if (!out || n_node < 3 || n_cell < 1 || !node_xy || !cell_ptr || !cell2node || n_bnd < 0 ||
-        (n_bnd > 0 && (!bnd2node || !bnd_bc)))
+        (n_bnd > 0 && (!bnd2node || !bnd_bc))) {
+        err = "invalid mesh arguments";
         return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
     *out = nullptr;
     dndsmi_mesh *m = new (std::nothrow) dndsmi_mesh();
     if (!m)
@@ -288,7 +290,6 @@ extern "C" int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int
             auto it = bmap.find(edge_key(a, b));
             return it == bmap.end() ? DNDSMI_BC_INNER : it->second;
         };
-        std::string err;
         int rc = build_topology(*m, canon, bc_of, err);
         if (rc != DNDSMI_OK) {
             delete m;
@@ -296,13 +297,22 @@ extern "C" int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int
         }
     } catch (const std::bad_alloc &) {
         delete m;
+        err = "out of memory";
         return DNDSMI_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception &) {
+    } catch (const std::exception &e) {
         delete m;
+        err = e.what();
         return DNDSMI_ERR_INVALID_ARGUMENT;
     }
     *out = m;
     return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const int64_t *cell_ptr,
+                                      const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node,
+                                      const int32_t *bnd_bc, dndsmi_mesh **out) {
+    std::string err;
+    return mesh_from_cells(n_node, node_xy, n_cell, cell_ptr, cell2node, n_bnd, bnd2node, bnd_bc, out, err);
 }
 
 extern "C" int dndsmi_mesh_get_view(const dndsmi_mesh *m, dndsmi_mesh_view *v) {

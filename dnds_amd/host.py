@@ -19,6 +19,7 @@ host layouts.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -87,6 +88,32 @@ class CompactFacedMeshSerialRW:
                                              A.i64ptr(cell_ptr), A.i64ptr(cell2node), bnd_bc.shape[0],
                                              A.i64ptr(bnd2node), A.i32ptr(bnd_bc), C.byref(h)), "mesh from cells")
         return cls(h)
+
+    @classmethod
+    def read_gmsh(cls, fname: str):
+        """A Gmsh 2.2 ASCII mesh as CompactFacedMeshSerialRWBuild reads it
+        (GmshReader::FileRead + InterpolateTopology, DNDS_Mesh.hpp:87-350,
+        760-835); boundary groups named bc-1..bc-4, bc-DMRFar, bc-RTFar,
+        bc-IVFar, bc-2DRiemannFar."""
+        h = C.c_void_p()
+        buf = C.create_string_buffer(1024)
+        rc = _lib().dndsmi_mesh_read_gmsh(os.fsencode(fname), C.byref(h), buf, 1024)
+        if rc != A.OK:
+            raise DNDSError(rc, f"Gmsh read: {buf.value.decode(errors='replace')}")
+        return cls(h)
+
+    def PrintSerialPartPltBinaryDataArray(self, fname: str, names, data, t: float = 0.0):
+        """Tecplot binary file of the mesh with cell-centred data
+        (DNDS_Mesh.hpp:1634-1814): names[k] and data[k][n_cell]."""
+        names = list(names)
+        data = np.ascontiguousarray(data, dtype=np.float64).reshape(len(names), self.n_cell) if names else \
+            np.zeros((0, self.n_cell))
+        enc = [os.fsencode(n) for n in names]
+        arr = (C.c_char_p * max(len(enc), 1))(*enc)
+        buf = C.create_string_buffer(1024)
+        rc = _lib().dndsmi_mesh_write_plt(self._h, os.fsencode(fname), len(names), arr, A.dptr(data), t, buf, 1024)
+        if rc != A.OK:
+            raise DNDSError(rc, f"Tecplot write: {buf.value.decode(errors='replace')}")
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -377,6 +404,28 @@ class ArrayScalar(_DeviceArray):
 RS = {"Roe": A.RS_ROE, "HLLC": A.RS_HLLC}
 
 
+def solver_output(u, ifUseLimiter=None, rhs=None, gamma: float = 1.4, Rgas: float = 1.0,
+                  smooth_indicator_scale: float = 1e-10):
+    """(names, data[8][n_cell]) as EulerSolver::PrintData composes them for
+    NS_2D (DNDS_FV_EulerSolver.hpp:1436-1492; nOUTS = nVars + 4, :55): rho,
+    u, v, p, T = p / rho / Rgas, Mach, ifUseLimiter / (WBAP_SmoothIndicatorScale
+    + 1e-200), rhs of rho.  The names are the first eight of the reference's
+    list, R U V W P T M ifUseLimiter, as its 2D files carry them (the slots
+    from the fourth on hold p, T, M, the indicator and the rhs)."""
+    u = np.asarray(u, dtype=np.float64).reshape(-1, 4)
+    n = u.shape[0]
+    rho = u[:, 0]
+    vel = u[:, 1:3] / rho[:, None]
+    vsqr = np.sum(vel * vel, axis=1)
+    p = (gamma - 1) * (u[:, 3] - 0.5 * rho * vsqr)
+    asqr = gamma * p / rho
+    ind = np.zeros(n) if ifUseLimiter is None else np.asarray(ifUseLimiter, dtype=np.float64).reshape(n)
+    r0 = np.zeros(n) if rhs is None else np.asarray(rhs, dtype=np.float64).reshape(n, 4)[:, 0]
+    data = np.stack([rho, vel[:, 0], vel[:, 1], p, p / rho / Rgas, np.sqrt(vsqr / asqr),
+                     ind / (smooth_indicator_scale + 1e-200), r0])
+    return ["R", "U", "V", "W", "P", "T", "M", "ifUseLimiter"], data
+
+
 class EulerEvaluator:
     """EulerEvaluator<NS_2D> (Euler flux, muGas = 0)."""
 
@@ -394,6 +443,15 @@ class EulerEvaluator:
 
     def BuildScalar(self):
         return ArrayScalar(self.device)
+
+    def PrintData(self, fname: str, u: "ArrayDOFV", ifUseLimiter: "ArrayScalar | None" = None,
+                  rhs: "ArrayDOFV | None" = None, t: float = 0.0, Rgas: float = 1.0):
+        """EulerSolver::PrintData (DNDS_FV_EulerSolver.hpp:1436-1492): the
+        solver's cell fields (solver_output) as a Tecplot binary file."""
+        names, data = solver_output(u.download(), None if ifUseLimiter is None else ifUseLimiter.download(),
+                                    None if rhs is None else rhs.download(), self.settings.gamma, Rgas,
+                                    self.vfv.setting.wbap_smooth_indicator_scale)
+        self.mesh.PrintSerialPartPltBinaryDataArray(fname, names, data, t)
 
     def EvaluateDt(self, dt: ArrayScalar, u: ArrayDOFV, CFL: float, MaxDt: float = 1.0,
                    UseLocaldt: bool = False) -> float:

@@ -42,7 +42,8 @@ enum {
     DNDSMI_ERR_NONPOSITIVE_STATE = 4,/* rho <= 0 or negative sound speed squared in a flux (DNDS_Gas.hpp:322, 438) */
     DNDSMI_ERR_UNSUPPORTED = 5,      /* option outside the implemented set */
     DNDSMI_ERR_OUT_OF_MEMORY = 6,
-    DNDSMI_ERR_SETUP = 7             /* degenerate geometry during setup (DNDS_FV_VR.hpp:67-69, DNDS_FV_VR.cpp:579) */
+    DNDSMI_ERR_SETUP = 7,            /* degenerate geometry during setup (DNDS_FV_VR.hpp:67-69, DNDS_FV_VR.cpp:579) */
+    DNDSMI_ERR_IO = 8                /* a file could not be opened, read or written */
 };
 
 /* ---- boundary kinds: the values of BoundaryType, DNDS_Mesh.hpp:21-35 ----- */
@@ -115,6 +116,22 @@ int dndsmi_mesh_generate(const dndsmi_mesh_spec *spec, dndsmi_mesh **out);
 int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const int64_t *cell_ptr,
                            const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node, const int32_t *bnd_bc,
                            dndsmi_mesh **out);
+/* GmshReader::FileRead + InterpolateTopology as CompactFacedMeshSerialRWBuild
+ * calls them (DNDS_Mesh.hpp:87-350, 760-835): a Gmsh 2.2 ASCII file of
+ * first-order triangles and quadrangles (physical groups of dimension 2) and
+ * boundary lines (dimension 1) named bc-1 (Wall), bc-2 (Farfield), bc-3
+ * (Wall_Euler), bc-4 (Wall_NoSlip), bc-DMRFar, bc-RTFar, bc-IVFar or
+ * bc-2DRiemannFar.  Cells in element-id order, counter-clockwise; point
+ * elements (dimension 0) are skipped.  err (may be NULL) receives a
+ * description of a failure. */
+int dndsmi_mesh_read_gmsh(const char *path, dndsmi_mesh **out, char *err, size_t err_len);
+/* CompactFacedMeshSerialRW::PrintSerialPartPltBinaryDataArray(fname, 0,
+ * n_data, names, data, t) on one rank (DNDS_Mesh.hpp:1634-1814): a Tecplot
+ * binary file (#!TDV112), one FEQUADRILATERAL zone, X/Y/Z at the nodes, the
+ * n_data arrays data[k][n_cell] and iPart at the cell centres; triangles
+ * repeat their third node. */
+int dndsmi_mesh_write_plt(const dndsmi_mesh *mesh, const char *path, int32_t n_data, const char *const *names,
+                          const double *data, double t, char *err, size_t err_len);
 int dndsmi_mesh_get_view(const dndsmi_mesh *mesh, dndsmi_mesh_view *view);
 void dndsmi_mesh_destroy(dndsmi_mesh *mesh);
 

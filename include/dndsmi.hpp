@@ -79,6 +79,35 @@ class CompactFacedMeshSerialRW {
                       "mesh from cells");
         return CompactFacedMeshSerialRW(m);
     }
+    /* CompactFacedMeshSerialRWBuild's reading step: a Gmsh 2.2 ASCII file */
+    static CompactFacedMeshSerialRW read_gmsh(const std::string &fname) {
+        dndsmi_mesh *m = nullptr;
+        char buf[1024] = {0};
+        const int rc = dndsmi_mesh_read_gmsh(fname.c_str(), &m, buf, sizeof(buf));
+        if (rc != DNDSMI_OK)
+            throw Error(rc, "Gmsh read failed: " + std::string(buf));
+        return CompactFacedMeshSerialRW(m);
+    }
+    /* PrintSerialPartPltBinaryDataArray(fname, 0, arraySiz, names, data, t)
+     * on one rank: names(k) and data(k, iCell) as in the reference */
+    template <class FNAMES, class FDATA>
+    void PrintSerialPartPltBinaryDataArray(const std::string &fname, int arraySiz, FNAMES &&names, FDATA &&data,
+                                           double t) const {
+        std::vector<std::string> nm(static_cast<size_t>(arraySiz));
+        std::vector<const char *> np(static_cast<size_t>(arraySiz));
+        std::vector<double> d(static_cast<size_t>(arraySiz) * static_cast<size_t>(NumCell()));
+        for (int k = 0; k < arraySiz; k++) {
+            nm[size_t(k)] = names(k);
+            np[size_t(k)] = nm[size_t(k)].c_str();
+            for (index i = 0; i < NumCell(); i++)
+                d[size_t(k) * size_t(NumCell()) + size_t(i)] = data(k, i);
+        }
+        char buf[1024] = {0};
+        const int rc = dndsmi_mesh_write_plt(h_.get(), fname.c_str(), arraySiz, np.data(), d.data(), t, buf,
+                                             sizeof(buf));
+        if (rc != DNDSMI_OK)
+            throw Error(rc, "Tecplot write failed: " + std::string(buf));
+    }
     const dndsmi_mesh_view &view() const { return view_; }
     index NumCell() const { return view_.n_cell; }
     index NumFace() const { return view_.n_face; }
