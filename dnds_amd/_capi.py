@@ -99,6 +99,8 @@ class VRView(C.Structure):
 class EulerSetting(C.Structure):
     _fields_ = [
         ("riemann_solver", C.c_int32), ("gamma", C.c_double), ("far_field", C.c_double * 4),
+        ("mu_gas", C.c_double), ("pr_gas", C.c_double), ("r_gas", C.c_double), ("t_ref", C.c_double),
+        ("c_sutherland", C.c_double),
     ]
 
 

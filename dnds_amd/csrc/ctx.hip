@@ -1234,6 +1234,9 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         return DNDSMI_ERR_INVALID_ARGUMENT; // the device evaluates the face bases from these
     if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC)
         return DNDSMI_ERR_UNSUPPORTED;
+    if (!(es->mu_gas >= 0) ||
+        (es->mu_gas > 0 && !(es->pr_gas > 0 && es->r_gas > 0 && es->t_ref > 0 && es->c_sutherland >= 0)))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
     dndsmi_ctx *c = new (std::nothrow) dndsmi_ctx();
     if (!c)
         return DNDSMI_ERR_OUT_OF_MEMORY;
@@ -1257,6 +1260,11 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     for (int k = 0; k < 4; k++)
         c->gas.far[k] = es->far_field[k];
     c->gas.rs = es->riemann_solver;
+    c->gas.mu = es->mu_gas;
+    c->gas.pr = es->pr_gas;
+    c->gas.cp = es->r_gas * es->gamma / (es->gamma - 1);
+    c->gas.tref = es->t_ref;
+    c->gas.csuth = es->c_sutherland;
     int nfmax = 0;
     for (int64_t i = 0; i < nc; i++)
         nfmax = std::max<int>(nfmax, int(mesh->cell_ptr[i + 1] - mesh->cell_ptr[i]));

@@ -56,6 +56,8 @@ struct GasParams {
     double gamma;
     double far[4];
     int32_t rs; // 1 Roe, 2 HLLC
+    // laminar viscosity (Sutherland) and conduction; mu = 0: inviscid
+    double mu, pr, cp, tref, csuth;
 };
 
 // ---------------------------------------------------------------- gas ----

@@ -427,14 +427,19 @@ def solver_output(u, ifUseLimiter=None, rhs=None, gamma: float = 1.4, Rgas: floa
 
 
 class EulerEvaluator:
-    """EulerEvaluator<NS_2D> (Euler flux, muGas = 0)."""
+    """EulerEvaluator<NS_2D>: the Euler model with muGas = 0 (the default
+    here), the laminar Navier-Stokes terms with Sutherland's law for
+    muGas > 0 (idealGasProperty, DNDS_FV_EulerEvaluator.hpp:151-160)."""
 
     def __init__(self, mesh: CompactFacedMeshSerialRW, vfv: VRFiniteVolume2D, riemannSolverType: str = "Roe",
-                 gamma: float = 1.4, farFieldStaticValue=(1.0, 0.0, 0.0, 2.5), stream: int | None = None):
+                 gamma: float = 1.4, farFieldStaticValue=(1.0, 0.0, 0.0, 2.5), stream: int | None = None,
+                 muGas: float = 0.0, prGas: float = 0.72, Rgas: float = 1.0, TRef: float = 273.15,
+                 CSutherland: float = 110.4):
         self.mesh = mesh
         self.vfv = vfv
         self.settings = A.EulerSetting(riemann_solver=RS[riemannSolverType], gamma=gamma,
-                                       far_field=(C.c_double * 4)(*farFieldStaticValue))
+                                       far_field=(C.c_double * 4)(*farFieldStaticValue), mu_gas=muGas,
+                                       pr_gas=prGas, r_gas=Rgas, t_ref=TRef, c_sutherland=CSutherland)
         self.device = Device(mesh, vfv, self.settings, stream)
         vfv.device = self.device
 

@@ -203,11 +203,22 @@ void dndsmi_vr_destroy(dndsmi_vr *vr);
 /* Device context and arrays                                                */
 /* ======================================================================== */
 
-/* EulerEvaluator<NS_2D>::Setting, DNDS_FV_EulerEvaluator.hpp:139-196 (Euler subset) */
+/* EulerEvaluator<NS_2D>::Setting, DNDS_FV_EulerEvaluator.hpp:139-196: the
+ * inviscid subset, then idealGasProperty's viscous constants (:151-160).
+ * mu_gas = 0 is the Euler model (the viscous terms are skipped; with the
+ * reference's formulas they are exactly zero then); mu_gas > 0 adds the
+ * laminar Navier-Stokes terms of NS_2D with Sutherland's law, and then
+ * pr_gas, r_gas and t_ref must be positive.  CpGas = r_gas gamma / (gamma - 1)
+ * as the solver sets it (DNDS_FV_EulerSolver.hpp:334-339). */
 typedef struct dndsmi_euler_setting {
     int32_t riemann_solver;           /* 1 Roe, 2 HLLC (RiemannSolverType)      */
     double gamma;                     /* idealGasProperty.gamma                 */
     double far_field[4];              /* farFieldStaticValue (rho, rhou, rhov, E) */
+    double mu_gas;                    /* idealGasProperty.muGas (reference default 1) */
+    double pr_gas;                    /* prGas (0.72)                           */
+    double r_gas;                     /* Rgas (1)                               */
+    double t_ref;                     /* TRef (273.15)                          */
+    double c_sutherland;              /* CSutherland (110.4)                    */
 } dndsmi_euler_setting;
 
 enum { DNDSMI_RS_ROE = 1, DNDSMI_RS_HLLC = 2 };

@@ -243,10 +243,12 @@ inline ArrayRecV VRFiniteVolume2D::BuildRec() const { return ArrayRecV(dev_); }
 
 class EulerEvaluator {
   public:
+    /* muGas = 0: the Euler model; muGas > 0: laminar NS_2D with Sutherland's law */
     struct Setting {
         int riemannSolverType = DNDSMI_RS_ROE;
         real gamma = 1.4;
         std::array<real, 4> farFieldStaticValue{1.0, 0.0, 0.0, 2.5};
+        real muGas = 0, prGas = 0.72, Rgas = 1, TRef = 273.15, CSutherland = 110.4;
     };
     EulerEvaluator(const CompactFacedMeshSerialRW &mesh, VRFiniteVolume2D &vfv, const Setting &s,
                    void *stream = nullptr) {
@@ -255,6 +257,8 @@ class EulerEvaluator {
         es.gamma = s.gamma;
         for (int k = 0; k < 4; k++)
             es.far_field[k] = s.farFieldStaticValue[k];
+        es.mu_gas = s.muGas, es.pr_gas = s.prGas, es.r_gas = s.Rgas, es.t_ref = s.TRef;
+        es.c_sutherland = s.CSutherland;
         dev_ = std::make_shared<Device>(mesh, vfv, es, stream);
         vfv.dev_ = dev_;
     }

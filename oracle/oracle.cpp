@@ -653,7 +653,16 @@ extern "C" int oracle_evaluate_dt(const oracle_mesh *m, const oracle_setup *s, c
         double aMean = std::sqrt(gamma * pMean / uMean[0]);
         double lambdaConvection = std::abs(veloNMean) + aMean;
         const double area = s->face_area[f];
-        double lf = lambdaConvection; // + lamVis * area * (...), zero for muGas = 0
+        double lf = lambdaConvection;
+        if (cfg->mu_gas > 0) { // DNDS_FV_EulerEvaluator.cpp:79-119 (NS_2D: no SA factor)
+            const double cp = cfg->r_gas * gamma / (gamma - 1);
+            const double T = pMean / ((gamma - 1) / gamma * cp * uMean[0]);
+            const double muf = cfg->mu_gas * std::pow(T / cfg->t_ref, 1.5) * (cfg->t_ref + cfg->c_sutherland) /
+                               (T + cfg->c_sutherland);
+            const double lamVis = muf / uMean[0] * std::max(4. / 3., gamma / cfg->pr_gas);
+            const double volR = R != kNoCell ? s->volume[R] : s->volume[L];
+            lf = lambdaConvection + lamVis * area * (1. / s->volume[L] + 1. / volR);
+        }
         if (lambda_face)
             lambda_face[f] = lf;
         lambdaCell[L] += lf * area;
@@ -677,6 +686,59 @@ extern "C" int oracle_evaluate_dt(const oracle_mesh *m, const oracle_setup *s, c
             dt[i] = dtMin;
     return ERR_OK;
 }
+
+namespace {
+
+// Rows 1 and 2 (d/dx, d/dy) of the DiBj block of `cell` at a face point from
+// the point's scaled coordinates (X, Y) and the cell's scales (sx, sy)
+// (FDiffBaseValue, global coordinates, DNDS_FV_VR.hpp:328-356): for base
+// ib = c + 1 with exponents (bx, by), d/dx = bx X^(bx-1) Y^by / sx and
+// d/dy = by X^bx Y^(by-1) / sy.
+void face_grad_rows(const double XY[2], const double sc[2], double D[2][9]) {
+    static const int BX[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0}, BY[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+    double pX[4], pY[4];
+    pX[0] = 1.0, pX[1] = XY[0], pX[2] = XY[0] * XY[0], pX[3] = pX[2] * XY[0];
+    pY[0] = 1.0, pY[1] = XY[1], pY[2] = XY[1] * XY[1], pY[3] = pY[2] * XY[1];
+    for (int c = 0; c < 9; c++) {
+        const int bx = BX[c + 1], by = BY[c + 1];
+        D[0][c] = bx > 0 ? double(bx) * pX[bx - 1] * pY[by] * 1.0 / (sc[0] * 1.0 * 1.0) : 0.0;
+        D[1][c] = by > 0 ? double(by) * pX[bx] * pY[by - 1] * 1.0 / (1.0 * sc[1] * 1.0) : 0.0;
+    }
+}
+
+// Gas::ViscousFlux_IdealGas<2> (DNDS_Gas.hpp:795-826): U the mean state,
+// G[d][v] the gradient of the conservative variables, Flux[d][v].
+void viscous_flux(const double U[4], const double G[2][4], const double n[2], bool adiabatic, double gamma,
+                  double mu, double k, double Cp, double Flux[2][4]) {
+    const double velo[2] = {U[1] / U[0], U[2] / U[0]};
+    const double lambda = -2. / 3.;
+    double S[2][2]; // S[i][j] = d u_j / d x_i
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            S[i][j] = (1.0 / (U[0] * U[0])) * (U[0] * G[i][1 + j] - G[i][0] * U[1 + j]);
+    double GradP[2];
+    for (int i = 0; i < 2; i++)
+        GradP[i] = (gamma - 1) * (G[i][3] - 0.5 * ((G[i][1] * velo[0] + G[i][2] * velo[1]) +
+                                                   (S[i][0] * U[1] + S[i][1] * U[2])));
+    const double vSqr = velo[0] * velo[0] + velo[1] * velo[1];
+    const double p = (gamma - 1) * (U[3] - U[0] * 0.5 * vSqr);
+    double GradT[2];
+    for (int i = 0; i < 2; i++)
+        GradT[i] = (gamma / ((gamma - 1) * Cp * U[0] * U[0])) * (U[0] * GradP[i] - p * G[i][0]);
+    if (adiabatic) {
+        const double d = GradT[0] * n[0] + GradT[1] * n[1];
+        GradT[0] -= d * n[0], GradT[1] -= d * n[1];
+    }
+    const double tr = S[0][0] + S[1][1];
+    for (int i = 0; i < 2; i++) {
+        Flux[i][0] = 0.0;
+        for (int j = 0; j < 2; j++)
+            Flux[i][1 + j] = (S[i][j] + S[j][i]) * mu + (i == j ? 1.0 : 0.0) * (lambda * mu * tr);
+        Flux[i][3] = (Flux[i][1] * velo[0] + Flux[i][2] * velo[1]) + k * GradT[i];
+    }
+}
+
+} // namespace
 
 // EvaluateRHS, DNDS_FV_EulerEvaluator.cpp:197-640 (source term zero for NS_2D
 // with constMassForce = 0)
@@ -730,6 +792,55 @@ extern "C" int oracle_evaluate_rhs(const oracle_mesh *m, const oracle_setup *s, 
             int rc = flux_face(*cfg, ULxy, URxy, ULMeanXy, URMeanXy, un, delta_lambda_face[f], F, FL, FR);
             if (rc)
                 return rc;
+            if (cfg->mu_gas > 0) {
+                // gradients at the point (DNDS_FV_EulerEvaluator.cpp:297-346), the
+                // face gradient with the jump term (:381-395), fluxFace's viscous
+                // part (DNDS_FV_EulerEvaluator.hpp:397-430, 574-577)
+                double GL[2][4], GR[2][4], D[2][9];
+                face_grad_rows(s->face_gauss_xy + 12 * f + 4 * g, s->cell_scale + 2 * L, D);
+                for (int d = 0; d < 2; d++)
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += D[d][c] * urec[36 * L + c + 9 * v];
+                        GL[d][v] = acc;
+                    }
+                double minVol = s->volume[L];
+                if (R != kNoCell) {
+                    face_grad_rows(s->face_gauss_xy + 12 * f + 4 * g + 2, s->cell_scale + 2 * R, D);
+                    for (int d = 0; d < 2; d++)
+                        for (int v = 0; v < 4; v++) {
+                            double acc = 0;
+                            for (int c = 0; c < 9; c++)
+                                acc += D[d][c] * urec[36 * R + c + 9 * v];
+                            GR[d][v] = acc;
+                        }
+                    minVol = std::min(minVol, s->volume[R]);
+                } else {
+                    std::memcpy(GR, GL, sizeof(GL));
+                }
+                const int32_t bc = m->face_bc[f];
+                double distGRP = minVol / s->face_area[f] * 2;
+                distGRP += (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) ? kVeryLargeReal : 0.0;
+                double GM[2][4], UM[4];
+                for (int d = 0; d < 2; d++)
+                    for (int v = 0; v < 4; v++)
+                        GM[d][v] = (GR[d][v] + GL[d][v]) * 0.5 + (1.0 / distGRP) * (un[d] * (URxy[v] - ULxy[v]));
+                for (int v = 0; v < 4; v++)
+                    UM[v] = 0.5 * (ULxy[v] + URxy[v]);
+                const double gamma = cfg->gamma, cp = cfg->r_gas * gamma / (gamma - 1);
+                const double vm0 = UM[1] / UM[0], vm1 = UM[2] / UM[0];
+                double pMean, asqrMean, Hmean;
+                ideal_gas_thermal(UM[3], UM[0], vm0 * vm0 + vm1 * vm1, gamma, pMean, asqrMean, Hmean);
+                const double T = pMean / ((gamma - 1) / gamma * cp * UM[0]);
+                const double muf = cfg->mu_gas * std::pow(T / cfg->t_ref, 1.5) * (cfg->t_ref + cfg->c_sutherland) /
+                                   (T + cfg->c_sutherland);
+                const double k = cp * (muf - muf) / 0.9 + cp * muf / cfg->pr_gas;
+                double VF[2][4];
+                viscous_flux(UM, GM, un, bc == BC_WALL_NOSLIP, gamma, muf, k, cp, VF);
+                for (int v = 0; v < 4; v++)
+                    F[v] += (VF[0][v] * un[0] + VF[1][v] * un[1]) * 1;
+            }
             const double nl = std::sqrt(nx * nx + ny * ny);
             for (int v = 0; v < 4; v++) {
                 double a = F[v] * nl, b = FL[v] * nl, c = FR[v] * nl;

@@ -42,6 +42,8 @@ typedef struct oracle_setup {
     const double *ainv;             /* [n_cell][9][9]      */
     const double *ainv_b;           /* [n_c2f][9][9]       */
     const double *ainv_bvec;        /* [n_c2f][9]          */
+    const double *face_gauss_xy;    /* [n_face][3][2][2] scaled (X, Y) of each LINE_3 point per side (viscous terms) */
+    const double *cell_scale;       /* [n_cell][2]         */
 } oracle_setup;
 
 typedef struct oracle_setting {
@@ -57,6 +59,8 @@ typedef struct oracle_setting {
     int32_t n_gmres_space;         /* Krylov subspace size      */
     int32_t n_gmres_iter;          /* restarts                  */
     int32_t limiter_procedure;     /* 0 WBAP V2, 1 WBAP V3 (limiterProcedure) */
+    double mu_gas;                 /* idealGasProperty: 0 = Euler            */
+    double pr_gas, r_gas, t_ref, c_sutherland;
 } oracle_setting;
 
 /* VRFiniteVolume2D::ReconstructionJacobiStep, SOR_Instead = false

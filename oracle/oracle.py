@@ -28,7 +28,8 @@ class OMesh(C.Structure):
 class OSetup(C.Structure):
     _fields_ = [("volume", PD), ("face_area", PD), ("face_center", PD), ("face_norm_center", PD),
                 ("face_norm", PD), ("face_weight", PD), ("face_d0", PD), ("face_dibj_g0", PD),
-                ("matrix_secondary", PD), ("ainv", PD), ("ainv_b", PD), ("ainv_bvec", PD)]
+                ("matrix_secondary", PD), ("ainv", PD), ("ainv_b", PD), ("ainv_bvec", PD), ("face_gauss_xy", PD),
+                ("cell_scale", PD)]
 
 
 class OSetting(C.Structure):
@@ -36,7 +37,8 @@ class OSetting(C.Structure):
                 ("jacobi_relax", C.c_double), ("tang_weight", C.c_double), ("tang_weight_direction", C.c_int32),
                 ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double),
                 ("gmres_code", C.c_int32), ("n_gmres_space", C.c_int32), ("n_gmres_iter", C.c_int32),
-                ("limiter_procedure", C.c_int32)]
+                ("limiter_procedure", C.c_int32), ("mu_gas", C.c_double), ("pr_gas", C.c_double),
+                ("r_gas", C.c_double), ("t_ref", C.c_double), ("c_sutherland", C.c_double)]
 
 
 def build(quiet: bool = True) -> str:
@@ -118,7 +120,8 @@ class Oracle:
                  far_field=(1.0, 0.0, 0.0, 2.5), jacobi_relax: float = 1.0, tang_weight: float = 1.0,
                  tang_weight_direction: int = 0, wbap_smooth_indicator_scale: float = 1e-10,
                  wbap_n_std: float = 10.0, gmres_code: int = 0, n_gmres_space: int = 10, n_gmres_iter: int = 2,
-                 limiter_procedure: int = 0):
+                 limiter_procedure: int = 0, mu_gas: float = 0.0, pr_gas: float = 0.72, r_gas: float = 1.0,
+                 t_ref: float = 273.15, c_sutherland: float = 110.4):
         self._keep = {}
 
         def keep(name, a, dtype):
@@ -141,7 +144,7 @@ class Oracle:
         ff = (C.c_double * 4)(*far_field)
         self.cfg = OSetting(riemann_solver, gamma, ff, jacobi_relax, tang_weight, tang_weight_direction,
                             wbap_smooth_indicator_scale, wbap_n_std, gmres_code, n_gmres_space, n_gmres_iter,
-                            limiter_procedure)
+                            limiter_procedure, mu_gas, pr_gas, r_gas, t_ref, c_sutherland)
         self.lib = lib()
 
     def _check(self, rc, what):
