@@ -397,8 +397,10 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__re
 // deltaLambdaFace per face (muGas = 0).
 __global__ __launch_bounds__(kBlock) void k_dt_face(int32_t nf, int64_t ldf, const int32_t *__restrict__ f2c,
                                                     const double *__restrict__ fnc, const double *__restrict__ u,
-                                                    double gamma, double *__restrict__ lf, double *__restrict__ dlf,
-                                                    int32_t *errw) {
+                                                    GasParams gp, const double *__restrict__ farea,
+                                                    const double *__restrict__ vol, double *__restrict__ lf,
+                                                    double *__restrict__ dlf, int32_t *errw) {
+    const double gamma = gp.gamma;
     const int32_t f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf)
         return;
@@ -428,7 +430,14 @@ __global__ __launch_bounds__(kBlock) void k_dt_face(int32_t nf, int64_t ldf, con
     const double veloNMean = 0.5 * ((vL0 + vR0) * n0 + (vL1 + vR1) * n1);
     const double pMean = (pL + pR) * 0.5;
     const double aMean = sqrt(gamma * pMean / uMean[0]);
-    lf[f] = fabs(veloNMean) + aMean;
+    const double lambdaConvection = fabs(veloNMean) + aMean;
+    if (gp.mu > 0) { // DNDS_FV_EulerEvaluator.cpp:79-119, NS_2D
+        const double lamVis = sutherland_mu(gp, pMean, uMean[0]) / uMean[0] * fmax(4. / 3., gamma / gp.pr);
+        const double volL = vol[L], volR = R >= 0 ? vol[R] : volL;
+        lf[f] = lambdaConvection + lamVis * farea[f] * (1. / volL + 1. / volR);
+    } else {
+        lf[f] = lambdaConvection;
+    }
     dlf[f] = fabs((vR0 - vL0) * n0 + (vR1 - vL1) * n1) + sqrt(fabs(asqrR - asqrL)) * 0.7071;
 }
 
@@ -508,7 +517,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_from(int32_t n, const double *_
 // three points, as on straight edges): the unit normal, |n| and the
 // flux-balance fluxes are computed at point 0 and reused, which yields the
 // same bits as recomputing them from the same inputs.
-template <bool BOUNDARY, bool MIXED = false, bool STRAIGHT = false>
+//
+// VIS (muGas > 0, FP64 only): also the gradients of both sides at each point
+// from rows 1, 2 of the DiBj blocks (DNDS_FV_EulerEvaluator.cpp:297-346), the
+// face gradient with the jump term over distGRP = 2 min(V) / A, which walls
+// push to 3e200 (:356-395, USE_DISABLE_DIST_GRP_FIX_AT_WALL), and fluxFace's
+// viscous flux (DNDS_FV_EulerEvaluator.hpp:397-430, 574-577).
+template <bool BOUNDARY, bool MIXED = false, bool STRAIGHT = false, bool VIS = false>
 __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int64_t ldc,
                                                      const int32_t *__restrict__ list,
                                                      const int32_t *__restrict__ f2c,
@@ -519,7 +534,11 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                                                      const double *__restrict__ x, GasParams gp, double t,
                                                      const uint8_t *__restrict__ fpos,
                                                      double *__restrict__ fslot, int32_t *errw,
-                                                     float *__restrict__ fslot32) {
+                                                     float *__restrict__ fslot32,
+                                                     const double *__restrict__ cscale = nullptr,
+                                                     const double *__restrict__ vol = nullptr,
+                                                     const double *__restrict__ farea = nullptr) {
+    static_assert(!(VIS && MIXED), "the viscous terms run in FP64 only");
     const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     if (tid >= n)
         return;
@@ -537,6 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     // would be read.  MIXED: the bases are rounded to FP32 and the products
     // accumulated in FP32.
     double incL[3][4], incR[3][4];
+    double gradL[VIS ? 3 : 1][2][4], gradR[VIS ? 3 : 1][2][4]; // [point][d/dx, d/dy][var]
 #pragma unroll
     for (int g = 0; g < 3; g++)
 #pragma unroll
@@ -555,11 +575,22 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
         }
         double inc[3][4];
         float acc[3][4];
+        double grd[VIS ? 3 : 1][2][4];
+        double isx = 0, isy = 0;
 #pragma unroll
         for (int g = 0; g < 3; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 inc[g][v] = 0.0, acc[g][v] = 0.f;
+        if (VIS) {
+#pragma unroll
+            for (int g = 0; g < (VIS ? 3 : 1); g++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    grd[g][0][v] = 0.0, grd[g][1][v] = 0.0;
+            const dbl2 sc = reinterpret_cast<const dbl2 *>(cscale)[cell];
+            isx = sc.x * 1.0 * 1.0, isy = 1.0 * sc.y * 1.0; // the derivative rows' denominators
+        }
 #pragma unroll
         for (int c = 0; c < 9; c++) {
             const double m = cmom[c * ldc + cell];
@@ -567,6 +598,25 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 xc[v] = x[(c + 9 * v) * ldc + cell];
+            if (VIS) {
+                // d/dx: bx X^(bx-1) Y^by / sx, d/dy: by X^bx Y^(by-1) / sy
+                const int bx = kBX[c + 1], by = kBY[c + 1];
+#pragma unroll
+                for (int g = 0; g < (VIS ? 3 : 1); g++) {
+                    if (bx > 0) {
+                        const double dx = double(bx) * pX[g][bx - 1] * pY[g][by] * 1.0 / isx;
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            grd[g][0][v] += dx * xc[v];
+                    }
+                    if (by > 0) {
+                        const double dy = double(by) * pX[g][bx] * pY[g][by - 1] * 1.0 / isy;
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            grd[g][1][v] += dy * xc[v];
+                    }
+                }
+            }
 #pragma unroll
             for (int g = 0; g < 3; g++) {
                 const double d = pX[g][kBX[c + 1]] * pY[g][kBY[c + 1]] - m;
@@ -592,6 +642,19 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                 else
                     incL[g][v] = r;
             }
+        if (VIS) {
+#pragma unroll
+            for (int g = 0; g < (VIS ? 3 : 1); g++)
+#pragma unroll
+                for (int d = 0; d < 2; d++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        if (sd)
+                            gradR[g][d][v] = grd[g][d][v];
+                        else
+                            gradL[g][d][v] = grd[g][d][v];
+                    }
+        }
     }
     if (!BOUNDARY) {
 #pragma unroll
@@ -629,6 +692,37 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
         if (!STRAIGHT || g == 0)
             okb = ok && balance_fluxes(gp, uL, URm, n0, n1, FL, FR);
         ok &= ok && okb && riemann_face(gp, ULxy, URxy, n0, n1, dl, F);
+        if (VIS) {
+            const int gi = VIS ? g : 0;
+            double minVol = vol[L];
+            if (!BOUNDARY)
+                minVol = fmin(minVol, vol[R]);
+            double distGRP = minVol / farea[f] * 2;
+            distGRP += (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) ? 3e200 : 0.0;
+            double GM[2][4], UM[4];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const double nd = d ? n1 : n0;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const double gr = BOUNDARY ? gradL[gi][d][v] : gradR[gi][d][v];
+                    GM[d][v] = (gr + gradL[gi][d][v]) * 0.5 + (1.0 / distGRP) * (nd * (URxy[v] - ULxy[v]));
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                UM[v] = 0.5 * (ULxy[v] + URxy[v]);
+            const double vm0 = UM[1] / UM[0], vm1 = UM[2] / UM[0];
+            double pM, aM, HM;
+            ideal_gas_thermal(UM[3], UM[0], vm0 * vm0 + vm1 * vm1, gp.gamma, pM, aM, HM);
+            const double muf = sutherland_mu(gp, pM, UM[0]);
+            const double k = gp.cp * (muf - muf) / 0.9 + gp.cp * muf / gp.pr;
+            double vn[4];
+            viscous_flux_n(UM, GM, n0, n1, bc == BC_WALL_NOSLIP, gp.gamma, muf, k, gp.cp, vn);
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                F[v] += vn[v];
+        }
         const double w = g == 1 ? kLine3W1 : kLine3W0;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -1712,7 +1806,7 @@ int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double 
     const int nc = int(c->nc), nf = int(c->nf);
     prof_begin(c, DNDSMI_K_DT_FACE);
     hipLaunchKernelGGL(k_dt_face, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->f2c, c->fnc, u,
-                       c->gas.gamma, c->lambda_face, c->dlambda_face, c->errw);
+                       c->gas, c->farea, c->vol, c->lambda_face, c->dlambda_face, c->errw);
     prof_end(c);
     const unsigned g = grid_for(nc);
     prof_begin(c, DNDSMI_K_DT_CELL);
@@ -1731,15 +1825,31 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
                const double *dt, double coefA, double coefB) {
     const int nc = int(c->nc), nf = int(c->nf);
     prof_begin(c, DNDSMI_K_RHS_FACE);
-    hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, false, true> : k_rhs_face<false, false, false>),
-                       dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc,
-                       c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw,
-                       nullptr);
+    if (c->gas.mu > 0) {
+        hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, false, true, true>
+                                              : k_rhs_face<false, false, false, true>),
+                           dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c,
+                           c->fbc, c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos,
+                           c->fslot, c->errw, nullptr, c->cscale, c->vol, c->farea);
+    } else {
+        hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, false, true> : k_rhs_face<false, false, false>),
+                           dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c,
+                           c->fbc, c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos,
+                           c->fslot, c->errw, nullptr, nullptr, nullptr, nullptr);
+    }
     prof_end(c);
-    if (c->n_bface > 0)
-        hipLaunchKernelGGL(k_rhs_face<true>, dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
-                           int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
-                           c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw, nullptr);
+    if (c->n_bface > 0) {
+        if (c->gas.mu > 0)
+            hipLaunchKernelGGL((k_rhs_face<true, false, false, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0,
+                               c->stream, int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy,
+                               c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw,
+                               nullptr, c->cscale, c->vol, c->farea);
+        else
+            hipLaunchKernelGGL(k_rhs_face<true>, dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
+                               int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
+                               c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw, nullptr,
+                               nullptr, nullptr, nullptr);
+    }
     prof_begin(c, DNDSMI_K_RHS_CELL);
     if (c->nfmax == 3)
         hipLaunchKernelGGL(k_rhs_cell<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->gat,
@@ -1888,12 +1998,13 @@ int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, 
     hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, true, true> : k_rhs_face<false, true, false>),
                        dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc,
                        c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw,
-                       c->fslot32);
+                       c->fslot32, nullptr, nullptr, nullptr);
     prof_end(c);
     if (c->n_bface > 0)
         hipLaunchKernelGGL((k_rhs_face<true, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
                            int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
-                           c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw, c->fslot32);
+                           c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw, c->fslot32,
+                           nullptr, nullptr, nullptr);
     prof_begin(c, DNDSMI_K_RHS_CELL32);
     if (c->nfmax == 3)
         hipLaunchKernelGGL((k_rhs_cell<3, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
@@ -2036,6 +2147,10 @@ extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_a
         c->err = "WBAP needs a setup built with build_limiter_data";
         return DNDSMI_ERR_UNSUPPORTED;
     }
+    if (c->gas.mu > 0) {
+        c->err = "the mixed-precision step covers the Euler model (muGas = 0) only";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
     DNDSMI_TRY(ensure_mixed(c));
     DNDSMI_TRY(launch_dt(c, u->d, CFL, MaxDt, c->dt_tmp));
     double dtmin = 0;
@@ -2110,8 +2225,9 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
     case DNDSMI_K_DT_CELL: // gather index, lambda + area per slot, volume, dt out
         return c2f * (4 + 16) + nc * (8 + 8);
     case DNDSMI_K_RHS_FACE: // f2c, slot positions, point coordinates, normals, dlambda, 2 x 8 terms out;
-        // x, u and the moments once per cell
-        return nif * (8 + 1 + 96 + (c->straight_faces ? 16 : 48) + 8 + 128) + nc * (36 * 8 + 32 + 72);
+        // x, u and the moments once per cell; viscous: face area, both volumes and cell scales
+        return nif * (8 + 1 + 96 + (c->straight_faces ? 16 : 48) + 8 + 128 + (c->gas.mu > 0 ? 8 + 16 + 32 : 0)) +
+               nc * (36 * 8 + 32 + 72);
     case DNDSMI_K_RHS_CELL: // 8 terms per slot, volume, x read+write, xLast (+ last-slot index for 4 slots)
         return c2f * 64 + nc * (8 + 32 * 3 + (c->nfmax == 4 ? 4 : 0));
     case DNDSMI_K_WBAP_FACE: // point-0 coordinates of both sides, f2c, weights, normal, 8 terms out;

@@ -407,6 +407,57 @@ __device__ inline bool flux_face(const GasParams &gp, const double ULxy[4], cons
     return balance_fluxes(gp, ULm, URm, n0, n1, FLfix, FRfix) && riemann_face(gp, ULxy, URxy, n0, n1, dLambda, out);
 }
 
+// ---------------------------------------------------------- viscous ----
+// Sutherland's law at the mean state (DNDS_FV_EulerEvaluator.hpp:397-412,
+// DNDS_FV_EulerEvaluator.cpp:79-85), T = p / ((gamma - 1) / gamma Cp rho).
+__device__ inline double sutherland_mu(const GasParams &gp, double p, double rho) {
+    const double T = p / ((gp.gamma - 1) / gp.gamma * gp.cp * rho);
+    return gp.mu * pow(T / gp.tref, 1.5) * (gp.tref + gp.csuth) / (T + gp.csuth);
+}
+
+// Gas::ViscousFlux_IdealGas<2> (DNDS_Gas.hpp:795-826) dotted with the unit
+// normal: vn[v] = sum_d Flux[d][v] n_d for the mean state U and the
+// conservative gradient G[d][v]; adiabatic removes the normal part of grad T.
+__device__ inline void viscous_flux_n(const double U[4], const double G[2][4], double n0, double n1, bool adiabatic,
+                                      double gamma, double mu, double k, double Cp, double vn[4]) {
+    const double velo0 = U[1] / U[0], velo1 = U[2] / U[0];
+    const double lambda = -2. / 3.;
+    double S[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            S[i][j] = (1.0 / (U[0] * U[0])) * (U[0] * G[i][1 + j] - G[i][0] * U[1 + j]);
+    double GradP[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+        GradP[i] = (gamma - 1) *
+                   (G[i][3] - 0.5 * ((G[i][1] * velo0 + G[i][2] * velo1) + (S[i][0] * U[1] + S[i][1] * U[2])));
+    const double vSqr = velo0 * velo0 + velo1 * velo1;
+    const double p = (gamma - 1) * (U[3] - U[0] * 0.5 * vSqr);
+    double GradT[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+        GradT[i] = (gamma / ((gamma - 1) * Cp * U[0] * U[0])) * (U[0] * GradP[i] - p * G[i][0]);
+    if (adiabatic) {
+        const double d = GradT[0] * n0 + GradT[1] * n1;
+        GradT[0] -= d * n0, GradT[1] -= d * n1;
+    }
+    const double tr = S[0][0] + S[1][1];
+    double F[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        F[i][0] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            F[i][1 + j] = (S[i][j] + S[j][i]) * mu + (i == j ? 1.0 : 0.0) * (lambda * mu * tr);
+        F[i][3] = (F[i][1] * velo0 + F[i][2] * velo1) + k * GradT[i];
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        vn[v] = (F[0][v] * n0 + F[1][v] * n1) * 1;
+}
+
 // ------------------------------------------------------------ bases ----
 // The scaled cubic monomials of FDiffBaseValue in global coordinates
 // (DNDS_FV_VR.hpp:300-356), evaluated at a face point from its scaled

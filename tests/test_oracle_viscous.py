@@ -165,3 +165,14 @@ def test_viscous_time_step_on_a_uniform_mesh_at_rest():
     a = np.sqrt(GAMMA * p / rho)
     assert rel_err(dt, np.full(mesh.n_cell, 0.5 * h / (4 * a + 8 * lam_vis / h))) < 1e-13
     assert rel_err(lf, np.full(mesh.n_face, a + lam_vis * 2 / h)) < 1e-13
+
+
+@pytest.mark.parametrize("bad", [dict(muGas=-1.0), dict(muGas=0.1, prGas=0.0), dict(muGas=0.1, Rgas=0.0),
+                                 dict(muGas=0.1, TRef=-1.0), dict(muGas=0.1, CSutherland=-0.1)])
+def test_viscous_settings_are_checked_before_the_device_is_touched(bad):
+    from dnds_amd import _capi as A
+    from dnds_amd.host import DNDSError, EulerEvaluator
+    mesh, vfv = mesh_and_setup(kind=0, nx=4, ny=4)
+    with pytest.raises(DNDSError) as e:
+        EulerEvaluator(mesh, vfv, **bad)
+    assert e.value.code == A.ERR_INVALID_ARGUMENT
