@@ -192,7 +192,7 @@ K_WBAP_CELL = 7
 K_WBAP_LIM = 8
 KERNEL_NAMES = ["k_jacobi", "k_wall_corr", "k_dt_face", "k_dt_cell", "k_rhs_face", "k_rhs_cell", "k_wbap_face",
                 "k_wbap_cell", "k_wbap_lim", "k_lusgs_sweep", "k_delta32", "k_rhs_face32", "k_rhs_cell32",
-                "k_lusgs_matvec"]
+                "k_lusgs_matvec", "k_rhs_bface"]
 
 _lib = None
 

@@ -589,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                 for (int v = 0; v < 4; v++)
                     grd[g][0][v] = 0.0, grd[g][1][v] = 0.0;
             const dbl2 sc = reinterpret_cast<const dbl2 *>(cscale)[cell];
-            isx = sc.x * 1.0 * 1.0, isy = 1.0 * sc.y * 1.0; // the derivative rows' denominators
+            isx = 1.0 / sc.x, isy = 1.0 / sc.y; // the derivative rows' scales, as reciprocals
         }
 #pragma unroll
         for (int c = 0; c < 9; c++) {
@@ -599,18 +599,19 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
             for (int v = 0; v < 4; v++)
                 xc[v] = x[(c + 9 * v) * ldc + cell];
             if (VIS) {
-                // d/dx: bx X^(bx-1) Y^by / sx, d/dy: by X^bx Y^(by-1) / sy
+                // d/dx: bx X^(bx-1) Y^by / sx, d/dy: by X^bx Y^(by-1) / sy (the
+                // divisions as products with the reciprocal: within rounding)
                 const int bx = kBX[c + 1], by = kBY[c + 1];
 #pragma unroll
                 for (int g = 0; g < (VIS ? 3 : 1); g++) {
                     if (bx > 0) {
-                        const double dx = double(bx) * pX[g][bx - 1] * pY[g][by] * 1.0 / isx;
+                        const double dx = double(bx) * pX[g][bx - 1] * pY[g][by] * isx;
 #pragma unroll
                         for (int v = 0; v < 4; v++)
                             grd[g][0][v] += dx * xc[v];
                     }
                     if (by > 0) {
-                        const double dy = double(by) * pX[g][bx] * pY[g][by - 1] * 1.0 / isy;
+                        const double dy = double(by) * pX[g][bx] * pY[g][by - 1] * isy;
 #pragma unroll
                         for (int v = 0; v < 4; v++)
                             grd[g][1][v] += dy * xc[v];
@@ -1839,6 +1840,7 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
     }
     prof_end(c);
     if (c->n_bface > 0) {
+        prof_begin(c, DNDSMI_K_RHS_BFACE);
         if (c->gas.mu > 0)
             hipLaunchKernelGGL((k_rhs_face<true, false, false, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0,
                                c->stream, int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy,
@@ -1849,6 +1851,7 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
                                int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
                                c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw, nullptr,
                                nullptr, nullptr, nullptr);
+        prof_end(c);
     }
     prof_begin(c, DNDSMI_K_RHS_CELL);
     if (c->nfmax == 3)
@@ -2237,6 +2240,10 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
+    case DNDSMI_K_RHS_BFACE: // per boundary face: index, f2c, slot position, kind, one side's point
+        // coordinates, normals, centre, dlambda, 8 terms out, the cell's x, u and moments
+        return c->n_bface * (4 + 8 + 1 + 4 + 48 + 48 + 16 + 8 + 64 + 36 * 8 + 32 + 72 +
+                             (c->gas.mu > 0 ? 8 + 8 + 16 : 0));
     case DNDSMI_K_LUSGS_MATVEC: // per scan position cell, D, volume, x own in, Ax out; per slot both
         // neighbour indices, face, normal, area, u and x of the neighbour, lambda
         return nc * (4 + 8 + 8 + 32 + 32) + c2f * (8 + 4 + 16 + 8 + 32 + 32 + 8);

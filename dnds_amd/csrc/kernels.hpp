@@ -409,10 +409,12 @@ __device__ inline bool flux_face(const GasParams &gp, const double ULxy[4], cons
 
 // ---------------------------------------------------------- viscous ----
 // Sutherland's law at the mean state (DNDS_FV_EulerEvaluator.hpp:397-412,
-// DNDS_FV_EulerEvaluator.cpp:79-85), T = p / ((gamma - 1) / gamma Cp rho).
+// DNDS_FV_EulerEvaluator.cpp:79-85), T = p / ((gamma - 1) / gamma Cp rho);
+// (T / TRef)^1.5 is formed as t sqrt(t), within rounding of the power.
 __device__ inline double sutherland_mu(const GasParams &gp, double p, double rho) {
     const double T = p / ((gp.gamma - 1) / gp.gamma * gp.cp * rho);
-    return gp.mu * pow(T / gp.tref, 1.5) * (gp.tref + gp.csuth) / (T + gp.csuth);
+    const double tr = T / gp.tref;
+    return gp.mu * (tr * sqrt(tr)) * (gp.tref + gp.csuth) / (T + gp.csuth);
 }
 
 // Gas::ViscousFlux_IdealGas<2> (DNDS_Gas.hpp:795-826) dotted with the unit

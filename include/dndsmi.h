@@ -484,7 +484,8 @@ enum {
     DNDSMI_K_RHS_FACE32 = 11,/* mixed precision: inner-face fluxes, FP32 reconstruction */
     DNDSMI_K_RHS_CELL32 = 12,/* mixed precision: FP32 gather + stage update  */
     DNDSMI_K_LUSGS_MATVEC = 13,/* LUSGSMatrixVec (GMRES operator)            */
-    DNDSMI_K_COUNT = 14
+    DNDSMI_K_RHS_BFACE = 14, /* EvaluateRHS: boundary-face fluxes (FP64)     */
+    DNDSMI_K_COUNT = 15
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the
