@@ -537,7 +537,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                                                      float *__restrict__ fslot32,
                                                      const double *__restrict__ cscale = nullptr,
                                                      const double *__restrict__ vol = nullptr,
-                                                     const double *__restrict__ farea = nullptr) {
+                                                     const double *__restrict__ farea = nullptr,
+                                                     const double *__restrict__ xalt = nullptr,
+                                                     const uint8_t *__restrict__ limf = nullptr) {
     static_assert(!(VIS && MIXED), "the viscous terms run in FP64 only");
     const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     if (tid >= n)
@@ -566,6 +568,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
 #pragma unroll 1
     for (int sd = 0; sd < (BOUNDARY ? 1 : 2); sd++) {
         const int32_t cell = sd ? R : L;
+        // the steps' limiter leaves its unlimited cells in x and flags the others
+        const double *xs = (limf && limf[cell]) ? xalt : x;
         double pX[3][4], pY[3][4];
 #pragma unroll
         for (int g = 0; g < 3; g++) {
@@ -597,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
             double xc[4];
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                xc[v] = x[(c + 9 * v) * ldc + cell];
+                xc[v] = xs[(c + 9 * v) * ldc + cell];
             if (VIS) {
                 // d/dx: bx X^(bx-1) Y^by / sx, d/dy: by X^bx Y^(by-1) / sy (the
                 // divisions as products with the reciprocal: within rounding)
@@ -962,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, in
                                                      const double *__restrict__ wsum, const double *__restrict__ x,
                                                      double *__restrict__ xo, double *__restrict__ ind, double scale,
                                                      int if_all, int32_t *__restrict__ count,
-                                                     int32_t *__restrict__ list) {
+                                                     int32_t *__restrict__ list, uint8_t *__restrict__ limf) {
     const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
     bool lim = false;
     if (i < nc) {
@@ -983,7 +987,9 @@ __global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, in
         const double flag = sqrt(fmax(fabs(si0), fabs(si1))) * (3 * 3);
         ind[i] = flag;
         lim = !(flag < scale && !if_all);
-        if (!lim) {
+        if (limf)
+            limf[i] = uint8_t(lim);
+        else if (!lim) {
 #pragma unroll
             for (int k = 0; k < 36; k++)
                 xo[k * ldc + i] = x[k * ldc + i];
@@ -1161,7 +1167,8 @@ __global__ __launch_bounds__(kBlock) void k_wbap_lim_v2(int64_t ldc, int64_t ldf
                                                         const double *__restrict__ msec,
                                                         const double *__restrict__ u, const double *__restrict__ x,
                                                         double *__restrict__ xo, double gamma, double nStd,
-                                                        double eps4, int32_t *errw) {
+                                                        double eps4, int32_t *errw,
+                                                        const uint8_t *__restrict__ limf) {
     constexpr int ls = ORDER == 3 ? 5 : (ORDER == 2 ? 2 : 0);
     constexpr int le = ORDER == 3 ? 8 : (ORDER == 2 ? 4 : 1);
     constexpr int nr = le - ls + 1;
@@ -1193,6 +1200,8 @@ __global__ __launch_bounds__(kBlock) void k_wbap_lim_v2(int64_t ldc, int64_t ldf
         double un0, un1;
         stable_normalized(fnc[f], fnc[ldf + f], un0, un1);
         const double *ms = msec + int64_t(81 * side) * ldf + f;
+        // the neighbour's rows above le: limited (xo) or, with limf, as given (x)
+        const double *above = (limf && !limf[j]) ? x : xo;
         double oIn[nr][4];
 #pragma unroll
         for (int r = 0; r < nr; r++)
@@ -1201,7 +1210,7 @@ __global__ __launch_bounds__(kBlock) void k_wbap_lim_v2(int64_t ldc, int64_t ldf
                 oIn[r][v] = 0.0;
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-            const double *src = c <= le ? x : xo;
+            const double *src = c <= le ? x : above;
             double uj[4];
 #pragma unroll
             for (int v = 0; v < 4; v++)
@@ -1566,6 +1575,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             DNDSMI_TRY(c->alloc(c->uF, int64_t(nfmax) * 28 * ldc));
             DNDSMI_TRY(c->alloc(c->ind, ldc));
             DNDSMI_TRY(c->alloc(c->xlim, 36 * ldc));
+            DNDSMI_TRY(c->alloc(c->limf, ldc));
             DNDSMI_TRY(c->alloc(c->lim_list, ldc));
             DNDSMI_TRY(c->alloc(c->lim_count, 1));
         }
@@ -1823,7 +1833,7 @@ int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double 
 }
 
 int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
-               const double *dt, double coefA, double coefB) {
+               const double *dt, double coefA, double coefB, const double *xalt, const uint8_t *limf) {
     const int nc = int(c->nc), nf = int(c->nf);
     prof_begin(c, DNDSMI_K_RHS_FACE);
     if (c->gas.mu > 0) {
@@ -1831,12 +1841,12 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
                                               : k_rhs_face<false, false, false, true>),
                            dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c,
                            c->fbc, c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos,
-                           c->fslot, c->errw, nullptr, c->cscale, c->vol, c->farea);
+                           c->fslot, c->errw, nullptr, c->cscale, c->vol, c->farea, xalt, limf);
     } else {
         hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, false, true> : k_rhs_face<false, false, false>),
                            dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c,
                            c->fbc, c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos,
-                           c->fslot, c->errw, nullptr, nullptr, nullptr, nullptr);
+                           c->fslot, c->errw, nullptr, nullptr, nullptr, nullptr, xalt, limf);
     }
     prof_end(c);
     if (c->n_bface > 0) {
@@ -1845,12 +1855,12 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
             hipLaunchKernelGGL((k_rhs_face<true, false, false, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0,
                                c->stream, int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy,
                                c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw,
-                               nullptr, c->cscale, c->vol, c->farea);
+                               nullptr, c->cscale, c->vol, c->farea, xalt, limf);
         else
             hipLaunchKernelGGL(k_rhs_face<true>, dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
                                int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
                                c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, c->fslot, c->errw, nullptr,
-                               nullptr, nullptr, nullptr);
+                               nullptr, nullptr, nullptr, xalt, limf);
         prof_end(c);
     }
     prof_begin(c, DNDSMI_K_RHS_CELL);
@@ -1865,7 +1875,8 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
 }
 
 int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all,
-                int procedure) {
+                int procedure, bool full_copy) {
+    c->wbap_full_copy = full_copy;
     if (!c->has_limiter) {
         c->err = "WBAP needs a setup built with build_limiter_data";
         return DNDSMI_ERR_UNSUPPORTED;
@@ -1881,11 +1892,11 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
     if (c->nfmax == 3)
         hipLaunchKernelGGL(k_wbap_ind<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->nbr,
                            c->sface, c->f2c, c->wsum, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count,
-                           c->lim_list);
+                           c->lim_list, full_copy ? nullptr : c->limf);
     else
         hipLaunchKernelGGL(k_wbap_ind<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->nbr,
                            c->sface, c->f2c, c->wsum, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count,
-                           c->lim_list);
+                           c->lim_list, full_copy ? nullptr : c->limf);
     prof_end(c);
     prof_begin(c, DNDSMI_K_WBAP_LIM);
     if (procedure == 1) {
@@ -1904,7 +1915,7 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
             constexpr int NF = decltype(nf_c)::value, O = decltype(order_c)::value;
             hipLaunchKernelGGL((k_wbap_lim_v2<NF, O>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, c->ldc,
                                c->ldf, c->lim_count, c->lim_list, c->nbr, c->sface, c->f2c, c->fnc, c->msec, u, x,
-                               xo, c->gas.gamma, c->wbap_nstd, eps4, c->errw);
+                               xo, c->gas.gamma, c->wbap_nstd, eps4, c->errw, full_copy ? nullptr : c->limf);
         };
         using I3 = std::integral_constant<int, 3>;
         using I4 = std::integral_constant<int, 4>;
@@ -1995,19 +2006,19 @@ int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps,
 }
 
 int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
-                     const double *dt, double coefA, double coefB) {
+                     const double *dt, double coefA, double coefB, const double *xalt, const uint8_t *limf) {
     const int nc = int(c->nc), nf = int(c->nf);
     prof_begin(c, DNDSMI_K_RHS_FACE32);
     hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, true, true> : k_rhs_face<false, true, false>),
                        dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc,
                        c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw,
-                       c->fslot32, nullptr, nullptr, nullptr);
+                       c->fslot32, nullptr, nullptr, nullptr, xalt, limf);
     prof_end(c);
     if (c->n_bface > 0)
         hipLaunchKernelGGL((k_rhs_face<true, true>), dim3(grid_for(c->n_bface)), dim3(kBlock), 0, c->stream,
                            int32_t(c->n_bface), c->ldf, c->ldc, c->bfaces, c->f2c, c->fbc, c->fxy, c->cmom, c->fn,
                            c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos, nullptr, c->errw, c->fslot32,
-                           nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, xalt, limf);
     prof_begin(c, DNDSMI_K_RHS_CELL32);
     if (c->nfmax == 3)
         hipLaunchKernelGGL((k_rhs_cell<3, float>), dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc,
@@ -2130,12 +2141,12 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
         }
         const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
         const double cB = stage ? 1 - Coef[stage] : 0.0;
-        const double *xr = uRec->d;
-        if (use_limiter) { // the RHS sees the limited copy; uRec keeps the unlimited iterate
-            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure));
-            xr = c->xlim;
-        }
-        DNDSMI_TRY(launch_rhs(c, u->d, xr, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA, cB));
+        // the RHS sees the limited cells from xlim (flagged), the others from
+        // uRec, which keeps the unlimited iterate
+        if (use_limiter)
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure, false));
+        DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,
+                              cB, use_limiter ? c->xlim : nullptr, use_limiter ? c->limf : nullptr));
     }
     return DNDSMI_OK;
 }
@@ -2169,13 +2180,10 @@ extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_a
         DNDSMI_TRY(launch_jacobi_mixed(c, u->d, uRec->d, n_sweeps, ts));
         const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
         const double cB = stage ? 1 - Coef[stage] : 0.0;
-        const double *xr = uRec->d;
-        if (use_limiter) {
-            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure));
-            xr = c->xlim;
-        }
-        DNDSMI_TRY(launch_rhs_mixed(c, u->d, xr, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,
-                                    cB));
+        if (use_limiter)
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure, false));
+        DNDSMI_TRY(launch_rhs_mixed(c, u->d, uRec->d, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr,
+                                    cA, cB, use_limiter ? c->xlim : nullptr, use_limiter ? c->limf : nullptr));
     }
     return DNDSMI_OK;
 }
@@ -2236,8 +2244,9 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
     case DNDSMI_K_WBAP_FACE: // point-0 coordinates of both sides, f2c, weights, normal, 8 terms out;
         // x (rho, E), u (rho, E), moments and scales once per cell
         return nif * (32 + 8 + 80 + 16 + 64) + nc * (18 * 8 + 16 + 72 + 16);
-    case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out, x read and copied out
-        return c2f * (32 + 12) + nc * (8 + 36 * 8 * 2);
+    case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out; x read and copied out for the unlimited
+        // cells of a full copy (an upper bound: as if no cell were limited), else a flag per cell
+        return c2f * (32 + 12) + nc * (8 + (c->wbap_full_copy ? 36 * 8 * 2 : 1));
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
     case DNDSMI_K_RHS_BFACE: // per boundary face: index, f2c, slot position, kind, one side's point

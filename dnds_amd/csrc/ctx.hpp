@@ -81,7 +81,9 @@ struct dndsmi_ctx {
     double *wsum = nullptr;    // [ldf][8] per-face indicator terms of both cells
     double *uF = nullptr;      // [nfmax*28][ldc] WBAP per-cell face scratch
     double *ind = nullptr;     // [ldc] indicator when the caller passes none
-    double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps
+    double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps (limited cells only)
+    uint8_t *limf = nullptr;   // [ldc] 1 where the steps' limiter wrote the cell into xlim
+    bool wbap_full_copy = true; // the last WBAP launch copied the unlimited cells (byte model)
     int32_t *lim_list = nullptr;  // [ldc] cells to limit
     int32_t *lim_count = nullptr; // [1]
     double tang_weight = 1.0;
@@ -204,14 +206,19 @@ bool same_ctx(const dndsmi_ctx *c, const dndsmi_array *a, int32_t kind);
 // the explicit path's launches (ctx.hip)
 int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t);
 int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double *dt_out);
+// xalt/limf (may be null): cells with limf set are read from xalt instead of x
 int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
-               const double *dt, double coefA, double coefB);
+               const double *dt, double coefA, double coefB, const double *xalt = nullptr,
+               const uint8_t *limf = nullptr);
+// full_copy: xo receives every cell (x for those not limited); otherwise only
+// the limited cells, flagged in c->limf
 int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, double *ind, int if_all,
-                int procedure);
+                int procedure, bool full_copy = true);
 // the mixed-precision pieces (ctx.hip)
 int ensure_mixed(dndsmi_ctx *c);
 int launch_jacobi_mixed(dndsmi_ctx *c, const double *u, double *x, int n_sweeps, double t);
 int launch_rhs_mixed(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,
-                     const double *dt, double coefA, double coefB);
+                     const double *dt, double coefA, double coefB, const double *xalt = nullptr,
+                     const uint8_t *limf = nullptr);
 
 } // namespace dndsmi_host
