@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_from(int32_t n, const double *_
 // push to 3e200 (:356-395, USE_DISABLE_DIST_GRP_FIX_AT_WALL), and fluxFace's
 // viscous flux (DNDS_FV_EulerEvaluator.hpp:397-430, 574-577).
 template <bool BOUNDARY, bool MIXED = false, bool STRAIGHT = false, bool VIS = false>
-__global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int64_t ldc,
+__global__ __launch_bounds__(kBlock, VIS && !BOUNDARY ? 2 : 1) void k_rhs_face(int32_t n, int64_t ldf, int64_t ldc,
                                                      const int32_t *__restrict__ list,
                                                      const int32_t *__restrict__ f2c,
                                                      const int32_t *__restrict__ fbc, const double *__restrict__ fxy,
@@ -558,7 +558,6 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
     // would be read.  MIXED: the bases are rounded to FP32 and the products
     // accumulated in FP32.
     double incL[3][4], incR[3][4];
-    double gradL[VIS ? 3 : 1][2][4], gradR[VIS ? 3 : 1][2][4]; // [point][d/dx, d/dy][var]
 #pragma unroll
     for (int g = 0; g < 3; g++)
 #pragma unroll
@@ -579,22 +578,11 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
         }
         double inc[3][4];
         float acc[3][4];
-        double grd[VIS ? 3 : 1][2][4];
-        double isx = 0, isy = 0;
 #pragma unroll
         for (int g = 0; g < 3; g++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 inc[g][v] = 0.0, acc[g][v] = 0.f;
-        if (VIS) {
-#pragma unroll
-            for (int g = 0; g < (VIS ? 3 : 1); g++)
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    grd[g][0][v] = 0.0, grd[g][1][v] = 0.0;
-            const dbl2 sc = reinterpret_cast<const dbl2 *>(cscale)[cell];
-            isx = 1.0 / sc.x, isy = 1.0 / sc.y; // the derivative rows' scales, as reciprocals
-        }
 #pragma unroll
         for (int c = 0; c < 9; c++) {
             const double m = cmom[c * ldc + cell];
@@ -602,26 +590,6 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 xc[v] = xs[(c + 9 * v) * ldc + cell];
-            if (VIS) {
-                // d/dx: bx X^(bx-1) Y^by / sx, d/dy: by X^bx Y^(by-1) / sy (the
-                // divisions as products with the reciprocal: within rounding)
-                const int bx = kBX[c + 1], by = kBY[c + 1];
-#pragma unroll
-                for (int g = 0; g < (VIS ? 3 : 1); g++) {
-                    if (bx > 0) {
-                        const double dx = double(bx) * pX[g][bx - 1] * pY[g][by] * isx;
-#pragma unroll
-                        for (int v = 0; v < 4; v++)
-                            grd[g][0][v] += dx * xc[v];
-                    }
-                    if (by > 0) {
-                        const double dy = double(by) * pX[g][bx] * pY[g][by - 1] * isy;
-#pragma unroll
-                        for (int v = 0; v < 4; v++)
-                            grd[g][1][v] += dy * xc[v];
-                    }
-                }
-            }
 #pragma unroll
             for (int g = 0; g < 3; g++) {
                 const double d = pX[g][kBX[c + 1]] * pY[g][kBY[c + 1]] - m;
@@ -647,19 +615,6 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                 else
                     incL[g][v] = r;
             }
-        if (VIS) {
-#pragma unroll
-            for (int g = 0; g < (VIS ? 3 : 1); g++)
-#pragma unroll
-                for (int d = 0; d < 2; d++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        if (sd)
-                            gradR[g][d][v] = grd[g][d][v];
-                        else
-                            gradL[g][d][v] = grd[g][d][v];
-                    }
-        }
     }
     if (!BOUNDARY) {
 #pragma unroll
@@ -674,7 +629,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
         acc[k] = 0.0;
     bool ok = true, okb = true;
     double n0, n1, nl, FL[4], FR[4];
-#pragma unroll
+    // the viscous variant keeps its points apart: unrolled, their gradient
+    // work would be scheduled together and crowd the registers
+#pragma unroll(VIS && !BOUNDARY ? 1 : 3)
     for (int g = 0; g < 3; g++) {
         double ULxy[4], URxy[4], URm[4];
         if (!STRAIGHT || g == 0) {
@@ -698,7 +655,47 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
             okb = ok && balance_fluxes(gp, uL, URm, n0, n1, FL, FR);
         ok &= ok && okb && riemann_face(gp, ULxy, URxy, n0, n1, dl, F);
         if (VIS) {
-            const int gi = VIS ? g : 0;
+            // the gradients of both sides at this point, formed here rather
+            // than kept for all three points
+            auto grad_at = [&](int32_t cell, int sd, double G[2][4]) {
+                const dbl2 q = xy[(sd * 3 + g) * ldf];
+                double px[4], py[4];
+                int_powers(q.x, px);
+                int_powers(q.y, py);
+                const dbl2 sc = reinterpret_cast<const dbl2 *>(cscale)[cell];
+                const double isx = 1.0 / sc.x, isy = 1.0 / sc.y; // the rows' scales as reciprocals
+                const double *xs = (limf && limf[cell]) ? xalt : x;
+#pragma unroll
+                for (int d = 0; d < 2; d++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        G[d][v] = 0.0;
+                // d/dx: bx X^(bx-1) Y^by / sx, d/dy: by X^bx Y^(by-1) / sy
+#pragma unroll
+                for (int c = 0; c < 9; c++) {
+                    const int bx = kBX[c + 1], by = kBY[c + 1];
+                    double xc[4];
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        xc[v] = xs[(c + 9 * v) * ldc + cell];
+                    if (bx > 0) {
+                        const double dx = double(bx) * px[bx - 1] * py[by] * isx;
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            G[0][v] += dx * xc[v];
+                    }
+                    if (by > 0) {
+                        const double dy = double(by) * px[bx] * py[by - 1] * isy;
+#pragma unroll
+                        for (int v = 0; v < 4; v++)
+                            G[1][v] += dy * xc[v];
+                    }
+                }
+            };
+            double gL[2][4], gR[2][4];
+            grad_at(L, 0, gL);
+            if (!BOUNDARY)
+                grad_at(R, 1, gR);
             double minVol = vol[L];
             if (!BOUNDARY)
                 minVol = fmin(minVol, vol[R]);
@@ -710,8 +707,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_face(int32_t n, int64_t ldf, int
                 const double nd = d ? n1 : n0;
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const double gr = BOUNDARY ? gradL[gi][d][v] : gradR[gi][d][v];
-                    GM[d][v] = (gr + gradL[gi][d][v]) * 0.5 + (1.0 / distGRP) * (nd * (URxy[v] - ULxy[v]));
+                    const double gr = BOUNDARY ? gL[d][v] : gR[d][v];
+                    GM[d][v] = (gr + gL[d][v]) * 0.5 + (1.0 / distGRP) * (nd * (URxy[v] - ULxy[v]));
                 }
             }
 #pragma unroll
