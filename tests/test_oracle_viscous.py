@@ -37,22 +37,30 @@ P = np.array([[1.0, 0.10, -0.05, 0.08, 0.03, -0.06, 0.05, -0.02, 0.04, 0.03],
               [0.3, 0.20, 0.15, -0.10, 0.12, 0.05, 0.07, -0.04, 0.02, 0.06],
               [-0.2, 0.10, 0.25, 0.06, -0.08, 0.10, -0.03, 0.05, 0.04, -0.05],
               [2.5, 0.30, -0.20, 0.15, 0.10, -0.12, 0.05, 0.08, -0.06, 0.04]])
+# a Mach-2 state flowing in through x = 0 (u near 2, a near 1)
+P_SUP = np.array([[1.0, 0.05, -0.03, 0.04, 0.02, -0.03, 0.02, -0.01, 0.02, 0.01],
+                  [2.0, 0.10, 0.08, -0.05, 0.06, 0.03, 0.04, -0.02, 0.01, 0.03],
+                  [0.1, 0.05, 0.12, 0.03, -0.04, 0.05, -0.02, 0.03, 0.02, -0.03],
+                  [3.8, 0.15, -0.10, 0.08, 0.05, -0.06, 0.03, 0.04, -0.03, 0.02]])
+FAR_SUP = (1.1, 1.1 * 2.2, 1.1 * 0.05, 0.75 / 0.4 + 0.5 * 1.1 * (2.2 ** 2 + 0.05 ** 2))
 GAMMA = 1.4
+COEF = {"sub": P, "sup": P_SUP}
 
 
-def poly(x, y, a=0, b=0):
+def poly(x, y, a=0, b=0, coef=None):
     """d^(a+b) P / dx^a dy^b at (x, y): [..., 4]."""
+    coef = P if coef is None else coef
     X, Y = np.asarray(x) - 0.5, np.asarray(y) - 0.5
     out = np.zeros(np.shape(X) + (4,))
     for k, (p, q) in enumerate(MONO):
         if p >= a and q >= b:
             t = (factorial(p) / factorial(p - a) * factorial(q) / factorial(q - b) *
                  X ** (p - a) * Y ** (q - b))
-            out += t[..., None] * P[:, k]
+            out += t[..., None] * coef[:, k]
     return out
 
 
-def exact_state(m, s):
+def exact_state(m, s, coef=None):
     """Exact cell means (degree-3 triangle rule over a fan) and coefficients."""
     xy = m["node_xy"].reshape(-1, 2)
     cp, c2n = m["cell_ptr"], m["cell2node"]
@@ -65,8 +73,8 @@ def exact_state(m, s):
             A, B, C = pts[0], pts[k], pts[k + 1]
             ar = 0.5 * ((B[0] - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (B[1] - A[1]))
             q = (np.array([A, B, C]), np.array([(A + B) / 2, (B + C) / 2, (C + A) / 2]), (A + B + C) / 3)
-            val = (poly(q[0][:, 0], q[0][:, 1]).sum(0) / 20 + poly(q[1][:, 0], q[1][:, 1]).sum(0) * 2 / 15 +
-                   poly(q[2][0], q[2][1]) * 9 / 20)
+            val = (poly(q[0][:, 0], q[0][:, 1], coef=coef).sum(0) / 20 +
+                   poly(q[1][:, 0], q[1][:, 1], coef=coef).sum(0) * 2 / 15 + poly(q[2][0], q[2][1], coef=coef) * 9 / 20)
             tot += val * ar
             area += ar
         u[i] = tot / area
@@ -74,7 +82,7 @@ def exact_state(m, s):
     c = s["cell_bary"].reshape(-1, 2)
     urec = np.zeros((n, 36))
     for r, (a, b) in enumerate(MONO[1:]):
-        d = poly(c[:, 0], c[:, 1], a, b)
+        d = poly(c[:, 0], c[:, 1], a, b, coef=coef)
         for v in range(4):
             urec[:, r + 9 * v] = d[:, v] * sc[:, 0] ** a * sc[:, 1] ** b / (factorial(a) * factorial(b))
     return u, urec
@@ -108,7 +116,12 @@ def viscous_flux(U, G, mu_gas, pr, R, tref, cs, wall=None, n=None):
     return F
 
 
-def expected_viscous_rhs(m, s, consts):
+def expected_viscous_rhs(m, s, consts, coef=None, far=None):
+    """Inner and wall faces as the module docstring says; a farfield face
+    where the flow enters supersonically has the far state as its ghost, so
+    its face state is the mean of the inner value and the far state and its
+    gradient the inner one plus n (U_far - U) / distGRP, distGRP = 2 V / A
+    (DNDS_FV_EulerEvaluator.cpp:356-395)."""
     xy = m["node_xy"].reshape(-1, 2)
     f2n, f2c, bc = m["face2node"].reshape(-1, 2), m["face2cell"].reshape(-1, 2), m["face_bc"]
     vol = s["volume"]
@@ -121,8 +134,13 @@ def expected_viscous_rhs(m, s, consts):
         nrm = np.array([d[1], -d[0]])
         nl = np.linalg.norm(nrm)
         pts = 0.5 * (a + b)[None, :] + xi[:, None] * d[None, :]
-        U = poly(pts[:, 0], pts[:, 1])
-        G = np.stack([poly(pts[:, 0], pts[:, 1], 1, 0), poly(pts[:, 0], pts[:, 1], 0, 1)], 1)
+        U = poly(pts[:, 0], pts[:, 1], coef=coef)
+        G = np.stack([poly(pts[:, 0], pts[:, 1], 1, 0, coef=coef), poly(pts[:, 0], pts[:, 1], 0, 1, coef=coef)], 1)
+        if bc[f] == 2:
+            dist = vol[f2c[f, 0]] / (2 * nl) * 2  # minVol / faceArea * 2, faceArea = 2 |n|
+            jump = np.asarray(far)[None, :] - U
+            G = G + (nrm / nl)[None, :, None] * jump[:, None, :] / dist
+            U = 0.5 * (U + np.asarray(far)[None, :])
         F = viscous_flux(U, G, *consts, wall=int(bc[f]) if bc[f] in (3, 4) else None, n=nrm / nl)
         flux = np.einsum("g,gdv,d->v", w * nl, F, nrm / nl)
         L, R = f2c[f]
@@ -138,19 +156,20 @@ CONSTS = {"unit": (0.2, 0.72, 1.0, 1.0, 0.5),            # mu, Pr, R, TRef, CSut
 
 @pytest.mark.parametrize("kind", [0, 1, 2])
 @pytest.mark.parametrize("consts", ["unit", "air"])
-@pytest.mark.parametrize("walls", [(4, 4, 4, 4), (3, 4, 3, 3)])
-def test_viscous_rhs_of_an_exact_cubic_state(kind, consts, walls):
+@pytest.mark.parametrize("walls,state", [((4, 4, 4, 4), "sub"), ((3, 4, 3, 3), "sub"), ((2, 4, 3, 4), "sup")])
+def test_viscous_rhs_of_an_exact_cubic_state(kind, consts, walls, state):
     mesh, vfv = mesh_and_setup(kind=kind, nx=7, ny=6, periodic=False, jitter=0.2, seed=4, bc=walls)
     m, s = mesh.arrays(), vfv.arrays()
     st = vfv.setting
     mu, pr, R, tref, cs = CONSTS[consts]
+    far = FAR_SUP if state == "sup" else (1.0, 0.0, 0.0, 2.5)
     kw = dict(riemann_solver=2, tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
-              pr_gas=pr, r_gas=R, t_ref=tref, c_sutherland=cs)
+              pr_gas=pr, r_gas=R, t_ref=tref, c_sutherland=cs, far_field=far)
     inv, vis = oracle.Oracle(m, s, mu_gas=0.0, **kw), oracle.Oracle(m, s, mu_gas=mu, **kw)
-    u, urec = exact_state(m, s)
+    u, urec = exact_state(m, s, COEF[state])
     _, _, _, dlf = inv.evaluate_dt(u, 0.5)
     d = vis.evaluate_rhs(u, urec, dlf) - inv.evaluate_rhs(u, urec, dlf)
-    ref = expected_viscous_rhs(m, s, CONSTS[consts])
+    ref = expected_viscous_rhs(m, s, CONSTS[consts], COEF[state], far)
     assert np.abs(ref).max() > 1e-2
     assert rel_err(d, ref) < 1e-10
 
