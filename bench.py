@@ -91,7 +91,7 @@ def main():
                          "6.3M-cell case)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     from dnds_amd import _capi as A
@@ -194,14 +194,14 @@ def main():
         # the per-unit algorithmic figure of the hot-path scope table: 1384 B per
         # face (both sides' stored row-0 bases at 3 points, normals, weights,
         # indices) and 360 B per cell (uRec, u, volume, rhs written)
-        survey_bytes = 1384 * case.mesh.n_face + 360 * nc
-        survey_gbs = survey_bytes / (ms * 1e-3) / 1e9
+        stored_bytes = 1384 * case.mesh.n_face + 360 * nc
+        stored_gbs = stored_bytes / (ms * 1e-3) / 1e9
         rhs_obj = {"ms": ms, "algorithmic_bytes": nbytes, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": gbs / HBM_PEAK_GBS, "kernels": ks,
-                   "stored_basis_bytes": survey_bytes, "achieved_stored_basis": survey_gbs,
-                   "frac_stored_basis": survey_gbs / HBM_PEAK_GBS}
+                   "stored_basis_bytes": stored_bytes, "achieved_stored_basis": stored_gbs,
+                   "frac_stored_basis": stored_gbs / HBM_PEAK_GBS}
         log(f"EvaluateRHS: {ms:.3f} ms, {nbytes / 1e9:.2f} GB, {gbs:.0f} GB/s = {gbs / HBM_PEAK_GBS:.2f} of peak; "
-            f"{survey_bytes / 1e9:.2f} GB with stored bases: {survey_gbs / HBM_PEAK_GBS:.2f}")
+            f"{stored_bytes / 1e9:.2f} GB with stored bases: {stored_gbs / HBM_PEAK_GBS:.2f}")
 
     cpu = None
     if args.cpu_baseline:

@@ -1,8 +1,8 @@
 // Learned warm start of the VR reconstruction (gfx950).  No counterpart in
 // the reference: a small MLP maps a cell's stencil of cell means and its
 // geometry moments to an initial guess of its 9 x 4 rec coefficients; plain
-// Jacobi sweeps then iterate to the same tolerance, so the method stays the
-// judge and the fallback.
+// Jacobi sweeps then iterate to the same tolerance, so the method decides the
+// result and remains the fallback.
 //
 // Network (58 -> 128 -> 128 -> 36, GELU): inputs are the mean differences
 // u_j - u_i of 12 stencil cells (per variable divided by s_v, the largest of

@@ -420,7 +420,8 @@ int dndsmi_bdf_step(dndsmi_bdf *bdf, dndsmi_array *u, dndsmi_array *uRec, dndsmi
  * 0.5 log(volume).  Outputs: rec coefficients [r + 9 v] divided by s_v.
  * Training and inference run on the device with BF16 matrix-core GEMMs (FP32
  * accumulation) and Adam.  Plain Jacobi sweeps continue from the guess to
- * the caller's tolerance, so the method stays the judge and the fallback. */
+ * the caller's tolerance, so the method decides the result and remains the
+ * fallback. */
 typedef struct dndsmi_warmstart dndsmi_warmstart;
 
 /* capacity = training samples (cells) the set can hold; seed for the weights
