@@ -222,8 +222,9 @@ def main():
             timed[threads] = time.perf_counter() - c0
             log(f"CPU baseline {nc * 3 / timed[threads]:.3e} cell-updates/s ({timed[threads]:.1f}s)")
         cpu = {"value": nc * 3 / timed[args.threads], "unit": "cell-updates/s", "cores": args.threads, "kind": "port",
-               "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps + WBAP + RHS) on the same {nc}-cell "
-                         f"mesh: {timed[args.threads]:.1f}s on {args.threads} threads, {timed[1]:.1f}s on 1",
+               "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps{' + WBAP' if case.limiter else ''} + RHS) "
+                         f"on the same {nc}-cell mesh: {timed[args.threads]:.3g} s on {args.threads} threads, "
+                         f"{timed[1]:.3g} s on 1",
                "value_1_core": nc * 3 / timed[1]}
 
     implicit = None
