@@ -193,7 +193,7 @@ def mach3_step(n=2048, seed=5, jitter=0.15, x_step=0.5, noise=1e-3, nthreads=0) 
     stands at x = 0.5: left (rho, u, v, p) = (3.857, 2.629, 0, 10.33), right
     (1, 0, 0, 1), with seeded density noise of relative amplitude 1e-3 (u, p
     kept).  Far-field inflow on the left (the left state, supersonic), walls
-    (Wall_Euler) elsewhere.  HLLC, WBAP V3, SSP-RK3, 100 steps."""
+    (Wall_Euler) elsewhere.  HLLC, WBAP, SSP-RK3, 100 steps."""
     bcs = (A.BC_FARFIELD, A.BC_WALL_EULER, A.BC_WALL_EULER, A.BC_WALL_EULER)
     mesh = CompactFacedMeshSerialRW.generate(A.MESH_MIXED, n, n, 0.0, 4.0, 0.0, 1.0, jitter=jitter, seed=seed, bc=bcs)
     vfv = VRFiniteVolume2D(mesh, vr_setting(**dict(RIEMANN_VR_SETTING, far_weight=1.0), build_limiter_data=1))

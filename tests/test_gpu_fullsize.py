@@ -1,9 +1,13 @@
-"""The 2,097,152-cell configuration at its full size on the GPU.
+"""The 2,097,152-cell and 6,293,459-cell configurations at full size on the GPU.
 
 * One FP64 SSP-RK3 step of the four-quadrant Riemann problem (1024 x 1024
   quads cut into triangles, jitter 0.2h, HLLC, WBAP V2, 5 sweeps per stage)
   against the oracle on the host's threads: <= 1e-12 relative.
-* The periodic isentropic vortex on a mesh of the same size: SSP-RK3 with a
+* One FP64 SSP-RK3 step of the Mach-3 step on the largest configuration:
+  [0,4] x [0,1], 2048 x 2048 quads half cut into triangles (6,293,459 cells,
+  the quads on the 4-slot kernel instances), HLLC, WBAP V2, farfield and
+  Wall_Euler faces: <= 1e-12 relative to the oracle.
+* The periodic isentropic vortex on a mesh of the 2.1M-cell size: SSP-RK3 with a
   global time step keeps the totals sum V u of all four variables (every
   face flux enters its two cells with opposite signs, and the flux-balance
   terms of a closed straight-edged cell sum to zero), so after 10 steps they
@@ -38,6 +42,29 @@ def test_riemann_step_at_full_size_matches_oracle():
     assert abs(t - rt) <= 1e-12 * rt
     assert rel_err(du.download(), ru) <= 1e-12
     assert rel_err(dr.download(), rr) <= 1e-12
+
+
+def test_mach3_step_at_full_size_matches_oracle():
+    import oracle
+    from dnds_amd import cases
+    from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
+    case = cases.mach3_step(2048, nthreads=16)
+    assert case.n_cell > 6_000_000
+    ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType="HLLC", farFieldStaticValue=case.far)
+    urec0 = np.zeros((case.n_cell, 36))
+    du, dr, dn = ev.BuildMean().upload(case.u0), case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+    t = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=5, CFL=0.5, useLimiter=True).Step(du, dr, dn, 0.0)
+    got_u, got_r = du.download(), dr.download()
+    del du, dr, dn, ev
+    st = case.vfv.setting
+    orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2, far_field=case.far,
+                        tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
+                        wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale)
+    oracle.set_threads(16)
+    ru, rr, rt = orc.ssprk3_steps(case.u0, urec0, 1, 5, 0.5, use_limiter=True)
+    assert abs(t - rt) <= 1e-12 * rt
+    assert rel_err(got_u, ru) <= 1e-12
+    assert rel_err(got_r, rr) <= 1e-12
 
 
 def test_periodic_vortex_at_full_size_conserves_totals():
