@@ -113,6 +113,7 @@ SIGNATURES = [
     ("dndsmi_mesh_write_plt", C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), PD, C.c_double,
                                         C.c_char_p, C.c_size_t]),
     ("dndsmi_mesh_get_view", C.c_int, [C.c_void_p, C.POINTER(MeshView)]),
+    ("dndsmi_mesh_view_check", C.c_int, [C.POINTER(MeshView), C.c_char_p, C.c_size_t]),
     ("dndsmi_mesh_destroy", None, [C.c_void_p]),
     ("dndsmi_vr_setting_default", None, [C.POINTER(VRSetting)]),
     ("dndsmi_vr_create", C.c_int, [C.POINTER(MeshView), C.POINTER(VRSetting), C.c_int32, C.POINTER(C.c_void_p)]),

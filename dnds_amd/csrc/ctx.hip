@@ -1328,7 +1328,12 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     if (!mesh || !vr || !vs || !es || !out)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     *out = nullptr;
-    if (vr->n_cell != mesh->n_cell || vr->n_face != mesh->n_face || mesh->n_cell < 1 ||
+    {
+        std::string why;
+        if (check_mesh_view(mesh, why) != DNDSMI_OK)
+            return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
+    if (vr->n_cell != mesh->n_cell || vr->n_face != mesh->n_face || vr->n_c2f != mesh->cell_ptr[mesh->n_cell] ||
         mesh->n_cell > (int64_t(1) << 30) || mesh->n_face > (int64_t(1) << 30))
         return DNDSMI_ERR_INVALID_ARGUMENT;
     if (!vr->face_gauss_xy || !vr->cell_scale || !vr->base_moment || !vr->face_d0)

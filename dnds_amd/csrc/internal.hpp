@@ -26,6 +26,11 @@ int mesh_from_cells(int64_t n_node, const double *node_xy, int64_t n_cell, const
                     const int64_t *cell2node, int64_t n_bnd, const int64_t *bnd2node, const int32_t *bnd_bc,
                     dndsmi_mesh **out, std::string &err);
 
+// dndsmi_mesh_view_check with the failure's description in err: every
+// index in range, cell2face and face2cell / face2cell_ref naming each other,
+// triangles and quadrilaterals only, a known kind on every boundary face.
+int check_mesh_view(const dndsmi_mesh_view *m, std::string &err);
+
 struct dndsmi_vr {
     int64_t n_cell = 0, n_face = 0, n_c2f = 0;
     std::vector<double> volume, face_area, cell_bary, base_moment;

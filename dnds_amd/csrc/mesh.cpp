@@ -8,6 +8,7 @@
 // the local index of the face in each of its cells (DNDS_Mesh.hpp:1330-1361).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -313,6 +314,75 @@ extern "C" int dndsmi_mesh_from_cells(int64_t n_node, const double *node_xy, int
                                       const int32_t *bnd_bc, dndsmi_mesh **out) {
     std::string err;
     return mesh_from_cells(n_node, node_xy, n_cell, cell_ptr, cell2node, n_bnd, bnd2node, bnd_bc, out, err);
+}
+
+int check_mesh_view(const dndsmi_mesh_view *m, std::string &err) {
+    auto bad = [&](const std::string &why) {
+        err = why;
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    };
+    if (!m)
+        return bad("null mesh view");
+    if (!m->node_xy || !m->cell_ptr || !m->cell2node || !m->cell2face || !m->face2node || !m->face2cell ||
+        !m->face2cell_ref || !m->face_bc || !m->face_shift)
+        return bad("null array in the mesh view");
+    const int64_t nn = m->n_node, nc = m->n_cell, nf = m->n_face;
+    if (nn < 3 || nc < 1 || nf < 3)
+        return bad("mesh view without cells");
+    if (m->cell_ptr[0] != 0)
+        return bad("cell_ptr[0] is not 0");
+    auto cell = [](int64_t i) { return "cell " + std::to_string(i); };
+    auto face = [](int64_t f) { return "face " + std::to_string(f); };
+    for (int64_t i = 0; i < nc; i++) {
+        const int64_t b = m->cell_ptr[i], n = m->cell_ptr[i + 1] - b;
+        if (n != 3 && n != 4)
+            return bad(cell(i) + " has " + std::to_string(n) + " faces (triangles and quadrilaterals only)");
+        for (int64_t s = 0; s < n; s++) {
+            const int64_t v = m->cell2node[b + s], f = m->cell2face[b + s];
+            if (v < 0 || v >= nn)
+                return bad(cell(i) + ": node index " + std::to_string(v) + " out of range");
+            if (f < 0 || f >= nf)
+                return bad(cell(i) + ": face index " + std::to_string(f) + " out of range");
+            const bool left = m->face2cell[2 * f] == i && m->face2cell_ref[2 * f] == s;
+            const bool right = m->face2cell[2 * f + 1] == i && m->face2cell_ref[2 * f + 1] == s;
+            if (!left && !right)
+                return bad(cell(i) + " lists " + face(f) + " as its local face " + std::to_string(s) +
+                           ", but the face's face2cell / face2cell_ref do not name it there");
+        }
+    }
+    for (int64_t f = 0; f < nf; f++) {
+        for (int k = 0; k < 2; k++) {
+            const int64_t v = m->face2node[2 * f + k];
+            if (v < 0 || v >= nn)
+                return bad(face(f) + ": node index " + std::to_string(v) + " out of range");
+        }
+        for (int side = 0; side < 2; side++) {
+            const int64_t c = m->face2cell[2 * f + side];
+            if (side == 1 && c == DNDSMI_NO_CELL)
+                break;
+            if (c < 0 || c >= nc)
+                return bad(face(f) + ": cell index " + std::to_string(c) + " out of range");
+            const int64_t ref = m->face2cell_ref[2 * f + side];
+            if (ref < 0 || ref >= m->cell_ptr[c + 1] - m->cell_ptr[c] || m->cell2face[m->cell_ptr[c] + ref] != f)
+                return bad(face(f) + ": face2cell_ref does not point back to the face in " + cell(c));
+        }
+        const int32_t bc = m->face_bc[f];
+        if (m->face2cell[2 * f + 1] == DNDSMI_NO_CELL) {
+            if (bc != DNDSMI_BC_WALL && bc != DNDSMI_BC_FARFIELD && bc != DNDSMI_BC_WALL_EULER &&
+                bc != DNDSMI_BC_WALL_NOSLIP && bc != DNDSMI_BC_SPECIAL_DMRFAR && bc != DNDSMI_BC_SPECIAL_RTFAR &&
+                bc != DNDSMI_BC_SPECIAL_IVFAR && bc != DNDSMI_BC_SPECIAL_2DRIEMANNFAR)
+                return bad(face(f) + ": boundary face with unknown boundary kind " + std::to_string(bc));
+        }
+    }
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_mesh_view_check(const dndsmi_mesh_view *view, char *err, size_t err_len) {
+    std::string why;
+    const int rc = check_mesh_view(view, why);
+    if (rc != DNDSMI_OK && err && err_len)
+        std::snprintf(err, err_len, "%s", why.c_str());
+    return rc;
 }
 
 extern "C" int dndsmi_mesh_get_view(const dndsmi_mesh *m, dndsmi_mesh_view *v) {

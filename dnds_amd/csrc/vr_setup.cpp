@@ -493,6 +493,11 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
         setting->tang_weight_mod_min != 1.0 || setting->weight_scheme_geom < 0 || setting->weight_scheme_geom > 3 ||
         setting->weight_scheme_dir < 0 || setting->weight_scheme_dir > 1)
         return DNDSMI_ERR_UNSUPPORTED;
+    {
+        std::string why;
+        if (check_mesh_view(mesh, why) != DNDSMI_OK)
+            return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
     if (nthreads <= 0)
         nthreads = int32_t(std::max(1u, std::thread::hardware_concurrency()));
     const dndsmi_mesh_view &m = *mesh;

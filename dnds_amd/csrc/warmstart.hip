@@ -474,6 +474,11 @@ extern "C" int dndsmi_warmstart_create(dndsmi_ctx *c, const dndsmi_mesh_view *me
                                        uint64_t seed, int64_t capacity, dndsmi_warmstart **out) {
     if (!c || !mesh || !vr || !out || mesh->n_cell != c->nc || vr->n_cell != c->nc || capacity < 0)
         return DNDSMI_ERR_INVALID_ARGUMENT;
+    {
+        std::string why;
+        if (check_mesh_view(mesh, why) != DNDSMI_OK)
+            return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
     *out = nullptr;
     auto *w = new (std::nothrow) dndsmi_warmstart();
     if (!w)

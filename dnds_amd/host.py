@@ -186,6 +186,22 @@ class ExternalMesh:
                                d("node_xy", np.float64), d("cell_ptr", np.int64), d("cell2node", np.int64),
                                d("cell2face", np.int64), d("face2node", np.int64), d("face2cell", np.int64),
                                d("face2cell_ref", np.int32), d("face_bc", np.int32), d("face_shift", np.float64))
+        sizes = {"node_xy": 2 * self.view.n_node, "cell_ptr": self.view.n_cell + 1, "face2node": 2 * self.view.n_face,
+                 "face2cell": 2 * self.view.n_face, "face2cell_ref": 2 * self.view.n_face,
+                 "face_bc": self.view.n_face, "face_shift": 2 * self.view.n_face}
+        for name, n in sizes.items():
+            if self._keep[name].size != n:
+                raise DNDSError(A.ERR_INVALID_ARGUMENT, f"mesh array {name}: {self._keep[name].size} entries, "
+                                                        f"expected {n}")
+        nc2f = int(self._keep["cell_ptr"][-1])
+        for name in ("cell2node", "cell2face"):
+            if self._keep[name].size != nc2f:
+                raise DNDSError(A.ERR_INVALID_ARGUMENT, f"mesh array {name}: {self._keep[name].size} entries, "
+                                                        f"expected cell_ptr[-1] = {nc2f}")
+        buf = C.create_string_buffer(512)
+        rc = A.load().dndsmi_mesh_view_check(C.byref(self.view), buf, len(buf))
+        if rc != A.OK:
+            raise DNDSError(rc, f"mesh view: {buf.value.decode(errors='replace')}")
         self._arrays = dict(arrays)
 
     @property

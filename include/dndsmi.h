@@ -133,6 +133,15 @@ int dndsmi_mesh_read_gmsh(const char *path, dndsmi_mesh **out, char *err, size_t
 int dndsmi_mesh_write_plt(const dndsmi_mesh *mesh, const char *path, int32_t n_data, const char *const *names,
                           const double *data, double t, char *err, size_t err_len);
 int dndsmi_mesh_get_view(const dndsmi_mesh *mesh, dndsmi_mesh_view *view);
+/* Checks a mesh view before use (no counterpart in the reference, whose
+ * meshes come from its own reader): every node, face and cell index in
+ * range, each cell's cell2face entries and the faces' face2cell /
+ * face2cell_ref naming each other, 3 or 4 faces per cell, and a
+ * DNDSMI_BC_* kind on every boundary face.  dndsmi_vr_create,
+ * dndsmi_ctx_create and dndsmi_warmstart_create run the same check and
+ * return DNDSMI_ERR_INVALID_ARGUMENT on a view that fails it; this entry
+ * also describes the failure in err (may be NULL). */
+int dndsmi_mesh_view_check(const dndsmi_mesh_view *view, char *err, size_t err_len);
 void dndsmi_mesh_destroy(dndsmi_mesh *mesh);
 
 /* ======================================================================== */

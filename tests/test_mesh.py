@@ -126,3 +126,71 @@ def test_from_cells_rejects_boundary_without_kind():
     xy = np.array([[0, 0], [1, 0], [0, 1.0]])
     with pytest.raises(DNDSError):
         CompactFacedMeshSerialRW.from_cells(xy, [0, 3], [0, 1, 2], np.zeros((0, 2), np.int64), np.zeros(0, np.int32))
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("periodic", [False, True])
+def test_mesh_view_check_accepts_generated_meshes(kind, periodic):
+    from dnds_amd import _capi as A
+    from dnds_amd.host import ExternalMesh
+    m = gen(kind, periodic)
+    assert A.load().dndsmi_mesh_view_check(m.view, None, 0) == A.OK
+    ExternalMesh(m.arrays())
+
+
+def _corrupt(fn):
+    def apply(a):
+        a = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in a.items()}
+        fn(a)
+        return a
+    return apply
+
+
+def _inner_face(a):
+    from dnds_amd import _capi as A
+    return int(np.nonzero(a["face2cell"][:, 1] != A.NO_CELL)[0][0])
+
+
+def _boundary_face(a):
+    from dnds_amd import _capi as A
+    return int(np.nonzero(a["face2cell"][:, 1] == A.NO_CELL)[0][0])
+
+
+CORRUPTIONS = [
+    pytest.param(_corrupt(lambda a: a["cell2face"].__setitem__(0, a["n_face"])), "face index", id="face-index"),
+    pytest.param(_corrupt(lambda a: a["cell2node"].__setitem__(1, -1)), "node index -1 out of range",
+                 id="node-index"),
+    pytest.param(_corrupt(lambda a: a["face2node"].__setitem__((2, 0), a["n_node"])), "face 2: node index",
+                 id="face-node"),
+    pytest.param(_corrupt(lambda a: a["face2cell_ref"].__setitem__((0, 0), (a["face2cell_ref"][0, 0] + 1) % 3)),
+                 "do not name it there", id="ref"),
+    pytest.param(_corrupt(lambda a: a["face2cell"].__setitem__((_inner_face(a), 1), a["n_cell"] + 5)),
+                 "do not name it there", id="cell-index"),
+    pytest.param(_corrupt(lambda a: a["face_bc"].__setitem__(_boundary_face(a), 7)), "unknown boundary kind 7",
+                 id="bc-kind"),
+    pytest.param(_corrupt(lambda a: a["cell_ptr"].__setitem__(1, 5)), "cell 0 has 5 faces", id="five-faces"),
+]
+
+
+@pytest.mark.parametrize("corrupt,message", CORRUPTIONS)
+def test_mesh_view_check_names_the_broken_index(corrupt, message):
+    """A caller-owned mesh view is checked before any array built from it
+    reaches the device: each broken index is refused with its description,
+    by the check itself and by dndsmi_vr_create."""
+    import ctypes as C
+    from dnds_amd import _capi as A
+    from dnds_amd.host import DNDSError, ExternalMesh, vr_setting
+    good = gen(0).arrays()
+    bad = corrupt(good)
+    with pytest.raises(DNDSError) as e:
+        ExternalMesh(bad)
+    assert e.value.code == A.ERR_INVALID_ARGUMENT and message in str(e.value), str(e.value)
+    # the same view handed straight to the setup
+    m = ExternalMesh(good)
+    for k, v in bad.items():
+        if isinstance(v, np.ndarray):
+            m._keep[k][:] = v.reshape(-1)
+    out = C.c_void_p()
+    s = vr_setting()
+    assert A.load().dndsmi_vr_create(C.byref(m.view), C.byref(s), 1, C.byref(out)) == A.ERR_INVALID_ARGUMENT
+    assert not out.value
