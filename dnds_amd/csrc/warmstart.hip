@@ -167,16 +167,18 @@ __global__ __launch_bounds__(256) void k_splitk_sum(int64_t n, int32_t nz, const
     C[i] = acc;
 }
 
-// column sums of X [rows][n] (bias gradients), one thread per column, fixed order
-__global__ __launch_bounds__(256) void k_colsum(int32_t rows, int32_t n, const float *__restrict__ X,
-                                                float *__restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// partial column sums of X [rows][n] (bias gradients, n <= 256): block z
+// sums rows [z chunk, (z + 1) chunk) in order, one thread per column
+__global__ __launch_bounds__(256) void k_colsum(int32_t rows, int32_t n, int32_t chunk, const float *__restrict__ X,
+                                                float *__restrict__ part) {
+    const int c = threadIdx.x;
     if (c >= n)
         return;
+    const int32_t r0 = int32_t(blockIdx.x) * chunk, r1 = min(rows, r0 + chunk);
     float s = 0.f;
-    for (int32_t r = 0; r < rows; r++)
+    for (int32_t r = r0; r < r1; r++)
         s += X[int64_t(r) * n + c];
-    out[c] = s;
+    part[int64_t(blockIdx.x) * n + c] = s;
 }
 
 // MSE loss and its gradient: dY = 2 (Y - T) / (rows * 36); per-block sums of (Y - T)^2
@@ -405,6 +407,17 @@ void gemm_wgrad(hipStream_t s, int32_t M, int32_t N, int32_t K, const float *A, 
     hipLaunchKernelGGL(k_splitk_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, nz, part, C);
 }
 
+// column sums of X [rows][n] into out (n <= 256, rows >= 1): the rows split
+// into at most 1024 chunks, partial sums per chunk, then added in chunk order
+// (deterministic); part holds the chunks' sums
+void colsum(hipStream_t s, int32_t rows, int32_t n, const float *X, float *out, float *part) {
+    static_assert(1024 * kHid <= kMaxSplit * kHid * kHid, "the chunk sums fit in the split-K partials");
+    const int32_t chunk = std::max<int32_t>(64, (rows + 1023) / 1024);
+    const int32_t nz = (rows + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_colsum, dim3(unsigned(nz)), dim3(256), 0, s, rows, n, chunk, X, part);
+    hipLaunchKernelGGL(k_splitk_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, int64_t(n), nz, part, out);
+}
+
 // forward pass of `rows` rows of X into Z1, H1, Z2, H2, Y
 void forward(dndsmi_warmstart *w, const float *X, int32_t rows) {
     hipStream_t s = w->ctx->stream;
@@ -425,15 +438,15 @@ int loss_grad(dndsmi_warmstart *w, const float *X, const float *T, int32_t rows,
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, ny, w->Y, T, float(2.0 / double(ny)), w->dY, w->part);
     // layer 3
     gemm_wgrad(s, kHid, kOut, rows, w->H2, kHid, w->dY, kOut, G + oW3, w->kpart);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kOut, w->dY, G + ob3);
+    colsum(s, rows, kOut, w->dY, G + ob3, w->kpart);
     gemm<EPI_GELU_GRAD>(s, rows, kHid, kOut, w->dY, kOut, 0, P + oW3, kOut, 1, w->dZ2, kHid, nullptr, nullptr, w->Z2);
     // layer 2
     gemm_wgrad(s, kHid, kHid, rows, w->H1, kHid, w->dZ2, kHid, G + oW2, w->kpart);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kHid, w->dZ2, G + ob2);
+    colsum(s, rows, kHid, w->dZ2, G + ob2, w->kpart);
     gemm<EPI_GELU_GRAD>(s, rows, kHid, kHid, w->dZ2, kHid, 0, P + oW2, kHid, 1, w->dZ1, kHid, nullptr, nullptr, w->Z1);
     // layer 1
     gemm_wgrad(s, kIn, kHid, rows, X, kIn, w->dZ1, kHid, G + oW1, w->kpart);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, rows, kHid, w->dZ1, G + ob1);
+    colsum(s, rows, kHid, w->dZ1, G + ob1, w->kpart);
     DNDSMI_TRY(w->ctx->launch_check("warm-start backward"));
     if (loss) {
         std::vector<float> h(nb);
