@@ -10,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -98,6 +99,10 @@ bool blank(const std::string &s) { return s.find_first_not_of(" \t\r") == std::s
 
 } // namespace
 
+// node and element counts above this are refused before anything is sized
+// by them (2^40: 16 TiB of node coordinates)
+constexpr int64_t kMaxCount = int64_t(1) << 40;
+
 extern "C" int dndsmi_mesh_read_gmsh(const char *path, dndsmi_mesh **out, char *err, size_t err_len) {
     if (!path || !out) {
         put_err(err, err_len, "null argument");
@@ -160,7 +165,7 @@ extern "C" int dndsmi_mesh_read_gmsh(const char *path, dndsmi_mesh **out, char *
                     return fail("$Nodes not concluded");
                 LineReader h(line);
                 const int64_t n = h.i();
-                if (!h.ok || n < 0)
+                if (!h.ok || n < 0 || n > kMaxCount)
                     return fail("bad node count");
                 xy.assign(size_t(2 * n), 0.0);
                 node_seen.assign(size_t(n), 0);
@@ -190,7 +195,7 @@ extern "C" int dndsmi_mesh_read_gmsh(const char *path, dndsmi_mesh **out, char *
                     return fail("$Elements not concluded");
                 LineReader h(line);
                 const int64_t n = h.i();
-                if (!h.ok || n < 0)
+                if (!h.ok || n < 0 || n > kMaxCount)
                     return fail("bad element count");
                 elems.assign(size_t(n), Element());
                 bool closed = false;
@@ -322,6 +327,8 @@ extern "C" int dndsmi_mesh_read_gmsh(const char *path, dndsmi_mesh **out, char *
     } catch (const std::bad_alloc &) {
         put_err(err, err_len, "out of memory");
         return DNDSMI_ERR_OUT_OF_MEMORY;
+    } catch (const std::length_error &) {
+        return fail("a node or element count too large to hold");
     }
     return DNDSMI_OK;
 }

@@ -163,6 +163,8 @@ def edit(src, dst, old, new):
     ("26  1 2 2 2  1  2", "26  1 2 2 2  8  9", "not on a boundary face"),
     ("26  1 2 2 2  1  2", "26  1 2 2 2  1  99", "outside"),
     ("$EndNodes", "$EndNodez", "not concluded"),
+    ("$Nodes\n36\n", "$Nodes\n99999999999999\n", "bad node count"),
+    ("$Elements\n", "$Elements\n-3\n", "bad element count"),
 ])
 def test_reader_errors(tmp_path, old, new, msg):
     from dnds_amd import _capi as A
