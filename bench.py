@@ -85,10 +85,12 @@ def main():
     ap.add_argument("--mu-gas", type=float, default=0.0,
                     help="muGas > 0 times the laminar Navier-Stokes model on the same workload (Sutherland with "
                          "TRef 1, CSutherland 0.5, Prandtl 0.72, Rgas 1; the mixed-precision line is skipped)")
-    ap.add_argument("--case", default="riemann", choices=["riemann", "vortex", "step"],
+    ap.add_argument("--case", default="riemann", choices=["riemann", "vortex", "step", "plate"],
                     help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale; "
                          "step: Mach-3 step on a mixed tri/quad mesh of [0,4]x[0,1] (n x n quads, --n 2048 for the "
-                         "6.3M-cell case)")
+                         "6.3M-cell case); plate: laminar flat plate at M 0.2, Re 5e6, (1.5 n) x (0.3125 n) quads "
+                         "stretched to y+ = 1 (491,520 at the default n), steady: with --implicit N, N LU-SGS "
+                         "iterations from the free stream at CFL 20 with local time steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1"}))
@@ -96,7 +98,7 @@ def main():
 
     from dnds_amd import _capi as A
     from dnds_amd import cases
-    from dnds_amd.host import EulerEvaluator, ExplicitSSPRK3LocalDt
+    from dnds_amd.host import DNDSError, EulerEvaluator, ExplicitSSPRK3LocalDt
 
     t0 = time.time()
     log(f"building mesh + VR setup, {args.n}x{args.n} quads ({args.case})")
@@ -104,12 +106,14 @@ def main():
         case = cases.riemann_2d(args.n, nthreads=args.threads, build_limiter_data=True)
     elif args.case == "step":
         case = cases.mach3_step(args.n, nthreads=args.threads)
+    elif args.case == "plate":
+        case = cases.flat_plate(args.n * 3 // 2, args.n * 5 // 16, nthreads=args.threads)
     else:
         case = cases.isentropic_vortex(args.n, nthreads=args.threads)
     nc = case.n_cell
     log(f"setup done: {nc} cells, {case.mesh.n_face} faces, {time.time() - t0:.1f}s")
-    gas = dict(muGas=args.mu_gas, prGas=0.72, Rgas=1.0, TRef=1.0, CSutherland=0.5)
-    if args.mu_gas > 0:
+    gas = case.gas or dict(muGas=args.mu_gas, prGas=0.72, Rgas=1.0, TRef=1.0, CSutherland=0.5)
+    if gas["muGas"] > 0:
         args.mixed = 0  # the mixed-precision step covers the Euler model only
     ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType=case.riemann, farFieldStaticValue=case.far, **gas)
     log(f"device context: {ev.device.device_bytes() / 2**30:.2f} GiB")
@@ -211,8 +215,8 @@ def main():
                             far_field=case.far, tang_weight=st.tang_weight,
                             tang_weight_direction=st.tang_weight_direction,
                             wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
-                            limiter_procedure=args.limiter_procedure, mu_gas=args.mu_gas, pr_gas=0.72, r_gas=1.0,
-                            t_ref=1.0, c_sutherland=0.5)
+                            limiter_procedure=args.limiter_procedure, mu_gas=gas["muGas"], pr_gas=gas["prGas"],
+                            r_gas=gas["Rgas"], t_ref=gas["TRef"], c_sutherland=gas["CSutherland"])
         timed = {}
         for threads in (args.threads, 1):
             oracle.set_threads(threads)
@@ -235,13 +239,31 @@ def main():
         dts = ev.BuildScalar()
         dt_exp = ev.EvaluateDt(dts, u, args.cfl, 1e100, False)
         ev.SetLinearSolver(args.gmres, 10, 2)
-        imp = ImplicitEulerDualTimeStep(ev, nInternalRecStep=1, CFL=10.0, nTimeStepInternal=args.implicit,
+        steady = args.case == "plate"
+        # steady: pseudo-time iterations from the free stream (physical dt infinite)
+        imp_cfl, imp_dt = (case.cfl, 1e100) if steady else (10.0, 5 * dt_exp)
+        # steady: chunks of 100 iterations (with an infinite dt, the next chunk
+        # continues the same pseudo-time iteration), so that the history up to
+        # a failure is kept
+        chunk = min(args.implicit, 100) if steady else args.implicit
+        imp = ImplicitEulerDualTimeStep(ev, nInternalRecStep=1, CFL=imp_cfl, nTimeStepInternal=chunk,
                                         rhsThresholdInternal=0.0, UseLocaldt=True)
-        imp.Step(u, r, rn, dt=5 * dt_exp)  # warm-up step
+        if not steady:
+            imp.Step(u, r, rn, dt=imp_dt)  # warm-up step
         ev.device.sync()
         lib.dndsmi_profile_enable(ev.device.ctx, 1)
         ts = time.perf_counter()
-        hist = imp.Step(u, r, rn, dt=5 * dt_exp)
+        parts, failure = [], None
+        while sum(len(h) for h in parts) < args.implicit:
+            try:
+                parts.append(imp.Step(u, r, rn, dt=imp_dt))
+            except DNDSError as e:
+                if not steady:
+                    raise
+                failure = f"after {sum(len(h) for h in parts)} iterations: {e}"
+                log(f"implicit: stopped {failure}")
+                break
+        hist = np.concatenate(parts)
         ev.device.sync()
         el = time.perf_counter() - ts
         iprof = {}
@@ -263,7 +285,12 @@ def main():
                                       "LU-SGS then GMRES(10) x 2 restarts"][args.gmres],
                     "inner_iterations": len(hist), "ms_per_iteration": el / len(hist) * 1e3,
                     "cell_iterations_per_s": nc * len(hist) / el, "colours": ncol,
-                    "increment_drop": float(hist[-1, 0] / hist[0, 0]), "kernels": iprof}
+                    "increment_drop": float(hist[-1, 0] / hist[0, 0]), "cfl": imp_cfl,
+                    "dt": "steady (infinite)" if steady else imp_dt, "kernels": iprof}
+        if steady:
+            implicit["increment_history"] = {k: float(hist[k, 0]) for k in
+                                             sorted({0, *range(0, len(hist), max(1, len(hist) // 20)), len(hist) - 1})}
+            implicit["failure"] = failure
         lus = iprof.get("k_lusgs_sweep", {})
         log(f"implicit: {implicit['ms_per_iteration']:.3f} ms per inner iteration, {ncol} colours, "
             f"LU-SGS {lus.get('ms_per_sweep', 0):.3f} ms per sweep, {lus.get('gbs', 0):.0f} GB/s algorithmic")
@@ -308,10 +335,12 @@ def main():
                                   f"p=3 VR, Roe, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage, no limiter"),
                        "step": (f"Mach-3 step, [0,4]x[0,1], {args.n}x{args.n} quads half split: {nc} mixed cells, "
                                 f"jitter 0.15h, p=3 VR, HLLC, WBAP {wbap}, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
-                                "sweeps/stage")}[args.case],
+                                "sweeps/stage"),
+                       "plate": (f"laminar flat plate, M 0.2, Re 5e6, {nc} quads stretched to y+ = 1, p=3 VR, Roe + "
+                                 f"viscous, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage")}[args.case],
                    "cells": nc, "sweeps_per_stage": args.sweeps,
                    "limiter": f"WBAP {wbap}" if case.limiter else "none", "precision": "fp64",
-                   "model": "laminar Navier-Stokes, muGas %g" % args.mu_gas if args.mu_gas > 0 else "Euler"},
+                   "model": "laminar Navier-Stokes, muGas %g" % gas["muGas"] if gas["muGas"] > 0 else "Euler"},
         "kernels": prof,
         "roofline": roof,
         "evaluate_rhs": rhs_obj,

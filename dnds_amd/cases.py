@@ -8,6 +8,9 @@
   2D Riemann configuration (data/euler_config_2DRiemann.json: far-field
   state plus three box initializers, applied in order at cell barycentres;
   Special_2DRiemannFar boundaries, split at x = y = 0.8).
+* `bump_channel`, `flat_plate` -- steady cases for the implicit solver
+  (Euler over a bump; laminar Navier-Stokes over a flat plate).
+* `mach3_step` -- the Mach-3 step on a mixed tri/quad mesh.
 """
 from __future__ import annotations
 
@@ -103,8 +106,9 @@ def riemann_state(bary: np.ndarray) -> np.ndarray:
 class Case:
     """A mesh, its VR setup, the Euler settings and the initial state."""
 
-    def __init__(self, name, mesh, vfv, riemann, far, u0, steps, sweeps, cfl, limiter):
+    def __init__(self, name, mesh, vfv, riemann, far, u0, steps, sweeps, cfl, limiter, gas=None):
         self.name = name
+        self.gas = dict(gas or {})  # EulerEvaluator keyword arguments of the viscous model
         self.mesh = mesh
         self.vfv = vfv
         self.riemann = riemann
@@ -176,6 +180,50 @@ def bump_channel(nx=48, ny=16, kind=A.MESH_TRI_SPLIT, height=0.1, mach=0.5, seed
     far = bump_far(mach)
     u0 = np.tile(np.array(far), (mesh.n_cell, 1))
     return Case("bump_channel", mesh, vfv, "Roe", far, u0, 0, 1, 10.0, False)
+
+
+def flat_plate(nx=1536, ny=320, mach=0.2, reynolds=5e6, y_plus=1.0, height=1.0, upstream=0.0, nthreads=0) -> Case:
+    """Laminar flat plate (the NS_2D model: the 2D solver has no turbulence
+    model): [0,1] x [0,height], nx x ny quads, the plate (Wall_NoSlip) along
+    y = 0 from the leading edge at x = 0, Farfield on the other three sides.
+    Free stream rho = 1, sound speed 1, u = mach; Re = rho u L / mu per unit
+    length, mu from Sutherland's law with TRef the free-stream temperature and
+    CSutherland 110.4 K / 273.15 K of it (air), Prandtl 0.72, Rgas 1.  The y
+    lines are spaced geometrically (one ratio for the whole height) so that
+    the first cell's height puts the wall at y+ = y_plus for the turbulent
+    skin friction cf = 0.0576 Re^-0.2 at x = 1.  Roe flux, no limiter, VR
+    weights as the bump channel; initial state the free stream.  1 Jacobi
+    sweep per iteration; the implicit LU-SGS solver with local time steps at
+    CFL 20 drives it to steady state."""
+    far = bump_far(mach)
+    mu = mach / reynolds
+    u_tau = mach * np.sqrt(0.5 * 0.0576 * reynolds ** -0.2)
+    h0 = y_plus * mu / u_tau
+    # ratio r with h0 (r^ny - 1) / (r - 1) = height, by bisection
+    lo, hi = 1.0 + 1e-12, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        if h0 * (r ** ny - 1.0) / (r - 1.0) > height:
+            hi = r
+        else:
+            lo = r
+    bcs = (A.BC_FARFIELD, A.BC_FARFIELD, A.BC_WALL_NOSLIP, A.BC_FARFIELD)
+    mesh = CompactFacedMeshSerialRW.generate(A.MESH_QUAD, nx, ny, -upstream, 1.0, 0.0, height, y_ratio=r, bc=bcs)
+    if upstream > 0:
+        # symmetry (Wall_Euler) ahead of the leading edge
+        m = mesh.arrays()
+        bnd = m["face2cell"][:, 1] == A.NO_CELL
+        f2n, bc = m["face2node"][bnd], m["face_bc"][bnd].copy()
+        xm = 0.5 * (m["node_xy"][f2n[:, 0], 0] + m["node_xy"][f2n[:, 1], 0])
+        bc[(bc == A.BC_WALL_NOSLIP) & (xm < 0.0)] = A.BC_WALL_EULER
+        mesh = CompactFacedMeshSerialRW.from_cells(m["node_xy"], m["cell_ptr"], m["cell2node"], f2n, bc)
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**dict(RIEMANN_VR_SETTING, far_weight=1.0, wall_weight=0.5),
+                                            build_limiter_data=0))
+    vfv.Initialization(nthreads)
+    t_inf = (far[3] - 0.5 * mach * mach) * (GAMMA - 1.0)  # p / (Rgas rho) with rho = Rgas = 1
+    gas = dict(muGas=mu, prGas=0.72, Rgas=1.0, TRef=t_inf, CSutherland=110.4 / 273.15 * t_inf)
+    u0 = np.tile(np.array(far), (mesh.n_cell, 1))
+    return Case("flat_plate", mesh, vfv, "Roe", far, u0, 0, 1, 20.0, False, gas)
 
 
 MACH3_LEFT = (3.857, 2.629, 0.0, 10.33)   # (rho, u, v, p) behind the Mach-3 shock
