@@ -1338,6 +1338,9 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         return DNDSMI_ERR_INVALID_ARGUMENT;
     if (!vr->face_gauss_xy || !vr->cell_scale || !vr->base_moment || !vr->face_d0)
         return DNDSMI_ERR_INVALID_ARGUMENT; // the device evaluates the face bases from these
+    if (!vr->volume || !vr->face_area || !vr->face_center || !vr->face_norm_center || !vr->face_norm ||
+        !vr->face_weight || !vr->ainv || !vr->ainv_b || !vr->ainv_bvec)
+        return DNDSMI_ERR_INVALID_ARGUMENT; // matrix_secondary alone may be absent (no WBAP then)
     if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC)
         return DNDSMI_ERR_UNSUPPORTED;
     if (!(es->mu_gas >= 0) ||

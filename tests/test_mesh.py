@@ -194,3 +194,23 @@ def test_mesh_view_check_names_the_broken_index(corrupt, message):
     s = vr_setting()
     assert A.load().dndsmi_vr_create(C.byref(m.view), C.byref(s), 1, C.byref(out)) == A.ERR_INVALID_ARGUMENT
     assert not out.value
+
+
+@pytest.mark.parametrize("field", ["volume", "face_norm", "ainv", "ainv_b", "ainv_bvec", "face_gauss_xy"])
+def test_context_refuses_a_setup_view_without_a_required_array(field):
+    """dndsmi_ctx_create checks the caller's setup view before anything is
+    read from it or allocated on the device: a missing array it needs is an
+    invalid argument (matrix_secondary and face_dibj_g0 alone may be absent).
+    Host-only: the check returns before any device call."""
+    import ctypes as C
+    from conftest import mesh_and_setup
+    from dnds_amd import _capi as A
+    mesh, vfv = mesh_and_setup(nx=4, ny=3)
+    vv = A.VRView()
+    C.memmove(C.byref(vv), C.byref(vfv.view), C.sizeof(A.VRView))
+    setattr(vv, field, A.PD())
+    es = A.EulerSetting(riemann_solver=A.RS_ROE, gamma=1.4, far_field=(C.c_double * 4)(1.0, 0.3, 0.0, 2.6))
+    out = C.c_void_p()
+    rc = A.load().dndsmi_ctx_create(C.byref(mesh.view), C.byref(vv), C.byref(vfv.setting), C.byref(es), None,
+                                    C.byref(out))
+    assert rc == A.ERR_INVALID_ARGUMENT and not out.value
