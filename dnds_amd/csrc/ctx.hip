@@ -1733,7 +1733,7 @@ extern "C" void dndsmi_array_destroy(dndsmi_array *a) {
 }
 
 extern "C" int dndsmi_array_upload(dndsmi_array *a, const double *host) {
-    if (!a || !host)
+    if (!a || !host || !a->ctx) // no context: it was destroyed before the array
         return DNDSMI_ERR_INVALID_ARGUMENT;
     dndsmi_ctx *c = a->ctx;
     const int64_t nc = c->nc, ldc = c->ldc;
@@ -1754,7 +1754,7 @@ extern "C" int dndsmi_array_upload(dndsmi_array *a, const double *host) {
 }
 
 extern "C" int dndsmi_array_download(const dndsmi_array *a, double *host) {
-    if (!a || !host)
+    if (!a || !host || !a->ctx)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     dndsmi_ctx *c = a->ctx;
     const int64_t nc = c->nc, ldc = c->ldc;
@@ -1781,7 +1781,7 @@ extern "C" int dndsmi_array_download(const dndsmi_array *a, double *host) {
 }
 
 extern "C" int dndsmi_array_copy(dndsmi_array *dst, const dndsmi_array *src) {
-    if (!dst || !src || dst->ctx != src->ctx || dst->kind != src->kind)
+    if (!dst || !src || !dst->ctx || dst->ctx != src->ctx || dst->kind != src->kind)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     return dst->ctx->hip(hipMemcpyAsync(dst->d, src->d, sizeof(double) * dst->elems, hipMemcpyDeviceToDevice,
                                         dst->ctx->stream),
@@ -1789,7 +1789,7 @@ extern "C" int dndsmi_array_copy(dndsmi_array *dst, const dndsmi_array *src) {
 }
 
 extern "C" int dndsmi_array_set_constant(dndsmi_array *a, double value) {
-    if (!a)
+    if (!a || !a->ctx)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     hipLaunchKernelGGL(k_set, dim3(grid_for(a->elems)), dim3(kBlock), 0, a->ctx->stream, a->elems, a->d, value);
     return a->ctx->launch_check("k_set");

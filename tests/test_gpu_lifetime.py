@@ -3,6 +3,7 @@ and warm starts destroyed after their context must not touch it (the
 context frees the arrays' storage and detaches them)."""
 import ctypes as C
 
+import numpy as np
 import pytest
 
 from conftest import mesh_and_setup
@@ -27,6 +28,12 @@ def test_arrays_and_warm_start_outlive_their_context():
     assert lib.dndsmi_warmstart_create(ctx, C.byref(mesh.view), C.byref(vfv.view), 1, 0, C.byref(ws)) == 0
     lib.dndsmi_array_destroy(arrays.pop())  # one before the context, the rest after
     lib.dndsmi_ctx_destroy(ctx)
+    # an array whose context is gone refuses every call but destroy
+    host = np.zeros(4 * mesh.n_cell)
+    assert lib.dndsmi_array_upload(arrays[0], A.dptr(host)) == A.ERR_INVALID_ARGUMENT
+    assert lib.dndsmi_array_download(arrays[0], A.dptr(host)) == A.ERR_INVALID_ARGUMENT
+    assert lib.dndsmi_array_set_constant(arrays[0], 1.0) == A.ERR_INVALID_ARGUMENT
+    assert lib.dndsmi_array_copy(arrays[1], arrays[1]) == A.ERR_INVALID_ARGUMENT
     for h in arrays:
         lib.dndsmi_array_destroy(h)
     lib.dndsmi_warmstart_destroy(ws)
