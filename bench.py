@@ -159,13 +159,23 @@ def main():
         return elapsed, prof, uf, r
 
     elapsed, prof, u_fp64, r = timed_run(False)
+    copy = None
     if args.copy_probe:
         rn = case.vfv.BuildRec()
         nbytes = 36 * ((nc + 63) // 64 * 64) * 8
         log(f"copy probe: {args.copy_probe} device copies of {nbytes} bytes")
+        rn.copy_from(r)
+        ev.device.sync()
+        ts = time.perf_counter()
         for _ in range(args.copy_probe):
             rn.copy_from(r)
         ev.device.sync()
+        el = time.perf_counter() - ts
+        # read + write of each copy, wall clock over back-to-back copies on the library's stream
+        copy = {"copies": args.copy_probe, "bytes_per_copy": 2 * nbytes,
+                "gbs": 2 * nbytes * args.copy_probe / el / 1e9, "frac_of_peak": 2 * nbytes * args.copy_probe / el / 1e9
+                / HBM_PEAK_GBS}
+        log(f"copy probe: {copy['gbs']:.0f} GB/s read + write ({copy['frac_of_peak']:.2f} of peak)")
     mixed = None
     if args.mixed:
         m_elapsed, m_prof, u_mixed, _ = timed_run(True)
@@ -345,6 +355,7 @@ def main():
         "roofline": roof,
         "evaluate_rhs": rhs_obj,
         "cpu_baseline": cpu,
+        "copy_probe": copy,
         "mixed_precision": mixed,
         "implicit_lusgs": implicit,
         "learned_warm_start": warm,
