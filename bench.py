@@ -64,9 +64,10 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1), help="host threads for the setup")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on one step (0 = skip)")
-    ap.add_argument("--copy-probe", type=int, default=0,
+    ap.add_argument("--copy-probe", type=int, default=10,
                     help="after the timed steps, N device copies of one rec array (a plain copy of known size "
-                         "for scaling profiler byte counters)")
+                         "for scaling profiler byte counters), timed: the copy_probe object of the JSON line "
+                         "(0 = skip)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="also time the mixed-precision step (FP32 correction sweeps + FP64 refinement) "
                          "from the same initial state and report it beside the FP64 line")
