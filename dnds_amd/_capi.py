@@ -161,7 +161,8 @@ SIGNATURES = [
     ("dndsmi_warmstart_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("dndsmi_reconstruct_to_tolerance", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                                   C.c_int32, C.c_double, PI32, PD]),
-    ("dndsmi_lusgs_matrix_init", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double]),
+    ("dndsmi_lusgs_matrix_init", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                           C.c_int32, C.c_double]),
     ("dndsmi_update_lusgs_forward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p]),
     ("dndsmi_update_lusgs_backward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,

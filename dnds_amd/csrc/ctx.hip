@@ -2085,10 +2085,11 @@ extern "C" int dndsmi_reconstruction_jacobi_step(dndsmi_ctx *c, const dndsmi_arr
         !same_ctx(c, uRecNewBuf, DNDSMI_ARRAY_REC) || uRec == uRecNewBuf)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNewBuf->d, t));
-    // uRec = uRecNewBuf (DNDS_FV_VR.hpp:1707-1711) by exchanging the storage:
-    // uRec holds the new iterate, uRecNewBuf the previous one.
-    std::swap(uRec->d, uRecNewBuf->d);
-    return DNDSMI_OK;
+    // uRec = uRecNewBuf (DNDS_FV_VR.hpp:1707-1711): afterwards both arrays
+    // hold the new iterate, as in the reference.
+    return c->hip(hipMemcpyAsync(uRec->d, uRecNewBuf->d, sizeof(double) * uRec->elems, hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "copy uRecNewBuf to uRec");
 }
 
 extern "C" int dndsmi_evaluate_dt(dndsmi_ctx *c, dndsmi_array *dt, const dndsmi_array *u, double CFL,

@@ -686,9 +686,17 @@ extern "C" int dndsmi_lusgs_colouring(const dndsmi_ctx *c, int32_t *colour, int3
     return DNDSMI_OK;
 }
 
-extern "C" int dndsmi_lusgs_matrix_init(dndsmi_ctx *c, const dndsmi_array *dTau, double dt, double alphaDiag) {
-    if (!c || !same_ctx(c, dTau, DNDSMI_ARRAY_SCALAR))
+extern "C" int dndsmi_lusgs_matrix_init(dndsmi_ctx *c, const dndsmi_array *dTau, double dt, double alphaDiag,
+                                        const dndsmi_array *u, const dndsmi_array *uRec, int32_t jacobianCode,
+                                        double t) {
+    (void)t; // the NS_2D diagonal has no time-dependent source term
+    if (!c || !same_ctx(c, dTau, DNDSMI_ARRAY_SCALAR) || (u && !same_ctx(c, u, DNDSMI_ARRAY_DOF)) ||
+        (uRec && !same_ctx(c, uRec, DNDSMI_ARRAY_REC)))
         return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (jacobianCode != 0) {
+        c->err = "LUSGSMatrixInit: only jacobianCode 0 (the analytic diagonal) is implemented";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
     return launch_lusgs_init(c, dTau->d, dt, alphaDiag);
 }
 

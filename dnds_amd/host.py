@@ -295,8 +295,8 @@ class VRFiniteVolume2D:
         return ArrayRecV(self.device)
 
     def ReconstructionJacobiStep(self, u: "ArrayDOFV", uRec: "ArrayRecV", uRecNewBuf: "ArrayRecV", t: float = 0.0):
-        """One Jacobi sweep with SOR_Instead = false; afterwards uRec holds the
-        new iterate and uRecNewBuf the previous one (its storage is swapped)."""
+        """One Jacobi sweep with SOR_Instead = false into uRecNewBuf, then
+        uRec = uRecNewBuf (DNDS_FV_VR.hpp:1707-1711): both hold the new iterate."""
         d = self.device
         _check(_lib().dndsmi_reconstruction_jacobi_step(d.ctx, u.h, uRec.h, uRecNewBuf.h, t), "Jacobi step", d.ctx)
 
@@ -474,9 +474,16 @@ class EulerEvaluator:
                                     self.vfv.setting.wbap_smooth_indicator_scale)
         self.mesh.PrintSerialPartPltBinaryDataArray(fname, names, data, t)
 
-    def EvaluateDt(self, dt: ArrayScalar, u: ArrayDOFV, CFL: float, MaxDt: float = 1.0,
+    def EvaluateDt(self, dt: "ArrayScalar | list", u: ArrayDOFV, CFL: float, MaxDt: float = 1.0,
                    UseLocaldt: bool = False) -> float:
-        """Returns dtMinall (the reference's output argument)."""
+        """Returns dtMinall (the reference's output argument).  dt is a device
+        ArrayScalar, or a Python list (the reference's std::vector<real>&,
+        DNDS_FV_EulerEvaluator.hpp:1408-1411), which is refilled on the host."""
+        if isinstance(dt, list):
+            d = self.BuildScalar()
+            dtmin = self.EvaluateDt(d, u, CFL, MaxDt, UseLocaldt)
+            dt[:] = d.download().tolist()
+            return dtmin
         dtmin = C.c_double()
         _check(_lib().dndsmi_evaluate_dt(self.device.ctx, dt.h, u.h, CFL, C.byref(dtmin), MaxDt, int(UseLocaldt)),
                "EvaluateDt", self.device.ctx)
@@ -499,10 +506,17 @@ class EulerEvaluator:
         col, _ = self.LUSGSColouring()
         return np.argsort(col, kind="stable").astype(np.int64)
 
-    def LUSGSMatrixInit(self, dTau: ArrayScalar, dt: float, alphaDiag: float):
-        """Diagonal from lambdaFace of the last EvaluateDt (jacobianCode 0)."""
-        _check(_lib().dndsmi_lusgs_matrix_init(self.device.ctx, dTau.h, dt, alphaDiag), "LUSGSMatrixInit",
-               self.device.ctx)
+    def LUSGSMatrixInit(self, dTau: ArrayScalar, dt: float, alphaDiag: float, u: "ArrayDOFV | None" = None,
+                        uRec: "ArrayRecV | None" = None, jacobianCode: int = 0, t: float = 0.0):
+        """Diagonal from lambdaFace of the last EvaluateDt; u, uRec and t feed
+        only the source-term Jacobian, absent in NS_2D; jacobianCode != 0 is
+        unsupported (DNDSError).  dTau may be a host sequence (the reference's
+        std::vector<real>&), uploaded for the call."""
+        if not isinstance(dTau, ArrayScalar):
+            dTau = self.BuildScalar().upload(np.asarray(dTau, dtype=np.float64))
+        _check(_lib().dndsmi_lusgs_matrix_init(self.device.ctx, dTau.h, dt, alphaDiag,
+                                               u.h if u is not None else None, uRec.h if uRec is not None else None,
+                                               int(jacobianCode), t), "LUSGSMatrixInit", self.device.ctx)
 
     def UpdateLUSGSForward(self, alphaDiag: float, rhs: ArrayDOFV, u: ArrayDOFV, uInc: ArrayDOFV,
                            uIncNew: ArrayDOFV):

@@ -340,11 +340,14 @@ int dndsmi_ssprk3_step_mixed(dndsmi_ctx *ctx, dndsmi_array *u, dndsmi_array *uRe
  * otherwise it receives n_cell colours. */
 int dndsmi_lusgs_colouring(const dndsmi_ctx *ctx, int32_t *colour, int32_t *n_colours);
 
-/* EulerEvaluator::LUSGSMatrixInit(dTau, dt, alphaDiag, u, uRec, 0, t)
- * (DNDS_FV_EulerEvaluator.cpp:995-1047; declared DNDS_FV_EulerEvaluator.hpp:1429-1433),
- * jacobianCode 0, Euler model: the diagonal from lambdaFace of the last
- * dndsmi_evaluate_dt on this context. */
-int dndsmi_lusgs_matrix_init(dndsmi_ctx *ctx, const dndsmi_array *dTau, double dt, double alphaDiag);
+/* EulerEvaluator::LUSGSMatrixInit(dTau, dt, alphaDiag, u, uRec, jacobianCode, t)
+ * (DNDS_FV_EulerEvaluator.cpp:995-1047; declared DNDS_FV_EulerEvaluator.hpp:1429-1433):
+ * the diagonal from lambdaFace of the last dndsmi_evaluate_dt on this
+ * context.  u and uRec (either may be NULL) and t enter the reference only
+ * through the source-term Jacobian, which the NS_2D model lacks; jacobianCode
+ * other than 0 (the analytic diagonal) is DNDSMI_ERR_UNSUPPORTED. */
+int dndsmi_lusgs_matrix_init(dndsmi_ctx *ctx, const dndsmi_array *dTau, double dt, double alphaDiag,
+                             const dndsmi_array *u, const dndsmi_array *uRec, int32_t jacobianCode, double t);
 
 /* EulerEvaluator::UpdateLUSGSForward / UpdateLUSGSBackward(alphaDiag, rhs, u,
  * uInc, uIncNew) (DNDS_FV_EulerEvaluator.cpp:1149-1357; declared

@@ -9,7 +9,8 @@
  *   VRFiniteVolume2D           DNDS_FV_VR.hpp:143  VRFiniteVolume2D
  *   EulerEvaluator<NS_2D>      DNDS_FV_EulerEvaluator.hpp:71   EulerEvaluator
  *   ArrayDOFV<4>, ArrayRecV    DNDS_DerivedArray.hpp:302, 446  ArrayDOFV, ArrayRecV
- *   std::vector<real> dt       (EvaluateDt, LUSGSMatrixInit)   ArrayScalar
+ *   std::vector<real> dt       (EvaluateDt, LUSGSMatrixInit)   ArrayScalar, or std::vector<real>
+ *                              through an overload that moves it across PCIe
  *   ODE::ExplicitSSPRK3LocalDt DNDS_ODE.hpp:95-179 ExplicitSSPRK3LocalDt
  *   ODE::ImplicitEulerDualTimeStep DNDS_ODE.hpp:201-265 ImplicitEulerDualTimeStep
  *
@@ -269,11 +270,29 @@ class EulerEvaluator {
     void EvaluateDt(ArrayScalar &dt, ArrayDOFV &u, real CFL, real &dtMinall, real MaxDt = 1, bool UseLocaldt = false) {
         detail::check(dndsmi_evaluate_dt(ctx(), dt.h(), u.h(), CFL, &dtMinall, MaxDt, UseLocaldt), "EvaluateDt", ctx());
     }
+    /* the reference's std::vector<real> form (DNDS_FV_EulerEvaluator.hpp:1408-1411): dt is resized and
+     * filled on the host */
+    void EvaluateDt(std::vector<real> &dt, ArrayDOFV &u, real CFL, real &dtMinall, real MaxDt = 1,
+                    bool UseLocaldt = false) {
+        ArrayScalar d(dev_);
+        EvaluateDt(d, u, CFL, dtMinall, MaxDt, UseLocaldt);
+        dt = d.download();
+    }
     void EvaluateRHS(ArrayDOFV &rhs, ArrayDOFV &u, ArrayRecV &uRec, real t) {
         detail::check(dndsmi_evaluate_rhs(ctx(), rhs.h(), u.h(), uRec.h(), t), "EvaluateRHS", ctx());
     }
-    void LUSGSMatrixInit(ArrayScalar &dTau, real dt, real alphaDiag) {
-        detail::check(dndsmi_lusgs_matrix_init(ctx(), dTau.h(), dt, alphaDiag), "LUSGSMatrixInit", ctx());
+    /* DNDS_FV_EulerEvaluator.hpp:1429-1433; jacobianCode != 0 throws (unsupported) */
+    void LUSGSMatrixInit(ArrayScalar &dTau, real dt, real alphaDiag, ArrayDOFV &u, ArrayRecV &uRec, int jacobianCode,
+                         real t) {
+        detail::check(dndsmi_lusgs_matrix_init(ctx(), dTau.h(), dt, alphaDiag, u.h(), uRec.h(), jacobianCode, t),
+                      "LUSGSMatrixInit", ctx());
+    }
+    /* the reference's std::vector<real> form: dTau uploaded from the host */
+    void LUSGSMatrixInit(std::vector<real> &dTau, real dt, real alphaDiag, ArrayDOFV &u, ArrayRecV &uRec,
+                         int jacobianCode, real t) {
+        ArrayScalar d(dev_);
+        d.upload(dTau);
+        LUSGSMatrixInit(d, dt, alphaDiag, u, uRec, jacobianCode, t);
     }
     void UpdateLUSGSForward(real alphaDiag, ArrayDOFV &rhs, ArrayDOFV &u, ArrayDOFV &uInc, ArrayDOFV &uIncNew) {
         detail::check(dndsmi_update_lusgs_forward(ctx(), alphaDiag, rhs.h(), u.h(), uInc.h(), uIncNew.h()),

@@ -55,10 +55,31 @@ int main(int argc, char **argv) {
         dndsmi::ImplicitSDIRK4DualTimeStep sdirk(ev, 1, 5.0, 2, 0.0, true);
         sdirk.Step(u, uRec, uRecNew, 0.05, t);
         t += 0.05;
-        std::array<double, 4> res{};
+        std::array<double, 4> res{}, resInc{};
         auto rhs = ev.BuildMean();
         ev.EvaluateRHS(rhs, u, uRec, t);
         ev.EvaluateResidual(res, rhs, 1, false);
+        // one LU-SGS solve through the reference's host-vector forms of
+        // EvaluateDt and LUSGSMatrixInit
+        std::vector<double> dTau;
+        double dtMin = 0;
+        ev.EvaluateDt(dTau, u, 5.0, dtMin, 1e100, true);
+        auto inc = ev.BuildMean();
+        inc.setConstant(0.0);
+        ev.LUSGSMatrixInit(dTau, 1e100, 1.0, u, uRec, 0, t);
+        ev.UpdateLUSGSForward(1.0, rhs, u, inc, inc);
+        ev.UpdateLUSGSBackward(1.0, rhs, u, inc, inc);
+        ev.EvaluateResidual(resInc, inc, 1, false);
+        bool refused = false;
+        try {
+            ev.LUSGSMatrixInit(dTau, 1e100, 1.0, u, uRec, 1, t);
+        } catch (const dndsmi::Error &e) {
+            refused = e.code == DNDSMI_ERR_UNSUPPORTED;
+        }
+        if (!refused) {
+            std::fprintf(stderr, "facade_run: jacobianCode 1 was not refused\n");
+            return 4;
+        }
         const auto hu = u.download(), hr = uRec.download();
         FILE *f = std::fopen(argv[1], "wb");
         if (!f)
@@ -66,6 +87,8 @@ int main(int argc, char **argv) {
         std::fwrite(hu.data(), sizeof(double), hu.size(), f);
         std::fwrite(hr.data(), sizeof(double), hr.size(), f);
         std::fwrite(res.data(), sizeof(double), 4, f);
+        std::fwrite(resInc.data(), sizeof(double), 4, f);
+        std::fwrite(dTau.data(), sizeof(double), dTau.size(), f);
         std::fclose(f);
         std::printf("facade_run: %lld cells, t = %.6f, %zu implicit iterations, |rhs|_1 = %.6e\n", (long long)n, t,
                     hist.size() / 4, res[0]);

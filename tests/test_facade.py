@@ -29,7 +29,7 @@ def test_example_program_is_built_and_links():
 
 def _python_run():
     from dnds_amd import _capi as A
-    from dnds_amd.host import (CompactFacedMeshSerialRW, EulerEvaluator, ExplicitSSPRK3LocalDt,
+    from dnds_amd.host import (CompactFacedMeshSerialRW, DNDSError, EulerEvaluator, ExplicitSSPRK3LocalDt,
                                ImplicitBDFDualTimeStep, ImplicitEulerDualTimeStep, ImplicitSDIRK4DualTimeStep,
                                VRFiniteVolume2D, vr_setting)
     mesh = CompactFacedMeshSerialRW.generate(A.MESH_TRI_SPLIT, 10, 8, 0.0, 1.0, 0.0, 1.0, jitter=0.2, seed=5,
@@ -63,7 +63,16 @@ def _python_run():
     rhs = ev.BuildMean()
     ev.EvaluateRHS(rhs, u, r, t)
     res = ev.EvaluateResidual(rhs, 1, False)
-    return mesh.n_cell, u.download(), r.download(), res
+    dtau = []  # the reference's std::vector<real> form
+    ev.EvaluateDt(dtau, u, 5.0, 1e100, True)
+    inc = ev.BuildMean().setConstant(0.0)
+    ev.LUSGSMatrixInit(dtau, 1e100, 1.0, u, r, 0, t)
+    ev.UpdateLUSGSForward(1.0, rhs, u, inc, inc)
+    ev.UpdateLUSGSBackward(1.0, rhs, u, inc, inc)
+    res_inc = ev.EvaluateResidual(inc, 1, False)
+    with pytest.raises(DNDSError):
+        ev.LUSGSMatrixInit(dtau, 1e100, 1.0, u, r, 1, t)
+    return mesh.n_cell, u.download(), r.download(), np.r_[res, res_inc], np.array(dtau)
 
 
 @pytest.mark.gpu
@@ -71,9 +80,10 @@ def test_cpp_host_classes_match_python_host(tmp_path):
     out = tmp_path / "facade.bin"
     r = subprocess.run([BIN, str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    n, u, urec, res = _python_run()
+    n, u, urec, res, dtau = _python_run()
     data = np.fromfile(out, dtype=np.float64)
-    assert data.size == n * 4 + n * 36 + 4
+    assert data.size == n * 4 + n * 36 + 8 + n
     assert np.array_equal(data[:4 * n].reshape(n, 4), u)
     assert np.array_equal(data[4 * n:40 * n].reshape(n, 36), urec)
-    assert np.array_equal(data[40 * n:], res)
+    assert np.array_equal(data[40 * n:40 * n + 8], res)
+    assert np.array_equal(data[40 * n + 8:], dtau)

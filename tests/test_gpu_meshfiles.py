@@ -1,13 +1,13 @@
-"""The reference's own mesh files, read by the Gmsh reader, through the
-device path against the oracle (bar 1e-12 relative, as for generated meshes).
+"""Mesh files read by the Gmsh reader, through the device path against the
+oracle (bar 1e-12 relative, as for generated meshes).  The files are written
+by tests/golden/make_meshes.py in an external exporter's layout.
 
-* tests/golden/UniformB0.msh (68 cells) and UniformB1.msh (256 cells), the
-  first meshes of the reference's static reconstruction test
-  (test/staticReconstructionTest.cpp:64-67), bounded by `Wall` faces: Jacobi
-  sweeps (a `Wall` face is valid for the sweep only, not as a flux boundary).
-* tests/golden/in.msh (25 cells, the mesh test/meshtest.cpp:42 reads),
-  bounded by `Farfield` faces: the time step, the right-hand side with Roe
-  and HLLC, and SSP-RK3 steps with the WBAP limiter.
+* tests/golden/tri_delaunay_a.msh (58 cells) and tri_delaunay_b.msh (238
+  cells), Delaunay triangulations bounded by `Wall` faces: Jacobi sweeps (a
+  `Wall` face is valid for the sweep only, not as a flux boundary).
+* tests/golden/quad_6x4.msh (24 quadrangles), bounded by `Farfield` faces:
+  the time step, the right-hand side with Roe and HLLC, and SSP-RK3 steps
+  with the WBAP limiter.
 """
 import os
 
@@ -40,8 +40,8 @@ def build(name, riemann="Roe"):
     return mesh, vfv, ev, orc
 
 
-@pytest.mark.parametrize("name", ["UniformB0.msh", "UniformB1.msh", "in.msh"])
-def test_jacobi_sweeps_on_the_reference_meshes_match_oracle(name):
+@pytest.mark.parametrize("name", ["tri_delaunay_a.msh", "tri_delaunay_b.msh", "quad_6x4.msh"])
+def test_jacobi_sweeps_on_the_mesh_files_match_oracle(name):
     mesh, vfv, ev, orc = build(name)
     u, urec = random_state(mesh.n_cell, 41)
     du, dr, dn = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec()
@@ -53,8 +53,8 @@ def test_jacobi_sweeps_on_the_reference_meshes_match_oracle(name):
 
 
 @pytest.mark.parametrize("riemann", ["Roe", "HLLC"])
-def test_dt_and_rhs_on_the_reference_farfield_mesh_match_oracle(riemann):
-    mesh, vfv, ev, orc = build("in.msh", riemann)
+def test_dt_and_rhs_on_the_farfield_mesh_file_match_oracle(riemann):
+    mesh, vfv, ev, orc = build("quad_6x4.msh", riemann)
     u, urec = random_state(mesh.n_cell, 42)
     du, dr, rhs, dt = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), ev.BuildMean(), ev.BuildScalar()
     dtmin = ev.EvaluateDt(dt, du, CFL=0.5, MaxDt=1e100, UseLocaldt=True)
@@ -65,9 +65,9 @@ def test_dt_and_rhs_on_the_reference_farfield_mesh_match_oracle(riemann):
 
 
 @pytest.mark.parametrize("procedure", [0, 1])
-def test_limited_ssprk3_on_the_reference_farfield_mesh_matches_oracle(procedure):
+def test_limited_ssprk3_on_the_farfield_mesh_file_matches_oracle(procedure):
     from dnds_amd.host import ExplicitSSPRK3LocalDt
-    mesh, vfv, ev, orc = build("in.msh", "HLLC")
+    mesh, vfv, ev, orc = build("quad_6x4.msh", "HLLC")
     u, _ = random_state(mesh.n_cell, 43, amp=0.3)
     urec = np.zeros((mesh.n_cell, 36))
     du, dr, dn = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec()

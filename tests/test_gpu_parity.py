@@ -21,15 +21,23 @@ BC_SETS = {
     "mixed_bc": (2, 3, 4, 3),       # Farfield, Wall_Euler, Wall_NoSlip, Wall_Euler
     "with_wall": (2, 3, 4, 1),      # ... and a Wall face (valid for the sweep only)
     "riemann": (-5, -5, -5, -5),    # Special_2DRiemannFar
+    "ivfar": (-4, -4, -4, -4),      # Special_IVFar (exact vortex), on [0,10]^2
+    "dmrfar": (-2, -2, -2, -2),     # Special_DMRFar (Mach-10 shock ghost), on [0,4]x[0,1]
+    "rtfar": (-3, -3, -3, -3),      # Special_RTFar (Rayleigh-Taylor far states)
 }
+
+# domain (x1, y1) of the special far-field sets: the vortex centre (5, 5)
+# inside the domain, the double-Mach shock line crossing the bottom edge
+DOMAIN = {"ivfar": (10.0, 10.0), "dmrfar": (4.0, 1.0)}
 
 
 def build(kind, bcname, riemann="Roe", nx=9, ny=7, jitter=0.2, seed=5):
     import oracle
     from dnds_amd.host import EulerEvaluator
     periodic = bcname == "periodic"
+    x1, y1 = DOMAIN.get(bcname, (1.0, 1.0))
     mesh, vfv = mesh_and_setup(kind=kind, nx=nx, ny=ny, periodic=periodic, jitter=0.0 if periodic else jitter,
-                               seed=seed, bc=BC_SETS[bcname])
+                               seed=seed, bc=BC_SETS[bcname], x1=x1, y1=y1)
     far = (1.0, 0.3, -0.2, 2.6)
     ev = EulerEvaluator(mesh, vfv, riemannSolverType=riemann, gamma=1.4, farFieldStaticValue=far)
     st = vfv.setting
@@ -52,7 +60,8 @@ def random_state(n, seed, amp=0.2, rec_amp=0.05):
     return u, urec
 
 
-CASES = [(k, b) for k in (0, 1, 2) for b in ("periodic", "farfield", "walls", "mixed_bc")] + [(0, "riemann")]
+CASES = ([(k, b) for k in (0, 1, 2) for b in ("periodic", "farfield", "walls", "mixed_bc")] + [(0, "riemann")]
+         + [(k, b) for k, b in ((0, "ivfar"), (2, "ivfar"), (0, "dmrfar"), (1, "dmrfar"), (2, "rtfar"), (0, "rtfar"))])
 
 
 @pytest.mark.parametrize("kind,bcname", CASES + [(2, "with_wall")])
@@ -95,7 +104,8 @@ def test_evaluate_rhs_matches_oracle(kind, bcname, riemann):
 
 
 @pytest.mark.parametrize("kind,bcname,riemann", [(0, "periodic", "Roe"), (2, "farfield", "HLLC"),
-                                                 (1, "walls", "Roe"), (0, "riemann", "HLLC")])
+                                                 (1, "walls", "Roe"), (0, "riemann", "HLLC"),
+                                                 (2, "ivfar", "Roe"), (0, "rtfar", "HLLC")])
 def test_ssprk3_steps_match_oracle(kind, bcname, riemann):
     from dnds_amd.host import ExplicitSSPRK3LocalDt
     mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
@@ -109,6 +119,29 @@ def test_ssprk3_steps_match_oracle(kind, bcname, riemann):
     assert abs(t - rt) <= TOL * rt
     assert rel_err(du.download(), ru) <= TOL
     assert rel_err(dr.download(), rr) <= TOL
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("riemann", ["Roe", "HLLC"])
+def test_positivity_fallback_matches_oracle(kind, riemann):
+    """CompressRecPart's fallback to the cell mean (DNDS_FV_EulerEvaluator.hpp:
+    1322-1351) forced at every face Gauss point of one cell (huge momentum
+    coefficients, e < 0 there): the device right-hand side equals the
+    oracle's, and equals the one with that cell's coefficients zeroed."""
+    from test_oracle_pins import fallback_state
+    mesh, vfv, ev, orc = build(kind, "farfield", riemann=riemann)
+    cell = mesh.n_cell // 2
+    u, bad, zeroed = fallback_state(mesh, vfv, cell)
+    du, dt, rhs = ev.BuildMean().upload(u), ev.BuildScalar(), ev.BuildMean()
+    ev.EvaluateDt(dt, du, CFL=0.5)
+    _, _, _, dlf = orc.evaluate_dt(u, 0.5)
+    out = []
+    for rec in (bad, zeroed):
+        ev.EvaluateRHS(rhs, du, vfv.BuildRec().upload(rec), t=0.0)
+        got = rhs.download()
+        assert rel_err(got, orc.evaluate_rhs(u, rec, dlf)) <= TOL
+        out.append(got)
+    assert np.array_equal(out[0], out[1])
 
 
 def test_vortex_config_full_run_matches_oracle():

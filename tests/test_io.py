@@ -1,13 +1,14 @@
 """Mesh and solution files: the Gmsh 2.2 reader (DNDS_Mesh.hpp:87-350,
 760-835) and the Tecplot binary writer (DNDS_Mesh.hpp:1634-1814).  CPU only.
 
-Fixtures: tests/golden/in.msh (the mesh test/meshtest.cpp:42 reads) and
-tests/golden/UniformB0.msh, UniformB1.msh (the first meshes of
-test/staticReconstructionTest.cpp:64-67), data files of the reference kept as
-they are.  The reader is checked against a plain Python parse of the same
-files fed to the topology builder, against the generator through a written
-file, and on its error paths; the writer against a parse of the Tecplot
-binary format (Tecplot 360 data format guide, version 112).
+Fixtures: tests/golden/quad_6x4.msh, tri_delaunay_a.msh and
+tri_delaunay_b.msh, written by tests/golden/make_meshes.py in an external
+exporter's layout (comment blocks, aligned columns, two physical groups of
+one name, boundary lines against the boundary's direction).  The reader is
+checked against a plain Python parse of the same files fed to the topology
+builder, against the generator through a written file, and on its error
+paths; the writer against a parse of the Tecplot binary format (Tecplot 360
+data format guide, version 112).
 """
 import os
 import struct
@@ -76,9 +77,10 @@ def same_topology(a, b):
         assert np.array_equal(A[key], B[key]), key
 
 
-@pytest.mark.parametrize("name,n_cell,n_face,bcs", [("in.msh", 25, 60, {2: 20}), ("UniformB0.msh", 68, 112, {1: 20}),
-                                                    ("UniformB1.msh", 256, 404, {1: 40})])
-def test_reads_the_reference_meshes(name, n_cell, n_face, bcs):
+@pytest.mark.parametrize("name,n_cell,n_face,bcs", [("quad_6x4.msh", 24, 58, {2: 20}),
+                                                    ("tri_delaunay_a.msh", 58, 97, {1: 20}),
+                                                    ("tri_delaunay_b.msh", 238, 377, {1: 40})])
+def test_reads_the_mesh_files(name, n_cell, n_face, bcs):
     from dnds_amd.host import CompactFacedMeshSerialRW
     path = os.path.join(GOLD, name)
     m = CompactFacedMeshSerialRW.read_gmsh(path)
@@ -157,20 +159,20 @@ def edit(src, dst, old, new):
     ('"bc-2"', '"bc-7"', "boundary group"),
     ("2.2 0 8", "2.2 1 8", "ASCII"),
     ("2.2 0 8", "4.1 0 8", "2.x"),
-    ("1  3 2 1 1  1  2  8  7", "1  9 2 1 1  1  2  8  7 3 4 5", "not supported"),
-    ("1  3 2 1 1  1  2  8  7", "1  3 2 1 1  1  7  8  2", "counter-clockwise"),
-    ("1  3 2 1 1  1  2  8  7", "1  3 2 5 1  1  2  8  7", "no $PhysicalNames"),
-    ("26  1 2 2 2  1  2", "26  1 2 2 2  8  9", "not on a boundary face"),
-    ("26  1 2 2 2  1  2", "26  1 2 2 2  1  99", "outside"),
+    ("1  3 2 1 1  1  2  9  8", "1  9 2 1 1  1  2  9  8 3 4 5", "not supported"),
+    ("1  3 2 1 1  1  2  9  8", "1  3 2 1 1  1  8  9  2", "counter-clockwise"),
+    ("1  3 2 1 1  1  2  9  8", "1  3 2 5 1  1  2  9  8", "no $PhysicalNames"),
+    ("25  1 2 2 2  1  2", "25  1 2 2 2  9 10", "not on a boundary face"),
+    ("25  1 2 2 2  1  2", "25  1 2 2 2  1  99", "outside"),
     ("$EndNodes", "$EndNodez", "not concluded"),
-    ("$Nodes\n36\n", "$Nodes\n99999999999999\n", "bad node count"),
+    ("$Nodes\n35\n", "$Nodes\n99999999999999\n", "bad node count"),
     ("$Elements\n", "$Elements\n-3\n", "bad element count"),
 ])
 def test_reader_errors(tmp_path, old, new, msg):
     from dnds_amd import _capi as A
     from dnds_amd.host import CompactFacedMeshSerialRW, DNDSError
     path = str(tmp_path / "bad.msh")
-    edit(os.path.join(GOLD, "in.msh"), path, old, new)
+    edit(os.path.join(GOLD, "quad_6x4.msh"), path, old, new)
     with pytest.raises(DNDSError) as e:
         CompactFacedMeshSerialRW.read_gmsh(path)
     assert msg in str(e.value) and e.value.code == A.ERR_INVALID_ARGUMENT
@@ -272,12 +274,12 @@ def test_cpp_host_classes_read_and_write(tmp_path):
                         "-Wl,-rpath," + lib], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     out = str(tmp_path / "o.plt")
-    r = subprocess.run([exe, os.path.join(GOLD, "in.msh"), out], capture_output=True, text=True, timeout=60)
+    r = subprocess.run([exe, os.path.join(GOLD, "quad_6x4.msh"), out], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.split() == ["25", "60"]
+    assert r.stdout.split() == ["24", "58"]
     p = read_plt(out)
     assert p["names"] == ["X", "Y", "Z", "A", "B", "iPart"] and p["time"] == 1.5
-    assert np.array_equal(p["values"][4], 1000.0 + np.arange(25))
+    assert np.array_equal(p["values"][4], 1000.0 + np.arange(24))
     r = subprocess.run([exe, str(tmp_path / "none.msh"), out], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
 
@@ -285,7 +287,7 @@ def test_cpp_host_classes_read_and_write(tmp_path):
 def test_solver_output_fields(tmp_path):
     """EulerSolver::PrintData's eight NS_2D slots, written and read back."""
     from dnds_amd.host import CompactFacedMeshSerialRW, solver_output
-    m = CompactFacedMeshSerialRW.read_gmsh(os.path.join(GOLD, "UniformB0.msh"))
+    m = CompactFacedMeshSerialRW.read_gmsh(os.path.join(GOLD, "tri_delaunay_a.msh"))
     n = m.n_cell
     rng = np.random.default_rng(3)
     rho, vx, vy, p = (rng.uniform(0.5, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(0.5, 2, n))
