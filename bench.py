@@ -1,21 +1,31 @@
-"""Benchmark: p = 3 VR + RHS cell-updates/s of the SSP-RK3 hot path on one GPU.
+"""Benchmark: p = 3 VR + RHS cell-updates/s of the SSP-RK3 hot path on one GPU,
+plain FP64 against FP32 + FP64 refinement at equal L2 error.
 
-Workload (default): the four-quadrant 2D Riemann problem on [0,1]^2,
-1024 x 1024 quads cut along seeded diagonals (2,097,152 triangles), interior
-nodes jittered by U(-0.2h, 0.2h); p = 3 VR, HLLC, WBAP limiter, SSP-RK3 at
-CFL 0.5 with a global time step, 5 Jacobi sweeps per stage, FP64.  One
-"step" is one SSP-RK3 step: EvaluateDt once, then three stages of (5 sweeps +
-WBAP + EvaluateRHS + stage update), all on the device with the state resident
-in HBM.
+Workload (default): the combined large case -- [0,4] x [0,1], 2048 x 2048
+quads of which half are cut into triangles (seeded): 6,293,459 mixed cells,
+interior nodes jittered by U(-0.15h, 0.15h); a Mach-3 shock at x = 0.5 with
+seeded density noise of 1e-3; p = 3 VR, HLLC, WBAP V2 (the reference
+solver's default limiterProcedure 0), SSP-RK3 at CFL 0.5 with a global time
+step, 5 Jacobi sweeps per stage, 100 steps.  One "step" is one SSP-RK3 step:
+EvaluateDt once, then three stages of (5 sweeps + WBAP + EvaluateRHS + stage
+update), all on the device with the state resident in HBM.
 
-value = cells x 3 stages x steps / wall time of the timed steps.
+value = cells x 3 stages x steps / wall time of the timed FP64 steps (the
+plain arm).  The mixed arm (first sweep of each stage in FP64, the others as
+FP32 corrections; FP32 face reconstruction and flux storage) runs the same
+steps from the same initial state; the line reports its throughput, the
+ratio, and the L2 difference of its state from the FP64 arm's after the
+same steps.  Both arms are also compared with the oracle after one step from
+the initial state (the CPU-baseline run), and the accuracy object gives both
+arms' L2 error against the exact solution on the small isentropic-vortex
+case.
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
 (the Jacobi sweep: algorithmic bytes per launch / mean launch time from HIP
-events recorded by the library on its stream) and a CPU baseline (the FP64
-oracle, single thread, one SSP-RK3 step on the same mesh).  The default
-workload carries the WBAP limiter in every stage: V2, the solver's default
-limiterProcedure 0, or V3 with --limiter-procedure 1.
+events recorded by the library on its stream), EvaluateRHS's fraction of the
+HBM peak, a CPU baseline (the oracle compiled -O3 -march=native on this host,
+one SSP-RK3 step of the same workload on the host's cores) and the
+four-quadrant Riemann case at its 200 steps.
 """
 from __future__ import annotations
 
@@ -32,20 +42,22 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
+# committed rocprofv3 FETCH_SIZE + WRITE_SIZE summaries, by workload
+PROFILES = {("riemann", 1024): os.path.join(ROOT, "profiles", "riemann_2M_kernels_traffic.md"),
+            ("step", 2048): os.path.join(ROOT, "profiles", "step_6M_kernels_traffic.md")}
 
-PROFILE = os.path.join(ROOT, "profiles", "riemann_2M_kernels_traffic.md")
 
-
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on the default workload, from the
-    committed rocprofv3 FETCH_SIZE + WRITE_SIZE summary (None if absent)."""
+def measured_traffic(case: str, n: int, kernel: str):
+    """HBM bytes per launch of `kernel` on a workload, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (None if absent)."""
+    path = PROFILES.get((case, n))
     try:
-        with open(PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 cols = [c.strip() for c in line.strip().strip("|").split("|")]
                 if len(cols) == 7 and cols[0] == f"`{kernel}`":
-                    return float(cols[4]) * 1e9, os.path.relpath(PROFILE, ROOT)
-    except OSError:
+                    return float(cols[4]) * 1e9, os.path.relpath(path, ROOT)
+    except (OSError, TypeError):
         pass
     return None, None
 
@@ -54,23 +66,158 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def rel_l2(a, b, vol):
+    """Per variable, sqrt(sum V (a - b)^2 / sum V b^2): the volume-weighted L2
+    difference of two cell-mean states relative to b."""
+    w = vol[:, None]
+    return (np.sqrt((w * (a - b) ** 2).sum(0) / (w * b ** 2).sum(0))).tolist()
+
+
+def make_case(name, n, threads):
+    from dnds_amd import cases
+    if name == "riemann":
+        return cases.riemann_2d(n, nthreads=threads, build_limiter_data=True)
+    if name == "step":
+        return cases.mach3_step(n, nthreads=threads)
+    if name == "plate":
+        return cases.flat_plate(n * 3 // 2, n * 5 // 16, nthreads=threads)
+    return cases.isentropic_vortex(n, nthreads=threads)
+
+
+def kernel_profile(lib, ctx, A):
+    import ctypes as C
+    prof = {}
+    for k, name in enumerate(A.KERNEL_NAMES):
+        cnt, tot = C.c_int64(), C.c_double()
+        lib.dndsmi_profile_query(ctx, k, C.byref(cnt), C.byref(tot))
+        if cnt.value:
+            b = int(lib.dndsmi_kernel_bytes(ctx, k))
+            avg_ms = tot.value / cnt.value
+            prof[name] = dict(launches=cnt.value, avg_ms=avg_ms, bytes=b, gbs=b / (avg_ms * 1e-3) / 1e9)
+    return prof
+
+
+class Arms:
+    """Runs the plain FP64 and the mixed-precision SSP-RK3 steps of one case
+    from its initial state on the device."""
+
+    def __init__(self, case, args, gas):
+        from dnds_amd import _capi as A
+        from dnds_amd.host import EulerEvaluator
+        self.A, self.case, self.args = A, case, args
+        self.ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType=case.riemann, farFieldStaticValue=case.far,
+                                 **gas)
+        self.lib = A.load()
+        self.urec0 = np.zeros((case.n_cell, 36))
+
+    def fresh(self):
+        c = self.case
+        return self.ev.BuildMean().upload(c.u0), c.vfv.BuildRec().upload(self.urec0), c.vfv.BuildRec()
+
+    def ode(self, mixed):
+        from dnds_amd.host import ExplicitSSPRK3LocalDt
+        return ExplicitSSPRK3LocalDt(self.ev, nInternalRecStep=self.args.sweeps, CFL=self.args.cfl,
+                                     useLimiter=self.case.limiter, mixedPrecision=mixed,
+                                     limiterProcedure=self.args.limiter_procedure)
+
+    def steps(self, mixed, n):
+        """n steps from the initial state, untimed: (u, uRec, t)."""
+        u, r, rn = self.fresh()
+        ode, t = self.ode(mixed), 0.0
+        for _ in range(n):
+            t += ode.Step(u, r, rn, t)
+        return u.download(), r.download(), t
+
+    def timed(self, mixed, steps, warmup):
+        """warmup steps, then `steps` timed steps continuing from them; the
+        per-kernel HIP-event times of the timed steps"""
+        tag = "mixed" if mixed else "fp64"
+        u, r, rn = self.fresh()
+        ode, t = self.ode(mixed), 0.0
+        for _ in range(warmup):
+            t += ode.Step(u, r, rn, t)
+        self.ev.device.sync()
+        self.lib.dndsmi_profile_enable(self.ev.device.ctx, 1)
+        self.ev.device.sync()
+        ts = time.perf_counter()
+        for _ in range(steps):
+            t += ode.Step(u, r, rn, t)
+        self.ev.device.sync()
+        elapsed = time.perf_counter() - ts
+        log(f"[{self.case.name} {tag}] timed {steps} steps after {warmup}: {elapsed:.3f}s, t = {t:.6f}")
+        prof = kernel_profile(self.lib, self.ev.device.ctx, self.A)
+        self.lib.dndsmi_profile_enable(self.ev.device.ctx, 0)
+        for name, p in prof.items():
+            log(f"[{self.case.name} {tag}] {name}: {p['launches']} launches, {p['avg_ms']:.4f} ms avg, "
+                f"{p['gbs']:.0f} GB/s algorithmic")
+        uf = u.download()
+        if not np.isfinite(uf).all():
+            raise RuntimeError("non-finite state after the timed steps")
+        return elapsed, prof, uf, r, t
+
+
+def evaluate_rhs_object(prof, case):
+    """EvaluateRHS = face kernel (inner faces) + boundary-face kernel + gather
+    kernel, with the bytes those kernels need"""
+    if "k_rhs_face" not in prof or "k_rhs_cell" not in prof:
+        return None
+    ks = [k for k in ("k_rhs_face", "k_rhs_bface", "k_rhs_cell") if k in prof]
+    ms = sum(prof[k]["avg_ms"] for k in ks)
+    nbytes = sum(prof[k]["bytes"] for k in ks)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"EvaluateRHS: {ms:.3f} ms, {nbytes / 1e9:.2f} GB, {gbs:.0f} GB/s = {gbs / HBM_PEAK_GBS:.2f} of peak")
+    return {"ms": ms, "algorithmic_bytes": nbytes, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "kernels": ks}
+
+
+def vortex_accuracy(args):
+    """The small isentropic-vortex case (40 x 40 quads split into 3200
+    triangles, Roe, no limiter, 5 sweeps, CFL 0.5, 100 steps): density L2
+    error against the exact solution for the FP64 and the mixed arm."""
+    from dnds_amd import cases
+    case = cases.isentropic_vortex(40, nthreads=args.threads)
+    arms = Arms(case, argparse.Namespace(sweeps=5, cfl=0.5, limiter_procedure=0), {})
+    vol = case.vfv.arrays(["volume"])["volume"]
+    out = {"workload": "periodic isentropic vortex, 40x40 quads split to 3200 triangles, Roe, no limiter, "
+                       "5 sweeps/stage, CFL 0.5, 100 steps"}
+    for tag, mixed in (("fp64", False), ("mixed", True)):
+        u, _, t = arms.steps(mixed, case.steps)
+        ex = cases.cell_means(case.mesh, vol, lambda x, y: cases.vortex_exact(x, y, t))
+        out[f"l2_density_error_{tag}"] = float(np.sqrt(np.sum((u[:, 0] - ex[:, 0]) ** 2 * vol) / vol.sum()))
+    out["mixed_over_fp64"] = out["l2_density_error_mixed"] / out["l2_density_error_fp64"]
+    log(f"vortex accuracy: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1024, help="quads per direction")
+    ap.add_argument("--case", default="step", choices=["step", "riemann", "vortex", "plate"],
+                    help="step: the combined large case, Mach-3 shock on a mixed tri/quad mesh of [0,4]x[0,1] "
+                         "(n x n quads, half split; 6.3M cells at n 2048); riemann: four-quadrant Riemann problem "
+                         "on [0,1]^2 (2 n^2 triangles); vortex: periodic isentropic vortex on [0,10]^2; plate: laminar "
+                         "flat plate at M 0.2, Re 5e6, (1.5 n) x (0.3125 n) quads stretched to y+ = 1, steady: with "
+                         "--implicit N, N LU-SGS iterations from the free stream at CFL 20 with local time steps")
+    ap.add_argument("--n", type=int, default=None, help="quads per direction (default 2048 for step, 1024 else)")
     ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--cfl", type=float, default=0.5)
-    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1), help="host threads for the setup")
-    ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on one step (0 = skip)")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads for the setup and the CPU baseline")
+    ap.add_argument("--cpu-baseline", type=int, default=1,
+                    help="time the oracle (built -O3 -march=native here) on one step of the workload (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=10,
                     help="after the timed steps, N device copies of one rec array (a plain copy of known size "
-                         "for scaling profiler byte counters), timed: the copy_probe object of the JSON line "
-                         "(0 = skip)")
+                         "for scaling profiler byte counters), timed (0 = skip)")
     ap.add_argument("--mixed", type=int, default=1,
-                    help="also time the mixed-precision step (FP32 correction sweeps + FP64 refinement) "
-                         "from the same initial state and report it beside the FP64 line")
+                    help="also run the mixed-precision arm (FP32 correction sweeps + FP64 refinement) over the same "
+                         "steps from the same initial state")
+    ap.add_argument("--accuracy", type=int, default=1,
+                    help="both arms' L2 error against the exact solution on the 3200-cell isentropic vortex")
+    ap.add_argument("--riemann-200", type=int, default=1,
+                    help="also run the four-quadrant Riemann case (1024x1024 quads, 2.1M triangles) for its 200 "
+                         "steps in both arms and report it in the riemann_2M object")
     ap.add_argument("--implicit", type=int, default=0,
                     help="also time N inner iterations of the implicit-Euler dual-time step with multicolour LU-SGS "
                          "(1 sweep, CFL 10, dt = 5 explicit steps) on the same mesh and report them")
@@ -85,81 +232,35 @@ def main():
                     help="WBAP of the limited cases: 0 V2 (the reference solver's default), 1 V3")
     ap.add_argument("--mu-gas", type=float, default=0.0,
                     help="muGas > 0 times the laminar Navier-Stokes model on the same workload (Sutherland with "
-                         "TRef 1, CSutherland 0.5, Prandtl 0.72, Rgas 1; the mixed-precision line is skipped)")
-    ap.add_argument("--case", default="riemann", choices=["riemann", "vortex", "step", "plate"],
-                    help="riemann: four-quadrant Riemann problem; vortex: periodic isentropic vortex at the same scale; "
-                         "step: Mach-3 step on a mixed tri/quad mesh of [0,4]x[0,1] (n x n quads, --n 2048 for the "
-                         "6.3M-cell case); plate: laminar flat plate at M 0.2, Re 5e6, (1.5 n) x (0.3125 n) quads "
-                         "stretched to y+ = 1 (491,520 at the default n), steady: with --implicit N, N LU-SGS "
-                         "iterations from the free stream at CFL 20 with local time steps")
+                         "TRef 1, CSutherland 0.5, Prandtl 0.72, Rgas 1; the mixed-precision arm is skipped)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.n is None:
+        args.n = 2048 if args.case == "step" else 1024
 
-    from dnds_amd import _capi as A
-    from dnds_amd import cases
-    from dnds_amd.host import DNDSError, EulerEvaluator, ExplicitSSPRK3LocalDt
+    from dnds_amd.host import DNDSError
 
     t0 = time.time()
     log(f"building mesh + VR setup, {args.n}x{args.n} quads ({args.case})")
-    if args.case == "riemann":
-        case = cases.riemann_2d(args.n, nthreads=args.threads, build_limiter_data=True)
-    elif args.case == "step":
-        case = cases.mach3_step(args.n, nthreads=args.threads)
-    elif args.case == "plate":
-        case = cases.flat_plate(args.n * 3 // 2, args.n * 5 // 16, nthreads=args.threads)
-    else:
-        case = cases.isentropic_vortex(args.n, nthreads=args.threads)
+    case = make_case(args.case, args.n, args.threads)
     nc = case.n_cell
     log(f"setup done: {nc} cells, {case.mesh.n_face} faces, {time.time() - t0:.1f}s")
     gas = case.gas or dict(muGas=args.mu_gas, prGas=0.72, Rgas=1.0, TRef=1.0, CSutherland=0.5)
     if gas["muGas"] > 0:
         args.mixed = 0  # the mixed-precision step covers the Euler model only
-    ev = EulerEvaluator(case.mesh, case.vfv, riemannSolverType=case.riemann, farFieldStaticValue=case.far, **gas)
+    arms = Arms(case, args, gas)
+    ev, lib = arms.ev, arms.lib
     log(f"device context: {ev.device.device_bytes() / 2**30:.2f} GiB")
-    urec0 = np.zeros((nc, 36))
-    lib = A.load()
-    import ctypes as C
+    vol = case.vfv.arrays(["volume"])["volume"]
 
-    def timed_run(mixed: bool):
-        """warmup + timed SSP-RK3 steps from the initial state; per-kernel
-        HIP-event times over the timed steps"""
-        u = ev.BuildMean().upload(case.u0)
-        r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
-        ode = ExplicitSSPRK3LocalDt(ev, nInternalRecStep=args.sweeps, CFL=args.cfl, useLimiter=case.limiter,
-                                    mixedPrecision=mixed, limiterProcedure=args.limiter_procedure)
-        t = 0.0
-        for _ in range(args.warmup):
-            t += ode.Step(u, r, rn, t)
-        ev.device.sync()
-        tag = "mixed" if mixed else "fp64"
-        log(f"[{tag}] warmup {args.warmup} steps done")
-        lib.dndsmi_profile_enable(ev.device.ctx, 1)
-        ev.device.sync()
-        ts = time.perf_counter()
-        for _ in range(args.steps):
-            t += ode.Step(u, r, rn, t)
-        ev.device.sync()
-        elapsed = time.perf_counter() - ts
-        log(f"[{tag}] timed {args.steps} steps: {elapsed:.3f}s, t = {t:.5f}")
-        prof = {}
-        for k, name in enumerate(A.KERNEL_NAMES):
-            cnt, tot = C.c_int64(), C.c_double()
-            lib.dndsmi_profile_query(ev.device.ctx, k, C.byref(cnt), C.byref(tot))
-            if cnt.value:
-                b = int(lib.dndsmi_kernel_bytes(ev.device.ctx, k))
-                avg_ms = tot.value / cnt.value
-                prof[name] = dict(launches=cnt.value, avg_ms=avg_ms, bytes=b, gbs=b / (avg_ms * 1e-3) / 1e9)
-        lib.dndsmi_profile_enable(ev.device.ctx, 0)
-        for name, p in prof.items():
-            log(f"[{tag}] {name}: {p['launches']} launches, {p['avg_ms']:.4f} ms avg, {p['gbs']:.0f} GB/s algorithmic")
-        uf = u.download()
-        if not np.isfinite(uf).all():
-            raise RuntimeError("non-finite state after the timed steps")
-        return elapsed, prof, uf, r
+    # one step of each arm from the initial state, for the comparison with the oracle
+    one = {"fp64": arms.steps(False, 1)}
+    if args.mixed:
+        one["mixed"] = arms.steps(True, 1)
 
-    elapsed, prof, u_fp64, r = timed_run(False)
+    elapsed, prof, u_fp64, r, t_end = arms.timed(False, args.steps, args.warmup)
     copy = None
     if args.copy_probe:
         rn = case.vfv.BuildRec()
@@ -179,13 +280,17 @@ def main():
         log(f"copy probe: {copy['gbs']:.0f} GB/s read + write ({copy['frac_of_peak']:.2f} of peak)")
     mixed = None
     if args.mixed:
-        m_elapsed, m_prof, u_mixed, _ = timed_run(True)
-        diff = float(np.abs(u_mixed - u_fp64).max() / np.abs(u_fp64).max())
+        m_elapsed, m_prof, u_mixed, _, m_t = arms.timed(True, args.steps, args.warmup)
+        if m_t != t_end:
+            log(f"note: the arms reached different times ({t_end} vs {m_t})")
+        l2 = rel_l2(u_mixed, u_fp64, vol)
         mixed = {"value": nc * 3 * args.steps / m_elapsed, "ms_per_step": m_elapsed / args.steps * 1e3,
-                 "speedup_vs_fp64": elapsed / m_elapsed, "rel_diff_u_vs_fp64": diff,
-                 "dtype": "f32 sweeps/face reconstruction + f64 refinement", "kernels": m_prof}
-        log(f"mixed: {mixed['value']:.3e} cell-updates/s, x{mixed['speedup_vs_fp64']:.2f} vs fp64, "
-            f"rel. diff of u {diff:.1e}")
+                 "speedup_vs_fp64": elapsed / m_elapsed, "rel_l2_vs_fp64_arm": l2,
+                 "rel_max_vs_fp64_arm": float(np.abs(u_mixed - u_fp64).max() / np.abs(u_fp64).max()),
+                 "dtype": "f32 sweeps/face reconstruction + f64 refinement", "learned_warm_start": None,
+                 "kernels": m_prof, "evaluate_rhs": evaluate_rhs_object(m_prof, case)}
+        log(f"mixed: {mixed['value']:.3e} cell-updates/s, x{mixed['speedup_vs_fp64']:.3f} vs fp64, "
+            f"rel. L2 vs the fp64 arm after {args.warmup + args.steps} steps {l2}")
 
     stages = 3 * args.steps
     value = nc * stages / elapsed
@@ -195,58 +300,50 @@ def main():
         roof = {"bound": "hbm", "achieved": jac["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": jac["gbs"] / HBM_PEAK_GBS, "traffic": None, "kernel": "k_jacobi",
                 "bytes_per_launch": jac["bytes"], "avg_ms": jac["avg_ms"]}
-        if args.case == "riemann" and args.n == 1024:
-            roof["traffic"], roof["traffic_source"] = measured_traffic("k_jacobi<3, false>")
-
-    # EvaluateRHS = face kernel (inner faces; the boundary-face launch is
-    # untimed and small) + gather kernel, with the bytes those kernels need
-    rhs_obj = None
-    if "k_rhs_face" in prof and "k_rhs_cell" in prof:
-        ks = [k for k in ("k_rhs_face", "k_rhs_bface", "k_rhs_cell") if k in prof]
-        ms = sum(prof[k]["avg_ms"] for k in ks)
-        nbytes = sum(prof[k]["bytes"] for k in ks)
-        gbs = nbytes / (ms * 1e-3) / 1e9
-        # the per-unit algorithmic figure of the hot-path scope table: 1384 B per
-        # face (both sides' stored row-0 bases at 3 points, normals, weights,
-        # indices) and 360 B per cell (uRec, u, volume, rhs written)
-        stored_bytes = 1384 * case.mesh.n_face + 360 * nc
-        stored_gbs = stored_bytes / (ms * 1e-3) / 1e9
-        rhs_obj = {"ms": ms, "algorithmic_bytes": nbytes, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": gbs / HBM_PEAK_GBS, "kernels": ks,
-                   "stored_basis_bytes": stored_bytes, "achieved_stored_basis": stored_gbs,
-                   "frac_stored_basis": stored_gbs / HBM_PEAK_GBS}
-        log(f"EvaluateRHS: {ms:.3f} ms, {nbytes / 1e9:.2f} GB, {gbs:.0f} GB/s = {gbs / HBM_PEAK_GBS:.2f} of peak; "
-            f"{stored_bytes / 1e9:.2f} GB with stored bases: {stored_gbs / HBM_PEAK_GBS:.2f}")
+        kname = "k_jacobi<4, false>" if args.case == "step" else "k_jacobi<3, false>"
+        roof["traffic"], roof["traffic_source"] = measured_traffic(args.case, args.n, kname)
+    rhs_obj = evaluate_rhs_object(prof, case)
 
     cpu = None
+    oracle_cmp = None
+    orc = None
     if args.cpu_baseline:
         import oracle
+        log("CPU baseline: building the oracle -O3 -march=native on this host")
+        oracle.build_baseline()
         st = case.vfv.setting
         orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2 if case.riemann == "HLLC" else 1,
                             far_field=case.far, tang_weight=st.tang_weight,
                             tang_weight_direction=st.tang_weight_direction,
                             wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
                             limiter_procedure=args.limiter_procedure, mu_gas=gas["muGas"], pr_gas=gas["prGas"],
-                            r_gas=gas["Rgas"], t_ref=gas["TRef"], c_sutherland=gas["CSutherland"])
-        timed = {}
-        for threads in (args.threads, 1):
-            oracle.set_threads(threads)
-            log(f"CPU baseline: oracle on {threads} thread(s), one SSP-RK3 step on the same mesh")
-            c0 = time.perf_counter()
-            orc.ssprk3_steps(case.u0, urec0, 1, args.sweeps, args.cfl, use_limiter=case.limiter)
-            timed[threads] = time.perf_counter() - c0
-            log(f"CPU baseline {nc * 3 / timed[threads]:.3e} cell-updates/s ({timed[threads]:.1f}s)")
-        cpu = {"value": nc * 3 / timed[args.threads], "unit": "cell-updates/s", "cores": args.threads, "kind": "port",
+                            r_gas=gas["Rgas"], t_ref=gas["TRef"], c_sutherland=gas["CSutherland"],
+                            variant="baseline")
+        oracle.set_threads(args.threads, variant="baseline")
+        log(f"CPU baseline: oracle on {args.threads} threads, one SSP-RK3 step on the same mesh")
+        c0 = time.perf_counter()
+        ou, _, ot = orc.ssprk3_steps(case.u0, arms.urec0, 1, args.sweeps, args.cfl, use_limiter=case.limiter)
+        el = time.perf_counter() - c0
+        log(f"CPU baseline {nc * 3 / el:.3e} cell-updates/s ({el:.1f}s)")
+        cpu = {"value": nc * 3 / el, "unit": "cell-updates/s", "cores": args.threads, "kind": "port",
                "sample": f"one SSP-RK3 step (3 stages x {args.sweeps} sweeps{' + WBAP' if case.limiter else ''} + RHS) "
-                         f"on the same {nc}-cell mesh: {timed[args.threads]:.3g} s on {args.threads} threads, "
-                         f"{timed[1]:.3g} s on 1",
-               "value_1_core": nc * 3 / timed[1]}
+                         f"of the same {nc}-cell workload from its initial state: {el:.3g} s on {args.threads} "
+                         "threads",
+               "build": "g++ " + " ".join(oracle.BASELINE_FLAGS)}
+        oracle_cmp = {"steps": 1, "oracle_build": "the CPU-baseline build",
+                      "t_oracle": ot, "t_gpu": one["fp64"][2]}
+        for tag, (gu, _, _) in one.items():
+            oracle_cmp[f"rel_l2_{tag}"] = rel_l2(gu, ou, vol)
+            oracle_cmp[f"rel_max_{tag}"] = float(np.abs(gu - ou).max() / np.abs(ou).max())
+        log(f"vs oracle after one step: {oracle_cmp}")
+
+    accuracy = vortex_accuracy(args) if args.accuracy else None
 
     implicit = None
     if args.implicit:
         from dnds_amd.host import ImplicitEulerDualTimeStep
         u = ev.BuildMean().upload(case.u0)
-        r, rn = case.vfv.BuildRec().upload(urec0), case.vfv.BuildRec()
+        r, rn = case.vfv.BuildRec().upload(arms.urec0), case.vfv.BuildRec()
         dts = ev.BuildScalar()
         dt_exp = ev.EvaluateDt(dts, u, args.cfl, 1e100, False)
         ev.SetLinearSolver(args.gmres, 10, 2)
@@ -324,9 +421,33 @@ def main():
                 "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
                 "eval_seed": 9, "tolerance": 1e-10, **ev_ws}
 
+    riemann = None
+    if args.riemann_200 and args.case != "riemann":
+        orc = None
+        del arms, ev  # frees the workload's device context before the next one
+        log("riemann_2M: building mesh + VR setup, 1024x1024 quads")
+        rcase = make_case("riemann", 1024, args.threads)
+        rargs = argparse.Namespace(**vars(args))
+        rarms = Arms(rcase, rargs, dict(muGas=0.0, prGas=0.72, Rgas=1.0, TRef=1.0, CSutherland=0.5))
+        re, rprof, ru, _, _ = rarms.timed(False, rcase.steps, 0)
+        riemann = {"workload": f"2D Riemann, 1024x1024 quads split to {rcase.n_cell} triangles, jitter 0.2h, p=3 VR, "
+                               f"HLLC, WBAP V2, SSP-RK3 CFL {args.cfl}, {args.sweeps} sweeps/stage, "
+                               f"{rcase.steps} steps from the initial state",
+                   "cells": rcase.n_cell, "steps": rcase.steps, "value_fp64": rcase.n_cell * 3 * rcase.steps / re,
+                   "ms_per_step_fp64": re / rcase.steps * 1e3,
+                   "jacobi_frac": rprof["k_jacobi"]["gbs"] / HBM_PEAK_GBS,
+                   "evaluate_rhs": evaluate_rhs_object(rprof, rcase)}
+        if args.mixed:
+            me, mprof, mu, _, _ = rarms.timed(True, rcase.steps, 0)
+            rvol = rcase.vfv.arrays(["volume"])["volume"]
+            riemann.update({"value_mixed": rcase.n_cell * 3 * rcase.steps / me,
+                            "ms_per_step_mixed": me / rcase.steps * 1e3, "speedup": re / me,
+                            "rel_l2_mixed_vs_fp64": rel_l2(mu, ru, rvol)})
+        log(f"riemann_2M: {riemann}")
+
     wbap = "V2" if args.limiter_procedure == 0 else "V3"
     out = {
-        "metric": "p=3 VR+RHS cell-updates/s",
+        "metric": "p=3 VR+RHS cell-updates/s; plain FP64 vs FP32+learned at equal L2 error",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -358,6 +479,9 @@ def main():
         "cpu_baseline": cpu,
         "copy_probe": copy,
         "mixed_precision": mixed,
+        "oracle_comparison": oracle_cmp,
+        "accuracy_vortex": accuracy,
+        "riemann_2M": riemann,
         "implicit_lusgs": implicit,
         "learned_warm_start": warm,
     }

@@ -168,6 +168,7 @@ SIGNATURES = [
     ("dndsmi_update_lusgs_backward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p]),
     ("dndsmi_evaluate_residual", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, PD]),
+    ("dndsmi_evaluate_residual_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, PD]),
     ("dndsmi_implicit_euler_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                                              C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, PI32, PD]),
     ("dndsmi_lusgs_matrix_vec", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -141,6 +141,9 @@ struct dndsmi_ctx {
     double *imp_stage = nullptr;      // [3][ldc][4] SDIRK4 stage right-hand sides (first use)
     double *res_part = nullptr;       // [4][grid]
     double *res = nullptr;            // [4]
+    // device address of this GPU's static scratch of the allocation-free
+    // residual (set at context creation; never freed: it is not an allocation)
+    double *noalloc_scratch = nullptr;
     // per-kernel HIP-event timing (dndsmi_profile_*)
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
@@ -202,6 +205,9 @@ void prof_begin(dndsmi_ctx *c, int id);
 void prof_end(dndsmi_ctx *c);
 // reads and clears the device error word; a status for the first bit set
 int check_errw(dndsmi_ctx *c);
+// looks up this GPU's static scratch of dndsmi_evaluate_residual_noalloc
+// (once per device; the device must be current) into c->noalloc_scratch
+int noalloc_scratch_init(dndsmi_ctx *c);
 bool same_ctx(const dndsmi_ctx *c, const dndsmi_array *a, int32_t kind);
 // the explicit path's launches (ctx.hip)
 int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t);

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "ctx.hpp"
@@ -661,14 +663,16 @@ int fsolve(dndsmi_ctx *c, const double *dtau, double dt, double alpha, const dou
 }
 
 // residual into c->res (device) and then into host res[4]; x += r first when x
-int residual(dndsmi_ctx *c, const double *r, double *x, int32_t P, int32_t vol_wise, double *res_host) {
+// part: [4][grid_for(nc)] block partials, res_dev: [4]; both written before read
+int residual(dndsmi_ctx *c, const double *r, double *x, int32_t P, int32_t vol_wise, double *res_host, double *part,
+             double *res_dev) {
     const unsigned g = grid_for(c->nc);
     hipLaunchKernelGGL(k_residual, dim3(g), dim3(kBlock), 0, c->stream, int32_t(c->nc), r, x, c->vol, P, vol_wise,
-                       c->res_part);
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, c->stream, int32_t(g), P, c->res_part, c->res);
+                       part);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kBlock), 0, c->stream, int32_t(g), P, part, res_dev);
     DNDSMI_TRY(c->launch_check("residual kernels"));
     double h[4];
-    DNDSMI_TRY(c->hip(hipMemcpyAsync(h, c->res, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync"));
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(h, res_dev, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync"));
     DNDSMI_TRY(check_errw(c));
     for (int v = 0; v < 4; v++)
         res_host[v] = P >= 3 ? h[v] : (P == 2 ? std::sqrt(h[v]) : h[v]);
@@ -742,7 +746,68 @@ extern "C" int dndsmi_evaluate_residual(dndsmi_ctx *c, const dndsmi_array *rhs, 
     if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !res || P < 1)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     DNDSMI_TRY(ensure_implicit_scratch(c));
-    return residual(c, rhs->d, nullptr, P, volWise, res);
+    return residual(c, rhs->d, nullptr, P, volWise, res, c->res_part, c->res);
+}
+
+// ---- the allocation-free residual ---------------------------------------
+//
+// dndsmi_evaluate_residual_noalloc computes what dndsmi_evaluate_residual
+// does, with the same kernels, but keeps the block partials and the four
+// results in `scratch`, a static array of the library on each GPU, instead
+// of the context's lazily allocated buffers.  Host code must not pass
+// `scratch` itself to a kernel (on the host the name is only a stand-in);
+// its device address is looked up once per device and kept in
+// g_scratch_dev.  Every k_residual block writes its partials before
+// k_residual_final reads them, so nothing needs clearing between calls.
+// All calls on one GPU share the array: calls that overlap in time on one
+// GPU (from other streams or host threads) race.
+constexpr int64_t kNoAllocMaxBlocks = 65536; // n_cell <= 65536 * 256 = 16,777,216
+constexpr int64_t kNoAllocScratch = 4 * kNoAllocMaxBlocks + 4;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+} // namespace
+
+namespace dndsmi_host {
+int noalloc_scratch_init(dndsmi_ctx *c) {
+    if (c->device < 0 || c->device >= kMaxDevices) {
+        c->err = "device ordinal beyond the allocation-free scratch table";
+        return DNDSMI_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            c->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return DNDSMI_ERR_HIP;
+        }
+        g_scratch_dev[c->device] = static_cast<double *>(p);
+    }
+    c->noalloc_scratch = g_scratch_dev[c->device];
+    return DNDSMI_OK;
+}
+} // namespace dndsmi_host
+
+extern "C" int dndsmi_evaluate_residual_noalloc(dndsmi_ctx *c, const dndsmi_array *rhs, int32_t P, int32_t volWise,
+                                                double *res) {
+    if (!c || !same_ctx(c, rhs, DNDSMI_ARRAY_DOF) || !res || P < 1)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    const int64_t g = grid_for(c->nc);
+    if (g > kNoAllocMaxBlocks) {
+        c->err = "dndsmi_evaluate_residual_noalloc: n_cell above the static scratch's maximum of " +
+                 std::to_string(kNoAllocMaxBlocks * kBlock) + " cells";
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    }
+    if (!c->noalloc_scratch) {
+        c->err = "dndsmi_evaluate_residual_noalloc: the static scratch was not looked up";
+        return DNDSMI_ERR_HIP;
+    }
+    double *part = c->noalloc_scratch;
+    return residual(c, rhs->d, nullptr, P, volWise, res, part, part + 4 * kNoAllocMaxBlocks);
 }
 
 extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
@@ -782,7 +847,7 @@ extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi
         DNDSMI_TRY(fsolve(c, c->imp_dtau, dt, alpha, c->imp_rhs, u->d, c->imp_xinc));
         // x += xinc, fstop on the L1 norm of xinc
         double res[4];
-        DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
+        DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res, c->res_part, c->res));
         if (res_hist)
             for (int v = 0; v < 4; v++)
                 res_hist[4 * (iter - 1) + v] = res[v];
@@ -899,7 +964,7 @@ extern "C" int dndsmi_sdirk4_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
             DNDSMI_TRY(c->launch_check("k_sdirk_rhs"));
             DNDSMI_TRY(fsolve(c, c->imp_dtau, dt, aii, c->imp_rhs, u->d, c->imp_xinc));
             double res[4];
-            DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
+            DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res, c->res_part, c->res));
             if (res_hist)
                 for (int v = 0; v < 4; v++)
                     res_hist[(int64_t(ib) * max_iter + (iter - 1)) * 4 + v] = res[v];
@@ -1027,7 +1092,7 @@ extern "C" int dndsmi_bdf_step(dndsmi_bdf *b, dndsmi_array *u, dndsmi_array *uRe
         // fsolve with alphaDiag = alpha0
         DNDSMI_TRY(fsolve(c, c->imp_dtau, dt, alpha0, c->imp_rhs, u->d, c->imp_xinc));
         double res[4];
-        DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res));
+        DNDSMI_TRY(residual(c, c->imp_xinc, u->d, 1, 0, res, c->res_part, c->res));
         if (res_hist)
             for (int v = 0; v < 4; v++)
                 res_hist[4 * (iter - 1) + v] = res[v];

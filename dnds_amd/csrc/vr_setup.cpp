@@ -489,7 +489,8 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
     if (!mesh || !setting || !outp)
         return DNDSMI_ERR_INVALID_ARGUMENT;
     *outp = nullptr;
-    if (setting->rec_order != 3 || setting->use_local_coord || setting->orthogonalize_base || setting->norm_wbap ||
+    if ((setting->rec_order != 3 && setting->rec_order != 1) || (setting->rec_order == 1 && setting->build_limiter_data) ||
+        setting->use_local_coord || setting->orthogonalize_base || setting->norm_wbap ||
         setting->tang_weight_mod_min != 1.0 || setting->weight_scheme_geom < 0 || setting->weight_scheme_geom > 3 ||
         setting->weight_scheme_dir < 0 || setting->weight_scheme_dir > 1)
         return DNDSMI_ERR_UNSUPPORTED;
@@ -710,10 +711,33 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
                     P[lf][g] = ff_params(C, f, g, &out->face_weight[10 * f]);
                 }
             }
+            // P_ORDER 1: NDOF = NDIFF = 3, 2 rec dofs; the functional is the
+            // plain weighted product DiffI^T diag(W)^2 DiffJ over the rows
+            // 0, d/dx, d/dy (FFaceFunctional's branch for 3 weights,
+            // DNDS_FV_VR.hpp:1283-1286) instead of the norm functional.  The
+            // 2 x 2 matrices sit in the top-left corner of the 9 x 9 slots,
+            // the rest zero, so the p = 3 kernels run them unchanged.
+            const bool p1 = setting->rec_order == 1;
+            const int nr = p1 ? 2 : 9;
+            auto conj_p1 = [&](int64_t f, const double Di[10][10], int i, const double Dj[10][10], int j) {
+                const double *W = &out->face_weight[10 * f];
+                double c = 0;
+                for (int k = 0; k < 3; k++)
+                    c += ((Di[k][i] * W[k]) * W[k]) * Dj[k][j];
+                return c;
+            };
             for (int lf = 0; lf < nfc; lf++) {
                 const int64_t f = m.cell2face[b0 + lf];
                 const double nl = norm2(out->face_norm[6 * f], out->face_norm[6 * f + 1]);
                 for (int g = 0; g < 3; g++) {
+                    if (p1) {
+                        for (int i = 1; i <= nr; i++)
+                            for (int j = 1; j <= nr; j++) {
+                                double inc = conj_p1(f, DI[lf][g], i, DI[lf][g], j) * nl;
+                                A[9 * (i - 1) + (j - 1)] += inc * kLine3W[g];
+                            }
+                        continue;
+                    }
                     double a[10][10];
                     for (int i = 0; i < 10; i++)
                         ff_project(P[lf][g], DI[lf][g], i, a[i]);
@@ -724,11 +748,18 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
                         }
                 }
             }
-            double I9[81];
-            for (int i = 0; i < 81; i++)
-                I9[i] = (i % 10 == 0) ? 1.0 : 0.0;
-            double Ainv[81];
-            lstsq(9, A, I9, 9, Ainv);
+            double Ainv[81] = {0};
+            {
+                // the pseudo-inverse of the nr x nr block (row-major, stride 9 in A)
+                double Ab[81], Ib[81], Xb[81];
+                for (int i = 0; i < nr; i++)
+                    for (int j = 0; j < nr; j++)
+                        Ab[nr * i + j] = A[9 * i + j], Ib[nr * i + j] = i == j ? 1.0 : 0.0;
+                lstsq(nr, Ab, Ib, nr, Xb);
+                for (int i = 0; i < nr; i++)
+                    for (int j = 0; j < nr; j++)
+                        Ainv[9 * i + j] = Xb[nr * i + j];
+            }
             std::memcpy(&out->ainv[81 * ic], Ainv, sizeof(Ainv));
             double rows[4][9];
             for (int lf = 0; lf < nfc; lf++) {
@@ -746,6 +777,14 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
                             face_dibj(C, f, 1 - side, kLine3X[g], DJ);
                         else
                             std::memcpy(DJ, DI[lf][g], sizeof(DJ));
+                        if (p1) {
+                            for (int i = 1; i <= nr; i++)
+                                for (int j = 1; j <= nr; j++) {
+                                    double inc = conj_p1(f, DI[lf][g], i, DJ, j) * nl;
+                                    B[9 * (i - 1) + (j - 1)] += inc * kLine3W[g];
+                                }
+                            continue;
+                        }
                         double a[10][10], bb[10][10];
                         for (int i = 0; i < 10; i++) {
                             ff_project(P[lf][g], DI[lf][g], i, a[i]);
@@ -760,7 +799,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
                     for (int r = 0; r < 9; r++)
                         for (int c = 0; c < 9; c++) {
                             double acc = 0;
-                            for (int k = 0; k < 9; k++)
+                            for (int k = 0; k < nr; k++)
                                 acc += Ainv[9 * r + k] * B[9 * k + c];
                             M[9 * r + c] = acc;
                         }
@@ -774,7 +813,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
                 const double w0 = out->face_weight[10 * f];
                 double row[9] = {0};
                 for (int g = 0; g < 3; g++)
-                    for (int c = 0; c < 9; c++) {
+                    for (int c = 0; c < nr; c++) {
                         double inc = ((1.0 * w0) * w0) * DI[lf][g][0][c + 1];
                         inc *= nl;
                         row[c] += inc * kLine3W[g];
@@ -786,7 +825,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
             for (int lf = 0; lf < nfc; lf++)
                 for (int r = 0; r < 9; r++) {
                     double acc = 0;
-                    for (int k = 0; k < 9; k++)
+                    for (int k = 0; k < nr; k++)
                         acc += rows[lf][k] * Ainv[9 * r + k];
                     out->ainv_bvec[9 * (b0 + lf) + r] = acc;
                 }

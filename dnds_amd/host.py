@@ -256,11 +256,14 @@ class ExternalVR:
 class VRFiniteVolume2D:
     """Geometry + VR setup (host FP64) and the Jacobi sweep (device)."""
 
-    def __init__(self, mesh: CompactFacedMeshSerialRW, setting: A.VRSetting | None = None, P_ORDER: int = 3):
+    def __init__(self, mesh: CompactFacedMeshSerialRW, setting: A.VRSetting | None = None, P_ORDER: int | None = None):
+        """P_ORDER (DNDS_FV_VR.hpp:244): 3 or 1; None keeps the setting's
+        rec_order (3 by default)."""
         self.mesh = mesh
         self.setting = setting if setting is not None else vr_setting()
-        self.setting.rec_order = P_ORDER
-        self.P_ORDER = P_ORDER
+        if P_ORDER is not None:
+            self.setting.rec_order = P_ORDER
+        self.P_ORDER = int(self.setting.rec_order)
         self._h = None
         self.view = None
         self.device = None  # set by EulerEvaluator

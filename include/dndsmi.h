@@ -363,6 +363,18 @@ int dndsmi_update_lusgs_backward(dndsmi_ctx *ctx, double alphaDiag, const dndsmi
  * for P = 1, 2, max |r| for P >= 3; res receives 4 doubles (synchronises). */
 int dndsmi_evaluate_residual(dndsmi_ctx *ctx, const dndsmi_array *rhs, int32_t P, int32_t volWise, double *res);
 
+/* No counterpart in the reference.  dndsmi_evaluate_residual for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same result from the same
+ * kernels, with the block partials kept in a static scratch array of the
+ * library (2 MiB per GPU, held until the process ends) instead of buffers the
+ * context allocates on first use.  It calls no hipMalloc or hipFree.  n_cell
+ * at most 16,777,216 (DNDSMI_ERR_INVALID_ARGUMENT above that).  All calls on
+ * one GPU share the scratch: calls that overlap in time on one GPU (other
+ * streams or host threads) race. */
+int dndsmi_evaluate_residual_noalloc(dndsmi_ctx *ctx, const dndsmi_array *rhs, int32_t P, int32_t volWise,
+                                     double *res);
+
 /* ODE::ImplicitEulerDualTimeStep::Step (DNDS_ODE.hpp:201-265) with the
  * solver's lambdas (DNDS_FV_EulerSolver.hpp:713-1000) for jacobianTypeCode 0,
  * gmresCode 0, nSGSIterationInternal 0, no limiter: per iteration

@@ -14,6 +14,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the CPU-baseline copy: the same source compiled for speed on the machine
+# that runs it (-O3 -march=native, contraction to FMA allowed); built where it
+# is timed, since -march=native code need not run on another host
+BASELINE_LIB = os.path.join(HERE, "_baseline", "liboracle_baseline.so")
+BASELINE_FLAGS = ["-O3", "-march=native", "-ffp-contract=fast", "-std=c++17", "-fPIC", "-shared", "-pthread"]
 
 PD = C.POINTER(C.c_double)
 PI64 = C.POINTER(C.c_int64)
@@ -49,15 +54,38 @@ def build(quiet: bool = True) -> str:
     return LIB
 
 
-_lib = None
+def build_baseline() -> str:
+    """Compile the CPU-baseline copy (BASELINE_FLAGS) into oracle/_baseline/
+    on this host; it is timed by the benchmark, never used as the checker."""
+    os.makedirs(os.path.dirname(BASELINE_LIB), exist_ok=True)
+    tmp = BASELINE_LIB + ".tmp"
+    out = subprocess.run(["g++", *BASELINE_FLAGS, os.path.join(HERE, "oracle.cpp"), "-o", tmp], capture_output=True,
+                         text=True)
+    if out.returncode != 0:
+        raise RuntimeError("baseline oracle build failed:\n" + out.stdout + out.stderr)
+    os.replace(tmp, BASELINE_LIB)
+    return BASELINE_LIB
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            build()
-        L = C.CDLL(LIB)
+_libs = {}
+
+
+def lib(variant: str = "parity") -> C.CDLL:
+    """The parity build (liboracle.so, -O2 -ffp-contract=off: every product
+    and sum rounded as written) or, with variant="baseline", the speed build
+    of the CPU baseline (build_baseline first)."""
+    if variant not in _libs:
+        if variant == "parity":
+            if not os.path.exists(LIB):
+                build()
+            path = LIB
+        elif variant == "baseline":
+            if not os.path.exists(BASELINE_LIB):
+                raise RuntimeError("the baseline oracle is not built: call oracle.build_baseline() first")
+            path = BASELINE_LIB
+        else:
+            raise ValueError(variant)
+        L = C.CDLL(path)
         sig = {
             "oracle_jacobi_step": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, C.c_double],
             "oracle_evaluate_dt": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, C.c_double,
@@ -97,8 +125,8 @@ def lib() -> C.CDLL:
             f = getattr(L, name)
             f.restype = C.c_int
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _d(a):
@@ -121,7 +149,7 @@ class Oracle:
                  tang_weight_direction: int = 0, wbap_smooth_indicator_scale: float = 1e-10,
                  wbap_n_std: float = 10.0, gmres_code: int = 0, n_gmres_space: int = 10, n_gmres_iter: int = 2,
                  limiter_procedure: int = 0, mu_gas: float = 0.0, pr_gas: float = 0.72, r_gas: float = 1.0,
-                 t_ref: float = 273.15, c_sutherland: float = 110.4):
+                 t_ref: float = 273.15, c_sutherland: float = 110.4, variant: str = "parity"):
         self._keep = {}
 
         def keep(name, a, dtype):
@@ -145,7 +173,7 @@ class Oracle:
         self.cfg = OSetting(riemann_solver, gamma, ff, jacobi_relax, tang_weight, tang_weight_direction,
                             wbap_smooth_indicator_scale, wbap_n_std, gmres_code, n_gmres_space, n_gmres_iter,
                             limiter_procedure, mu_gas, pr_gas, r_gas, t_ref, c_sutherland)
-        self.lib = lib()
+        self.lib = lib(variant)
 
     def _check(self, rc, what):
         if rc != 0:
@@ -341,6 +369,6 @@ def flux_jacobian_times_du(U, unit_norm, dU, lambda_main=0.0, gamma=1.4):
     return dF
 
 
-def set_threads(n: int):
+def set_threads(n: int, variant: str = "parity"):
     """Worker threads of the oracle's per-cell / per-face loops."""
-    lib().oracle_set_threads(int(n))
+    lib(variant).oracle_set_threads(int(n))

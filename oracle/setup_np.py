@@ -1,5 +1,7 @@
 """numpy restatement of the geometry and VR setup (DNDS_FV_VR.hpp:29-102,
-DNDS_FV_VR.cpp:82-1154) for small meshes, global-coordinate basis, p = 3.
+DNDS_FV_VR.cpp:82-1154) for small meshes, global-coordinate basis, p = 3 or
+p = 1 (setting "rec_order"; p = 1 matrices in the top-left 2 x 2 corner of
+the 9 x 9 blocks, the rest zero).
 
 TEST INFRASTRUCTURE ONLY: an independent second implementation that the
 tests hold the product's host setup against.  It is written from the
@@ -199,6 +201,8 @@ def build(mesh: dict, setting: dict) -> dict:
             ms[f, 1] = pinv(AR) @ AL
 
     tn = setting["tang_weight_direction"] == 0
+    p1 = setting.get("rec_order", 3) == 1
+    nr = 2 if p1 else 9
     ainv = np.zeros((nc, 9, 9))
     ainv_b = np.zeros((cp[-1], 9, 9))
     ainv_bvec = np.zeros((cp[-1], 9))
@@ -209,7 +213,15 @@ def build(mesh: dict, setting: dict) -> dict:
         for f in faces:
             side = 0 if f2c[f, 0] == i else 1
             nl = np.linalg.norm(fn[f])
-            P, q = functional_rows(W[f], fn[f], setting["tang_weight"], tn)
+            if p1:
+                # FFaceFunctional with 3 weights: DiffI^T diag(W)^2 DiffJ over
+                # the rows 0, d/dx, d/dy (DNDS_FV_VR.hpp:1283-1286)
+                P = np.zeros((10, 10))
+                P[0, 0] = P[1, 1] = P[2, 2] = 1.0
+                q = np.zeros(10)
+                q[:3] = W[f, :3] ** 2
+            else:
+                P, q = functional_rows(W[f], fn[f], setting["tang_weight"], tn)
             other = f2c[f, 1 - side]
             B = np.zeros((9, 9))
             row = np.zeros(9)
@@ -224,9 +236,13 @@ def build(mesh: dict, setting: dict) -> dict:
                     B += (PI.T @ (q[:, None] * PI))[1:, 1:] * nl * W3[g]
                 row += W[f, 0] ** 2 * DI[0, 1:] * nl * W3[g]
             blocks.append((B, row))
-        Ai = pinv(A)
+        Ai = np.zeros((9, 9))
+        Ai[:nr, :nr] = pinv(A[:nr, :nr])
         ainv[i] = Ai
         for k, (B, row) in enumerate(blocks):
+            B[nr:, :] = 0.0
+            B[:, nr:] = 0.0
+            row[nr:] = 0.0
             ainv_b[cp[i] + k] = Ai @ B
             ainv_bvec[cp[i] + k] = Ai @ row
     return dict(volume=vol, face_area=area, cell_bary=bary.ravel(), base_moment=mom.ravel(), face_center=fc.ravel(),
