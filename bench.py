@@ -42,6 +42,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
+# the learned warm start's plain check: steps per check, steps between a
+# passed check and the next, tolerance on the relative L2 difference from
+# the plain run (1 % of the vortex case's L2 discretisation error, 2.5e-5)
+LEARNED_CHECK = (5, 20, 2.5e-7)
+
 # committed rocprofv3 FETCH_SIZE + WRITE_SIZE summaries, by workload
 PROFILES = {("riemann", 1024): os.path.join(ROOT, "profiles", "riemann_2M_kernels_traffic.md"),
             ("step", 2048): os.path.join(ROOT, "profiles", "step_6M_kernels_traffic.md")}
@@ -114,26 +119,36 @@ class Arms:
         c = self.case
         return self.ev.BuildMean().upload(c.u0), c.vfv.BuildRec().upload(self.urec0), c.vfv.BuildRec()
 
-    def ode(self, mixed):
+    def ode(self, mixed, ws=None):
         from dnds_amd.host import ExplicitSSPRK3LocalDt
-        return ExplicitSSPRK3LocalDt(self.ev, nInternalRecStep=self.args.sweeps, CFL=self.args.cfl,
-                                     useLimiter=self.case.limiter, mixedPrecision=mixed,
-                                     limiterProcedure=self.args.limiter_procedure)
+        self.last_ode = ExplicitSSPRK3LocalDt(self.ev, nInternalRecStep=self.args.sweeps, CFL=self.args.cfl,
+                                              useLimiter=self.case.limiter, mixedPrecision=mixed,
+                                              limiterProcedure=self.args.limiter_procedure, warmStart=ws,
+                                              checkSteps=LEARNED_CHECK[0], recheckInterval=LEARNED_CHECK[1],
+                                              tolerance=LEARNED_CHECK[2])
+        return self.last_ode
 
-    def steps(self, mixed, n):
+    def warm_start(self, params):
+        """the trained network on this workload's context"""
+        from dnds_amd.host import WarmStart
+        ws = WarmStart(self.ev, seed=0, capacity=0)
+        ws.set_params(params)
+        return ws
+
+    def steps(self, mixed, n, ws=None):
         """n steps from the initial state, untimed: (u, uRec, t)."""
         u, r, rn = self.fresh()
-        ode, t = self.ode(mixed), 0.0
+        ode, t = self.ode(mixed, ws), 0.0
         for _ in range(n):
             t += ode.Step(u, r, rn, t)
         return u.download(), r.download(), t
 
-    def timed(self, mixed, steps, warmup):
+    def timed(self, mixed, steps, warmup, ws=None):
         """warmup steps, then `steps` timed steps continuing from them; the
         per-kernel HIP-event times of the timed steps"""
-        tag = "mixed" if mixed else "fp64"
+        tag = ("mixed+learned" if ws is not None else "mixed") if mixed else "fp64"
         u, r, rn = self.fresh()
-        ode, t = self.ode(mixed), 0.0
+        ode, t = self.ode(mixed, ws), 0.0
         for _ in range(warmup):
             t += ode.Step(u, r, rn, t)
         self.ev.device.sync()
@@ -170,23 +185,50 @@ def evaluate_rhs_object(prof, case):
             "frac": gbs / HBM_PEAK_GBS, "kernels": ks}
 
 
-def vortex_accuracy(args):
+def vortex_accuracy(args, params=None):
     """The small isentropic-vortex case (40 x 40 quads split into 3200
     triangles, Roe, no limiter, 5 sweeps, CFL 0.5, 100 steps): density L2
-    error against the exact solution for the FP64 and the mixed arm."""
+    error against the exact solution for the FP64 arm, the mixed arm and
+    (with trained parameters) the mixed arm with the learned warm start."""
     from dnds_amd import cases
     case = cases.isentropic_vortex(40, nthreads=args.threads)
     arms = Arms(case, argparse.Namespace(sweeps=5, cfl=0.5, limiter_procedure=0), {})
     vol = case.vfv.arrays(["volume"])["volume"]
     out = {"workload": "periodic isentropic vortex, 40x40 quads split to 3200 triangles, Roe, no limiter, "
                        "5 sweeps/stage, CFL 0.5, 100 steps"}
-    for tag, mixed in (("fp64", False), ("mixed", True)):
-        u, _, t = arms.steps(mixed, case.steps)
+    runs = [("fp64", False, None), ("mixed", True, None)]
+    if params is not None:
+        runs.append(("mixed_learned", True, arms.warm_start(params)))
+    for tag, mixed, ws in runs:
+        u, _, t = arms.steps(mixed, case.steps, ws)
         ex = cases.cell_means(case.mesh, vol, lambda x, y: cases.vortex_exact(x, y, t))
         out[f"l2_density_error_{tag}"] = float(np.sqrt(np.sum((u[:, 0] - ex[:, 0]) ** 2 * vol) / vol.sum()))
+        if ws is not None:
+            out["warm_start_stats"] = arms.last_ode.WarmStartStats()
     out["mixed_over_fp64"] = out["l2_density_error_mixed"] / out["l2_density_error_fp64"]
+    if params is not None:
+        out["mixed_learned_over_fp64"] = out["l2_density_error_mixed_learned"] / out["l2_density_error_fp64"]
     log(f"vortex accuracy: {out}")
     return out
+
+
+def train_warm_start(args):
+    """The learned warm start's network: converged reconstructions of the
+    40 x 40 vortex at mesh seeds 1..8 (`args.learned_snapshots` snapshots
+    each, one step apart), 12 epochs on the device; (parameters, summary)."""
+    from dnds_amd import learned
+    w0 = time.time()
+    X, T = learned.harvest(range(1, 9), n=40, snapshots=args.learned_snapshots, steps_between=1,
+                           nthreads=args.threads, log=log)
+    w1 = time.time()
+    params, hist = learned.train(X, T, epochs=12, batch=4096, lr=2e-3, lr_final=1e-5, seed=0, n=40,
+                                 nthreads=args.threads)
+    w2 = time.time()
+    log(f"warm start: {X.shape[0]} pairs harvested in {w1 - w0:.1f}s, trained 12 epochs in {w2 - w1:.1f}s, "
+        f"loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    return params, {"pairs": int(X.shape[0]), "harvest_s": w1 - w0, "train_s": w2 - w1,
+                    "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
+                    "training_data": "converged reconstructions of the 40x40 isentropic vortex, mesh seeds 1-8"}
 
 
 def main():
@@ -210,9 +252,17 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=10,
                     help="after the timed steps, N device copies of one rec array (a plain copy of known size "
                          "for scaling profiler byte counters), timed (0 = skip)")
+    ap.add_argument("--read-probe", type=int, default=10,
+                    help="after the timed steps, N launches of a kernel that streams the sweep's A^-1 B pairs as the "
+                         "sweep does, without its gathers: the read ceiling of the sweep's streaming part (0 = skip)")
     ap.add_argument("--mixed", type=int, default=1,
                     help="also run the mixed-precision arm (FP32 correction sweeps + FP64 refinement) over the same "
                          "steps from the same initial state")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also train the learned warm start (vortex data, on the device) and run the mixed arm with "
+                         "it and its plain fallback over the same steps")
+    ap.add_argument("--learned-snapshots", type=int, default=40,
+                    help="training snapshots per mesh seed (8 seeds of the 40x40 vortex, 3200 pairs each)")
     ap.add_argument("--accuracy", type=int, default=1,
                     help="both arms' L2 error against the exact solution on the 3200-cell isentropic vortex")
     ap.add_argument("--riemann-200", type=int, default=1,
@@ -278,6 +328,19 @@ def main():
                 "gbs": 2 * nbytes * args.copy_probe / el / 1e9, "frac_of_peak": 2 * nbytes * args.copy_probe / el / 1e9
                 / HBM_PEAK_GBS}
         log(f"copy probe: {copy['gbs']:.0f} GB/s read + write ({copy['frac_of_peak']:.2f} of peak)")
+    probe = None
+    if args.read_probe:
+        ev.device.sync()
+        lib.dndsmi_profile_enable(ev.device.ctx, 1)
+        lib.dndsmi_read_probe(ev.device.ctx, args.read_probe)
+        pp = kernel_profile(lib, ev.device.ctx, arms.A).get("k_read_probe")
+        lib.dndsmi_profile_enable(ev.device.ctx, 0)
+        if pp:
+            probe = {"launches": pp["launches"], "avg_ms": pp["avg_ms"], "bytes": pp["bytes"], "gbs": pp["gbs"],
+                     "frac_of_peak": pp["gbs"] / HBM_PEAK_GBS,
+                     "what": "16-byte non-temporal loads of the sweep's A^-1 B pairs, as the sweep issues them, "
+                             "one double written per cell"}
+            log(f"read probe: {pp['gbs']:.0f} GB/s ({pp['gbs'] / HBM_PEAK_GBS:.2f} of peak)")
     mixed = None
     if args.mixed:
         m_elapsed, m_prof, u_mixed, _, m_t = arms.timed(True, args.steps, args.warmup)
@@ -291,6 +354,27 @@ def main():
                  "kernels": m_prof, "evaluate_rhs": evaluate_rhs_object(m_prof, case)}
         log(f"mixed: {mixed['value']:.3e} cell-updates/s, x{mixed['speedup_vs_fp64']:.3f} vs fp64, "
             f"rel. L2 vs the fp64 arm after {args.warmup + args.steps} steps {l2}")
+    params = None
+    if args.mixed and args.learned:
+        # the network is trained on the vortex and carried to this workload
+        # (its inputs are scale-free stencil differences and cell moments)
+        params, train_info = train_warm_start(args)
+        ws = arms.warm_start(params)
+        l_elapsed, l_prof, u_learned, _, _ = arms.timed(True, args.steps, args.warmup, ws)
+        stats = arms.last_ode.WarmStartStats()
+        l2l = rel_l2(u_learned, u_fp64, vol)
+        mixed["learned_warm_start"] = {
+            "value": nc * 3 * args.steps / l_elapsed, "ms_per_step": l_elapsed / args.steps * 1e3,
+            "speedup_vs_fp64": elapsed / l_elapsed, "rel_l2_vs_fp64_arm": l2l,
+            "rel_max_vs_fp64_arm": float(np.abs(u_learned - u_fp64).max() / np.abs(u_fp64).max()),
+            "fallback": (f"the first {LEARNED_CHECK[0]} steps, and {LEARNED_CHECK[0]} steps every "
+                         f"{LEARNED_CHECK[1]} after a passed check, run plain on shadow copies too; a corrected state "
+                         f"more than {LEARNED_CHECK[2]:g} (relative L2) from the plain one, or a failed corrected "
+                         "run, puts the plain state back and ends the correction"),
+            "stats_warmup_and_timed": stats, "training": train_info}
+        log(f"mixed + learned: {mixed['learned_warm_start']['value']:.3e} cell-updates/s, "
+            f"x{mixed['learned_warm_start']['speedup_vs_fp64']:.3f} vs fp64, stats {stats}, rel. L2 vs fp64 {l2l}")
+        del ws
 
     stages = 3 * args.steps
     value = nc * stages / elapsed
@@ -302,6 +386,10 @@ def main():
                 "bytes_per_launch": jac["bytes"], "avg_ms": jac["avg_ms"]}
         kname = "k_jacobi<4, false>" if args.case == "step" else "k_jacobi<3, false>"
         roof["traffic"], roof["traffic_source"] = measured_traffic(args.case, args.n, kname)
+        if probe:
+            roof["algorithmic_rate_over_read_probe"] = jac["gbs"] / probe["gbs"]
+            if roof["traffic"]:
+                roof["traffic_rate_over_read_probe"] = roof["traffic"] / (jac["avg_ms"] * 1e-3) / 1e9 / probe["gbs"]
     rhs_obj = evaluate_rhs_object(prof, case)
 
     cpu = None
@@ -337,10 +425,13 @@ def main():
             oracle_cmp[f"rel_max_{tag}"] = float(np.abs(gu - ou).max() / np.abs(ou).max())
         log(f"vs oracle after one step: {oracle_cmp}")
 
-    accuracy = vortex_accuracy(args) if args.accuracy else None
+    accuracy = vortex_accuracy(args, params) if args.accuracy else None
 
     implicit = None
     if args.implicit:
+        import ctypes as C
+
+        from dnds_amd import _capi as A
         from dnds_amd.host import ImplicitEulerDualTimeStep
         u = ev.BuildMean().upload(case.u0)
         r, rn = case.vfv.BuildRec().upload(arms.urec0), case.vfv.BuildRec()
@@ -429,20 +520,24 @@ def main():
         rcase = make_case("riemann", 1024, args.threads)
         rargs = argparse.Namespace(**vars(args))
         rarms = Arms(rcase, rargs, dict(muGas=0.0, prGas=0.72, Rgas=1.0, TRef=1.0, CSutherland=0.5))
-        re, rprof, ru, _, _ = rarms.timed(False, rcase.steps, 0)
         riemann = {"workload": f"2D Riemann, 1024x1024 quads split to {rcase.n_cell} triangles, jitter 0.2h, p=3 VR, "
                                f"HLLC, WBAP V2, SSP-RK3 CFL {args.cfl}, {args.sweeps} sweeps/stage, "
-                               f"{rcase.steps} steps from the initial state",
-                   "cells": rcase.n_cell, "steps": rcase.steps, "value_fp64": rcase.n_cell * 3 * rcase.steps / re,
-                   "ms_per_step_fp64": re / rcase.steps * 1e3,
-                   "jacobi_frac": rprof["k_jacobi"]["gbs"] / HBM_PEAK_GBS,
-                   "evaluate_rhs": evaluate_rhs_object(rprof, rcase)}
-        if args.mixed:
-            me, mprof, mu, _, _ = rarms.timed(True, rcase.steps, 0)
-            rvol = rcase.vfv.arrays(["volume"])["volume"]
-            riemann.update({"value_mixed": rcase.n_cell * 3 * rcase.steps / me,
-                            "ms_per_step_mixed": me / rcase.steps * 1e3, "speedup": re / me,
-                            "rel_l2_mixed_vs_fp64": rel_l2(mu, ru, rvol)})
+                               f"{rcase.steps} steps from the initial state", "cells": rcase.n_cell,
+                   "steps": rcase.steps}
+        try:
+            re, rprof, ru, _, _ = rarms.timed(False, rcase.steps, 0)
+            riemann.update({"value_fp64": rcase.n_cell * 3 * rcase.steps / re,
+                            "ms_per_step_fp64": re / rcase.steps * 1e3,
+                            "jacobi_frac": rprof["k_jacobi"]["gbs"] / HBM_PEAK_GBS,
+                            "evaluate_rhs": evaluate_rhs_object(rprof, rcase)})
+            if args.mixed:
+                me, mprof, mu, _, _ = rarms.timed(True, rcase.steps, 0)
+                rvol = rcase.vfv.arrays(["volume"])["volume"]
+                riemann.update({"value_mixed": rcase.n_cell * 3 * rcase.steps / me,
+                                "ms_per_step_mixed": me / rcase.steps * 1e3, "speedup": re / me,
+                                "rel_l2_mixed_vs_fp64": rel_l2(mu, ru, rvol)})
+        except DNDSError as e:
+            riemann["failure"] = str(e)
         log(f"riemann_2M: {riemann}")
 
     wbap = "V2" if args.limiter_procedure == 0 else "V3"
@@ -478,6 +573,7 @@ def main():
         "evaluate_rhs": rhs_obj,
         "cpu_baseline": cpu,
         "copy_probe": copy,
+        "read_probe": probe,
         "mixed_precision": mixed,
         "oracle_comparison": oracle_cmp,
         "accuracy_vortex": accuracy,

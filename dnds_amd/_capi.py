@@ -93,6 +93,7 @@ class VRView(C.Structure):
         ("face_center", PD), ("face_norm_center", PD), ("face_norm", PD), ("face_weight", PD),
         ("face_d0", PD), ("face_dibj_g0", PD), ("matrix_secondary", PD),
         ("ainv", PD), ("ainv_b", PD), ("ainv_bvec", PD), ("face_gauss_xy", PD), ("cell_scale", PD),
+        ("cell_frame", PD),
     ]
 
 
@@ -159,6 +160,9 @@ SIGNATURES = [
     ("dndsmi_warmstart_adam_step", C.c_int, [C.c_void_p, PF, C.c_double]),
     ("dndsmi_warmstart_train", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, PD]),
     ("dndsmi_warmstart_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("dndsmi_ssprk3_set_warm_start", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
+    ("dndsmi_read_probe", C.c_int, [C.c_void_p, C.c_int32]),
+    ("dndsmi_ssprk3_warm_start_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), PD]),
     ("dndsmi_reconstruct_to_tolerance", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                                   C.c_int32, C.c_double, PI32, PD]),
     ("dndsmi_lusgs_matrix_init", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
@@ -195,7 +199,7 @@ K_WBAP_CELL = 7
 K_WBAP_LIM = 8
 KERNEL_NAMES = ["k_jacobi", "k_wall_corr", "k_dt_face", "k_dt_cell", "k_rhs_face", "k_rhs_cell", "k_wbap_face",
                 "k_wbap_cell", "k_wbap_lim", "k_lusgs_sweep", "k_delta32", "k_rhs_face32", "k_rhs_cell32",
-                "k_lusgs_matvec", "k_rhs_bface"]
+                "k_lusgs_matvec", "k_rhs_bface", "k_read_probe"]
 
 _lib = None
 

@@ -136,13 +136,18 @@ def isentropic_vortex(n=40, seed=1, nthreads=0, build_limiter_data=False) -> Cas
     return Case("isentropic_vortex", mesh, vfv, "Roe", (1.0, 1.0, 1.0, 2.5 + 1.0), u0, 100, 5, 0.5, False)
 
 
-def riemann_2d(n=1024, seed=2, jitter=0.2, nthreads=0, build_limiter_data=True) -> Case:
+def riemann_2d(n=1024, seed=2, jitter=0.2, nthreads=0, build_limiter_data=True, wbap_scale=None) -> Case:
     """[0,1]^2, n x n quads cut along seeded diagonals (2 n^2 cells), interior
-    nodes jittered by U(-0.2h, 0.2h); HLLC, SSP-RK3, 200 steps."""
+    nodes jittered by U(-0.2h, 0.2h); HLLC, SSP-RK3, 200 steps.  wbap_scale:
+    WBAP_SmoothIndicatorScale (default that of the reference's 2D Riemann
+    configuration, 0.1)."""
     bc = A.BC_SPECIAL_2DRIEMANNFAR
     mesh = CompactFacedMeshSerialRW.generate(A.MESH_TRI_SPLIT, n, n, 0.0, 1.0, 0.0, 1.0, jitter=jitter, seed=seed,
                                              bc=(bc, bc, bc, bc))
-    vfv = VRFiniteVolume2D(mesh, vr_setting(**RIEMANN_VR_SETTING, build_limiter_data=int(build_limiter_data)))
+    st = dict(RIEMANN_VR_SETTING)
+    if wbap_scale is not None:
+        st["wbap_smooth_indicator_scale"] = wbap_scale
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**st, build_limiter_data=int(build_limiter_data)))
     vfv.Initialization(nthreads)
     bary = vfv.arrays(["cell_bary"])["cell_bary"].reshape(-1, 2)
     u0 = riemann_state(bary)
@@ -182,7 +187,16 @@ def bump_channel(nx=48, ny=16, kind=A.MESH_TRI_SPLIT, height=0.1, mach=0.5, seed
     return Case("bump_channel", mesh, vfv, "Roe", far, u0, 0, 1, 10.0, False)
 
 
-def flat_plate(nx=1536, ny=320, mach=0.2, reynolds=5e6, y_plus=1.0, height=1.0, upstream=0.0, nthreads=0) -> Case:
+# the reference's boundary-layer vfvSetting (data/eulerSA_config_BL.json:61-77):
+# isotropic lengths, no tangential weight, no geometric weighting, far-field
+# weight 1, wall weight 0.5
+BL_VR_SETTING = dict(anisotropic_lengths=0, tang_weight=0.0, tang_weight_direction=0, wall_weight=0.5,
+                     far_weight=1.0, weight_len_scale=1.0, weight_scheme_geom=0, weight_scheme_dir=0,
+                     jacobi_relax=1.0)
+
+
+def flat_plate(nx=1536, ny=320, mach=0.2, reynolds=1e5, y_plus=1.0, height=1.0, upstream=0.0, rec_order=3,
+               cfl=20.0, nthreads=0) -> Case:
     """Laminar flat plate (the NS_2D model: the 2D solver has no turbulence
     model): [0,1] x [0,height], nx x ny quads, the plate (Wall_NoSlip) along
     y = 0 from the leading edge at x = 0, Farfield on the other three sides.
@@ -191,16 +205,18 @@ def flat_plate(nx=1536, ny=320, mach=0.2, reynolds=5e6, y_plus=1.0, height=1.0, 
     CSutherland 110.4 K / 273.15 K of it (air), Prandtl 0.72, Rgas 1.  The y
     lines are spaced geometrically (one ratio for the whole height) so that
     the first cell's height puts the wall at y+ = y_plus for the turbulent
-    skin friction cf = 0.0576 Re^-0.2 at x = 1.  Roe flux, no limiter, VR
-    weights as the bump channel; initial state the free stream.  1 Jacobi
-    sweep per iteration; the implicit LU-SGS solver with local time steps at
-    CFL 20 drives it to steady state."""
+    skin friction cf = 0.0576 Re^-0.2 at x = 1.  Roe flux, no limiter, the
+    reference's boundary-layer VR settings (BL_VR_SETTING), P_ORDER
+    rec_order (3 or 1); initial state the free stream.  1 Jacobi sweep per
+    iteration; the implicit LU-SGS solver with local time steps at CFL `cfl`
+    drives it to steady state.  At Re 5e6 the laminar case diverges (the
+    wall cells are dominated by the inviscid terms); Re 1e5 converges."""
     far = bump_far(mach)
     mu = mach / reynolds
     u_tau = mach * np.sqrt(0.5 * 0.0576 * reynolds ** -0.2)
     h0 = y_plus * mu / u_tau
     # ratio r with h0 (r^ny - 1) / (r - 1) = height, by bisection
-    lo, hi = 1.0 + 1e-12, 2.0
+    lo, hi = 1.0 + 1e-12, 3.0
     for _ in range(200):
         r = 0.5 * (lo + hi)
         if h0 * (r ** ny - 1.0) / (r - 1.0) > height:
@@ -217,13 +233,12 @@ def flat_plate(nx=1536, ny=320, mach=0.2, reynolds=5e6, y_plus=1.0, height=1.0, 
         xm = 0.5 * (m["node_xy"][f2n[:, 0], 0] + m["node_xy"][f2n[:, 1], 0])
         bc[(bc == A.BC_WALL_NOSLIP) & (xm < 0.0)] = A.BC_WALL_EULER
         mesh = CompactFacedMeshSerialRW.from_cells(m["node_xy"], m["cell_ptr"], m["cell2node"], f2n, bc)
-    vfv = VRFiniteVolume2D(mesh, vr_setting(**dict(RIEMANN_VR_SETTING, far_weight=1.0, wall_weight=0.5),
-                                            build_limiter_data=0))
+    vfv = VRFiniteVolume2D(mesh, vr_setting(**BL_VR_SETTING, rec_order=rec_order, build_limiter_data=0))
     vfv.Initialization(nthreads)
     t_inf = (far[3] - 0.5 * mach * mach) * (GAMMA - 1.0)  # p / (Rgas rho) with rho = Rgas = 1
     gas = dict(muGas=mu, prGas=0.72, Rgas=1.0, TRef=t_inf, CSutherland=110.4 / 273.15 * t_inf)
     u0 = np.tile(np.array(far), (mesh.n_cell, 1))
-    return Case("flat_plate", mesh, vfv, "Roe", far, u0, 0, 1, 20.0, False, gas)
+    return Case("flat_plate", mesh, vfv, "Roe", far, u0, 0, 1, cfl, False, gas)
 
 
 MACH3_LEFT = (3.857, 2.629, 0.0, 10.33)   # (rho, u, v, p) behind the Mach-3 shock

@@ -238,6 +238,31 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
         xn[k * ldc + i] = nw[k];
 }
 
+// Read-bandwidth probe of the sweep's dominant stream: every cell streams its
+// slots' A^-1 B pairs exactly as k_jacobi does (16-byte non-temporal loads,
+// the same slots skipped) with none of the sweep's gathers, and writes one
+// double.  Its rate is the ceiling of the sweep's streaming part.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_read_probe(int32_t nc, int64_t ldc, const int32_t *__restrict__ nbr,
+                                                       const double *__restrict__ M, double *__restrict__ out) {
+    const int32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll 1
+    for (int s = 0; s < NF; s++) {
+        if (nbr[s * ldc + i] == -2)
+            continue;
+        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + int64_t(s) * kMPairs * ldc + i;
+#pragma unroll
+        for (int e = 0; e < kMPairs; e++) {
+            const dbl2 m = __builtin_nontemporal_load(Ms + e * ldc);
+            a0 += m.x, a1 += m.y;
+        }
+    }
+    out[i] = a0 + a1;
+}
+
 // Linear part of the Wall_Euler / Wall_NoSlip correction (k_wall_corr) for a
 // correction d of the reconstruction: the wall state is the reconstructed
 // value D d with the normal momentum removed (Wall_Euler) or the momentum
@@ -500,155 +525,6 @@ __global__ __launch_bounds__(kBlock) void k_fill_from(int32_t n, const double *_
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
         dst[i] = src[0];
-}
-
-// The value of v on lane SRC of this lane's quad (lanes 4k .. 4k+3): a DPP
-// quad permutation of the two halves, no LDS.
-template <int SRC>
-__device__ __forceinline__ double quad_bcast(double v) {
-    constexpr int ctrl = SRC | (SRC << 2) | (SRC << 4) | (SRC << 6); // quad_perm [SRC, SRC, SRC, SRC]
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-// a[g] (g = 0..2): the entry a lane keeps after a quad broadcast of three values
-__device__ __forceinline__ double quad_pick(const double a[3], int g) {
-    return g == 0 ? a[0] : (g == 1 ? a[1] : a[2]);
-}
-
-// EvaluateRHS over the inner faces of a mesh with straight faces, FP64,
-// inviscid (DNDS_FV_EulerEvaluator.cpp:215-492): the same arithmetic as
-// k_rhs_face<false, false, true>, spread over the four lanes of a quad per
-// face so that each lane gathers a quarter of the reconstruction data.
-//  * Reconstruction: lane q owns variable q and forms the increments D0 . x
-//    of both sides at the three points from its 9 coefficients per side, in
-//    the same order (c = 0..8) as the one-lane form.
-//  * Fluxes: a quad transpose gives lane g (g = 0..2) the four variables at
-//    point g; it computes the Riemann flux there.  Lane 3 computes the
-//    flux-balance fluxes of the two cell means (one per straight face).
-//  * Gauss sum: lane q collects variable q of the three point fluxes and of
-//    the balance fluxes and sums them in point order, as the one-lane form
-//    does, then writes its variable into both cells' slots.
-// Every value is therefore bit for bit the one-lane kernel's.
-__global__ __launch_bounds__(kBlock) void k_rhs_face_quad(int32_t nf, int64_t ldf, int64_t ldc,
-                                                          const int32_t *__restrict__ f2c,
-                                                          const double *__restrict__ fxy,
-                                                          const double *__restrict__ cmom,
-                                                          const double *__restrict__ fn,
-                                                          const double *__restrict__ dlf,
-                                                          const double *__restrict__ u, const double *__restrict__ x,
-                                                          GasParams gp, const uint8_t *__restrict__ fpos,
-                                                          double *__restrict__ fslot, int32_t *errw,
-                                                          const double *__restrict__ xalt,
-                                                          const uint8_t *__restrict__ limf) {
-    const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
-    const int32_t f = tid >> 2;
-    const int q = threadIdx.x & 3;
-    // the four lanes of a quad share f, so they leave together and every
-    // quad permutation below reads an active lane
-    if (f >= nf)
-        return;
-    const int32_t L = f2c[f], R = f2c[ldf + f];
-    if (R < 0)
-        return; // boundary faces: k_rhs_face<true>
-    const dbl2 *xy = reinterpret_cast<const dbl2 *>(fxy) + f;
-    double inc[2][3];
-#pragma unroll
-    for (int sd = 0; sd < 2; sd++) {
-        const int32_t cell = sd ? R : L;
-        const double *xs = (limf && limf[cell]) ? xalt : x;
-        double pX[3][4], pY[3][4];
-#pragma unroll
-        for (int g = 0; g < 3; g++) {
-            const dbl2 p = xy[(sd * 3 + g) * ldf];
-            int_powers(p.x, pX[g]);
-            int_powers(p.y, pY[g]);
-        }
-        double a[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int c = 0; c < 9; c++) {
-            const double m = cmom[c * ldc + cell];
-            const double xc = xs[(c + 9 * q) * ldc + cell];
-#pragma unroll
-            for (int g = 0; g < 3; g++) {
-                const double d = pX[g][kBX[c + 1]] * pY[g][kBY[c + 1]] - m;
-                a[g] += d * xc;
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < 3; g++)
-            inc[sd][g] = a[g];
-    }
-    // lane g <- the four variables at point g (lane 3 takes point 2's, unused)
-    double iL[4], iR[4];
-    {
-        const int g = q < 3 ? q : 2;
-        double t[3];
-#define DNDSMI_QT(V)                                                                                                   \
-    t[0] = quad_bcast<V>(inc[0][0]), t[1] = quad_bcast<V>(inc[0][1]), t[2] = quad_bcast<V>(inc[0][2]);                \
-    iL[V] = quad_pick(t, g);                                                                                           \
-    t[0] = quad_bcast<V>(inc[1][0]), t[1] = quad_bcast<V>(inc[1][1]), t[2] = quad_bcast<V>(inc[1][2]);                \
-    iR[V] = quad_pick(t, g);
-        DNDSMI_QT(0)
-        DNDSMI_QT(1)
-        DNDSMI_QT(2)
-        DNDSMI_QT(3)
-#undef DNDSMI_QT
-    }
-    double uL[4], uR[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        uL[k] = u[4 * int64_t(L) + k], uR[k] = u[4 * int64_t(R) + k];
-    double n0, n1;
-    const double nx = fn[f], ny = fn[ldf + f];
-    normalized(nx, ny, n0, n1);
-    const double nl = sqrt(nx * nx + ny * ny);
-    bool ok;
-    double pa[4], pb[4]; // lanes 0-2: F n_l at the point; lane 3: FLfix n_l, FRfix n_l
-    if (q < 3) {
-        double ULxy[4], URxy[4], F[4];
-        compress_rec_part(uL, iL, ULxy);
-        compress_rec_part(uR, iR, URxy);
-        // the one-lane form skips the Riemann flux once the balance fluxes
-        // failed; the result is discarded then, so only the flag matters
-        ok = riemann_face(gp, ULxy, URxy, n0, n1, dlf[f], F);
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-            pa[v] = F[v] * nl, pb[v] = 0.0;
-    } else {
-        double FL[4], FR[4];
-        ok = balance_fluxes(gp, uL, uR, n0, n1, FL, FR);
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-            pa[v] = FL[v] * nl, pb[v] = FR[v] * nl;
-    }
-    if (!ok)
-        atomicOr(errw, ERRBIT_NONPOSITIVE);
-    // lane q <- variable q of the three point fluxes and of the balance fluxes
-    double a0, a1, a2, bl, br;
-    {
-        double t[4];
-#define DNDSMI_QV(DST, SRC, ARR)                                                                                       \
-    t[0] = quad_bcast<SRC>(ARR[0]), t[1] = quad_bcast<SRC>(ARR[1]), t[2] = quad_bcast<SRC>(ARR[2]),                   \
-    t[3] = quad_bcast<SRC>(ARR[3]);                                                                                    \
-    DST = q == 0 ? t[0] : (q == 1 ? t[1] : (q == 2 ? t[2] : t[3]));
-        DNDSMI_QV(a0, 0, pa)
-        DNDSMI_QV(a1, 1, pa)
-        DNDSMI_QV(a2, 2, pa)
-        DNDSMI_QV(bl, 3, pa)
-        DNDSMI_QV(br, 3, pb)
-#undef DNDSMI_QV
-    }
-    // the Gauss sum in point order (LINE_3 weights), as k_rhs_face accumulates it
-    double accF = 0.0, accL = 0.0, accR = 0.0;
-    accF += a0 * kLine3W0, accL += bl * kLine3W0, accR += br * kLine3W0;
-    accF += a1 * kLine3W1, accL += bl * kLine3W1, accR += br * kLine3W1;
-    accF += a2 * kLine3W0, accL += bl * kLine3W0, accR += br * kLine3W0;
-    const int32_t pos = fpos[f];
-    const int64_t oL = int64_t(pos & 15) * 8 * ldc + L, oR = int64_t(pos >> 4) * 8 * ldc + R;
-    fslot[oL + q * ldc] = accF, fslot[oL + (4 + q) * ldc] = -accL;
-    fslot[oR + q * ldc] = -accF, fslot[oR + (4 + q) * ldc] = accR;
 }
 
 // EvaluateRHS face loop (DNDS_FV_EulerEvaluator.cpp:215-492): left/right
@@ -1492,6 +1368,9 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         return DNDSMI_ERR_INVALID_ARGUMENT; // matrix_secondary alone may be absent (no WBAP then)
     if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC)
         return DNDSMI_ERR_UNSUPPORTED;
+    // local coordinates: the kernels evaluate the bases' values only
+    if (vr->cell_frame && (es->mu_gas > 0 || vr->matrix_secondary))
+        return DNDSMI_ERR_UNSUPPORTED;
     if (!(es->mu_gas >= 0) ||
         (es->mu_gas > 0 && !(es->pr_gas > 0 && es->r_gas > 0 && es->t_ref > 0 && es->c_sutherland >= 0)))
         return DNDSMI_ERR_INVALID_ARGUMENT;
@@ -1998,11 +1877,6 @@ int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double
                            dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c,
                            c->fbc, c->fxy, c->cmom, c->fn, c->fctr, c->dlambda_face, u, x, c->gas, t, c->fpos,
                            c->fslot, c->errw, nullptr, c->cscale, c->vol, c->farea, xalt, limf);
-    } else if (c->straight_faces && int64_t(4) * nf < (int64_t(1) << 31)) {
-        // four lanes per face
-        hipLaunchKernelGGL(k_rhs_face_quad, dim3(grid_for(int64_t(4) * nf)), dim3(kBlock), 0, c->stream, nf, c->ldf,
-                           c->ldc, c->f2c, c->fxy, c->cmom, c->fn, c->dlambda_face, u, x, c->gas, c->fpos, c->fslot,
-                           c->errw, xalt, limf);
     } else {
         hipLaunchKernelGGL((c->straight_faces ? k_rhs_face<false, false, true> : k_rhs_face<false, false, false>),
                            dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->bfaces, c->f2c,
@@ -2313,6 +2187,32 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
     return DNDSMI_OK;
 }
 
+namespace {
+
+// the three stages of a mixed-precision SSP-RK3 step on (u, x) after
+// EvaluateDt (c->dt_tmp and dtmin set); `ws`: the stages run the learned
+// warm start's hook, adding its correction when `correct`
+int mixed_stages(dndsmi_ctx *c, double *u, double *x, int32_t n_sweeps, int32_t use_limiter, int32_t UseLocaldt,
+                 double t, double dtmin, bool ws, bool correct) {
+    const double Coef[3] = {1.0, 0.25, 2.0 / 3.0};
+    const double ct[3] = {1, 0.25, 2. / 3.};
+    for (int stage = 0; stage < 3; stage++) {
+        const double ts = t + ct[stage] * dtmin;
+        if (ws)
+            DNDSMI_TRY(ws_stage_begin(c, u, x, correct));
+        DNDSMI_TRY(launch_jacobi_mixed(c, u, x, n_sweeps, ts));
+        const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
+        const double cB = stage ? 1 - Coef[stage] : 0.0;
+        if (use_limiter)
+            DNDSMI_TRY(launch_wbap(c, u, x, c->xlim, nullptr, 0, c->limiter_procedure, false));
+        DNDSMI_TRY(launch_rhs_mixed(c, u, x, ts, nullptr, stage + 1, u, UseLocaldt ? c->dt_tmp : nullptr, cA, cB,
+                                    use_limiter ? c->xlim : nullptr, use_limiter ? c->limf : nullptr));
+    }
+    return DNDSMI_OK;
+}
+
+} // namespace
+
 extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *uRec, dndsmi_array *uRecNew,
                                         int32_t n_sweeps, int32_t use_limiter, double CFL, double MaxDt,
                                         int32_t UseLocaldt, double t, double *dt_out) {
@@ -2328,29 +2228,68 @@ extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_a
         return DNDSMI_ERR_UNSUPPORTED;
     }
     DNDSMI_TRY(ensure_mixed(c));
-    DNDSMI_TRY(launch_dt(c, u->d, CFL, MaxDt, c->dt_tmp));
+    // the learned warm start: is this step in a check window (the plain run
+    // on shadow copies beside the corrected one)?
+    const bool ws = c->step_ws && !c->ws_rejected;
+    bool window = false;
+    if (ws) {
+        if (c->ws_win_left == 0 && (c->ws_first || (c->ws_recheck > 0 && c->ws_since_check >= c->ws_recheck))) {
+            DNDSMI_TRY(ws_window_begin(c, u->d, uRec->d));
+            c->ws_win_left = c->ws_check_steps;
+            c->ws_first = false;
+        }
+        window = c->ws_win_left > 0;
+    }
+    // the step's dt: from the plain run's state in a window, used by both runs
+    DNDSMI_TRY(launch_dt(c, window ? c->ws_sh_u : u->d, CFL, MaxDt, c->dt_tmp));
     double dtmin = 0;
     DNDSMI_TRY(c->hip(hipMemcpyAsync(&dtmin, c->dt_min, sizeof(double), hipMemcpyDeviceToHost, c->stream),
                       "hipMemcpyAsync"));
     DNDSMI_TRY(check_errw(c));
     if (dt_out)
         *dt_out = dtmin;
-    const double Coef[3] = {1.0, 0.25, 2.0 / 3.0};
-    const double ct[3] = {1, 0.25, 2. / 3.};
-    for (int stage = 0; stage < 3; stage++) {
-        const double ts = t + ct[stage] * dtmin;
-        DNDSMI_TRY(launch_jacobi_mixed(c, u->d, uRec->d, n_sweeps, ts));
-        const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
-        const double cB = stage ? 1 - Coef[stage] : 0.0;
-        if (use_limiter)
-            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure, false));
-        DNDSMI_TRY(launch_rhs_mixed(c, u->d, uRec->d, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr,
-                                    cA, cB, use_limiter ? c->xlim : nullptr, use_limiter ? c->limf : nullptr));
+    if (!window) {
+        if (ws)
+            c->ws_since_check++;
+        return mixed_stages(c, u->d, uRec->d, n_sweeps, use_limiter, UseLocaldt, t, dtmin, ws, ws);
+    }
+    DNDSMI_TRY(mixed_stages(c, c->ws_sh_u, c->ws_sh_x, n_sweeps, use_limiter, UseLocaldt, t, dtmin, false, false));
+    DNDSMI_TRY(check_errw(c)); // the plain run's own failures are the method's
+    int rc = mixed_stages(c, u->d, uRec->d, n_sweeps, use_limiter, UseLocaldt, t, dtmin, true, true);
+    if (rc == DNDSMI_OK)
+        rc = check_errw(c);
+    const bool failed = rc == DNDSMI_ERR_NONPOSITIVE_STATE || rc == DNDSMI_ERR_NAN;
+    if (rc != DNDSMI_OK && !failed)
+        return rc;
+    c->ws_win_left--;
+    if (failed || c->ws_win_left == 0) {
+        DNDSMI_TRY(ws_window_end(c, u->d, uRec->d, failed));
+        c->ws_since_check = 0;
+        c->ws_win_left = 0;
     }
     return DNDSMI_OK;
 }
 
 // ---- profiling ---------------------------------------------------------
+
+extern "C" int dndsmi_read_probe(dndsmi_ctx *c, int32_t reps) {
+    if (!c || reps < 1)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (!c->probe_out)
+        DNDSMI_TRY(c->alloc(c->probe_out, c->ldc));
+    for (int k = 0; k < reps; k++) {
+        prof_begin(c, DNDSMI_K_READ_PROBE);
+        if (c->nfmax == 3)
+            hipLaunchKernelGGL(k_read_probe<3>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, int32_t(c->nc),
+                               c->ldc, c->nbr, c->M, c->probe_out);
+        else
+            hipLaunchKernelGGL(k_read_probe<4>, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, int32_t(c->nc),
+                               c->ldc, c->nbr, c->M, c->probe_out);
+        prof_end(c);
+        DNDSMI_TRY(c->launch_check("k_read_probe"));
+    }
+    return DNDSMI_OK;
+}
 
 extern "C" int dndsmi_profile_enable(dndsmi_ctx *c, int32_t on) {
     if (!c)
@@ -2428,6 +2367,8 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
         return nif * (8 + 1 + 96 + (c->straight_faces ? 16 : 48) + 8 + 64) + nc * (36 * 8 + 32 + 72);
     case DNDSMI_K_RHS_CELL32: // 8 FP32 terms per slot, volume, x read+write, xLast (+ last-slot index)
         return c2f * 32 + nc * (8 + 32 * 3 + (c->nfmax == 4 ? 4 : 0));
+    case DNDSMI_K_READ_PROBE: // the sweep's A^-1 B pairs (41 x 16 B per slot) and slot index, one double out
+        return c2f * (kMPairs * 16 + 4) + nc * 8;
     default:
         return 0;
     }

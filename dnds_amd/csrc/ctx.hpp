@@ -141,6 +141,18 @@ struct dndsmi_ctx {
     double *imp_stage = nullptr;      // [3][ldc][4] SDIRK4 stage right-hand sides (first use)
     double *res_part = nullptr;       // [4][grid]
     double *res = nullptr;            // [4]
+    // the learned warm start inside the mixed-precision step
+    // (dndsmi_ssprk3_set_warm_start): the previous stage's cell means and
+    // their change [ldc][4], the plain shadow run of a check window
+    // ([ldc][4], [36][ldc]) and its reduction partials (all first use)
+    struct dndsmi_warmstart *step_ws = nullptr;
+    double *ws_uprev = nullptr, *ws_du = nullptr;
+    double *ws_sh_u = nullptr, *ws_sh_x = nullptr, *ws_part = nullptr;
+    bool ws_have_prev = false, ws_rejected = false, ws_first = true;
+    int32_t ws_check_steps = 1, ws_recheck = 0, ws_win_left = 0, ws_since_check = 0;
+    double ws_tol = 0.0, ws_last_dev = -1.0;
+    int64_t ws_stats[3] = {0, 0, 0}; // stages corrected, windows passed, windows failed (fallbacks)
+    double *probe_out = nullptr; // [ldc] output of the read-bandwidth probe (first use)
     // device address of this GPU's static scratch of the allocation-free
     // residual (set at context creation; never freed: it is not an allocation)
     double *noalloc_scratch = nullptr;
@@ -205,6 +217,14 @@ void prof_begin(dndsmi_ctx *c, int id);
 void prof_end(dndsmi_ctx *c);
 // reads and clears the device error word; a status for the first bit set
 int check_errw(dndsmi_ctx *c);
+// the learned warm start inside the mixed-precision step (warmstart.hip):
+// before a stage's sweeps, the change of the means since the previous stage
+// and, when `use`, x += the network's prediction on it; a check window's
+// shadow copies, and its verdict at the end (restores the plain run's u and
+// x when the corrected run failed or strayed beyond the tolerance)
+int ws_stage_begin(dndsmi_ctx *c, const double *u, double *x, bool use);
+int ws_window_begin(dndsmi_ctx *c, const double *u, const double *x);
+int ws_window_end(dndsmi_ctx *c, double *u, double *x, bool failed);
 // looks up this GPU's static scratch of dndsmi_evaluate_residual_noalloc
 // (once per device; the device must be current) into c->noalloc_scratch
 int noalloc_scratch_init(dndsmi_ctx *c);

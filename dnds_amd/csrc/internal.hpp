@@ -36,6 +36,6 @@ struct dndsmi_vr {
     std::vector<double> volume, face_area, cell_bary, base_moment;
     std::vector<double> face_center, face_norm_center, face_norm, face_weight;
     std::vector<double> face_d0, face_dibj_g0, matrix_secondary;
-    std::vector<double> face_gauss_xy, cell_scale;
+    std::vector<double> face_gauss_xy, cell_scale, cell_frame;
     std::vector<double> ainv, ainv_b, ainv_bvec;
 };

@@ -1,6 +1,7 @@
 // Host FP64 geometry and VR setup: the ImplicitFiniteVolume2D constructor
 // (DNDS_FV_VR.hpp:29-102) and VRFiniteVolume2D::Initialization
-// (DNDS_FV_VR.cpp:3-1154) for p = 3 with the global-coordinate basis.
+// (DNDS_FV_VR.cpp:3-1154) for p = 3 and p = 1 with the global-coordinate or
+// the local-inertia-coordinate basis.
 //
 // The reference keeps 10x10 basis-derivative matrices at every face and cell
 // quadrature point (DNDS_FV_VR.cpp:195-532), about 19 KB per triangle.  Here
@@ -163,6 +164,7 @@ struct Ctx {
     const dndsmi_vr_setting *s;
     dndsmi_vr *out;
     std::vector<double> scale; // [n_cell][2] CoordMinMaxScale (after the isotropic option)
+    std::vector<double> frame; // [n_cell][2][2] invPJacobi of the local coordinates (useLocalCoord only)
 };
 
 Cell load_cell(const dndsmi_mesh_view &m, int64_t ic, double shx = 0, double shy = 0) {
@@ -213,6 +215,110 @@ void diff_base(double px, double py, double cx, double cy, double sx, double sy,
     }
     for (int ib = 0; ib < 10; ib++)
         D[0][ib] -= moment[ib];
+}
+
+// FDiffBaseValue, local inertia coordinates (useLocalCoord, DNDS_FV_VR.hpp:358-1018
+// without USE_LOCAL_COORD_CURVILINEAR, which DNDS_Experimentals.h leaves off):
+// the cubic monomials of xi = A (p - c), A = invPJacobi (row-major), their
+// derivatives with respect to xi mapped to x and y by the chain rule with
+// dXijdXi = A^T (Elem::Convert2dDiffsLinMap, DNDS_Elements.hpp:1491-1528), row
+// 0 minus the base moments.  XY (optional) receives xi.
+void diff_base_local(double px, double py, double cx, double cy, const double A[4], const double *moment,
+                     double D[10][10], double *XY = nullptr) {
+    const double dx = px - cx, dy = py - cy;
+    const double X = A[0] * dx + A[1] * dy;
+    const double Y = A[2] * dx + A[3] * dy;
+    if (XY)
+        XY[0] = X, XY[1] = Y;
+    double powX[4], powY[4];
+    int_powers(X, powX);
+    int_powers(Y, powY);
+    for (int id = 0; id < 10; id++) {
+        const int ndx = kDiffX[id], ndy = kDiffY[id];
+        for (int ib = 0; ib < 10; ib++) {
+            const int bx = kDiffX[ib], by = kDiffY[ib];
+            int c = (ndx <= bx ? kDFact[bx][ndx] : 0) * (ndy <= by ? kDFact[by][ndy] : 0);
+            D[id][ib] = c ? double(c) * powX[bx - ndx] * powY[by - ndy] : 0.;
+        }
+    }
+    // M = dXijdXi = A^T: grad_x = M grad_xi, H_x = M H_xi M^T, and each index of
+    // the third-derivative tensor contracted with M
+    const double M[2][2] = {{A[0], A[2]}, {A[1], A[3]}};
+    for (int ib = 0; ib < 10; ib++) {
+        double T[2][2][2];
+        T[0][0][0] = D[6][ib];
+        T[0][0][1] = T[0][1][0] = T[1][0][0] = D[7][ib];
+        T[0][1][1] = T[1][0][1] = T[1][1][0] = D[8][ib];
+        T[1][1][1] = D[9][ib];
+        double U[2][2][2];
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++)
+                for (int k = 0; k < 2; k++) {
+                    double acc = 0.0;
+                    for (int a = 0; a < 2; a++)
+                        for (int b = 0; b < 2; b++)
+                            for (int e = 0; e < 2; e++)
+                                acc += M[i][a] * M[j][b] * M[k][e] * T[a][b][e];
+                    U[i][j][k] = acc;
+                }
+        D[6][ib] = U[0][0][0], D[7][ib] = U[0][0][1], D[8][ib] = U[0][1][1], D[9][ib] = U[1][1][1];
+        const double H[2][2] = {{D[3][ib], D[4][ib]}, {D[4][ib], D[5][ib]}};
+        double G[2][2];
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) {
+                double acc = 0.0;
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++)
+                        acc += M[i][a] * H[a][b] * M[j][b];
+                G[i][j] = acc;
+            }
+        D[3][ib] = G[0][0], D[4][ib] = G[0][1], D[5][ib] = G[1][1];
+        const double g0 = D[1][ib], g1 = D[2][ib];
+        D[1][ib] = M[0][0] * g0 + M[0][1] * g1;
+        D[2][ib] = M[1][0] * g0 + M[1][1] * g1;
+    }
+    for (int ib = 0; ib < 10; ib++)
+        D[0][ib] -= moment[ib];
+}
+
+// The local frame of a cell (DNDS_FV_VR.cpp:118-139 and DNDS_FV_VR.hpp:358-412):
+// the principal axes of the cell's second-moment tensor I = (1/V) int (p - b)
+// (p - b)^T (the cell rule of the moments), largest first, as unit columns
+// of pJacobi; each column scaled by the cell's half extent along it
+// (anisotropicLengths) or both by the diagonal of those half extents; A =
+// pJacobi^-1.  The axes' orientation is a convention of this build: the
+// first axis is a = (cos t, sin t) with t = atan2(2 Ixy, Ixx - Iyy) / 2,
+// negated if its larger component is negative, and t = 0 (the x axis) when
+// the tensor is isotropic to 1e-10 of its trace; the second axis is a turned
+// by +90 degrees.  (The reference takes whatever signs Eigen's
+// SelfAdjointEigenSolver returns; the basis spans the same polynomials
+// either way, so the reconstruction's point values do not depend on it.)
+void local_frame(const Cell &c, double bx, double by, double Ixx, double Ixy, double Iyy, bool aniso, double A[4]) {
+    const double d = Ixx - Iyy, o = 2.0 * Ixy;
+    const double t = std::sqrt(d * d + o * o) <= 1e-10 * (Ixx + Iyy) ? 0.0 : 0.5 * std::atan2(o, d);
+    double a0 = std::cos(t), a1 = std::sin(t);
+    if (std::fabs(a1) > std::fabs(a0) && a1 < 0)
+        a0 = -a0, a1 = -a1;
+    double P[2][2] = {{a0, -a1}, {a1, a0}}; // columns: axes
+    // invPJacobi of the unit axes, the nodes' coordinates along them
+    double det = P[0][0] * P[1][1] - P[0][1] * P[1][0];
+    double Q[2][2] = {{P[1][1] / det, -P[0][1] / det}, {-P[1][0] / det, P[0][0] / det}};
+    double lo[2] = {0, 0}, hi[2] = {0, 0};
+    for (int n = 0; n < c.nv; n++) {
+        const double dx = c.x[n] - bx, dy = c.y[n] - by;
+        for (int k = 0; k < 2; k++) {
+            const double v = Q[k][0] * dx + Q[k][1] * dy;
+            lo[k] = n ? std::min(lo[k], v) : v;
+            hi[k] = n ? std::max(hi[k], v) : v;
+        }
+    }
+    const double L0 = (hi[0] - lo[0]) * 0.5, L1 = (hi[1] - lo[1]) * 0.5;
+    const double uni = std::sqrt(sqr(L0) + sqr(L1));
+    const double s0 = aniso ? L0 : uni, s1 = aniso ? L1 : uni;
+    for (int r = 0; r < 2; r++)
+        P[r][0] *= s0, P[r][1] *= s1;
+    det = P[0][0] * P[1][1] - P[0][1] * P[1][0];
+    A[0] = P[1][1] / det, A[1] = -P[0][1] / det, A[2] = -P[1][0] / det, A[3] = P[0][0] / det;
 }
 
 // Scale factors of FDiffBaseValue: CoordMinMaxScale, made isotropic unless anisotropicLengths.
@@ -276,8 +382,11 @@ void face_dibj(const Ctx &C, int64_t f, int side, double xi, double D[10][10], d
     face2vol(fs.c.nv, m.face2cell_ref[2 * f + side], face_inverted(C, f, side), xi, pxi, peta);
     double px, py, detJ;
     map_point(fs.c, pxi, peta, px, py, detJ);
-    diff_base(px, py, fs.cx, fs.cy, C.scale[2 * fs.cell], C.scale[2 * fs.cell + 1],
-              &C.out->base_moment[10 * fs.cell], D, XY);
+    if (!C.frame.empty())
+        diff_base_local(px, py, fs.cx, fs.cy, &C.frame[4 * fs.cell], &C.out->base_moment[10 * fs.cell], D, XY);
+    else
+        diff_base(px, py, fs.cx, fs.cy, C.scale[2 * fs.cell], C.scale[2 * fs.cell + 1],
+                  &C.out->base_moment[10 * fs.cell], D, XY);
 }
 
 inline bool is_far(int32_t bc) {
@@ -490,7 +599,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
         return DNDSMI_ERR_INVALID_ARGUMENT;
     *outp = nullptr;
     if ((setting->rec_order != 3 && setting->rec_order != 1) || (setting->rec_order == 1 && setting->build_limiter_data) ||
-        setting->use_local_coord || setting->orthogonalize_base || setting->norm_wbap ||
+        (setting->use_local_coord && setting->build_limiter_data) || setting->orthogonalize_base || setting->norm_wbap ||
         setting->tang_weight_mod_min != 1.0 || setting->weight_scheme_geom < 0 || setting->weight_scheme_geom > 3 ||
         setting->weight_scheme_dir < 0 || setting->weight_scheme_dir > 1)
         return DNDSMI_ERR_UNSUPPORTED;
@@ -527,6 +636,8 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
             out->matrix_secondary.assign(size_t(162) * nf, 0.0);
         }
         C.scale.assign(2 * nc, 0.0);
+        if (setting->use_local_coord)
+            C.frame.assign(4 * nc, 0.0);
         std::atomic<int> failed{0};
 
         // ---- pass 1: cells -- volume (TRI_7 / QUAD_9), barycentre and
@@ -559,13 +670,29 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
             double sx, sy;
             cell_scale(c, setting->anisotropic_lengths != 0, sx, sy);
             C.scale[2 * ic] = sx, C.scale[2 * ic + 1] = sy;
+            if (!C.frame.empty()) {
+                // cellInertia (DNDS_FV_VR.cpp:118-128) with the moments' cell rule
+                double ixx = 0, ixy = 0, iyy = 0;
+                for (int g = 0; g < ng; g++) {
+                    double px, py, det;
+                    map_point(c, gx[g], gy[g], px, py, det);
+                    const double dx = px - bx, dy = py - by;
+                    ixx += ((dx * dx) * det) * gw[g];
+                    ixy += ((dx * dy) * det) * gw[g];
+                    iyy += ((dy * dy) * det) * gw[g];
+                }
+                local_frame(c, bx, by, ixx / v, ixy / v, iyy / v, setting->anisotropic_lengths != 0, &C.frame[4 * ic]);
+            }
             double zero[10] = {0};
             double mom[10] = {0};
             for (int g = 0; g < ng; g++) {
                 double px, py, det;
                 map_point(c, gx[g], gy[g], px, py, det);
                 double D[10][10];
-                diff_base(px, py, bx, by, sx, sy, zero, D);
+                if (!C.frame.empty())
+                    diff_base_local(px, py, bx, by, &C.frame[4 * ic], zero, D);
+                else
+                    diff_base(px, py, bx, by, sx, sy, zero, D);
                 for (int k = 0; k < 10; k++)
                     mom[k] += (D[0][k] * det) * gw[g];
             }
@@ -576,6 +703,7 @@ extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_se
         if (failed)
             throw SetupError("non-positive cell Jacobian (orientation wrong or distorted)");
         out->cell_scale = C.scale;
+        out->cell_frame = C.frame;
 
         // ---- pass 2: faces -- area (LINE_5), centre and normals, weights,
         // row 0 of the Gauss-point bases, limiter data (DNDS_FV_VR.hpp:75-101,
@@ -863,6 +991,7 @@ extern "C" int dndsmi_vr_get_view(const dndsmi_vr *vr, dndsmi_vr_view *v) {
     v->ainv_bvec = vr->ainv_bvec.data();
     v->face_gauss_xy = vr->face_gauss_xy.data();
     v->cell_scale = vr->cell_scale.data();
+    v->cell_frame = vr->cell_frame.empty() ? nullptr : vr->cell_frame.data();
     return DNDSMI_OK;
 }
 

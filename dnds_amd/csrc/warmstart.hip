@@ -297,6 +297,51 @@ __global__ __launch_bounds__(256) void k_gather_rows(int32_t rows, int32_t width
     dst[e] = src[int64_t(idx[r]) * width + c];
 }
 
+// x += the network's output, unscaled (the stage correction of the learned
+// warm start inside the mixed-precision step)
+__global__ __launch_bounds__(256) void k_unscale_add(int32_t nc, int64_t ldc, const float *__restrict__ Y,
+                                                     const float *__restrict__ S, double *__restrict__ x) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+#pragma unroll
+    for (int k = 0; k < kOut; k++)
+        x[k * ldc + i] += double(Y[int64_t(i) * kOut + k]) * double(S[4 * int64_t(i) + k / 9]);
+}
+
+// du = u - u_prev, then u_prev = u (cell means, [cell][4])
+__global__ __launch_bounds__(256) void k_mean_change(int32_t nc, const double *__restrict__ u,
+                                                     double *__restrict__ uprev, double *__restrict__ du) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * nc)
+        return;
+    const double v = u[i];
+    du[i] = v - uprev[i];
+    uprev[i] = v;
+}
+
+// per block: sum of (a - b)^2 and of b^2 over n doubles (the corrected state
+// against the plain one at the end of a check window)
+__global__ __launch_bounds__(256) void k_diff2(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                               double *__restrict__ part) {
+    __shared__ double red[2][256];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double d2 = 0.0, b2 = 0.0;
+    if (i < n) {
+        const double d = a[i] - b[i];
+        d2 = d * d, b2 = b[i] * b[i];
+    }
+    red[0][threadIdx.x] = d2, red[1][threadIdx.x] = b2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            red[0][threadIdx.x] += red[0][threadIdx.x + st], red[1][threadIdx.x] += red[1][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[2 * blockIdx.x] = red[0][0], part[2 * blockIdx.x + 1] = red[1][0];
+}
+
 // sum over all cells of |x_new - x_old|^2 (per block), for the rec residual
 __global__ __launch_bounds__(256) void k_rec_diff2(int32_t nc, int64_t ldc, const double *__restrict__ a,
                                                    const double *__restrict__ b, double *__restrict__ part) {
@@ -576,8 +621,130 @@ extern "C" void dndsmi_warmstart_destroy(dndsmi_warmstart *w) {
     if (w && w->ctx) {
         (void)hipStreamSynchronize(w->ctx->stream);
         w->ctx->warmstarts.erase(w);
+        if (w->ctx->step_ws == w)
+            w->ctx->step_ws = nullptr;
     }
     delete w;
+}
+
+// ---- the learned warm start inside the mixed-precision step ------------
+//
+// Every stage of dndsmi_ssprk3_step_mixed starts its sweeps from the uRec the
+// previous stage left (the reference's own warm start).  With a network
+// attached, the stage first adds the network's prediction on the change of
+// the cell means since the previous stage.  The plain method judges it: in a
+// check window every step runs twice, plain on shadow copies of u and uRec
+// and then with the correction, with the plain run's time step; at the
+// window's end the corrected state must lie within the tolerance of the
+// plain one (relative L2 over all cells), else u and uRec become the plain
+// run's and the correction stays off until the next attach.  A corrected
+// run that meets a non-positive state inside a window is replaced the same
+// way.
+
+namespace dndsmi_host {
+
+int ws_stage_begin(dndsmi_ctx *c, const double *u, double *x, bool use) {
+    dndsmi_warmstart *w = c->step_ws;
+    if (!c->ws_uprev) {
+        DNDSMI_TRY(c->alloc(c->ws_uprev, 4 * c->ldc));
+        DNDSMI_TRY(c->alloc(c->ws_du, 4 * c->ldc));
+    }
+    if (!c->ws_have_prev) {
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(c->ws_uprev, u, sizeof(double) * 4 * c->nc, hipMemcpyDeviceToDevice,
+                                         c->stream),
+                          "copy u"));
+        c->ws_have_prev = true;
+        return DNDSMI_OK;
+    }
+    hipLaunchKernelGGL(k_mean_change, dim3(grid_for(4 * c->nc)), dim3(256), 0, c->stream, int32_t(c->nc), u,
+                       c->ws_uprev, c->ws_du);
+    DNDSMI_TRY(c->launch_check("k_mean_change"));
+    if (!use)
+        return DNDSMI_OK;
+    DNDSMI_TRY(features(w, c->ws_du));
+    DNDSMI_TRY(ensure_rows(w, w->nc));
+    forward(w, w->fX, int32_t(w->nc));
+    hipLaunchKernelGGL(k_unscale_add, dim3(grid_for(w->nc)), dim3(256), 0, c->stream, int32_t(w->nc), w->ldc, w->Y,
+                       w->fS, x);
+    DNDSMI_TRY(c->launch_check("warm-start stage correction"));
+    c->ws_stats[0]++;
+    return DNDSMI_OK;
+}
+
+int ws_window_begin(dndsmi_ctx *c, const double *u, const double *x) {
+    if (!c->ws_sh_u) {
+        DNDSMI_TRY(c->alloc(c->ws_sh_u, 4 * c->ldc));
+        DNDSMI_TRY(c->alloc(c->ws_sh_x, 36 * c->ldc));
+        DNDSMI_TRY(c->alloc(c->ws_part, 2 * int64_t(grid_for(36 * c->ldc)) + 2));
+    }
+    DNDSMI_TRY(c->hip(hipMemcpyAsync(c->ws_sh_u, u, sizeof(double) * 4 * c->ldc, hipMemcpyDeviceToDevice, c->stream),
+                      "copy u to the shadow"));
+    return c->hip(hipMemcpyAsync(c->ws_sh_x, x, sizeof(double) * 36 * c->ldc, hipMemcpyDeviceToDevice, c->stream),
+                  "copy uRec to the shadow");
+}
+
+int ws_window_end(dndsmi_ctx *c, double *u, double *x, bool failed) {
+    bool reject = failed;
+    if (!reject) {
+        const int64_t n = 4 * c->nc;
+        const unsigned g = grid_for(n, 256);
+        hipLaunchKernelGGL(k_diff2, dim3(g), dim3(256), 0, c->stream, n, u, c->ws_sh_u, c->ws_part);
+        DNDSMI_TRY(c->launch_check("k_diff2"));
+        std::vector<double> part(2 * size_t(g));
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(part.data(), c->ws_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "hipMemcpyAsync"));
+        DNDSMI_TRY(c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+        double d2 = 0, b2 = 0;
+        for (unsigned k = 0; k < g; k++)
+            d2 += part[2 * k], b2 += part[2 * k + 1];
+        c->ws_last_dev = std::sqrt(d2 / (b2 > 0 ? b2 : 1.0));
+        reject = !(c->ws_last_dev <= c->ws_tol);
+    }
+    if (reject) {
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(u, c->ws_sh_u, sizeof(double) * 4 * c->ldc, hipMemcpyDeviceToDevice,
+                                         c->stream),
+                          "restore u"));
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(x, c->ws_sh_x, sizeof(double) * 36 * c->ldc, hipMemcpyDeviceToDevice,
+                                         c->stream),
+                          "restore uRec"));
+        c->ws_rejected = true;
+        c->ws_stats[2]++;
+    } else {
+        c->ws_stats[1]++;
+    }
+    return DNDSMI_OK;
+}
+
+} // namespace dndsmi_host
+
+extern "C" int dndsmi_ssprk3_set_warm_start(dndsmi_ctx *c, dndsmi_warmstart *ws, int32_t check_steps,
+                                            int32_t recheck_interval, double tolerance) {
+    if (!c || (ws && ws->ctx != c) || check_steps < 1 || recheck_interval < 0 || !(tolerance >= 0))
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    c->step_ws = ws;
+    c->ws_check_steps = check_steps;
+    c->ws_recheck = recheck_interval;
+    c->ws_tol = tolerance;
+    c->ws_have_prev = false;
+    c->ws_rejected = false;
+    c->ws_win_left = 0;
+    c->ws_since_check = 0;
+    c->ws_first = true;
+    c->ws_last_dev = -1.0;
+    for (int k = 0; k < 3; k++)
+        c->ws_stats[k] = 0;
+    return DNDSMI_OK;
+}
+
+extern "C" int dndsmi_ssprk3_warm_start_stats(const dndsmi_ctx *c, int64_t stats[3], double *last_deviation) {
+    if (!c || !stats)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < 3; k++)
+        stats[k] = c->ws_stats[k];
+    if (last_deviation)
+        *last_deviation = c->ws_last_dev;
+    return DNDSMI_OK;
 }
 
 extern "C" int64_t dndsmi_warmstart_num_params(void) { return kNParam; }

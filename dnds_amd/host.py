@@ -165,7 +165,7 @@ SETUP_SIZES = {  # name -> (per, count) where per is "cell", "face" or "c2f"
     "face_center": ("face", 2), "face_norm_center": ("face", 2), "face_norm": ("face", 6),
     "face_weight": ("face", 10), "face_d0": ("face", 54), "face_dibj_g0": ("face", 180),
     "matrix_secondary": ("face", 162), "ainv": ("cell", 81), "ainv_b": ("c2f", 81), "ainv_bvec": ("c2f", 9),
-    "face_gauss_xy": ("face", 12), "cell_scale": ("cell", 2),
+    "face_gauss_xy": ("face", 12), "cell_scale": ("cell", 2), "cell_frame": ("cell", 4),
 }
 
 
@@ -559,7 +559,14 @@ class ExplicitSSPRK3LocalDt:
 
     def __init__(self, evaluator: EulerEvaluator, nInternalRecStep: int, CFL: float, MaxDt: float = 1e100,
                  UseLocaldt: bool = False, useLimiter: bool = False, mixedPrecision: bool = False,
-                 limiterProcedure: int = 0):
+                 limiterProcedure: int = 0, warmStart: "WarmStart | None" = None, checkSteps: int = 5,
+                 recheckInterval: int = 0, tolerance: float = 1e-6):
+        """warmStart (mixedPrecision only; no reference counterpart): every
+        stage adds the network's prediction on the change of the means to
+        the previous stage's uRec; the plain method checks it over
+        checkSteps steps (and again every recheckInterval steps) and the run
+        falls back to the plain trajectory when the corrected state strays
+        beyond `tolerance` (dndsmi_ssprk3_set_warm_start)."""
         self.ev = evaluator
         self.n_sweeps = nInternalRecStep
         self.CFL = CFL
@@ -568,6 +575,23 @@ class ExplicitSSPRK3LocalDt:
         self.useLimiter = useLimiter
         self.mixedPrecision = mixedPrecision
         self.limiterProcedure = limiterProcedure
+        self.warmStart = warmStart
+        if warmStart is not None and not mixedPrecision:
+            raise DNDSError(A.ERR_INVALID_ARGUMENT, "the learned warm start runs inside the mixed-precision step")
+        if mixedPrecision:
+            d = evaluator.device
+            _check(_lib().dndsmi_ssprk3_set_warm_start(d.ctx, warmStart.h if warmStart is not None else None,
+                                                       int(checkSteps), int(recheckInterval), float(tolerance)),
+                   "warm start attach", d.ctx)
+
+    def WarmStartStats(self) -> dict:
+        """stages that added the correction, checks passed, checks failed
+        (fallbacks), and the relative L2 deviation the last check found"""
+        st = (C.c_int64 * 3)()
+        dev = C.c_double()
+        _check(_lib().dndsmi_ssprk3_warm_start_stats(self.ev.device.ctx, st, C.byref(dev)), "warm start stats")
+        return {"corrected_stages": st[0], "checks_passed": st[1], "fallbacks": st[2],
+                "last_check_deviation": dev.value}
 
     def Step(self, u: ArrayDOFV, uRec: ArrayRecV, uRecNew: ArrayRecV, t: float) -> float:
         """Advances u in place; returns the global minimum dt of the step.

@@ -150,7 +150,7 @@ void dndsmi_mesh_destroy(dndsmi_mesh *mesh);
 
 /* VRFiniteVolume2D::Setting, DNDS_FV_VR.hpp:170-230 (names kept) */
 typedef struct dndsmi_vr_setting {
-    int32_t rec_order;              /* P_ORDER, only 3 is implemented        */
+    int32_t rec_order;              /* P_ORDER: 3, or 1 (no limiter data)    */
     double jacobi_relax;            /* JacobiRelax                            */
     double wall_weight, far_weight; /* wallWeight, farWeight                  */
     double tang_weight;             /* tangWeight                             */
@@ -162,7 +162,7 @@ typedef struct dndsmi_vr_setting {
     double weight_len_scale;        /* weightLenScale                         */
     double wbap_smooth_indicator_scale; /* WBAP_SmoothIndicatorScale          */
     double wbap_n_std;              /* WBAP_nStd                              */
-    int32_t use_local_coord;        /* must be 0                              */
+    int32_t use_local_coord;        /* useLocalCoord (no limiter data)        */
     int32_t orthogonalize_base;     /* must be 0                              */
     int32_t norm_wbap;              /* must be 0                              */
     int32_t build_limiter_data;     /* also build the WBAP face data          */
@@ -192,11 +192,19 @@ typedef struct dndsmi_vr_view {
     const double *ainv_b;             /* [n_c2f][9][9] matrixBatch.m(ic2f+1)      */
     const double *ainv_bvec;          /* [n_c2f][9] vectorBatch rows              */
     /* [n_face][3][2][2] (X, Y) = ((px - cx) / sx, (py - cy) / sy) of LINE_3
-     * point g seen from side s: the arguments of the scaled monomials whose
-     * values (minus the cell's moments) and derivatives form face_d0 and
-     * face_dibj_g0; the device evaluates the bases from these. */
+     * point g seen from side s (local coordinates: xi = A (p - c), A the
+     * side's cell_frame): the arguments of the monomials whose values (minus
+     * the cell's moments) and derivatives form face_d0 and face_dibj_g0; the
+     * device evaluates the bases from these. */
     const double *face_gauss_xy;
     const double *cell_scale;         /* [n_cell][2] (sx, sy) of FDiffBaseValue (CoordMinMaxScale) */
+    /* [n_cell][2][2] A = invPJacobi of the local inertia coordinates, row-major
+     * (DNDS_FV_VR.hpp:358-412; DNDS_FV_VR.cpp:118-139); NULL in global
+     * coordinates.  With local coordinates the device evaluates the bases'
+     * values only: contexts on such a setup refuse mu_gas > 0 (the viscous
+     * terms need derivative rows) with DNDSMI_ERR_UNSUPPORTED, and the setup
+     * refuses build_limiter_data. */
+    const double *cell_frame;
 } dndsmi_vr_view;
 
 typedef struct dndsmi_vr dndsmi_vr;
@@ -481,6 +489,31 @@ int dndsmi_warmstart_train(dndsmi_warmstart *ws, int32_t epochs, int32_t batch, 
 /* uRec = the network's guess for the state u */
 int dndsmi_warmstart_predict(dndsmi_warmstart *ws, const dndsmi_array *u, dndsmi_array *uRec);
 
+/* No counterpart in the reference: the learned warm start inside
+ * dndsmi_ssprk3_step_mixed, whose stages otherwise start their sweeps from
+ * the uRec the previous stage left (the reference's own warm start,
+ * DNDS_ODE.hpp:131-178, DNDS_FV_EulerSolver.hpp:713-837).  With ws attached
+ * (ws on this context; NULL detaches), every stage first adds to uRec the
+ * network's prediction on the change of the cell means since the previous
+ * stage.  Plain fallback, judged by the plain method itself: the first
+ * check_steps steps after attaching, and again check_steps steps every
+ * recheck_interval steps after a passed check (0: no re-checks), run twice
+ * -- plain on shadow copies of u and uRec, then with the correction, both
+ * with the plain run's time step (returned in dt_out).  At the end of a
+ * check, if the corrected state differs from the plain one by more than
+ * tolerance in the relative L2 norm over all cells, or if the corrected run
+ * met a non-positive state, u and uRec become the plain run's and the
+ * correction stays off until the next attach.  Check steps cost two steps
+ * and synchronise once at their end; the shadow copies (u, uRec) are
+ * allocated on the first check. */
+int dndsmi_ssprk3_set_warm_start(dndsmi_ctx *ctx, dndsmi_warmstart *ws, int32_t check_steps,
+                                 int32_t recheck_interval, double tolerance);
+/* stats[0] stages that added the correction, stats[1] checks passed,
+ * stats[2] checks failed (fallbacks to the plain run), counted since
+ * dndsmi_ssprk3_set_warm_start; last_deviation (may be NULL) the relative
+ * L2 difference found by the last check (-1 before the first) */
+int dndsmi_ssprk3_warm_start_stats(const dndsmi_ctx *ctx, int64_t stats[3], double *last_deviation);
+
 /* Jacobi sweeps (as dndsmi_reconstruction_jacobi_step) until the reference's
  * rec increment sum_i |uRec_i - uRecNew_i|^2 / n_cell (DNDS_FV_VR.hpp:1684-1705)
  * falls below tol, or max_sweeps; *sweeps and *res receive the count and the
@@ -510,7 +543,8 @@ enum {
     DNDSMI_K_RHS_CELL32 = 12,/* mixed precision: FP32 gather + stage update  */
     DNDSMI_K_LUSGS_MATVEC = 13,/* LUSGSMatrixVec (GMRES operator)            */
     DNDSMI_K_RHS_BFACE = 14, /* EvaluateRHS: boundary-face fluxes (FP64)     */
-    DNDSMI_K_COUNT = 15
+    DNDSMI_K_READ_PROBE = 15,/* read-bandwidth probe of the sweep's A^-1 B stream */
+    DNDSMI_K_COUNT = 16
 };
 
 /* Turns HIP-event brackets around every launch on or off and clears the
@@ -522,6 +556,12 @@ int dndsmi_profile_query(dndsmi_ctx *ctx, int32_t kernel, int64_t *count, double
 /* Algorithmic HBM bytes of one launch of `kernel` on this context's mesh
  * (DNDSMI_K_LUSGS_SWEEP: of one whole sweep, i.e. all its colour launches). */
 int64_t dndsmi_kernel_bytes(const dndsmi_ctx *ctx, int32_t kernel);
+
+/* No counterpart in the reference (measurement): reps launches of a kernel
+ * that streams the sweep's A^-1 B pairs exactly as the Jacobi sweep does,
+ * without its gathers (DNDSMI_K_READ_PROBE in the profile): the read
+ * bandwidth the sweep's streaming part can reach on this device. */
+int dndsmi_read_probe(dndsmi_ctx *ctx, int32_t reps);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,8 @@
 """numpy restatement of the geometry and VR setup (DNDS_FV_VR.hpp:29-102,
-DNDS_FV_VR.cpp:82-1154) for small meshes, global-coordinate basis, p = 3 or
-p = 1 (setting "rec_order"; p = 1 matrices in the top-left 2 x 2 corner of
-the 9 x 9 blocks, the rest zero).
+DNDS_FV_VR.cpp:82-1154) for small meshes, global-coordinate basis or the
+local inertia coordinates (setting "use_local_coord", DNDS_FV_VR.hpp:358-1018),
+p = 3 or p = 1 (setting "rec_order"; p = 1 matrices in the top-left 2 x 2
+corner of the 9 x 9 blocks, the rest zero).
 
 TEST INFRASTRUCTURE ONLY: an independent second implementation that the
 tests hold the product's host setup against.  It is written from the
@@ -72,6 +73,63 @@ def basis(p, c, scale, moment):
     return D
 
 
+def _poly_mul(a, b):
+    """product of two polynomials in (dx, dy) held as 4 x 4 coefficient grids"""
+    out = np.zeros((4, 4))
+    for i in range(4):
+        for j in range(4):
+            if a[i, j]:
+                out[i:, j:] += a[i, j] * b[:4 - i, :4 - j]
+    return out
+
+
+def basis_local(p, c, A, moment):
+    """10 x 10 derivatives of the monomials of xi = A (p - c) (FDiffBaseValue,
+    local coordinates), by a route of its own: each xi-monomial is expanded
+    into monomials of (dx, dy) = p - c, whose derivatives in x and y are
+    those of the unscaled global basis."""
+    e0 = np.zeros((4, 4))
+    e0[1, 0], e0[0, 1] = A[0, 0], A[0, 1]
+    e1 = np.zeros((4, 4))
+    e1[1, 0], e1[0, 1] = A[1, 0], A[1, 1]
+    one = np.zeros((4, 4))
+    one[0, 0] = 1.0
+    pw0, pw1 = [one], [one]
+    for _ in range(3):
+        pw0.append(_poly_mul(pw0[-1], e0))
+        pw1.append(_poly_mul(pw1[-1], e1))
+    Dg = basis(p, c, np.ones(2), np.zeros(10))  # derivatives of dx^i dy^j at p
+    D = np.zeros((10, 10))
+    for j, (a, b) in enumerate(DIFF):
+        coef = _poly_mul(pw0[a], pw1[b])
+        for k, (i2, j2) in enumerate(DIFF):
+            if coef[i2, j2]:
+                D[:, j] += coef[i2, j2] * Dg[:, k]
+    D[0] -= moment
+    return D
+
+
+def local_frame(X, bary, inertia, anisotropic):
+    """invPJacobi of the local coordinates: the principal axes of the cell's
+    second-moment tensor (largest first, oriented so that the first axis's
+    larger component is positive; the x axis when the tensor is isotropic to
+    1e-10 of its trace; the second axis the first turned by +90 degrees),
+    scaled by the cell's half extents along them (or both by their
+    diagonal)."""
+    w, V = np.linalg.eigh(inertia)
+    if w[1] - w[0] <= 1e-10 * np.trace(inertia):
+        a0 = np.array([1.0, 0.0])
+    else:
+        a0 = V[:, np.argmax(w)]
+        if a0[np.argmax(np.abs(a0))] < 0:
+            a0 = -a0
+    P = np.stack([a0, np.array([-a0[1], a0[0]])], axis=1)
+    nc = np.linalg.solve(P, (X - bary).T)
+    half = (nc.max(1) - nc.min(1)) / 2
+    s = half if anisotropic else np.full(2, np.linalg.norm(half))
+    return np.linalg.inv(P * s[None, :])
+
+
 def functional_rows(W, n, tang_weight, tang_norm):
     """Rows P (10 x 10) and diagonal weights q with Conj(DI, DJ) = (P DI)^T diag(q) (P DJ)
     (FFaceFunctional with USE_NORM_FUNCTIONAL, tangWeightModMin = 1)."""
@@ -119,13 +177,23 @@ def build(mesh: dict, setting: dict) -> dict:
     nc, nf = mesh["n_cell"], mesh["n_face"]
     coords = [xy[c2n[cp[i]:cp[i + 1]]] for i in range(nc)]
     vol, bary, scale, mom = np.zeros(nc), np.zeros((nc, 2)), np.zeros((nc, 2)), np.zeros((nc, 10))
+    local = bool(setting.get("use_local_coord", 0))
+    frame = np.zeros((nc, 2, 2))
+
+    def cell_basis(i, p, c, moment):
+        return basis_local(p, c, frame[i], moment) if local else basis(p, c, scale[i], moment)
+
     for i, X in enumerate(coords):
         rule_v, rule_b = (TRI7, TRI4) if len(X) == 3 else (QUAD9, QUAD9)
         vol[i] = sum(point_det(X, a, b)[1] * w for a, b, w in rule_v)
         bary[i] = sum(point_det(X, a, b)[0] * point_det(X, a, b)[1] * w for a, b, w in rule_b) / vol[i]
         s = (X.max(0) - X.min(0)) / 2
         scale[i] = s if setting["anisotropic_lengths"] else np.full(2, np.linalg.norm(s))
-        mom[i] = sum(basis(point_det(X, a, b)[0], bary[i], scale[i], np.zeros(10))[0] * point_det(X, a, b)[1] * w
+        if local:
+            inertia = sum(np.outer(point_det(X, a, b)[0] - bary[i], point_det(X, a, b)[0] - bary[i])
+                          * point_det(X, a, b)[1] * w for a, b, w in rule_b) / vol[i]
+            frame[i] = local_frame(X, bary[i], inertia, setting["anisotropic_lengths"])
+        mom[i] = sum(cell_basis(i, point_det(X, a, b)[0], bary[i], np.zeros(10))[0] * point_det(X, a, b)[1] * w
                      for a, b, w in rule_b) / vol[i]
         mom[i, 0] = 1.0
 
@@ -146,7 +214,7 @@ def build(mesh: dict, setting: dict) -> dict:
     def side_basis(f, side, xi):
         p, c = side_point(f, side, xi)
         i = f2c[f, side]
-        return basis(p, c, scale[i], mom[i])
+        return cell_basis(i, p, c, mom[i])
 
     fn = np.zeros((nf, 2))
     area = np.zeros(nf)
@@ -192,7 +260,7 @@ def build(mesh: dict, setting: dict) -> dict:
                 Db = side_basis(f, side, G3[g])
                 d0[f, g, side] = Db[0, 1:]
                 p, c = side_point(f, side, G3[g])
-                gxy[f, g, side] = (p - c) / scale[f2c[f, side]]
+                gxy[f, g, side] = frame[f2c[f, side]] @ (p - c) if local else (p - c) / scale[f2c[f, side]]
                 if g == 0:
                     dg0[f, side] = Db[:, 1:]
         if f2c[f, 1] != NO_CELL:
@@ -248,4 +316,5 @@ def build(mesh: dict, setting: dict) -> dict:
     return dict(volume=vol, face_area=area, cell_bary=bary.ravel(), base_moment=mom.ravel(), face_center=fc.ravel(),
                 face_norm_center=fn.ravel(), face_weight=W.ravel(), face_d0=d0.ravel(), face_dibj_g0=dg0.ravel(),
                 matrix_secondary=ms.ravel(), ainv=ainv.ravel(), ainv_b=ainv_b.ravel(), ainv_bvec=ainv_bvec.ravel(),
-                face_gauss_xy=gxy.ravel(), cell_scale=np.asarray(scale, dtype=float).ravel())
+                face_gauss_xy=gxy.ravel(), cell_scale=np.asarray(scale, dtype=float).ravel(),
+                cell_frame=frame.ravel() if local else None)
