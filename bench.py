@@ -78,14 +78,14 @@ def rel_l2(a, b, vol):
     return (np.sqrt((w * (a - b) ** 2).sum(0) / (w * b ** 2).sum(0))).tolist()
 
 
-def make_case(name, n, threads):
+def make_case(name, n, threads, rec_order=3):
     from dnds_amd import cases
     if name == "riemann":
         return cases.riemann_2d(n, nthreads=threads, build_limiter_data=True)
     if name == "step":
         return cases.mach3_step(n, nthreads=threads)
     if name == "plate":
-        return cases.flat_plate(n * 3 // 2, n * 5 // 16, nthreads=threads)
+        return cases.flat_plate(n * 3 // 2, n * 5 // 16, rec_order=rec_order, nthreads=threads)
     return cases.isentropic_vortex(n, nthreads=threads)
 
 
@@ -143,9 +143,12 @@ class Arms:
             t += ode.Step(u, r, rn, t)
         return u.download(), r.download(), t
 
-    def timed(self, mixed, steps, warmup, ws=None):
+    def timed(self, mixed, steps, warmup, ws=None, until_failure=False):
         """warmup steps, then `steps` timed steps continuing from them; the
-        per-kernel HIP-event times of the timed steps"""
+        per-kernel HIP-event times of the timed steps.  until_failure: a step
+        that fails (a non-positive state) ends the run; self.completed holds
+        the steps before it, and the time covers them"""
+        from dnds_amd.host import DNDSError
         tag = ("mixed+learned" if ws is not None else "mixed") if mixed else "fp64"
         u, r, rn = self.fresh()
         ode, t = self.ode(mixed, ws), 0.0
@@ -154,12 +157,22 @@ class Arms:
         self.ev.device.sync()
         self.lib.dndsmi_profile_enable(self.ev.device.ctx, 1)
         self.ev.device.sync()
+        self.completed, self.failure = steps, None
         ts = time.perf_counter()
-        for _ in range(steps):
-            t += ode.Step(u, r, rn, t)
+        for k in range(steps):
+            try:
+                t += ode.Step(u, r, rn, t)
+            except DNDSError as e:
+                if not until_failure:
+                    raise
+                # the failure surfaces when step k reads the time step back:
+                # the state left by step k - 1 is not admissible
+                self.completed, self.failure = k - 1, f"step {k}: {e}"
+                break
         self.ev.device.sync()
         elapsed = time.perf_counter() - ts
-        log(f"[{self.case.name} {tag}] timed {steps} steps after {warmup}: {elapsed:.3f}s, t = {t:.6f}")
+        log(f"[{self.case.name} {tag}] timed {self.completed} steps after {warmup}: {elapsed:.3f}s, t = {t:.6f}"
+            + (f" (stopped: {self.failure})" if self.failure else ""))
         prof = kernel_profile(self.lib, self.ev.device.ctx, self.A)
         self.lib.dndsmi_profile_enable(self.ev.device.ctx, 0)
         for name, p in prof.items():
@@ -240,10 +253,13 @@ def main():
                     help="step: the combined large case, Mach-3 shock on a mixed tri/quad mesh of [0,4]x[0,1] "
                          "(n x n quads, half split; 6.3M cells at n 2048); riemann: four-quadrant Riemann problem "
                          "on [0,1]^2 (2 n^2 triangles); vortex: periodic isentropic vortex on [0,10]^2; plate: laminar "
-                         "flat plate at M 0.2, Re 5e6, (1.5 n) x (0.3125 n) quads stretched to y+ = 1, steady: with "
-                         "--implicit N, N LU-SGS iterations from the free stream at CFL 20 with local time steps")
+                         "flat plate at M 0.2, Re 1e5, (1.5 n) x (0.3125 n) quads stretched to y+ = 1, the reference's "
+                         "boundary-layer VR settings, steady: with --implicit N, N LU-SGS iterations from the free "
+                         "stream at CFL 20 with local time steps")
     ap.add_argument("--n", type=int, default=None, help="quads per direction (default 2048 for step, 1024 else)")
     ap.add_argument("--sweeps", type=int, default=5)
+    ap.add_argument("--rec-order", type=int, default=3, choices=[1, 3],
+                    help="P_ORDER of the plate case (the other cases run p = 3)")
     ap.add_argument("--cfl", type=float, default=0.5)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="host threads for the setup and the CPU baseline")
@@ -294,7 +310,7 @@ def main():
 
     t0 = time.time()
     log(f"building mesh + VR setup, {args.n}x{args.n} quads ({args.case})")
-    case = make_case(args.case, args.n, args.threads)
+    case = make_case(args.case, args.n, args.threads, args.rec_order)
     nc = case.n_cell
     log(f"setup done: {nc} cells, {case.mesh.n_face} faces, {time.time() - t0:.1f}s")
     gas = case.gas or dict(muGas=args.mu_gas, prGas=0.72, Rgas=1.0, TRef=1.0, CSutherland=0.5)
@@ -400,13 +416,16 @@ def main():
         log("CPU baseline: building the oracle -O3 -march=native on this host")
         oracle.build_baseline()
         st = case.vfv.setting
-        orc = oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(), riemann_solver=2 if case.riemann == "HLLC" else 1,
-                            far_field=case.far, tang_weight=st.tang_weight,
-                            tang_weight_direction=st.tang_weight_direction,
-                            wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
-                            limiter_procedure=args.limiter_procedure, mu_gas=gas["muGas"], pr_gas=gas["prGas"],
-                            r_gas=gas["Rgas"], t_ref=gas["TRef"], c_sutherland=gas["CSutherland"],
-                            variant="baseline")
+
+        def make_oracle(variant):
+            return oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(),
+                                 riemann_solver=2 if case.riemann == "HLLC" else 1, far_field=case.far,
+                                 tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
+                                 wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
+                                 limiter_procedure=args.limiter_procedure, mu_gas=gas["muGas"], pr_gas=gas["prGas"],
+                                 r_gas=gas["Rgas"], t_ref=gas["TRef"], c_sutherland=gas["CSutherland"],
+                                 variant=variant)
+        orc = make_oracle("baseline")
         oracle.set_threads(args.threads, variant="baseline")
         log(f"CPU baseline: oracle on {args.threads} threads, one SSP-RK3 step on the same mesh")
         c0 = time.perf_counter()
@@ -418,11 +437,19 @@ def main():
                          f"of the same {nc}-cell workload from its initial state: {el:.3g} s on {args.threads} "
                          "threads",
                "build": "g++ " + " ".join(oracle.BASELINE_FLAGS)}
-        oracle_cmp = {"steps": 1, "oracle_build": "the CPU-baseline build",
-                      "t_oracle": ot, "t_gpu": one["fp64"][2]}
+        # the parity build (every product and sum rounded as written, as on
+        # the device) is the checker; the speed build's contractions differ
+        # within rounding, which the limiter's switches can amplify
+        log("oracle (parity build): one SSP-RK3 step for the comparison")
+        orc = make_oracle("parity")
+        oracle.set_threads(args.threads)
+        pu, _, pt = orc.ssprk3_steps(case.u0, arms.urec0, 1, args.sweeps, args.cfl, use_limiter=case.limiter)
+        oracle_cmp = {"steps": 1, "oracle_build": "parity (-O2 -ffp-contract=off)", "t_oracle": pt,
+                      "t_gpu": one["fp64"][2]}
         for tag, (gu, _, _) in one.items():
-            oracle_cmp[f"rel_l2_{tag}"] = rel_l2(gu, ou, vol)
-            oracle_cmp[f"rel_max_{tag}"] = float(np.abs(gu - ou).max() / np.abs(ou).max())
+            oracle_cmp[f"rel_l2_{tag}"] = rel_l2(gu, pu, vol)
+            oracle_cmp[f"rel_max_{tag}"] = float(np.abs(gu - pu).max() / np.abs(pu).max())
+        oracle_cmp["speed_build_rel_max_vs_parity_build"] = float(np.abs(ou - pu).max() / np.abs(pu).max())
         log(f"vs oracle after one step: {oracle_cmp}")
 
     accuracy = vortex_accuracy(args, params) if args.accuracy else None
@@ -524,20 +551,21 @@ def main():
                                f"HLLC, WBAP V2, SSP-RK3 CFL {args.cfl}, {args.sweeps} sweeps/stage, "
                                f"{rcase.steps} steps from the initial state", "cells": rcase.n_cell,
                    "steps": rcase.steps}
-        try:
-            re, rprof, ru, _, _ = rarms.timed(False, rcase.steps, 0)
-            riemann.update({"value_fp64": rcase.n_cell * 3 * rcase.steps / re,
-                            "ms_per_step_fp64": re / rcase.steps * 1e3,
-                            "jacobi_frac": rprof["k_jacobi"]["gbs"] / HBM_PEAK_GBS,
-                            "evaluate_rhs": evaluate_rhs_object(rprof, rcase)})
-            if args.mixed:
-                me, mprof, mu, _, _ = rarms.timed(True, rcase.steps, 0)
-                rvol = rcase.vfv.arrays(["volume"])["volume"]
-                riemann.update({"value_mixed": rcase.n_cell * 3 * rcase.steps / me,
-                                "ms_per_step_mixed": me / rcase.steps * 1e3, "speedup": re / me,
-                                "rel_l2_mixed_vs_fp64": rel_l2(mu, ru, rvol)})
-        except DNDSError as e:
-            riemann["failure"] = str(e)
+        # the step that meets a non-positive cell mean ends the run (the
+        # scheme aborts there, as the reference's DNDS_assert would)
+        re, rprof, ru, _, _ = rarms.timed(False, rcase.steps, 0, until_failure=True)
+        n_ok = rarms.completed
+        riemann.update({"steps_completed_fp64": n_ok, "failure_fp64": rarms.failure,
+                        "value_fp64": rcase.n_cell * 3 * n_ok / re, "ms_per_step_fp64": re / max(n_ok, 1) * 1e3,
+                        "jacobi_frac": rprof["k_jacobi"]["gbs"] / HBM_PEAK_GBS,
+                        "evaluate_rhs": evaluate_rhs_object(rprof, rcase)})
+        if args.mixed:
+            me, mprof, mu, _, _ = rarms.timed(True, n_ok, 0, until_failure=True)
+            rvol = rcase.vfv.arrays(["volume"])["volume"]
+            riemann.update({"steps_mixed": rarms.completed, "value_mixed": rcase.n_cell * 3 * rarms.completed / me,
+                            "ms_per_step_mixed": me / max(rarms.completed, 1) * 1e3,
+                            "speedup": (re / max(n_ok, 1)) / (me / max(rarms.completed, 1)),
+                            "rel_l2_mixed_vs_fp64": rel_l2(mu, ru, rvol) if rarms.completed == n_ok else None})
         log(f"riemann_2M: {riemann}")
 
     wbap = "V2" if args.limiter_procedure == 0 else "V3"
@@ -563,8 +591,9 @@ def main():
                        "step": (f"Mach-3 step, [0,4]x[0,1], {args.n}x{args.n} quads half split: {nc} mixed cells, "
                                 f"jitter 0.15h, p=3 VR, HLLC, WBAP {wbap}, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi "
                                 "sweeps/stage"),
-                       "plate": (f"laminar flat plate, M 0.2, Re 5e6, {nc} quads stretched to y+ = 1, p=3 VR, Roe + "
-                                 f"viscous, SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage")}[args.case],
+                       "plate": (f"laminar flat plate, M 0.2, Re 1e5, {nc} quads stretched to y+ = 1, "
+                                 f"p={args.rec_order} VR with the reference's boundary-layer settings, Roe + viscous, "
+                                 f"SSP-RK3 CFL {args.cfl}, {args.sweeps} Jacobi sweeps/stage")}[args.case],
                    "cells": nc, "sweeps_per_stage": args.sweeps,
                    "limiter": f"WBAP {wbap}" if case.limiter else "none", "precision": "fp64",
                    "model": "laminar Navier-Stokes, muGas %g" % gas["muGas"] if gas["muGas"] > 0 else "Euler"},

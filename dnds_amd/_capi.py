@@ -162,6 +162,7 @@ SIGNATURES = [
     ("dndsmi_warmstart_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("dndsmi_ssprk3_set_warm_start", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     ("dndsmi_read_probe", C.c_int, [C.c_void_p, C.c_int32]),
+    ("dndsmi_set_limiter_if_all", C.c_int, [C.c_void_p, C.c_int32]),
     ("dndsmi_ssprk3_warm_start_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), PD]),
     ("dndsmi_reconstruct_to_tolerance", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                                   C.c_int32, C.c_double, PI32, PD]),

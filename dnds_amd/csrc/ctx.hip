@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
             bmul = 0.0;
         }
         // A^-1 B entries e = 9 r + c, two per 16-byte load (entry 81 pads)
-        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + int64_t(s) * kMPairs * ldc + i;
+        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + m_vec_index(s, 0, i, ldc, kMPairs);
         double du[4];
 #pragma unroll
         for (int v = 0; v < 4; v++)
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int32_t nc, int64_t ldc, cons
         for (int e = 0; e < 81; e++) {
             const int r = e / 9, c = e % 9;
             if (e % 2 == 0)
-                mp = __builtin_nontemporal_load(Ms + (e / 2) * ldc);
+                mp = __builtin_nontemporal_load(Ms + (e / 2) * 64);
             const double m = e % 2 == 0 ? mp.x : mp.y;
             if (c == 0)
                 tt[0] = tt[1] = tt[2] = tt[3] = 0.0;
@@ -253,10 +253,10 @@ __global__ __launch_bounds__(kBlock) void k_read_probe(int32_t nc, int64_t ldc, 
     for (int s = 0; s < NF; s++) {
         if (nbr[s * ldc + i] == -2)
             continue;
-        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + int64_t(s) * kMPairs * ldc + i;
+        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + m_vec_index(s, 0, i, ldc, kMPairs);
 #pragma unroll
         for (int e = 0; e < kMPairs; e++) {
-            const dbl2 m = __builtin_nontemporal_load(Ms + e * ldc);
+            const dbl2 m = __builtin_nontemporal_load(Ms + e * 64);
             a0 += m.x, a1 += m.y;
         }
     }
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, con
             j = i; // far field
         }
         // FP32 A^-1 B entries, four per 16-byte load (81..83 pad)
-        const flt4 *Ms = reinterpret_cast<const flt4 *>(M32) + int64_t(s) * kMQuads * ldc + i;
+        const flt4 *Ms = reinterpret_cast<const flt4 *>(M32) + m_vec_index(s, 0, i, ldc, kMQuads);
         float dj[36];
 #pragma unroll
         for (int k = 0; k < 36; k++)
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, con
         for (int e = 0; e < 81; e++) {
             const int r = e / 9, c = e % 9;
             if (e % 4 == 0)
-                mq = __builtin_nontemporal_load(Ms + (e / 4) * ldc);
+                mq = __builtin_nontemporal_load(Ms + (e / 4) * 64);
             const float m = mq[e % 4];
             if (c == 0)
                 tt[0] = tt[1] = tt[2] = tt[3] = 0.f;
@@ -397,17 +397,19 @@ __global__ __launch_bounds__(kBlock) void k_delta32(int32_t nc, int64_t ldc, con
     }
 }
 
-// A^-1 B from FP64 pairs [slot][41][ldc][2] to FP32 quads [slot][21][ldc][4]
+// A^-1 B from FP64 pairs to FP32 quads (both in the m_vec_index layout)
 __global__ __launch_bounds__(kBlock) void k_m_to_f32(int32_t nfmax, int64_t ldc, const double *__restrict__ M,
                                                      float *__restrict__ M32) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= int64_t(nfmax) * kMQuads * ldc)
         return;
-    const int64_t i = t % ldc, sq = t / ldc, s = sq / kMQuads, q = sq % kMQuads;
+    const int64_t i = t % ldc, sq = t / ldc;
+    const int s = int(sq / kMQuads), q = int(sq % kMQuads);
+    const int64_t o = m_vec_index(s, q, i, ldc, kMQuads) * 4;
 #pragma unroll
     for (int w = 0; w < 4; w++) {
-        const int k = int(4 * q) + w;
-        M32[t * 4 + w] = k < 81 ? float(M[((s * kMPairs + k / 2) * ldc + i) * 2 + k % 2]) : 0.f;
+        const int k = 4 * q + w;
+        M32[o + w] = k < 81 ? float(M[m_vec_index(s, k / 2, i, ldc, kMPairs) * 2 + k % 2]) : 0.f;
     }
 }
 
@@ -1417,15 +1419,30 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         // kept: on the structured-as-unstructured meshes it is row-major, and
         // measured faster for the sweeps than Morton or strip orders (whose
         // neighbour loads coalesce worse); a mesh read in a scattered order
-        // would be renumbered here.  Faces follow their first cell.
+        // would be renumbered here.  On a mesh of triangles and quads, each
+        // window of 64 consecutive cells (one wavefront) puts its triangles
+        // before its quads, keeping their order: the fourth slot's A^-1 B
+        // vectors of a block are then contiguous over its quads, and the
+        // 128-byte lines of that stream hold no unused triangle slots.
         std::vector<int32_t> &c2d = c->c2d, &f2d = c->f2d;
         {
             c2d.resize(static_cast<size_t>(nc));
-            for (int64_t i = 0; i < nc; i++)
-                c2d[i] = int32_t(i);
+            for (int64_t w0 = 0; w0 < nc; w0 += 64) {
+                const int64_t w1 = std::min<int64_t>(nc, w0 + 64);
+                int64_t d = w0;
+                for (int pass = 0; pass < 2; pass++)
+                    for (int64_t i = w0; i < w1; i++) {
+                        const bool quad = mesh->cell_ptr[i + 1] - mesh->cell_ptr[i] == 4;
+                        if (quad == (pass == 1))
+                            c2d[i] = int32_t(d++);
+                    }
+            }
+            // faces in the order of their first cell in the mesh (the
+            // partition of a window above moves cells by less than a
+            // wavefront, and the face kernels gather best in mesh order)
             std::vector<std::pair<int64_t, int64_t>> fk(static_cast<size_t>(nf));
             for (int64_t f = 0; f < nf; f++)
-                fk[f] = {c2d[mesh->face2cell[2 * f]], f};
+                fk[f] = {mesh->face2cell[2 * f], f};
             std::sort(fk.begin(), fk.end());
             f2d.assign(static_cast<size_t>(nf), 0);
             for (int64_t d = 0; d < nf; d++)
@@ -1455,7 +1472,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
                 sbc[s * ldc + ci] = mesh->face_bc[f];
                 sface[s * ldc + ci] = f2d[f];
                 for (int k = 0; k < 81; k++)
-                    M[((size_t(s) * kMPairs + k / 2) * ldc + ci) * 2 + k % 2] = vr->ainv_b[81 * (b0 + s) + k];
+                    M[size_t(m_vec_index(s, k / 2, ci, ldc, kMPairs)) * 2 + k % 2] = vr->ainv_b[81 * (b0 + s) + k];
                 for (int k = 0; k < 9; k++)
                     bv[(size_t(s) * 9 + k) * ldc + ci] = vr->ainv_bvec[9 * (b0 + s) + k];
                 order.push_back({f, side});
@@ -2109,6 +2126,13 @@ extern "C" int dndsmi_set_limiter_procedure(dndsmi_ctx *c, int32_t limiterProced
     return DNDSMI_OK;
 }
 
+extern "C" int dndsmi_set_limiter_if_all(dndsmi_ctx *c, int32_t ifAll) {
+    if (!c)
+        return DNDSMI_ERR_INVALID_ARGUMENT;
+    c->limiter_if_all = ifAll != 0;
+    return DNDSMI_OK;
+}
+
 extern "C" int dndsmi_reconstruction_jacobi_step(dndsmi_ctx *c, const dndsmi_array *u, dndsmi_array *uRec,
                                                  dndsmi_array *uRecNewBuf, double t) {
     if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
@@ -2180,7 +2204,7 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
         // the RHS sees the limited cells from xlim (flagged), the others from
         // uRec, which keeps the unlimited iterate
         if (use_limiter)
-            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, 0, c->limiter_procedure, false));
+            DNDSMI_TRY(launch_wbap(c, u->d, uRec->d, c->xlim, nullptr, c->limiter_if_all, c->limiter_procedure, false));
         DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, ts, nullptr, stage + 1, u->d, UseLocaldt ? c->dt_tmp : nullptr, cA,
                               cB, use_limiter ? c->xlim : nullptr, use_limiter ? c->limf : nullptr));
     }
@@ -2204,7 +2228,7 @@ int mixed_stages(dndsmi_ctx *c, double *u, double *x, int32_t n_sweeps, int32_t 
         const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
         const double cB = stage ? 1 - Coef[stage] : 0.0;
         if (use_limiter)
-            DNDSMI_TRY(launch_wbap(c, u, x, c->xlim, nullptr, 0, c->limiter_procedure, false));
+            DNDSMI_TRY(launch_wbap(c, u, x, c->xlim, nullptr, c->limiter_if_all, c->limiter_procedure, false));
         DNDSMI_TRY(launch_rhs_mixed(c, u, x, ts, nullptr, stage + 1, u, UseLocaldt ? c->dt_tmp : nullptr, cA, cB,
                                     use_limiter ? c->xlim : nullptr, use_limiter ? c->limf : nullptr));
     }

@@ -90,6 +90,7 @@ struct dndsmi_ctx {
     int twd_norm = 1;
     double wbap_scale = 1e-10, wbap_nstd = 10.0;
     int32_t limiter_procedure = 0; // the steps' limiter: 0 WBAP V2, 1 WBAP V3
+    int32_t limiter_if_all = 0;    // the steps' limiter limits every cell (the solver's ifAll)
     // evaluator scratch (EulerEvaluator members lambdaFace, deltaLambdaFace)
     double *lambda_face = nullptr;
     double *dlambda_face = nullptr;

@@ -52,6 +52,15 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 constexpr int kMPairs = 41, kMQuads = 21;
 
+// Layout of the A^-1 B stream: per slot, blocks of 64 consecutive cells,
+// and within a block the 16-byte vectors (pair e or quad q) of all 64 cells
+// contiguous, vector after vector.  A wavefront (64 cells) then reads one
+// contiguous 41 KiB (FP64) or 21 KiB (FP32) run per slot instead of 41 or 21
+// runs of 1 KiB spread over the whole array.  ldc is a multiple of 64.
+__host__ __device__ __forceinline__ int64_t m_vec_index(int s, int e, int64_t i, int64_t ldc, int nvec) {
+    return ((int64_t(s) * (ldc >> 6) + (i >> 6)) * nvec + e) * 64 + (i & 63);
+}
+
 struct GasParams {
     double gamma;
     double far[4];

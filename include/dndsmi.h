@@ -309,6 +309,12 @@ int dndsmi_wbap_limit_facial_v2(dndsmi_ctx *ctx, const dndsmi_array *u, const dn
  * steps below apply when use_limiter is set, 0 for V2 (the default) or 1 for
  * V3.  Any other value is DNDSMI_ERR_INVALID_ARGUMENT. */
 int dndsmi_set_limiter_procedure(dndsmi_ctx *ctx, int32_t limiterProcedure);
+/* The ifAll argument the SSP-RK3 steps pass to the limiter: every cell is
+ * limited, whatever its smoothness indicator.  The reference's frhs passes
+ * iter < nPartialLimiterStartLocal && step < nPartialLimiterStart
+ * (DNDS_FV_EulerSolver.hpp:796-806); the caller sets it per step (0 after
+ * the context is created). */
+int dndsmi_set_limiter_if_all(dndsmi_ctx *ctx, int32_t ifAll);
 
 /* One ExplicitSSPRK3LocalDt::Step over frhs = n_sweeps Jacobi sweeps, then
  * (use_limiter) WBAP V2 or V3 (dndsmi_set_limiter_procedure) into a context
