@@ -47,20 +47,21 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # the plain run (1 % of the vortex case's L2 discretisation error, 2.5e-5)
 LEARNED_CHECK = (5, 20, 2.5e-7)
 
-# committed rocprofv3 FETCH_SIZE + WRITE_SIZE summaries, by workload
+# committed rocprofv3 summaries (kernel trace, request-size traffic), by workload
 PROFILES = {("riemann", 1024): os.path.join(ROOT, "profiles", "riemann_2M_kernels_traffic.md"),
             ("step", 2048): os.path.join(ROOT, "profiles", "step_6M_kernels_traffic.md")}
 
 
 def measured_traffic(case: str, n: int, kernel: str):
-    """HBM bytes per launch of `kernel` on a workload, from the committed
-    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (None if absent)."""
+    """L2-to-fabric bytes per launch of `kernel` on a workload (an upper
+    bound of its HBM bytes), from the committed rocprofv3 request-size
+    summary (None if absent)."""
     path = PROFILES.get((case, n))
     try:
         with open(path) as f:
             for line in f:
                 cols = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cols) == 7 and cols[0] == f"`{kernel}`":
+                if len(cols) >= 7 and cols[0] == f"`{kernel}`":
                     return float(cols[4]) * 1e9, os.path.relpath(path, ROOT)
     except (OSError, TypeError):
         pass

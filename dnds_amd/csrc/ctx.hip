@@ -863,7 +863,7 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
                                                       const double *__restrict__ fw,
                                                       const double *__restrict__ fn, const double *__restrict__ u,
                                                       const double *__restrict__ x, double tang_weight, int twd_norm,
-                                                      double *__restrict__ wsum) {
+                                                      const uint8_t *__restrict__ wpos, double *__restrict__ wslot) {
     const int32_t f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= nf)
         return;
@@ -971,9 +971,13 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
             out[2 * q] = a * 2.0;
             out[2 * q + 1] = b * 2.0;
         }
+        // into the cell's own slot of this face, [slot][4][cell], so that the
+        // indicator reads its terms as a coalesced stream
+        const int32_t cell = cs ? R : L;
+        const int slot = cs ? (wpos[f] >> 4) : (wpos[f] & 15);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            wsum[8 * int64_t(f) + 4 * cs + k] = out[k];
+            wslot[(int64_t(slot) * 4 + k) * ldc + cell] = out[k];
     }
 }
 
@@ -983,11 +987,8 @@ __global__ __launch_bounds__(kBlock) void k_wbap_face(int32_t nf, int64_t ldf, i
 // (one atomic per wavefront; the order of the list does not affect any
 // result, each cell is limited independently).
 template <int NF>
-__global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, int64_t ldf,
-                                                     const int32_t *__restrict__ nbr,
-                                                     const int32_t *__restrict__ sface,
-                                                     const int32_t *__restrict__ f2c,
-                                                     const double *__restrict__ wsum, const double *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, const int32_t *__restrict__ nbr,
+                                                     const double *__restrict__ wslot, const double *__restrict__ x,
                                                      double *__restrict__ xo, double *__restrict__ ind, double scale,
                                                      int if_all, int32_t *__restrict__ count,
                                                      int32_t *__restrict__ list, uint8_t *__restrict__ limf) {
@@ -1000,11 +1001,9 @@ __global__ __launch_bounds__(kBlock) void k_wbap_ind(int32_t nc, int64_t ldc, in
             const int32_t j = nbr[s * ldc + i];
             if (j < 0)
                 continue;
-            const int32_t f = sface[s * ldc + i];
-            const int side = f2c[f] == i ? 0 : 1;
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                sum[k] += wsum[8 * int64_t(f) + 4 * side + k];
+                sum[k] += wslot[(int64_t(s) * 4 + k) * ldc + i];
         }
         const double si0 = sum[0] / (sum[1] + kVerySmallReal);
         const double si1 = sum[2] / (sum[3] + kVerySmallReal);
@@ -1451,6 +1450,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         // ---- cell streams
         std::vector<int32_t> nbr(size_t(nfmax) * ldc, -2), sbc(size_t(nfmax) * ldc, 0), sface(size_t(nfmax) * ldc, -1);
         std::vector<int32_t> gat(size_t(nfmax) * ldc, -1), widx(size_t(nfmax) * ldc, -1);
+        std::vector<uint8_t> wpos(ldf, 0);
         std::vector<double> M(size_t(nfmax) * kMPairs * 2 * ldc, 0.0), bv(size_t(nfmax) * 9 * ldc, 0.0), vol(ldc, 1.0);
         std::vector<double> cmom(9 * size_t(ldc), 0.0), cscale(2 * size_t(ldc), 1.0);
         std::vector<int64_t> wall_cell, wall_face, wall_c2f;
@@ -1471,6 +1471,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
                 nbr[s * ldc + ci] = j == DNDSMI_NO_CELL ? -1 : c2d[j];
                 sbc[s * ldc + ci] = mesh->face_bc[f];
                 sface[s * ldc + ci] = f2d[f];
+                if (j != DNDSMI_NO_CELL)
+                    wpos[f2d[f]] |= uint8_t(s << (4 * side));
                 for (int k = 0; k < 81; k++)
                     M[size_t(m_vec_index(s, k / 2, ci, ldc, kMPairs)) * 2 + k % 2] = vr->ainv_b[81 * (b0 + s) + k];
                 for (int k = 0; k < 9; k++)
@@ -1496,6 +1498,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         DNDSMI_TRY(c->upload(c->sface, sface));
         DNDSMI_TRY(c->alloc(c->gat, nfmax * ldc));
         DNDSMI_TRY(c->upload(c->gat, gat));
+        DNDSMI_TRY(c->alloc(c->wpos, ldf));
+        DNDSMI_TRY(c->upload(c->wpos, wpos));
         {
             std::vector<uint8_t> fpos(ldf, 0);
             for (int s = 0; s < nfmax; s++)
@@ -1623,7 +1627,7 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
                     buf[k * ldf + f2d[f]] = vr->face_weight[10 * f + k];
             DNDSMI_TRY(c->alloc(c->fw, 10 * ldf));
             DNDSMI_TRY(c->upload(c->fw, buf));
-            DNDSMI_TRY(c->alloc(c->wsum, 8 * ldf));
+            DNDSMI_TRY(c->alloc(c->wslot, int64_t(nfmax) * 4 * ldc));
             DNDSMI_TRY(c->alloc(c->uF, int64_t(nfmax) * 28 * ldc));
             DNDSMI_TRY(c->alloc(c->ind, ldc));
             DNDSMI_TRY(c->alloc(c->xlim, 36 * ldc));
@@ -1937,18 +1941,19 @@ int launch_wbap(dndsmi_ctx *c, const double *u, const double *x, double *xo, dou
     const double eps4 = std::pow(1e-200, 1.0 / 4); // verySmallReal_pDiP, DNDS_FV_Limiters.hpp:345
     prof_begin(c, DNDSMI_K_WBAP_FACE);
     hipLaunchKernelGGL(k_wbap_face, dim3(grid_for(nf)), dim3(kBlock), 0, c->stream, nf, c->ldf, c->ldc, c->f2c,
-                       c->fxy, c->cmom, c->cscale, c->fw, c->fn, u, x, c->tang_weight, c->twd_norm, c->wsum);
+                       c->fxy, c->cmom, c->cscale, c->fw, c->fn, u, x, c->tang_weight, c->twd_norm, c->wpos,
+                       c->wslot);
     prof_end(c);
     hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(64), 0, c->stream, c->lim_count);
     prof_begin(c, DNDSMI_K_WBAP_CELL);
     if (c->nfmax == 3)
-        hipLaunchKernelGGL(k_wbap_ind<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->nbr,
-                           c->sface, c->f2c, c->wsum, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count,
-                           c->lim_list, full_copy ? nullptr : c->limf);
+        hipLaunchKernelGGL(k_wbap_ind<3>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr,
+                           c->wslot, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count, c->lim_list,
+                           full_copy ? nullptr : c->limf);
     else
-        hipLaunchKernelGGL(k_wbap_ind<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->ldf, c->nbr,
-                           c->sface, c->f2c, c->wsum, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count,
-                           c->lim_list, full_copy ? nullptr : c->limf);
+        hipLaunchKernelGGL(k_wbap_ind<4>, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->ldc, c->nbr,
+                           c->wslot, x, xo, ind ? ind : c->ind, c->wbap_scale, if_all, c->lim_count, c->lim_list,
+                           full_copy ? nullptr : c->limf);
     prof_end(c);
     prof_begin(c, DNDSMI_K_WBAP_LIM);
     if (procedure == 1) {
@@ -2366,12 +2371,12 @@ extern "C" int64_t dndsmi_kernel_bytes(const dndsmi_ctx *c, int32_t kernel) {
                nc * (36 * 8 + 32 + 72);
     case DNDSMI_K_RHS_CELL: // 8 terms per slot, volume, x read+write, xLast (+ last-slot index for 4 slots)
         return c2f * 64 + nc * (8 + 32 * 3 + (c->nfmax == 4 ? 4 : 0));
-    case DNDSMI_K_WBAP_FACE: // point-0 coordinates of both sides, f2c, weights, normal, 8 terms out;
-        // x (rho, E), u (rho, E), moments and scales once per cell
-        return nif * (32 + 8 + 80 + 16 + 64) + nc * (18 * 8 + 16 + 72 + 16);
-    case DNDSMI_K_WBAP_CELL: // terms per slot, indicator out; x read and copied out for the unlimited
-        // cells of a full copy (an upper bound: as if no cell were limited), else a flag per cell
-        return c2f * (32 + 12) + nc * (8 + (c->wbap_full_copy ? 36 * 8 * 2 : 1));
+    case DNDSMI_K_WBAP_FACE: // point-0 coordinates of both sides, f2c, slot positions, weights, normal, 8 terms
+        // out; x (rho, E), u (rho, E), moments and scales once per cell
+        return nif * (32 + 8 + 1 + 80 + 16 + 64) + nc * (18 * 8 + 16 + 72 + 16);
+    case DNDSMI_K_WBAP_CELL: // terms and neighbour index per slot, indicator out; x read and copied out for the
+        // unlimited cells of a full copy (an upper bound: as if no cell were limited), else a flag per cell
+        return c2f * (32 + 4) + nc * (8 + (c->wbap_full_copy ? 36 * 8 * 2 : 1));
     case DNDSMI_K_WBAP_LIM: // depends on the number of limited cells; not a bandwidth kernel
         return 0;
     case DNDSMI_K_RHS_BFACE: // per boundary face: index, f2c, slot position, kind, one side's point

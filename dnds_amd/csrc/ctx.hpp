@@ -78,7 +78,8 @@ struct dndsmi_ctx {
     bool has_limiter = false;
     double *fw = nullptr;      // [10][ldf] face weights
     double *msec = nullptr;    // [162][ldf] matrixSecondary (R2L, L2R)
-    double *wsum = nullptr;    // [ldf][8] per-face indicator terms of both cells
+    double *wslot = nullptr;   // [nfmax][4][ldc] indicator terms of the cell's faces, cell2face slot order
+    uint8_t *wpos = nullptr;   // [ldf] the face's cell2face slot in cell L (low 4 bits) and in cell R (high)
     double *uF = nullptr;      // [nfmax*28][ldc] WBAP per-cell face scratch
     double *ind = nullptr;     // [ldc] indicator when the caller passes none
     double *xlim = nullptr;    // [36][ldc] limited reconstruction inside SSP-RK3 steps (limited cells only)
