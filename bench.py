@@ -420,7 +420,7 @@ def main():
 
         def make_oracle(variant):
             return oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(),
-                                 riemann_solver=2 if case.riemann == "HLLC" else 1, far_field=case.far,
+                                 riemann_solver={"Roe": 1, "HLLC": 2, "HLLEP": 3}[case.riemann], far_field=case.far,
                                  tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
                                  wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
                                  limiter_procedure=args.limiter_procedure, mu_gas=gas["muGas"], pr_gas=gas["prGas"],

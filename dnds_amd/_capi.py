@@ -43,6 +43,7 @@ MESH_MIXED = 2
 
 RS_ROE = 1
 RS_HLLC = 2
+RS_HLLEP = 3
 
 ARRAY_DOF = 0
 ARRAY_REC = 1

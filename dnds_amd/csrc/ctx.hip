@@ -1367,7 +1367,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     if (!vr->volume || !vr->face_area || !vr->face_center || !vr->face_norm_center || !vr->face_norm ||
         !vr->face_weight || !vr->ainv || !vr->ainv_b || !vr->ainv_bvec)
         return DNDSMI_ERR_INVALID_ARGUMENT; // matrix_secondary alone may be absent (no WBAP then)
-    if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC)
+    if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC &&
+        es->riemann_solver != DNDSMI_RS_HLLEP)
         return DNDSMI_ERR_UNSUPPORTED;
     // local coordinates: the kernels evaluate the bases' values only
     if (vr->cell_frame && (es->mu_gas > 0 || vr->matrix_secondary))

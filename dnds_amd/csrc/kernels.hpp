@@ -159,6 +159,66 @@ __device__ inline bool roe_flux(const double UL[4], const double UR[4], double g
     return true;
 }
 
+// HLLEP flux in the reference's V1 form (DNDS_Gas.hpp:178-310), face frame:
+// Roe's wave strengths and right eigenvectors with the eigenvalues replaced by
+// the HLL-type speeds eV = ((S+ + S-) lam - 2 (1 - delta) S+ S-) / (S+ - S-),
+// delta = 0 on the acoustic waves and a / (a + a + verySmallReal) on the
+// contact and shear waves.  Comparisons are written as std::min / std::max
+// evaluate them, so signed zeros come out as in the reference.
+__device__ inline bool hllep_flux(const double UL[4], const double UR[4], double gamma, double F[4]) {
+    if (!(UL[0] > 0 && UR[0] > 0))
+        return false;
+    const double vL0 = UL[1] / UL[0], vL1 = UL[2] / UL[0];
+    const double vR0 = UR[1] / UR[0], vR1 = UR[2] / UR[0];
+    double asqrL, asqrR, pL, pR, HL, HR;
+    ideal_gas_thermal(UL[3], UL[0], vL0 * vL0 + vL1 * vL1, gamma, pL, asqrL, HL);
+    ideal_gas_thermal(UR[3], UR[0], vR0 * vR0 + vR1 * vR1, gamma, pR, asqrR, HR);
+    const double sL = sqrt(UL[0]), sR = sqrt(UR[0]);
+    const double vRoe0 = (sL * vL0 + sR * vR0) / (sL + sR);
+    const double vRoe1 = (sL * vL1 + sR * vR1) / (sL + sR);
+    const double vsqrRoe = vRoe0 * vRoe0 + vRoe1 * vRoe1;
+    const double HRoe = (sL * HL + sR * HR) / (sL + sR);
+    const double asqrRoe = (gamma - 1) * (HRoe - 0.5 * vsqrRoe);
+    if (!(asqrRoe > 0 && asqrL > 0 && asqrR > 0))
+        return false;
+    const double aRoe = sqrt(asqrRoe);
+    const double lam0 = vRoe0 - aRoe, lam4 = vRoe0 + aRoe;
+    double inc[4];
+    for (int k = 0; k < 4; k++)
+        inc[k] = UR[k] - UL[k];
+    const double a2 = inc[2] - vRoe1 * inc[0];
+    const double inc4b = inc[3] - a2 * vRoe1;
+    const double a1 = (gamma - 1) / asqrRoe * (inc[0] * (HRoe - vRoe0 * vRoe0) + vRoe0 * inc[1] - inc4b);
+    const double a0 = (inc[0] * lam4 - inc[1] - aRoe * a1) / (2 * aRoe);
+    const double a3 = inc[0] - (a0 + a1);
+    const double cL = vL0 - sqrt(asqrL), cR = vR0 + sqrt(asqrR);
+    const double SL = cL < lam0 ? cL : lam0;
+    const double SR = lam4 < cR ? cR : lam4;
+    double FL[4], FR[4];
+    inviscid_flux(UL, vL0, pL, FL);
+    inviscid_flux(UR, vR0, pR, FR);
+    const double Sp = SR < 0.0 ? 0.0 : SR, Sn = 0.0 < SL ? 0.0 : SL;
+    const double delta1 = aRoe / (aRoe + aRoe + kVerySmallReal);
+    const double eV1 = ((Sp + Sn) * vRoe0 - 2.0 * (1.0 - delta1) * (Sp * Sn)) / (Sp - Sn);
+    const double eV2 = ((Sp + Sn) * (vRoe0 + aRoe) - 2.0 * (1.0 - 0.0) * (Sp * Sn)) / (Sp - Sn);
+    const double eV3 = ((Sp + Sn) * (vRoe0 - aRoe) - 2.0 * (1.0 - 0.0) * (Sp * Sn)) / (Sp - Sn);
+    const double la0 = fabs(eV3) * a0, la1 = fabs(eV1) * a1, la2 = fabs(eV1) * a2, la3 = fabs(eV2) * a3;
+    double r;
+    r = 0.0;
+    r += 1.0 * la0, r += 1.0 * la1, r += 0.0 * la2, r += 1.0 * la3;
+    F[0] = (FL[0] + FR[0]) * 0.5 - 0.5 * r;
+    r = 0.0;
+    r += (vRoe0 - aRoe) * la0, r += vRoe0 * la1, r += 0.0 * la2, r += (vRoe0 + aRoe) * la3;
+    F[1] = (FL[1] + FR[1]) * 0.5 - 0.5 * r;
+    r = 0.0;
+    r += vRoe1 * la0, r += vRoe1 * la1, r += 1.0 * la2, r += vRoe1 * la3;
+    F[2] = (FL[2] + FR[2]) * 0.5 - 0.5 * r;
+    r = 0.0;
+    r += (HRoe - vRoe0 * aRoe) * la0, r += (0.5 * vsqrRoe) * la1, r += vRoe1 * la2, r += (HRoe + vRoe0 * aRoe) * la3;
+    F[3] = (FL[3] + FR[3]) * 0.5 - 0.5 * r;
+    return true;
+}
+
 // HLLC flux with Einfeldt speeds, face frame (DNDS_Gas.hpp:312-403)
 __device__ inline bool hllc_flux(const double UL[4], const double UR[4], double gamma, double F[4]) {
     if (!(UL[0] > 0 && UR[0] > 0))
@@ -401,7 +461,9 @@ __device__ inline bool riemann_face(const GasParams &gp, const double ULxy[4], c
     UL[0] = ULxy[0], UL[1] = n0 * ULxy[1] + n1 * ULxy[2], UL[2] = (-n1) * ULxy[1] + n0 * ULxy[2], UL[3] = ULxy[3];
     UR[0] = URxy[0], UR[1] = n0 * URxy[1] + n1 * URxy[2], UR[2] = (-n1) * URxy[1] + n0 * URxy[2], UR[3] = URxy[3];
     double f[4];
-    const bool ok = gp.rs == 2 ? hllc_flux(UL, UR, gp.gamma, f) : roe_flux(UL, UR, gp.gamma, dLambda, f);
+    const bool ok = gp.rs == 2   ? hllc_flux(UL, UR, gp.gamma, f)
+                    : gp.rs == 3 ? hllep_flux(UL, UR, gp.gamma, f)
+                                 : roe_flux(UL, UR, gp.gamma, dLambda, f);
     if (!ok)
         return false;
     const double a = n0 * f[1] + (-n1) * f[2];

@@ -420,7 +420,7 @@ class ArrayScalar(_DeviceArray):
     WIDTH = 1
 
 
-RS = {"Roe": A.RS_ROE, "HLLC": A.RS_HLLC}
+RS = {"Roe": A.RS_ROE, "HLLC": A.RS_HLLC, "HLLEP": A.RS_HLLEP}
 
 
 def solver_output(u, ifUseLimiter=None, rhs=None, gamma: float = 1.4, Rgas: float = 1.0,

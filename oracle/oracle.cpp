@@ -269,6 +269,78 @@ int hllc_flux(const double UL[4], const double UR[4], double gamma, double dLamb
     return ERR_OK;
 }
 
+// Gas::HLLEPFlux_IdealGas<2> in its V1 form (DNDS_GAS_HLLEP_USE_V1 is
+// defined at DNDS_Gas.hpp:178), DNDS_Gas.hpp:179-310: Roe's wave strengths
+// and right eigenvectors, each eigenvalue replaced by
+//   eV = ((S+ + S-) lam - 2 (1 - delta) S+ S-) / (S+ - S-),
+// S+ = max(SR, 0), S- = min(SL, 0), SL = min(u - a, uL - aL),
+// SR = max(u + a, uR + aR) (Roe averages u, a); delta = 0 for the acoustic
+// waves and delta = a / (a + a + verySmallReal) for the contact and shear
+// waves (the reference sets the |u| of that coefficient to a, :291-292).
+// std::min / std::max are written out as the ternaries they are.
+int hllep_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4]) {
+    (void)dLambda; // the Harten-Yee lines are commented out in the reference (:224-231)
+    if (!(UL[0] > 0 && UR[0] > 0))
+        return ERR_NONPOSITIVE;
+    double veloL[2] = {UL[1] / UL[0], UL[2] / UL[0]};
+    double veloR[2] = {UR[1] / UR[0], UR[2] / UR[0]};
+    double asqrL, asqrR, pL, pR, HL, HR;
+    double vsqrL = veloL[0] * veloL[0] + veloL[1] * veloL[1];
+    double vsqrR = veloR[0] * veloR[0] + veloR[1] * veloR[1];
+    ideal_gas_thermal(UL[3], UL[0], vsqrL, gamma, pL, asqrL, HL);
+    ideal_gas_thermal(UR[3], UR[0], vsqrR, gamma, pR, asqrR, HR);
+    double sqrtRhoL = std::sqrt(UL[0]);
+    double sqrtRhoR = std::sqrt(UR[0]);
+    double veloRoe[2];
+    for (int k = 0; k < 2; k++)
+        veloRoe[k] = (sqrtRhoL * veloL[k] + sqrtRhoR * veloR[k]) / (sqrtRhoL + sqrtRhoR);
+    double vsqrRoe = veloRoe[0] * veloRoe[0] + veloRoe[1] * veloRoe[1];
+    double HRoe = (sqrtRhoL * HL + sqrtRhoR * HR) / (sqrtRhoL + sqrtRhoR);
+    double asqrRoe = (gamma - 1) * (HRoe - 0.5 * vsqrRoe);
+    if (!(asqrRoe > 0 && asqrL > 0 && asqrR > 0))
+        return ERR_NONPOSITIVE;
+    double aRoe = std::sqrt(asqrRoe);
+    double lam0 = veloRoe[0] - aRoe;
+    double lam4 = veloRoe[0] + aRoe;
+    double R[4][4];
+    right_eigen(veloRoe, vsqrRoe, HRoe, aRoe, R);
+    double incU[4];
+    for (int k = 0; k < 4; k++)
+        incU[k] = UR[k] - UL[k];
+    double alpha[4];
+    alpha[2] = incU[2] - veloRoe[1] * incU[0];
+    double incU4b = incU[3] - alpha[2] * veloRoe[1];
+    alpha[1] = (gamma - 1) / asqrRoe * (incU[0] * (HRoe - veloRoe[0] * veloRoe[0]) + veloRoe[0] * incU[1] - incU4b);
+    alpha[0] = (incU[0] * lam4 - incU[1] - aRoe * alpha[1]) / (2 * aRoe);
+    alpha[3] = incU[0] - (alpha[0] + alpha[1]);
+    const double aL = std::sqrt(asqrL), aR = std::sqrt(asqrR);
+    const double cL = veloL[0] - aL, cR = veloR[0] + aR;
+    double SL = cL < lam0 ? cL : lam0;     // std::min(lam0, uL - aL)
+    double SR = lam4 < cR ? cR : lam4;     // std::max(lam4, uR + aR)
+    double FL[4], FR[4];
+    inviscid_flux(UL, veloL[0], pL, FL);
+    inviscid_flux(UR, veloR[0], pR, FR);
+    double Sp = SR < 0.0 ? 0.0 : SR;       // std::max(SR, 0.0)
+    double Sn = 0.0 < SL ? 0.0 : SL;       // std::min(SL, 0.0)
+    double delta1 = aRoe / (aRoe + aRoe + kVerySmallReal);
+    double delta2 = 0.0, delta3 = 0.0;
+    double un = veloRoe[0];
+    double eV1 = ((Sp + Sn) * un - 2.0 * (1.0 - delta1) * (Sp * Sn)) / (Sp - Sn);
+    double eV2 = ((Sp + Sn) * (un + aRoe) - 2.0 * (1.0 - delta2) * (Sp * Sn)) / (Sp - Sn);
+    double eV3 = ((Sp + Sn) * (un - aRoe) - 2.0 * (1.0 - delta3) * (Sp * Sn)) / (Sp - Sn);
+    double lam[4] = {std::abs(eV3), std::abs(eV1), std::abs(eV1), std::abs(eV2)};
+    double la[4];
+    for (int k = 0; k < 4; k++)
+        la[k] = lam[k] * alpha[k];
+    for (int r = 0; r < 4; r++) {
+        double incF = 0;
+        for (int c = 0; c < 4; c++)
+            incF += R[r][c] * la[c];
+        F[r] = (FL[r] + FR[r]) * 0.5 - 0.5 * incF;
+    }
+    return ERR_OK;
+}
+
 // generateBoundaryValue, DNDS_FV_EulerEvaluator.hpp:958-1275 (dim = 2)
 int boundary_value(const oracle_setting &cfg, const double UL[4], const double n[2], const double p[2], double t,
                    int32_t bc, double UR[4]) {
@@ -478,8 +550,10 @@ int flux_face(const oracle_setting &cfg, const double ULxy[4], const double URxy
         rot(FRfix);
     }
     double finc[4];
-    int rc = cfg.riemann_solver == 2 ? hllc_flux(UL, UR, gamma, dLambda, finc)
-                                     : (cfg.riemann_solver == 1 ? roe_flux(UL, UR, gamma, dLambda, finc) : ERR_UNSUPPORTED);
+    int rc = cfg.riemann_solver == 2   ? hllc_flux(UL, UR, gamma, dLambda, finc)
+             : cfg.riemann_solver == 1 ? roe_flux(UL, UR, gamma, dLambda, finc)
+             : cfg.riemann_solver == 3 ? hllep_flux(UL, UR, gamma, dLambda, finc)
+                                       : ERR_UNSUPPORTED;
     if (rc)
         return rc;
     rot(finc);
@@ -498,6 +572,9 @@ extern "C" int oracle_roe_flux(const double *UL, const double *UR, double gamma,
 }
 extern "C" int oracle_hllc_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F) {
     return hllc_flux(UL, UR, gamma, dlambda, F);
+}
+extern "C" int oracle_hllep_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F) {
+    return hllep_flux(UL, UR, gamma, dlambda, F);
 }
 extern "C" int oracle_eigenvectors(const double *U, double gamma, double *L, double *R) {
     if (!(U[0] > 0))

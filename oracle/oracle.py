@@ -98,6 +98,7 @@ def lib(variant: str = "parity") -> C.CDLL:
             "oracle_wbap_v2": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, PD, C.c_int32],
             "oracle_roe_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_hllc_flux": [PD, PD, C.c_double, C.c_double, PD],
+            "oracle_hllep_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_eigenvectors": [PD, C.c_double, PD, PD],
             "oracle_boundary_value": [C.POINTER(OSetting), PD, PD, PD, C.c_double, C.c_int32, C.c_int32, PD],
             "oracle_flux_jacobian_times_du": [PD, PD, PD, C.c_double, C.c_double, PD],
@@ -337,6 +338,15 @@ def hllc_flux(UL, UR, gamma=1.4, dlambda=0.0):
                                 dlambda, _d(F))
     if rc:
         raise RuntimeError(f"hllc flux failed {rc}")
+    return F
+
+
+def hllep_flux(UL, UR, gamma=1.4, dlambda=0.0):
+    F = np.empty(4)
+    rc = lib().oracle_hllep_flux(_d(np.ascontiguousarray(UL, float)), _d(np.ascontiguousarray(UR, float)), gamma,
+                                dlambda, _d(F))
+    if rc:
+        raise RuntimeError(f"hllep flux failed {rc}")
     return F
 
 

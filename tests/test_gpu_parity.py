@@ -31,6 +31,9 @@ BC_SETS = {
 DOMAIN = {"ivfar": (10.0, 10.0), "dmrfar": (4.0, 1.0)}
 
 
+RS_CODE = {"Roe": 1, "HLLC": 2, "HLLEP": 3}
+
+
 def build(kind, bcname, riemann="Roe", nx=9, ny=7, jitter=0.2, seed=5):
     import oracle
     from dnds_amd.host import EulerEvaluator
@@ -41,7 +44,7 @@ def build(kind, bcname, riemann="Roe", nx=9, ny=7, jitter=0.2, seed=5):
     far = (1.0, 0.3, -0.2, 2.6)
     ev = EulerEvaluator(mesh, vfv, riemannSolverType=riemann, gamma=1.4, farFieldStaticValue=far)
     st = vfv.setting
-    orc = oracle.Oracle(mesh.arrays(), vfv.arrays(), riemann_solver=1 if riemann == "Roe" else 2, gamma=1.4,
+    orc = oracle.Oracle(mesh.arrays(), vfv.arrays(), riemann_solver=RS_CODE[riemann], gamma=1.4,
                         far_field=far, jacobi_relax=st.jacobi_relax, tang_weight=st.tang_weight,
                         tang_weight_direction=st.tang_weight_direction,
                         wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std)
@@ -91,7 +94,7 @@ def test_evaluate_dt_matches_oracle(kind, bcname, local):
 
 
 @pytest.mark.parametrize("kind,bcname", CASES)
-@pytest.mark.parametrize("riemann", ["Roe", "HLLC"])
+@pytest.mark.parametrize("riemann", ["Roe", "HLLC", "HLLEP"])
 def test_evaluate_rhs_matches_oracle(kind, bcname, riemann):
     mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
     u, urec = random_state(mesh.n_cell, 13)
@@ -105,7 +108,8 @@ def test_evaluate_rhs_matches_oracle(kind, bcname, riemann):
 
 @pytest.mark.parametrize("kind,bcname,riemann", [(0, "periodic", "Roe"), (2, "farfield", "HLLC"),
                                                  (1, "walls", "Roe"), (0, "riemann", "HLLC"),
-                                                 (2, "ivfar", "Roe"), (0, "rtfar", "HLLC")])
+                                                 (2, "ivfar", "Roe"), (0, "rtfar", "HLLC"),
+                                                 (0, "riemann", "HLLEP"), (1, "walls", "HLLEP")])
 def test_ssprk3_steps_match_oracle(kind, bcname, riemann):
     from dnds_amd.host import ExplicitSSPRK3LocalDt
     mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
@@ -122,7 +126,7 @@ def test_ssprk3_steps_match_oracle(kind, bcname, riemann):
 
 
 @pytest.mark.parametrize("kind", [0, 1, 2])
-@pytest.mark.parametrize("riemann", ["Roe", "HLLC"])
+@pytest.mark.parametrize("riemann", ["Roe", "HLLC", "HLLEP"])
 def test_positivity_fallback_matches_oracle(kind, riemann):
     """CompressRecPart's fallback to the cell mean (DNDS_FV_EulerEvaluator.hpp:
     1322-1351) forced at every face Gauss point of one cell (huge momentum
