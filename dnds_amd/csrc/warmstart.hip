@@ -13,10 +13,12 @@
 // cells met by a breadth-first walk over faces (cell2face order); missing
 // cells repeat the cell itself.
 //
-// Training and inference use hand-written kernels: one GEMM with BF16
-// operands on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
-// accumulation) and fused epilogues (bias, GELU, GELU backward), column
-// sums for the bias gradients, the MSE loss and Adam, all FP32 otherwise.
+// Training and inference use hand-written kernels on gfx950's matrix cores
+// (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation): a GEMM with
+// fused epilogues (bias, GELU, GELU backward) for training, column sums for
+// the bias gradients, the MSE loss and Adam, all FP32 otherwise; and one
+// fused kernel for inference that keeps the three layers' BF16 weights in
+// LDS and the hidden activations on chip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +40,8 @@ constexpr int kIn = 58, kHid = 128, kOut = 36, kSten = 12;
 constexpr int64_t oW1 = 0, ob1 = oW1 + kIn * kHid, oW2 = ob1 + kHid, ob2 = oW2 + kHid * kHid, oW3 = ob2 + kHid,
                   ob3 = oW3 + kHid * kOut, kNParam = ob3 + kOut;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (finite inputs)
@@ -58,9 +61,11 @@ enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2, EPI_GELU_GRAD = 3 };
 // C[M][N] = op(A) op(B) with BF16 operands and FP32 accumulation.
 // A(m, k) = transA ? A[k lda + m] : A[m lda + k]; B(k, n) likewise.
 // 256 threads; a 64 x 64 tile per workgroup, a 32 x 32 quarter per wave as
-// 2 x 2 MFMA 16x16x16 blocks; K in steps of 16 staged through LDS.  Operand
-// layout of v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + 0..3]
-// and B[4 (l / 16) + 0..3][l % 16]; it receives D[4 (l / 16) + 0..3][l % 16].
+// 2 x 2 MFMA 16x16x32 blocks; K in steps of 32 staged through LDS as BF16.
+// Operand layout of v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][k] and
+// B[k][l % 16] for the eight k = 8 (l / 16) + 0..7 (a product is the same
+// under any assignment of k that A and B share); it receives
+// D[4 (l / 16) + 0..3][l % 16].
 //
 // Split K (EPI_STORE only): block z sums k in [z kchunk, (z + 1) kchunk) and
 // writes its partial product to C + z M ldcm; k_splitk_sum adds the partials
@@ -71,8 +76,10 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
                                               int32_t transB, float *__restrict__ C, int64_t ldcm,
                                               const float *__restrict__ bias, float *__restrict__ C2,
                                               const float *__restrict__ Zin, int32_t kchunk) {
-    __shared__ unsigned short As[64][16 + 4];
-    __shared__ unsigned short Bs[16][64 + 4];
+    // As: rows m, k contiguous; Bs: columns n, k contiguous (so that a lane's
+    // eight k are one 16-byte LDS read); rows padded by 8 against bank conflicts
+    __shared__ __attribute__((aligned(16))) unsigned short As[64][32 + 8];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[64][32 + 8];
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int64_t m0 = int64_t(blockIdx.x) * 64, n0 = int64_t(blockIdx.y) * 64;
     const int wm = 32 * (wave / 2), wn = 32 * (wave % 2);
@@ -85,12 +92,13 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
 #pragma unroll
         for (int b = 0; b < 2; b++)
             acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int32_t k0 = kbeg; k0 < kend; k0 += 16) {
+    for (int32_t k0 = kbeg; k0 < kend; k0 += 32) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = tid + 256 * q;
+        for (int q = 0; q < 8; q++) {
+            const int e = tid + 256 * q; // 0 .. 2047
             {
-                const int r = e / 16, kk = e % 16;
+                // transA reads A[k lda + m]: consecutive threads take consecutive m
+                const int r = transA ? e % 64 : e / 32, kk = transA ? e / 64 : e % 32;
                 const int64_t m = m0 + r, k = k0 + kk;
                 float v = 0.f;
                 if (m < M && k < kend)
@@ -98,35 +106,28 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
                 As[r][kk] = to_bf16(v);
             }
             {
-                const int kk = e / 64, c = e % 64;
+                const int c = transB ? e / 32 : e % 64, kk = transB ? e % 32 : e / 64;
                 const int64_t n = n0 + c, k = k0 + kk;
                 float v = 0.f;
                 if (n < N && k < kend)
                     v = transB ? B[n * ldb + k] : B[k * ldb + n];
-                Bs[kk][c] = to_bf16(v);
+                Bs[c][kk] = to_bf16(v);
             }
         }
         __syncthreads();
-        bf16x4 af[2], bfr[2];
+        bf16x8 af[2], bfr[2];
+        const int kb = 8 * (lane / 16);
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-            const int row = wm + 16 * a + lane % 16, kb = 4 * (lane / 16);
+        for (int a = 0; a < 2; a++)
+            af[a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&As[wm + 16 * a + lane % 16][kb]));
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                af[a][j] = short(As[row][kb + j]);
-        }
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int col = wn + 16 * b + lane % 16, kb = 4 * (lane / 16);
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                bfr[b][j] = short(Bs[kb + j][col]);
-        }
+        for (int b = 0; b < 2; b++)
+            bfr[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&Bs[wn + 16 * b + lane % 16][kb]));
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
         __syncthreads();
     }
 #pragma unroll
@@ -153,6 +154,148 @@ __global__ __launch_bounds__(256) void k_gemm(int32_t M, int32_t N, int32_t K, c
                     C[o] = v * gelu_grad(Zin[o]);
                 }
             }
+}
+
+// Inference of the whole network, Y = W3^T g(W2^T g(W1^T x + b1) + b2) + b3
+// per row (g = GELU), in one kernel.  Each workgroup (8 waves) stages the
+// three weight matrices once, transposed to [n][k] and rounded to BF16, in
+// LDS (about 66 KB), then walks its rows 16 at a time per wave: layer 1 reads
+// the row's features from memory, every hidden layer's output (bias and GELU
+// applied in FP32, then rounded to BF16, as the operand of the next GEMM
+// would be) goes through the wave's own 16 x 128 LDS tile, and only the 36
+// outputs reach memory.  The arithmetic is that of the three k_gemm launches
+// (BF16 operands, FP32 accumulation, FP32 epilogues) without the hidden
+// activations' round trips through HBM.
+constexpr int kInPad = 64, kOutPad = 48, kLdW1 = kInPad + 8, kLdH = kHid + 8, kInferWaves = 16;
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(int32_t rows, const float *__restrict__ X,
+                                                                  const float *__restrict__ P,
+                                                                  float *__restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) unsigned short W1t[kHid][kLdW1];
+    __shared__ __attribute__((aligned(16))) unsigned short W2t[kHid][kLdH];
+    __shared__ __attribute__((aligned(16))) unsigned short W3t[kOutPad][kLdH];
+    __shared__ float bs[2 * kHid + kOutPad];
+    __shared__ __attribute__((aligned(16))) unsigned short H[kInferWaves][16][kLdH];
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    for (int e = tid; e < kHid * kInPad; e += 64 * kInferWaves) {
+        const int n = e % kHid, k = e / kHid; // consecutive threads: consecutive n of one row of W1
+        W1t[n][k] = to_bf16(k < kIn ? P[oW1 + k * kHid + n] : 0.f);
+    }
+    for (int e = tid; e < kHid * kHid; e += 64 * kInferWaves) {
+        const int n = e % kHid, k = e / kHid;
+        W2t[n][k] = to_bf16(P[oW2 + k * kHid + n]);
+    }
+    for (int e = tid; e < kOutPad * kHid; e += 64 * kInferWaves) {
+        const int n = e % kOutPad, k = e / kOutPad;
+        W3t[n][k] = to_bf16(n < kOut ? P[oW3 + k * kOut + n] : 0.f);
+    }
+    for (int e = tid; e < 2 * kHid + kOutPad; e += 64 * kInferWaves)
+        bs[e] = e < kHid ? P[ob1 + e] : (e < 2 * kHid ? P[ob2 + e - kHid] : (e - 2 * kHid < kOut ? P[ob3 + e - 2 * kHid] : 0.f));
+    __syncthreads();
+    const int kb = 8 * (lane / 16), lr = lane % 16;
+    const int64_t ntile = (int64_t(rows) + 15) / 16;
+    // the loop bound depends on the workgroup only, so every wave reaches
+    // every __syncthreads (a wave past the last tile computes on zeros and
+    // stores nothing)
+    float xc[kInPad / 32][8];
+    {
+        const int64_t row = (int64_t(blockIdx.x) * kInferWaves + wave) * 16 + lr;
+#pragma unroll
+        for (int s2 = 0; s2 < kInPad / 32; s2++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = 32 * s2 + kb + e;
+                xc[s2][e] = row < rows && k < kIn ? X[row * kIn + k] : 0.f;
+            }
+    }
+    for (int64_t t0 = int64_t(blockIdx.x) * kInferWaves; t0 < ntile; t0 += int64_t(gridDim.x) * kInferWaves) {
+        const int64_t r0 = (t0 + wave) * 16;
+        // layer 1: A operand straight from the features (58 per row, zero
+        // padded); the next tile's features are loaded while this one runs
+        floatx4 acc[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            acc[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        float xn[kInPad / 32][8];
+        {
+            const int64_t nrow = r0 + int64_t(gridDim.x) * kInferWaves * 16 + lr;
+#pragma unroll
+            for (int s2 = 0; s2 < kInPad / 32; s2++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int k = 32 * s2 + kb + e;
+                    xn[s2][e] = nrow < rows && k < kIn ? X[nrow * kIn + k] : 0.f;
+                }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < kInPad / 32; s2++) {
+            shortx8 av;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                av[e] = short(to_bf16(xc[s2][e]));
+            const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bf16x8 w = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&W1t[16 * b + lr][32 * s2 + kb]));
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[b], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                H[wave][4 * (lane / 16) + j][16 * b + lr] = to_bf16(gelu(acc[b][j] + bs[16 * b + lr]));
+        __syncthreads();
+        // layer 2
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            acc[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s2 = 0; s2 < kHid / 32; s2++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&H[wave][lr][32 * s2 + kb]));
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bf16x8 w = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&W2t[16 * b + lr][32 * s2 + kb]));
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[b], 0, 0, 0);
+            }
+        }
+        __syncthreads(); // every wave has read its layer-1 tile
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                H[wave][4 * (lane / 16) + j][16 * b + lr] = to_bf16(gelu(acc[b][j] + bs[kHid + 16 * b + lr]));
+        __syncthreads();
+        // layer 3
+        floatx4 out[3];
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            out[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s2 = 0; s2 < kHid / 32; s2++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&H[wave][lr][32 * s2 + kb]));
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const bf16x8 w = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8 *>(&W3t[16 * b + lr][32 * s2 + kb]));
+                out[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, out[b], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t m = r0 + 4 * (lane / 16) + j;
+                const int n = 16 * b + lr;
+                if (m < rows && n < kOut)
+                    Y[m * kOut + n] = out[b][j] + bs[2 * kHid + n];
+            }
+        __syncthreads(); // the tile is free for the next rows
+#pragma unroll
+        for (int s2 = 0; s2 < kInPad / 32; s2++)
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                xc[s2][e] = xn[s2][e];
+    }
 }
 
 // C[i] = sum over z of part[z][i], in order
@@ -217,14 +360,11 @@ __global__ __launch_bounds__(256) void k_adam(int64_t n, float *__restrict__ P, 
     P[i] -= lr_t * m / (sqrtf(v) + eps_t);
 }
 
-// features and scales of every cell (see the file comment)
-__global__ __launch_bounds__(256) void k_features(int32_t nc, int64_t ldc, const int32_t *__restrict__ sten,
-                                                  const double *__restrict__ u, const float *__restrict__ mom,
-                                                  const float *__restrict__ logsz, float *__restrict__ X,
-                                                  float *__restrict__ S) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc)
-        return;
+// features and scales of cell i: the row goes to `row` (58 floats), the
+// scales to S
+__device__ inline void features_row(int32_t i, int64_t ldc, const int32_t *__restrict__ sten,
+                                    const double *__restrict__ u, const float *__restrict__ mom,
+                                    const float *__restrict__ logsz, float *row, float *__restrict__ S) {
     double ui[4], d[kSten][4];
 #pragma unroll
     for (int v = 0; v < 4; v++)
@@ -250,41 +390,79 @@ __global__ __launch_bounds__(256) void k_features(int32_t nc, int64_t ldc, const
         s[v] = fmax(s[v], 1e-3 * umax);
         s[v] = s[v] > 0 ? s[v] : 1.0;
     }
-    float *x = X + int64_t(i) * kIn;
 #pragma unroll
     for (int k = 0; k < kSten; k++)
 #pragma unroll
         for (int v = 0; v < 4; v++)
-            x[4 * k + v] = float(d[k][v] / s[v]);
+            row[4 * k + v] = float(d[k][v] / s[v]);
 #pragma unroll
     for (int k = 0; k < 9; k++)
-        x[48 + k] = mom[k * ldc + i];
-    x[57] = logsz[i];
+        row[48 + k] = mom[k * ldc + i];
+    row[57] = logsz[i];
 #pragma unroll
     for (int v = 0; v < 4; v++)
         S[4 * int64_t(i) + v] = float(s[v]);
 }
 
-// targets: converged rec coefficients x [k][ldc] (k = r + 9 v) divided by s_v
-__global__ __launch_bounds__(256) void k_targets(int32_t nc, int64_t ldc, const double *__restrict__ x,
-                                                 const float *__restrict__ S, float *__restrict__ T) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc)
-        return;
-#pragma unroll
-    for (int k = 0; k < kOut; k++)
-        T[int64_t(i) * kOut + k] = float(x[k * ldc + i] / double(S[4 * int64_t(i) + k / 9]));
+// features and scales of every cell (see the file comment).  The block's
+// rows are formed in LDS and written out as one contiguous run: a row is 58
+// floats, so stores of whole rows by single threads would not coalesce.
+__global__ __launch_bounds__(256) void k_features(int32_t nc, int64_t ldc, const int32_t *__restrict__ sten,
+                                                  const double *__restrict__ u, const float *__restrict__ mom,
+                                                  const float *__restrict__ logsz, float *__restrict__ X,
+                                                  float *__restrict__ S) {
+    __shared__ float tile[256][kIn + 1];
+    const int64_t i0 = int64_t(blockIdx.x) * 256;
+    const int32_t i = int32_t(i0) + int32_t(threadIdx.x);
+    if (i < nc)
+        features_row(i, ldc, sten, u, mom, logsz, tile[threadIdx.x], S);
+    __syncthreads();
+    const int64_t nrow = min(int64_t(256), int64_t(nc) - i0);
+    for (int64_t e = threadIdx.x; e < nrow * kIn; e += 256)
+        X[i0 * kIn + e] = tile[e / kIn][e % kIn];
 }
 
-// prediction back to rec coefficients
+// targets: converged rec coefficients x [k][ldc] (k = r + 9 v) divided by
+// s_v, rows of 36 formed in LDS and written contiguously
+__global__ __launch_bounds__(256) void k_targets(int32_t nc, int64_t ldc, const double *__restrict__ x,
+                                                 const float *__restrict__ S, float *__restrict__ T) {
+    __shared__ float tile[256][kOut + 1];
+    const int64_t i0 = int64_t(blockIdx.x) * 256;
+    const int32_t i = int32_t(i0) + int32_t(threadIdx.x);
+    if (i < nc) {
+#pragma unroll
+        for (int k = 0; k < kOut; k++)
+            tile[threadIdx.x][k] = float(x[k * ldc + i] / double(S[4 * int64_t(i) + k / 9]));
+    }
+    __syncthreads();
+    const int64_t nrow = min(int64_t(256), int64_t(nc) - i0);
+    for (int64_t e = threadIdx.x; e < nrow * kOut; e += 256)
+        T[i0 * kOut + e] = tile[e / kOut][e % kOut];
+}
+
+// prediction back to rec coefficients, x = Y s_v (ADD: x += Y s_v, the stage
+// correction of the learned warm start inside the mixed-precision step).
+// The block's rows of Y are read contiguously into LDS first.
+template <bool ADD>
 __global__ __launch_bounds__(256) void k_unscale(int32_t nc, int64_t ldc, const float *__restrict__ Y,
                                                  const float *__restrict__ S, double *__restrict__ x) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float tile[256][kOut + 1];
+    const int64_t i0 = int64_t(blockIdx.x) * 256;
+    const int64_t nrow = min(int64_t(256), int64_t(nc) - i0);
+    for (int64_t e = threadIdx.x; e < nrow * kOut; e += 256)
+        tile[e / kOut][e % kOut] = Y[i0 * kOut + e];
+    __syncthreads();
+    const int32_t i = int32_t(i0) + int32_t(threadIdx.x);
     if (i >= nc)
         return;
 #pragma unroll
-    for (int k = 0; k < kOut; k++)
-        x[k * ldc + i] = double(Y[int64_t(i) * kOut + k]) * double(S[4 * int64_t(i) + k / 9]);
+    for (int k = 0; k < kOut; k++) {
+        const double v = double(tile[threadIdx.x][k]) * double(S[4 * int64_t(i) + k / 9]);
+        if (ADD)
+            x[k * ldc + i] += v;
+        else
+            x[k * ldc + i] = v;
+    }
 }
 
 // batch rows by index
@@ -295,18 +473,6 @@ __global__ __launch_bounds__(256) void k_gather_rows(int32_t rows, int32_t width
         return;
     const int64_t r = e / width, c = e % width;
     dst[e] = src[int64_t(idx[r]) * width + c];
-}
-
-// x += the network's output, unscaled (the stage correction of the learned
-// warm start inside the mixed-precision step)
-__global__ __launch_bounds__(256) void k_unscale_add(int32_t nc, int64_t ldc, const float *__restrict__ Y,
-                                                     const float *__restrict__ S, double *__restrict__ x) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc)
-        return;
-#pragma unroll
-    for (int k = 0; k < kOut; k++)
-        x[k * ldc + i] += double(Y[int64_t(i) * kOut + k]) * double(S[4 * int64_t(i) + k / 9]);
 }
 
 // du = u - u_prev, then u_prev = u (cell means, [cell][4])
@@ -470,6 +636,15 @@ void forward(dndsmi_warmstart *w, const float *X, int32_t rows) {
     gemm<EPI_BIAS_GELU>(s, rows, kHid, kIn, X, kIn, 0, P + oW1, kHid, 0, w->Z1, kHid, P + ob1, w->H1);
     gemm<EPI_BIAS_GELU>(s, rows, kHid, kHid, w->H1, kHid, 0, P + oW2, kHid, 0, w->Z2, kHid, P + ob2, w->H2);
     gemm<EPI_BIAS>(s, rows, kOut, kHid, w->H2, kHid, 0, P + oW3, kOut, 0, w->Y, kOut, P + ob3);
+}
+
+// the network's output for `rows` rows of X into Y (fused kernel, inference
+// only: the hidden activations are not kept)
+int infer(dndsmi_warmstart *w, const float *X, int32_t rows) {
+    const int64_t ntile = (int64_t(rows) + 15) / 16;
+    const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>((ntile + kInferWaves - 1) / kInferWaves, 512)));
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64 * kInferWaves), 0, w->ctx->stream, rows, X, w->P, w->Y);
+    return w->ctx->launch_check("k_mlp_infer");
 }
 
 // loss and gradients into G for rows of (X, T); returns the mean squared error
@@ -663,8 +838,8 @@ int ws_stage_begin(dndsmi_ctx *c, const double *u, double *x, bool use) {
         return DNDSMI_OK;
     DNDSMI_TRY(features(w, c->ws_du));
     DNDSMI_TRY(ensure_rows(w, w->nc));
-    forward(w, w->fX, int32_t(w->nc));
-    hipLaunchKernelGGL(k_unscale_add, dim3(grid_for(w->nc)), dim3(256), 0, c->stream, int32_t(w->nc), w->ldc, w->Y,
+    DNDSMI_TRY(infer(w, w->fX, int32_t(w->nc)));
+    hipLaunchKernelGGL(k_unscale<true>, dim3(grid_for(w->nc)), dim3(256), 0, c->stream, int32_t(w->nc), w->ldc, w->Y,
                        w->fS, x);
     DNDSMI_TRY(c->launch_check("warm-start stage correction"));
     c->ws_stats[0]++;
@@ -918,8 +1093,8 @@ extern "C" int dndsmi_warmstart_predict(dndsmi_warmstart *w, const dndsmi_array 
         return DNDSMI_ERR_INVALID_ARGUMENT;
     DNDSMI_TRY(features(w, u->d));
     DNDSMI_TRY(ensure_rows(w, w->nc));
-    forward(w, w->fX, int32_t(w->nc));
-    hipLaunchKernelGGL(k_unscale, dim3(grid_for(w->nc)), dim3(256), 0, w->ctx->stream, int32_t(w->nc), w->ldc, w->Y,
+    DNDSMI_TRY(infer(w, w->fX, int32_t(w->nc)));
+    hipLaunchKernelGGL(k_unscale<false>, dim3(grid_for(w->nc)), dim3(256), 0, w->ctx->stream, int32_t(w->nc), w->ldc, w->Y,
                        w->fS, uRec->d);
     return w->ctx->launch_check("warm-start predict");
 }

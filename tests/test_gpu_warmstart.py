@@ -51,6 +51,31 @@ def test_loss_and_gradient_match_restatement_with_bf16_operands():
     assert np.linalg.norm(g - g64) > np.linalg.norm(g - gr)
 
 
+@pytest.mark.parametrize("n", [10, 400])
+def test_fused_inference_matches_restatement_with_bf16_operands(n):
+    """The fused inference kernel (all three layers in one launch, weights in
+    LDS) behind predict(): the restatement's forward pass with BF16 operands,
+    unscaled per variable.  n = 10 leaves a partial 16-row tile; n = 400
+    (320 000 rows) makes the workgroups loop over more than one batch of
+    tiles."""
+    from oracle import mlp_np as M
+    c, ev, ws = _setup(n=n)
+    rng = np.random.default_rng(7)
+    p = (rng.standard_normal(M.N_PARAM) * 0.15).astype(np.float32)
+    ws.set_params(p)
+    u = ev.BuildMean().upload(c.u0 * (1 + 0.05 * rng.standard_normal(c.u0.shape)))
+    X, S = ws.features(u)
+    rec = c.vfv.BuildRec()
+    ws.predict(u, rec)
+    got = rec.download()
+    scale = np.repeat(S.astype(np.float64), 9, axis=1)  # column k = r + 9 v takes the scale of variable v
+    ref = M.forward(M.unpack(p.astype(np.float64)), X.astype(np.float64), round_bf16=True)[-1] * scale
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err <= 2e-3, err
+    exact = M.forward(M.unpack(p.astype(np.float64)), X.astype(np.float64))[-1] * scale
+    assert np.linalg.norm(got - exact) > np.linalg.norm(got - ref)
+
+
 def test_adam_matches_restatement():
     from oracle import mlp_np as M
     c, ev, ws = _setup()
