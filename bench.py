@@ -79,6 +79,10 @@ def rel_l2(a, b, vol):
     return (np.sqrt((w * (a - b) ** 2).sum(0) / (w * b ** 2).sum(0))).tolist()
 
 
+# the reference's RiemannSolverType codes, for the oracle
+RS_CODE = {"Roe": 1, "HLLC": 2, "HLLEP": 3, "Roe_M1": 11, "Roe_M2": 12, "Roe_M3": 13, "Roe_M4": 14, "Roe_M5": 15}
+
+
 def make_case(name, n, threads, rec_order=3):
     from dnds_amd import cases
     if name == "riemann":
@@ -420,7 +424,7 @@ def main():
 
         def make_oracle(variant):
             return oracle.Oracle(case.mesh.arrays(), case.vfv.arrays(),
-                                 riemann_solver={"Roe": 1, "HLLC": 2, "HLLEP": 3}[case.riemann], far_field=case.far,
+                                 riemann_solver=RS_CODE[case.riemann], far_field=case.far,
                                  tang_weight=st.tang_weight, tang_weight_direction=st.tang_weight_direction,
                                  wbap_smooth_indicator_scale=st.wbap_smooth_indicator_scale, wbap_n_std=st.wbap_n_std,
                                  limiter_procedure=args.limiter_procedure, mu_gas=gas["muGas"], pr_gas=gas["prGas"],

@@ -44,6 +44,7 @@ MESH_MIXED = 2
 RS_ROE = 1
 RS_HLLC = 2
 RS_HLLEP = 3
+RS_ROE_M = {1: 11, 2: 12, 3: 13, 4: 14, 5: 15}
 
 ARRAY_DOF = 0
 ARRAY_REC = 1

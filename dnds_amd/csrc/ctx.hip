@@ -1373,7 +1373,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         !vr->face_weight || !vr->ainv || !vr->ainv_b || !vr->ainv_bvec)
         return DNDSMI_ERR_INVALID_ARGUMENT; // matrix_secondary alone may be absent (no WBAP then)
     if (es->riemann_solver != DNDSMI_RS_ROE && es->riemann_solver != DNDSMI_RS_HLLC &&
-        es->riemann_solver != DNDSMI_RS_HLLEP)
+        es->riemann_solver != DNDSMI_RS_HLLEP &&
+        !(es->riemann_solver >= DNDSMI_RS_ROE_M1 && es->riemann_solver <= DNDSMI_RS_ROE_M5))
         return DNDSMI_ERR_UNSUPPORTED;
     // local coordinates: the kernels evaluate the bases' values only
     if (vr->cell_frame && (es->mu_gas > 0 || vr->matrix_secondary))

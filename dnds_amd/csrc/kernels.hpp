@@ -64,7 +64,7 @@ __host__ __device__ __forceinline__ int64_t m_vec_index(int s, int e, int64_t i,
 struct GasParams {
     double gamma;
     double far[4];
-    int32_t rs; // 1 Roe, 2 HLLC
+    int32_t rs; // RiemannSolverType: 1 Roe, 2 HLLC, 3 HLLEP, 11-15 Roe_M1-M5
     // laminar viscosity (Sutherland) and conduction; mu = 0: inviscid
     double mu, pr, cp, tref, csuth;
 };
@@ -103,8 +103,14 @@ __device__ inline void inviscid_flux(const double U[4], double vx, double p, dou
     F[3] += vx * p;
 }
 
-// Roe flux with the Harten-Yee entropy fix, face frame (DNDS_Gas.hpp:405-603)
-__device__ inline bool roe_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4]) {
+// Roe flux, face frame (DNDS_Gas.hpp:405-603).  scheme 0: the Harten-Yee
+// entropy fix (riemannSolverType Roe); 1..5: the eigenvalue variants
+// Roe_M1..Roe_M5 (:461-518): 1 cLLF_M (per-side speeds, sound speeds capped at
+// |u| / 0.2), 2 the local Lax flux, 3 / 4 the low- / increased-dissipation
+// Roe_M of Fleischmann et al. (2020), 5 no fix.  std::min / std::max are
+// written as the comparisons they evaluate.
+__device__ inline bool roe_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4],
+                                int scheme = 0) {
     if (!(UL[0] > 0 && UR[0] > 0))
         return false;
     const double vL0 = UL[1] / UL[0], vL1 = UL[2] / UL[0];
@@ -125,14 +131,38 @@ __device__ inline bool roe_flux(const double UL[4], const double UR[4], double g
         return false;
     const double aRoe = sqrt(asqrRoe);
     double lam0 = fabs(vRoe0 - aRoe);
-    const double lam123 = fabs(vRoe0);
+    double lam123 = fabs(vRoe0);
     double lam4 = fabs(vRoe0 + aRoe);
-    const double thr = fmax(0.05 * (sqrt(vsqrRoe) + aRoe), dLambda);
-    const double thrS = thr * thr;
-    if (lam0 < thr)
-        lam0 = (sqr(lam0) + thrS) / (2 * thr);
-    if (lam4 < thr)
-        lam4 = (sqr(lam4) + thrS) / (2 * thr);
+    auto dmin = [](double a, double b) { return b < a ? b : a; };
+    auto dmax = [](double a, double b) { return a < b ? b : a; };
+    if (scheme == 0) {
+        const double thr = fmax(0.05 * (sqrt(vsqrRoe) + aRoe), dLambda);
+        const double thrS = thr * thr;
+        if (lam0 < thr)
+            lam0 = (sqr(lam0) + thrS) / (2 * thr);
+        if (lam4 < thr)
+            lam4 = (sqr(lam4) + thrS) / (2 * thr);
+    } else if (scheme == 1) {
+        const double aL = dmin(sqrt(asqrL), fabs(vL0) / 0.2), aR = dmin(sqrt(asqrR), fabs(vR0) / 0.2);
+        lam0 = dmax(fabs(vL0 - aL), fabs(vR0 - aR));
+        lam123 = dmax(fabs(vL0), fabs(vR0));
+        lam4 = dmax(fabs(vL0 + aL), fabs(vR0 + aR));
+    } else if (scheme == 2) {
+        const double lam = dmax(fabs(vL0) + sqrt(asqrL), fabs(vR0) + sqrt(asqrR));
+        for (int k = 0; k < 4; k++)
+            F[k] = (FL[k] + FR[k]) * 0.5 - 0.5 * lam * (UR[k] - UL[k]);
+        return true;
+    } else if (scheme == 3) {
+        const double aStar = dmin(fabs(vRoe0) / 0.2, aRoe);
+        lam0 = fabs(vRoe0 - aStar);
+        lam4 = fabs(vRoe0 + aStar);
+    } else if (scheme == 4) {
+        const double sgn = vRoe0 > 0 ? 1.0 : (vRoe0 < 0 ? -1.0 : 0.0);
+        const double uStar = sgn * dmax(aRoe * 0.2, fabs(vRoe0));
+        lam0 = fabs(uStar - aRoe);
+        lam123 = fabs(uStar);
+        lam4 = fabs(uStar + aRoe);
+    }
     double inc[4];
     for (int k = 0; k < 4; k++)
         inc[k] = UR[k] - UL[k];
@@ -463,7 +493,7 @@ __device__ inline bool riemann_face(const GasParams &gp, const double ULxy[4], c
     double f[4];
     const bool ok = gp.rs == 2   ? hllc_flux(UL, UR, gp.gamma, f)
                     : gp.rs == 3 ? hllep_flux(UL, UR, gp.gamma, f)
-                                 : roe_flux(UL, UR, gp.gamma, dLambda, f);
+                                 : roe_flux(UL, UR, gp.gamma, dLambda, f, gp.rs > 10 ? gp.rs - 10 : 0);
     if (!ok)
         return false;
     const double a = n0 * f[1] + (-n1) * f[2];

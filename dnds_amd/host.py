@@ -420,7 +420,8 @@ class ArrayScalar(_DeviceArray):
     WIDTH = 1
 
 
-RS = {"Roe": A.RS_ROE, "HLLC": A.RS_HLLC, "HLLEP": A.RS_HLLEP}
+RS = {"Roe": A.RS_ROE, "HLLC": A.RS_HLLC, "HLLEP": A.RS_HLLEP,
+      **{f"Roe_M{k}": code for k, code in A.RS_ROE_M.items()}}
 
 
 def solver_output(u, ifUseLimiter=None, rhs=None, gamma: float = 1.4, Rgas: float = 1.0,

@@ -228,7 +228,7 @@ void dndsmi_vr_destroy(dndsmi_vr *vr);
  * pr_gas, r_gas and t_ref must be positive.  CpGas = r_gas gamma / (gamma - 1)
  * as the solver sets it (DNDS_FV_EulerSolver.hpp:334-339). */
 typedef struct dndsmi_euler_setting {
-    int32_t riemann_solver;           /* 1 Roe, 2 HLLC, 3 HLLEP (RiemannSolverType) */
+    int32_t riemann_solver;           /* RiemannSolverType: 1 Roe, 2 HLLC, 3 HLLEP, 11-15 Roe_M1-M5 */
     double gamma;                     /* idealGasProperty.gamma                 */
     double far_field[4];              /* farFieldStaticValue (rho, rhou, rhov, E) */
     double mu_gas;                    /* idealGasProperty.muGas (reference default 1) */
@@ -238,7 +238,17 @@ typedef struct dndsmi_euler_setting {
     double c_sutherland;              /* CSutherland (110.4)                    */
 } dndsmi_euler_setting;
 
-enum { DNDSMI_RS_ROE = 1, DNDSMI_RS_HLLC = 2, DNDSMI_RS_HLLEP = 3 };
+/* The reference's RiemannSolverType codes (DNDS_FV_EulerEvaluator.hpp:140-150). */
+enum {
+    DNDSMI_RS_ROE = 1,
+    DNDSMI_RS_HLLC = 2,
+    DNDSMI_RS_HLLEP = 3,
+    DNDSMI_RS_ROE_M1 = 11,
+    DNDSMI_RS_ROE_M2 = 12,
+    DNDSMI_RS_ROE_M3 = 13,
+    DNDSMI_RS_ROE_M4 = 14,
+    DNDSMI_RS_ROE_M5 = 15
+};
 
 typedef struct dndsmi_ctx dndsmi_ctx;
 typedef struct dndsmi_array dndsmi_array;

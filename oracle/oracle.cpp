@@ -147,9 +147,14 @@ void left_eigen(const double velo[2], double vsqr, double H, double a, double ga
             L[i][j] *= sc;
 }
 
-// Gas::RoeFlux_IdealGas_HartenYee<2, 0>, DNDS_Gas.hpp:405-603
-int roe_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4]) {
+// Gas::RoeFlux_IdealGas_HartenYee<2, eigScheme>, DNDS_Gas.hpp:405-603.
+// eigScheme 0 is the Roe solver with the Harten-Yee fix (riemannSolverType
+// Roe); 1..5 are Roe_M1..Roe_M5 (:461-518): 1 cLLF_M, 2 the local Lax
+// flux (early exit), 3 and 4 the low- and increased-dissipation Roe_M of
+// Fleischmann et al. (2020), 5 no fix.
+int roe_flux(const double UL[4], const double UR[4], double gamma, double dLambda, double F[4], int eigScheme) {
     const double scaleHartenYee = 0.05;
+    const double scaleLD = 0.2;
     if (!(UL[0] > 0 && UR[0] > 0))
         return ERR_NONPOSITIVE;
     double veloL[2] = {UL[1] / UL[0], UL[2] / UL[0]};
@@ -176,12 +181,39 @@ int roe_flux(const double UL[4], const double UR[4], double gamma, double dLambd
     double lam0 = std::abs(veloRoe[0] - aRoe);
     double lam123 = std::abs(veloRoe[0]);
     double lam4 = std::abs(veloRoe[0] + aRoe);
-    double thresholdHartenYee = std::max(scaleHartenYee * (std::sqrt(vsqrRoe) + aRoe), dLambda);
-    double thresholdHartenYeeS = thresholdHartenYee * thresholdHartenYee;
-    if (lam0 < thresholdHartenYee)
-        lam0 = (sqr(lam0) + thresholdHartenYeeS) / (2 * thresholdHartenYee);
-    if (lam4 < thresholdHartenYee)
-        lam4 = (sqr(lam4) + thresholdHartenYeeS) / (2 * thresholdHartenYee);
+    if (eigScheme == 0) {
+        double thresholdHartenYee = std::max(scaleHartenYee * (std::sqrt(vsqrRoe) + aRoe), dLambda);
+        double thresholdHartenYeeS = thresholdHartenYee * thresholdHartenYee;
+        if (lam0 < thresholdHartenYee)
+            lam0 = (sqr(lam0) + thresholdHartenYeeS) / (2 * thresholdHartenYee);
+        if (lam4 < thresholdHartenYee)
+            lam4 = (sqr(lam4) + thresholdHartenYeeS) / (2 * thresholdHartenYee);
+    } else if (eigScheme == 1) {
+        double aL = std::min(std::sqrt(asqrL), std::abs(veloL[0]) / scaleLD);
+        double aR = std::min(std::sqrt(asqrR), std::abs(veloR[0]) / scaleLD);
+        lam0 = std::max(std::abs(veloL[0] - aL), std::abs(veloR[0] - aR));
+        lam123 = std::max(std::abs(veloL[0]), std::abs(veloR[0]));
+        lam4 = std::max(std::abs(veloL[0] + aL), std::abs(veloR[0] + aR));
+    } else if (eigScheme == 2) {
+        lam0 = lam123 = lam4 = std::max(std::abs(veloL[0]) + std::sqrt(asqrL), std::abs(veloR[0]) + std::sqrt(asqrR));
+        for (int k = 0; k < 4; k++)
+            F[k] = (FL[k] + FR[k]) * 0.5 - 0.5 * lam0 * (UR[k] - UL[k]);
+        return ERR_OK;
+    } else if (eigScheme == 3) {
+        double LDthreshold = std::abs(veloRoe[0]) / scaleLD;
+        double aRoeStar = std::min(LDthreshold, aRoe);
+        lam0 = std::abs(veloRoe[0] - aRoeStar);
+        lam4 = std::abs(veloRoe[0] + aRoeStar);
+    } else if (eigScheme == 4) {
+        // sign(0) = 0 (DNDS_Defines.h:164-167; USE_SIGN_MINUS_AT_ROE_M4_FLUX is off)
+        const double sgn = veloRoe[0] > 0 ? 1 : (veloRoe[0] < 0 ? -1 : 0);
+        double uStar = sgn * std::max(aRoe * scaleLD, std::abs(veloRoe[0]));
+        lam0 = std::abs(uStar - aRoe);
+        lam123 = std::abs(uStar);
+        lam4 = std::abs(uStar + aRoe);
+    } else if (eigScheme != 5) {
+        return ERR_UNSUPPORTED;
+    }
     double lam[4] = {lam0, lam123, lam123, lam4};
     double R[4][4];
     right_eigen(veloRoe, vsqrRoe, HRoe, aRoe, R);
@@ -551,7 +583,9 @@ int flux_face(const oracle_setting &cfg, const double ULxy[4], const double URxy
     }
     double finc[4];
     int rc = cfg.riemann_solver == 2   ? hllc_flux(UL, UR, gamma, dLambda, finc)
-             : cfg.riemann_solver == 1 ? roe_flux(UL, UR, gamma, dLambda, finc)
+             : cfg.riemann_solver == 1 ? roe_flux(UL, UR, gamma, dLambda, finc, 0)
+             : (cfg.riemann_solver >= 11 && cfg.riemann_solver <= 15)
+                 ? roe_flux(UL, UR, gamma, dLambda, finc, cfg.riemann_solver - 10)
              : cfg.riemann_solver == 3 ? hllep_flux(UL, UR, gamma, dLambda, finc)
                                        : ERR_UNSUPPORTED;
     if (rc)
@@ -568,7 +602,11 @@ int flux_face(const oracle_setting &cfg, const double ULxy[4], const double URxy
 } // namespace
 
 extern "C" int oracle_roe_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F) {
-    return roe_flux(UL, UR, gamma, dlambda, F);
+    return roe_flux(UL, UR, gamma, dlambda, F, 0);
+}
+extern "C" int oracle_roe_m_flux(const double *UL, const double *UR, double gamma, double dlambda, int scheme,
+                                 double *F) {
+    return roe_flux(UL, UR, gamma, dlambda, F, scheme);
 }
 extern "C" int oracle_hllc_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F) {
     return hllc_flux(UL, UR, gamma, dlambda, F);

@@ -47,7 +47,7 @@ typedef struct oracle_setup {
 } oracle_setup;
 
 typedef struct oracle_setting {
-    int32_t riemann_solver;        /* 1 Roe, 2 HLLC, 3 HLLEP */
+    int32_t riemann_solver;        /* 1 Roe, 2 HLLC, 3 HLLEP, 11-15 Roe_M1-M5 */
     double gamma;
     double far_field[4];
     double jacobi_relax;
@@ -102,6 +102,7 @@ int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, const oracl
 
 /* Gas-dynamics kernels exposed for unit tests (rotated frame, 4 variables). */
 int oracle_roe_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F);
+int oracle_roe_m_flux(const double *UL, const double *UR, double gamma, double dlambda, int scheme, double *F);
 int oracle_hllc_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F);
 int oracle_hllep_flux(const double *UL, const double *UR, double gamma, double dlambda, double *F);
 /* Left / right eigenvector matrices of the x-direction Euler Jacobian

@@ -99,6 +99,7 @@ def lib(variant: str = "parity") -> C.CDLL:
             "oracle_roe_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_hllc_flux": [PD, PD, C.c_double, C.c_double, PD],
             "oracle_hllep_flux": [PD, PD, C.c_double, C.c_double, PD],
+            "oracle_roe_m_flux": [PD, PD, C.c_double, C.c_double, C.c_int, PD],
             "oracle_eigenvectors": [PD, C.c_double, PD, PD],
             "oracle_boundary_value": [C.POINTER(OSetting), PD, PD, PD, C.c_double, C.c_int32, C.c_int32, PD],
             "oracle_flux_jacobian_times_du": [PD, PD, PD, C.c_double, C.c_double, PD],
@@ -338,6 +339,16 @@ def hllc_flux(UL, UR, gamma=1.4, dlambda=0.0):
                                 dlambda, _d(F))
     if rc:
         raise RuntimeError(f"hllc flux failed {rc}")
+    return F
+
+
+def roe_m_flux(UL, UR, scheme, gamma=1.4, dlambda=0.0):
+    """Roe flux with eigenvalue scheme `scheme` (0 Harten-Yee, 1..5 Roe_M1..M5)"""
+    F = np.empty(4)
+    rc = lib().oracle_roe_m_flux(_d(np.ascontiguousarray(UL, float)), _d(np.ascontiguousarray(UR, float)), gamma,
+                                 dlambda, scheme, _d(F))
+    if rc:
+        raise RuntimeError(f"roe_m flux failed {rc}")
     return F
 
 

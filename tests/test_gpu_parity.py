@@ -31,7 +31,7 @@ BC_SETS = {
 DOMAIN = {"ivfar": (10.0, 10.0), "dmrfar": (4.0, 1.0)}
 
 
-RS_CODE = {"Roe": 1, "HLLC": 2, "HLLEP": 3}
+RS_CODE = {"Roe": 1, "HLLC": 2, "HLLEP": 3, "Roe_M1": 11, "Roe_M2": 12, "Roe_M3": 13, "Roe_M4": 14, "Roe_M5": 15}
 
 
 def build(kind, bcname, riemann="Roe", nx=9, ny=7, jitter=0.2, seed=5):
@@ -94,7 +94,7 @@ def test_evaluate_dt_matches_oracle(kind, bcname, local):
 
 
 @pytest.mark.parametrize("kind,bcname", CASES)
-@pytest.mark.parametrize("riemann", ["Roe", "HLLC", "HLLEP"])
+@pytest.mark.parametrize("riemann", ["Roe", "HLLC", "HLLEP", "Roe_M1", "Roe_M2", "Roe_M3", "Roe_M4", "Roe_M5"])
 def test_evaluate_rhs_matches_oracle(kind, bcname, riemann):
     mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
     u, urec = random_state(mesh.n_cell, 13)
@@ -109,7 +109,8 @@ def test_evaluate_rhs_matches_oracle(kind, bcname, riemann):
 @pytest.mark.parametrize("kind,bcname,riemann", [(0, "periodic", "Roe"), (2, "farfield", "HLLC"),
                                                  (1, "walls", "Roe"), (0, "riemann", "HLLC"),
                                                  (2, "ivfar", "Roe"), (0, "rtfar", "HLLC"),
-                                                 (0, "riemann", "HLLEP"), (1, "walls", "HLLEP")])
+                                                 (0, "riemann", "HLLEP"), (1, "walls", "HLLEP"),
+                                                 (2, "farfield", "Roe_M4"), (0, "periodic", "Roe_M1")])
 def test_ssprk3_steps_match_oracle(kind, bcname, riemann):
     from dnds_amd.host import ExplicitSSPRK3LocalDt
     mesh, vfv, ev, orc = build(kind, bcname, riemann=riemann)
