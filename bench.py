@@ -299,6 +299,10 @@ def main():
                     help="also run the learned warm start workload: harvest converged reconstructions of the "
                          "40x40 vortex at 8 mesh seeds (160 snapshots each, 4.1M pairs), train on the device, "
                          "report the sweeps to a 1e-10 rec increment at an unseen seed")
+    ap.add_argument("--sor", type=int, default=0,
+                    help="also reconstruct the workload's initial state from zero to a 1e-10 rec increment (at most "
+                         "N sweeps) with the Jacobi sweep, the SOR_Instead sweep and SOR_Instead with "
+                         "SOR_InverseScanning, and report sweeps and times")
     ap.add_argument("--limiter-procedure", type=int, default=0, choices=[0, 1],
                     help="WBAP of the limited cases: 0 V2 (the reference solver's default), 1 V3")
     ap.add_argument("--mu-gas", type=float, default=0.0,
@@ -526,6 +530,48 @@ def main():
         log(f"implicit: {implicit['ms_per_iteration']:.3f} ms per inner iteration, {ncol} colours, "
             f"LU-SGS {lus.get('ms_per_sweep', 0):.3f} ms per sweep, {lus.get('gbs', 0):.0f} GB/s algorithmic")
 
+    sor = None
+    if args.sor:
+        import ctypes as C
+        from dnds_amd import _capi as A
+        from dnds_amd.host import EulerEvaluator, VRFiniteVolume2D
+        sor = {"tolerance": 1e-10, "max_sweeps": args.sor,
+               "start": "the workload's initial means, uRec from zero, t = 0",
+               "note": "the SOR sweeps run colour by colour in place; each sweep of this loop first copies uRec "
+                       "(the loop measures the increment against the previous iterate)"}
+        kj = A.KERNEL_NAMES.index("k_jacobi")
+        for name, (on, inverse) in (("jacobi", (0, 0)), ("sor", (1, 0)), ("sor_inverse", (1, 1))):
+            if on:
+                st = A.VRSetting.from_buffer_copy(case.vfv.setting)
+                st.sor_instead, st.sor_inverse_scanning = on, inverse
+                s0 = time.time()
+                vfv2 = VRFiniteVolume2D(case.mesh, st)
+                vfv2.Initialization(args.threads)
+                ev2 = EulerEvaluator(case.mesh, vfv2, riemannSolverType=case.riemann, farFieldStaticValue=case.far,
+                                     **gas)
+                log(f"sor: {name} setup {time.time() - s0:.1f}s")
+            else:
+                vfv2, ev2 = case.vfv, ev
+            u = ev2.BuildMean().upload(case.u0)
+            r, rn = vfv2.BuildRec().setConstant(0.0), vfv2.BuildRec()
+            ctx = ev2.device.ctx
+            lib.dndsmi_profile_enable(ctx, 1)
+            ev2.device.sync()
+            ts = time.perf_counter()
+            n, res = vfv2.ReconstructToTolerance(u, r, rn, 1e-10, args.sor, 0.0)
+            ev2.device.sync()
+            el = time.perf_counter() - ts
+            cnt, tot = C.c_int64(), C.c_double()
+            lib.dndsmi_profile_query(ctx, kj, C.byref(cnt), C.byref(tot))
+            lib.dndsmi_profile_enable(ctx, 0)
+            sor[name] = {"sweeps": n, "last_increment": res, "converged": res < 1e-10, "s": el,
+                         "ms_per_sweep": el / n * 1e3, "kernel_ms_per_sweep": tot.value / n,
+                         "launches_per_sweep": cnt.value / n}
+            log(f"sor: {name}: {sor[name]}")
+            del u, r, rn
+            if on:
+                del ev2, vfv2
+
     warm = None
     if args.warmstart:
         from dnds_amd import learned
@@ -614,6 +660,7 @@ def main():
         "riemann_2M": riemann,
         "implicit_lusgs": implicit,
         "learned_warm_start": warm,
+        "sor_study": sor,
     }
     print(json.dumps(out), flush=True)
 

@@ -84,7 +84,7 @@ class VRSetting(C.Structure):
         ("weight_scheme_dir", C.c_int32), ("weight_len_scale", C.c_double),
         ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double),
         ("use_local_coord", C.c_int32), ("orthogonalize_base", C.c_int32), ("norm_wbap", C.c_int32),
-        ("build_limiter_data", C.c_int32),
+        ("build_limiter_data", C.c_int32), ("sor_instead", C.c_int32), ("sor_inverse_scanning", C.c_int32),
     ]
 
 

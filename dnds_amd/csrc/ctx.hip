@@ -60,29 +60,17 @@ void prof_end(dndsmi_ctx *c) {
 // kernels
 // ========================================================================
 
-// Boundary correction of a Wall_Euler / Wall_NoSlip face to its cell's sweep
-// (DNDS_FV_VR.hpp:1577-1657): corr = w A^-1 sum_g w_g |n_g| D0^T W0^2 (uBV - u_i).
-// Computed before the sweep from the current iterate; the sweep adds it in
-// the face's slot, which keeps the reference's accumulation order.
-__global__ __launch_bounds__(kBlock) void k_wall_corr(int32_t nw, int64_t ldc, const int32_t *__restrict__ wcell,
-                                                      const int32_t *__restrict__ wbc,
-                                                      const double *__restrict__ wall_ainv,
-                                                      const double *__restrict__ wall_d0,
-                                                      const double *__restrict__ wall_nrm,
-                                                      const double *__restrict__ wall_w0, int64_t ldw,
-                                                      const double *__restrict__ u, const double *__restrict__ x,
-                                                      double relax, GasParams gp, double t,
-                                                      double *__restrict__ corr, int32_t *errw) {
-    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nw)
-        return;
-    const int32_t i = wcell[w];
-    const int32_t bc = wbc[w];
-    double xi[36], ui[4];
-    for (int k = 0; k < 36; k++)
-        xi[k] = x[k * ldc + i];
-    for (int v = 0; v < 4; v++)
-        ui[v] = u[4 * int64_t(i) + v];
+// Boundary correction of a Wall_Euler / Wall_NoSlip face w to its cell's
+// sweep (DNDS_FV_VR.hpp:1577-1657), from the cell's coefficients xi and mean
+// ui: corr = relax A^-1 sum_g w_g |n_g| D0^T W0^2 (uBV - u_i).  ADD adds the
+// correction to corr (xi may then be corr itself: xi is read in full before
+// corr is written); otherwise it is stored there.
+template <bool ADD>
+__device__ inline void wall_correction(int32_t w, int32_t bc, const double *xi, const double ui[4],
+                                       const double *__restrict__ wall_ainv, const double *__restrict__ wall_d0,
+                                       const double *__restrict__ wall_nrm, const double *__restrict__ wall_w0,
+                                       int64_t ldw, double relax, const GasParams &gp, double t, int32_t *errw,
+                                       double *corr) {
     double BCc[36];
     for (int k = 0; k < 36; k++)
         BCc[k] = 0.0;
@@ -126,8 +114,151 @@ __global__ __launch_bounds__(kBlock) void k_wall_corr(int32_t nw, int64_t ldc, c
             double acc = 0;
             for (int c = 0; c < 9; c++)
                 acc += wall_ainv[(r * 9 + c) * ldw + w] * BCc[c + 9 * v];
-            corr[(r + 9 * v) * ldw + w] = relax * acc;
+            if (ADD)
+                corr[r + 9 * v] += relax * acc;
+            else
+                corr[r + 9 * v] = relax * acc;
         }
+}
+
+// The Jacobi sweep's wall corrections, computed before the sweep from the
+// current iterate; the sweep adds each in the face's slot, which keeps the
+// reference's accumulation order.
+__global__ __launch_bounds__(kBlock) void k_wall_corr(int32_t nw, int64_t ldc, const int32_t *__restrict__ wcell,
+                                                      const int32_t *__restrict__ wbc,
+                                                      const double *__restrict__ wall_ainv,
+                                                      const double *__restrict__ wall_d0,
+                                                      const double *__restrict__ wall_nrm,
+                                                      const double *__restrict__ wall_w0, int64_t ldw,
+                                                      const double *__restrict__ u, const double *__restrict__ x,
+                                                      double relax, GasParams gp, double t,
+                                                      double *__restrict__ corr, int32_t *errw) {
+    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw)
+        return;
+    const int32_t i = wcell[w];
+    double xi[36], ui[4], cw[36];
+    for (int k = 0; k < 36; k++)
+        xi[k] = x[k * ldc + i];
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+    wall_correction<false>(w, wbc[w], xi, ui, wall_ainv, wall_d0, wall_nrm, wall_w0, ldw, relax, gp, t, errw, cw);
+    for (int k = 0; k < 36; k++)
+        corr[k * ldw + w] = cw[k];
+}
+
+// One SOR pass over cells of one colour (SOR_Instead,
+// DNDS_FV_VR.hpp:1495-1662), in place on x.  No two cells of a colour are
+// neighbours, so the pass equals the reference's sequential in-place scan
+// over those cells, and the colours in turn equal its scan in colour-major
+// order.  As in the reference, every term is added to the cell's own
+// coefficients: the far-field product M x_i and the Wall_Euler / Wall_NoSlip
+// corrections take the partial sum after (1 - w) scaling and the faces before
+// them (the product is formed from a copy, then added).
+//
+// The cells are the device range [base, base + n): with SOR_Instead the
+// device numbering is colour-major.  SELF: the cells may have far-field,
+// Wall_Euler or Wall_NoSlip faces, the terms that read the cell's own partial
+// sum; the numbering puts those cells after the colour's others, so that the
+// others (all but the boundary layer) run without the registers of the copy
+// and of the wall correction.  x and xo are the
+// same array: the new coefficients go out through xo so that the compiler
+// does not keep the 36 addresses of the first reads live until the stores.
+template <int NF, bool SELF>
+__global__ __launch_bounds__(kBlock) void k_sor(int32_t n, int32_t base, int64_t ldc, const int32_t *__restrict__ nbr,
+                                                const int32_t *__restrict__ sbc, const int32_t *__restrict__ wall_idx,
+                                                const double *__restrict__ M, const double *__restrict__ bv,
+                                                const double *__restrict__ u, const double *x, double *xo,
+                                                double relax, const double *__restrict__ wall_ainv,
+                                                const double *__restrict__ wall_d0,
+                                                const double *__restrict__ wall_nrm,
+                                                const double *__restrict__ wall_w0, int64_t ldw, GasParams gp,
+                                                double t, int32_t *errw) {
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n)
+        return;
+    const int32_t i = base + tid;
+    double nw[36];
+#pragma unroll
+    for (int k = 0; k < 36; k++)
+        nw[k] = (1 - relax) * x[k * ldc + i];
+    double ui[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+        ui[v] = u[4 * int64_t(i) + v];
+#pragma unroll 1
+    for (int s = 0; s < NF; s++) {
+        const int32_t j = nbr[s * ldc + i];
+        if (j == -2)
+            continue;
+        const double *bs = bv + int64_t(s) * 9 * ldc + i;
+        double xj[36], du[4], bmul = 1.0;
+        if (j >= 0) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                du[v] = u[4 * int64_t(j) + v] - ui[v];
+#pragma unroll
+            for (int k = 0; k < 36; k++)
+                xj[k] = x[k * ldc + j];
+        } else {
+            const int32_t bc = sbc[s * ldc + i];
+            if (bc == BC_WALL) {
+#pragma unroll
+                for (int r = 0; r < 9; r++) {
+                    const double b = bs[r * ldc];
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        nw[r + 9 * v] += relax * ((0.0 - ui[v]) * b);
+                }
+                continue;
+            }
+            if (!SELF) { // not in this launch's lists
+                atomicOr(errw, ERRBIT_BC);
+                continue;
+            }
+            if (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) {
+                wall_correction<true>(wall_idx[s * ldc + i], bc, nw, ui, wall_ainv, wall_d0, wall_nrm, wall_w0, ldw,
+                                      relax, gp, t, errw, nw);
+                continue;
+            }
+            if (!is_far(bc)) {
+                atomicOr(errw, ERRBIT_BC);
+                continue;
+            }
+            // far field: M_if times the partial sum, no mean-value term
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                du[v] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 36; k++)
+                xj[k] = nw[k];
+            bmul = 0.0;
+        }
+        const dbl2 *Ms = reinterpret_cast<const dbl2 *>(M) + m_vec_index(s, 0, i, ldc, kMPairs);
+        dbl2 mp = {0.0, 0.0};
+        double tt[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < 81; e++) {
+            const int r = e / 9, c = e % 9;
+            if (e % 2 == 0)
+                mp = __builtin_nontemporal_load(Ms + (e / 2) * 64);
+            const double m = e % 2 == 0 ? mp.x : mp.y;
+            if (c == 0)
+                tt[0] = tt[1] = tt[2] = tt[3] = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                tt[v] += m * xj[c + 9 * v];
+            if (c == 8) {
+                const double b = bmul * bs[r * ldc];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 36; k++)
+        xo[k * ldc + i] = nw[k];
 }
 
 // One block-Jacobi VR sweep, SOR off (ReconstructionJacobiStep,
@@ -1403,6 +1534,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
     const int64_t nc = mesh->n_cell, nf = mesh->n_face;
     c->nc = nc, c->nf = nf, c->ldc = pad64(nc), c->ldf = pad64(nf);
     c->relax = vs->jacobi_relax;
+    c->sor = vs->sor_instead != 0;
+    c->sor_inverse = c->sor && vs->sor_inverse_scanning != 0;
     c->gas.gamma = es->gamma;
     for (int k = 0; k < 4; k++)
         c->gas.far[k] = es->far_field[k];
@@ -1430,18 +1563,79 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
         // before its quads, keeping their order: the fourth slot's A^-1 B
         // vectors of a block are then contiguous over its quads, and the
         // 128-byte lines of that stream hold no unused triangle slots.
+        //
+        // With SOR_Instead the numbering is colour-major instead: each colour
+        // is a contiguous range (its cells in mesh order), so that a colour's
+        // pass streams whole lines of the A^-1 B pairs and coefficients; in
+        // each colour the cells with terms on their own partial sum (far
+        // field, Wall_Euler, Wall_NoSlip) come last and take their own kernel
+        // variant.  The windows of 64 then start at each range's start.
+        //
+        // Colouring (multicolour LU-SGS and SOR): each cell takes the
+        // smallest colour that no lower-numbered neighbour holds.
+        std::vector<int32_t> col(size_t(nc), -1);
+        int32_t ncol = 0;
+        for (int64_t i = 0; i < nc; i++) { // mesh numbering
+            uint32_t used = 0;
+            for (int64_t q = mesh->cell_ptr[i]; q < mesh->cell_ptr[i + 1]; q++) {
+                const int64_t f = mesh->cell2face[q];
+                const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
+                const int64_t j = L == i ? R : L;
+                if (j != DNDSMI_NO_CELL && j < i && col[j] < 32)
+                    used |= 1u << col[j];
+            }
+            int32_t k = 0;
+            while (used & (1u << k))
+                k++;
+            col[i] = k;
+            ncol = std::max(ncol, k + 1);
+        }
         std::vector<int32_t> &c2d = c->c2d, &f2d = c->f2d;
         {
-            c2d.resize(static_cast<size_t>(nc));
-            for (int64_t w0 = 0; w0 < nc; w0 += 64) {
-                const int64_t w1 = std::min<int64_t>(nc, w0 + 64);
-                int64_t d = w0;
-                for (int pass = 0; pass < 2; pass++)
-                    for (int64_t i = w0; i < w1; i++) {
-                        const bool quad = mesh->cell_ptr[i + 1] - mesh->cell_ptr[i] == 4;
-                        if (quad == (pass == 1))
-                            c2d[i] = int32_t(d++);
+            // the numbering's segments (mesh cells in order) and their ends
+            std::vector<int64_t> seq(static_cast<size_t>(nc)), seg_end;
+            if (c->sor) {
+                std::vector<uint8_t> self(size_t(nc), 0);
+                for (int64_t i = 0; i < nc; i++)
+                    for (int64_t q = mesh->cell_ptr[i]; q < mesh->cell_ptr[i + 1]; q++) {
+                        const int64_t f = mesh->cell2face[q];
+                        const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
+                        if ((L == i ? R : L) == DNDSMI_NO_CELL && mesh->face_bc[f] != DNDSMI_BC_WALL)
+                            self[size_t(i)] = 1;
                     }
+                std::vector<int64_t> cnt(2 * size_t(ncol) + 1, 0);
+                for (int64_t i = 0; i < nc; i++)
+                    cnt[2 * col[i] + self[size_t(i)] + 1]++;
+                for (size_t g = 0; g + 1 < cnt.size(); g++)
+                    cnt[g + 1] += cnt[g];
+                seg_end.assign(cnt.begin() + 1, cnt.end());
+                std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+                for (int64_t i = 0; i < nc; i++)
+                    seq[fill[2 * col[i] + self[size_t(i)]]++] = i;
+                c->sor_start.assign(size_t(ncol) + 1, nc);
+                c->sor_wall.assign(size_t(ncol), nc);
+                for (int32_t k = 0; k < ncol; k++)
+                    c->sor_start[k] = cnt[2 * k], c->sor_wall[k] = cnt[2 * k + 1];
+            } else {
+                for (int64_t i = 0; i < nc; i++)
+                    seq[i] = i;
+                seg_end.push_back(nc);
+            }
+            c2d.resize(static_cast<size_t>(nc));
+            int64_t a = 0;
+            for (const int64_t b : seg_end) {
+                for (int64_t w0 = a; w0 < b; w0 += 64) {
+                    const int64_t w1 = std::min<int64_t>(b, w0 + 64);
+                    int64_t d = w0;
+                    for (int pass = 0; pass < 2; pass++)
+                        for (int64_t p = w0; p < w1; p++) {
+                            const int64_t i = seq[p];
+                            const bool quad = mesh->cell_ptr[i + 1] - mesh->cell_ptr[i] == 4;
+                            if (quad == (pass == 1))
+                                c2d[i] = int32_t(d++);
+                        }
+                }
+                a = b;
             }
             // faces in the order of their first cell in the mesh (the
             // partition of a window above moves cells by less than a
@@ -1642,26 +1836,8 @@ extern "C" int dndsmi_ctx_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_v
             DNDSMI_TRY(c->alloc(c->lim_list, ldc));
             DNDSMI_TRY(c->alloc(c->lim_count, 1));
         }
-        // ---- LU-SGS colouring: each cell takes the smallest colour that no
-        // lower-numbered neighbour holds
+        // ---- multicolour scan lists (col and ncol from before the numbering)
         {
-            std::vector<int32_t> col(size_t(nc), -1);
-            int32_t ncol = 0;
-            for (int64_t i = 0; i < nc; i++) { // mesh numbering
-                uint32_t used = 0;
-                for (int64_t q = mesh->cell_ptr[i]; q < mesh->cell_ptr[i + 1]; q++) {
-                    const int64_t f = mesh->cell2face[q];
-                    const int64_t L = mesh->face2cell[2 * f], R = mesh->face2cell[2 * f + 1];
-                    const int64_t j = L == i ? R : L;
-                    if (j != DNDSMI_NO_CELL && j < i && col[j] < 32)
-                        used |= 1u << col[j];
-                }
-                int32_t k = 0;
-                while (used & (1u << k))
-                    k++;
-                col[i] = k;
-                ncol = std::max(ncol, k + 1);
-            }
             c->n_colour = ncol;
             c->col_start.assign(size_t(ncol) + 1, 0);
             for (int64_t i = 0; i < nc; i++)
@@ -1855,7 +2031,38 @@ extern "C" int dndsmi_array_set_constant(dndsmi_array *a, double value) {
 
 namespace dndsmi_host {
 
+// SOR_Instead: colour by colour in place on x (forward; with
+// SOR_InverseScanning then backward, DNDS_FV_VR.hpp:1493-1502)
+int launch_sor(dndsmi_ctx *c, const double *u, double *x, double t) {
+    const int passes = c->sor_inverse ? 2 : 1;
+    for (int pass = 0; pass < passes; pass++)
+        for (int32_t kk = 0; kk < c->n_colour; kk++) {
+            const int32_t col = pass == 0 ? kk : c->n_colour - 1 - kk;
+            for (int w = 0; w < 2; w++) {
+                const int64_t b = w ? c->sor_wall[col] : c->sor_start[col];
+                const int64_t n = (w ? c->sor_start[col + 1] : c->sor_wall[col]) - b;
+                if (n == 0)
+                    continue;
+                auto kern = c->nfmax == 3 ? (w ? k_sor<3, true> : k_sor<3, false>)
+                                          : (w ? k_sor<4, true> : k_sor<4, false>);
+                prof_begin(c, DNDSMI_K_JACOBI);
+                hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, int32_t(n), int32_t(b),
+                                   c->ldc, c->nbr, c->sbc, c->wall_idx, c->M, c->bv, u, x, x, c->relax,
+                                   c->wall_ainv, c->wall_d0, c->wall_nrm, c->wall_w0, c->n_wall, c->gas, t, c->errw);
+                prof_end(c);
+                DNDSMI_TRY(c->launch_check("k_sor"));
+            }
+        }
+    return DNDSMI_OK;
+}
+
 int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t) {
+    if (c->sor) {
+        // the steps expect the new iterate in xn: start it from x, then update in place
+        DNDSMI_TRY(c->hip(hipMemcpyAsync(xn, x, sizeof(double) * 36 * c->ldc, hipMemcpyDeviceToDevice, c->stream),
+                          "copy uRec"));
+        return launch_sor(c, u, xn, t);
+    }
     const int nc = int(c->nc);
     if (c->n_wall_faces > 0) {
         prof_begin(c, DNDSMI_K_WALL_CORR);
@@ -2150,6 +2357,8 @@ extern "C" int dndsmi_reconstruction_jacobi_step(dndsmi_ctx *c, const dndsmi_arr
     if (!c || !same_ctx(c, u, DNDSMI_ARRAY_DOF) || !same_ctx(c, uRec, DNDSMI_ARRAY_REC) ||
         !same_ctx(c, uRecNewBuf, DNDSMI_ARRAY_REC) || uRec == uRecNewBuf)
         return DNDSMI_ERR_INVALID_ARGUMENT;
+    if (c->sor) // in place on uRec; uRecNewBuf is not touched (DNDS_FV_VR.hpp:1507-1510, 1707)
+        return launch_sor(c, u->d, uRec->d, t);
     DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNewBuf->d, t));
     // uRec = uRecNewBuf (DNDS_FV_VR.hpp:1707-1711): afterwards both arrays
     // hold the new iterate, as in the reference.
@@ -2257,6 +2466,10 @@ extern "C" int dndsmi_ssprk3_step_mixed(dndsmi_ctx *c, dndsmi_array *u, dndsmi_a
         return DNDSMI_ERR_INVALID_ARGUMENT;
     if (use_limiter && !c->has_limiter) {
         c->err = "WBAP needs a setup built with build_limiter_data";
+        return DNDSMI_ERR_UNSUPPORTED;
+    }
+    if (c->sor) {
+        c->err = "the mixed-precision step iterates the Jacobi map (SOR_Instead must be off)";
         return DNDSMI_ERR_UNSUPPORTED;
     }
     if (c->gas.mu > 0) {

@@ -36,6 +36,13 @@ struct dndsmi_ctx {
     int nfmax = 3;
     GasParams gas{};
     double relax = 1.0;
+    // SOR_Instead / SOR_InverseScanning: in-place sweeps over the colours of
+    // the multicolour scan (col_cells), forward (then backward)
+    bool sor = false, sor_inverse = false;
+    // SOR_Instead (colour-major device numbering): colour k holds the device
+    // cells [sor_start[k], sor_start[k + 1]), those with a far-field,
+    // Wall_Euler or Wall_NoSlip face from sor_wall[k] on
+    std::vector<int64_t> sor_start, sor_wall;
     int64_t bytes = 0;
     std::vector<void *> allocs;
     // live arrays and other objects bound to this context, detached when it
@@ -233,6 +240,8 @@ int noalloc_scratch_init(dndsmi_ctx *c);
 bool same_ctx(const dndsmi_ctx *c, const dndsmi_array *a, int32_t kind);
 // the explicit path's launches (ctx.hip)
 int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t);
+// one SOR sweep (SOR_Instead) in place on x, colour by colour
+int launch_sor(dndsmi_ctx *c, const double *u, double *x, double t);
 int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double *dt_out);
 // xalt/limf (may be null): cells with limf set are read from xalt instead of x
 int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,

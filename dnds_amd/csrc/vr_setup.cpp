@@ -591,6 +591,8 @@ extern "C" void dndsmi_vr_setting_default(dndsmi_vr_setting *s) {
     s->orthogonalize_base = 0;
     s->norm_wbap = 0;
     s->build_limiter_data = 0;
+    s->sor_instead = 0;          // SOR_Instead = false
+    s->sor_inverse_scanning = 0; // SOR_InverseScanning = false
 }
 
 extern "C" int dndsmi_vr_create(const dndsmi_mesh_view *mesh, const dndsmi_vr_setting *setting, int32_t nthreads,

@@ -298,8 +298,12 @@ class VRFiniteVolume2D:
         return ArrayRecV(self.device)
 
     def ReconstructionJacobiStep(self, u: "ArrayDOFV", uRec: "ArrayRecV", uRecNewBuf: "ArrayRecV", t: float = 0.0):
-        """One Jacobi sweep with SOR_Instead = false into uRecNewBuf, then
-        uRec = uRecNewBuf (DNDS_FV_VR.hpp:1707-1711): both hold the new iterate."""
+        """One reconstruction sweep (DNDS_FV_VR.hpp:1488-1712).  With
+        SOR_Instead = false a Jacobi sweep into uRecNewBuf, then
+        uRec = uRecNewBuf: both hold the new iterate.  With SOR_Instead an
+        in-place sweep on uRec in the colour-major scan order of
+        EulerEvaluator.LUSGSScanOrder (forward, then backward with
+        SOR_InverseScanning); uRecNewBuf is not touched."""
         d = self.device
         _check(_lib().dndsmi_reconstruction_jacobi_step(d.ctx, u.h, uRec.h, uRecNewBuf.h, t), "Jacobi step", d.ctx)
 

@@ -166,6 +166,10 @@ typedef struct dndsmi_vr_setting {
     int32_t orthogonalize_base;     /* must be 0                              */
     int32_t norm_wbap;              /* must be 0                              */
     int32_t build_limiter_data;     /* also build the WBAP face data          */
+    int32_t sor_instead;            /* SOR_Instead: in-place (Gauss-Seidel) sweeps,
+                                       over the cells colour by colour on the device */
+    int32_t sor_inverse_scanning;   /* SOR_InverseScanning: each SOR sweep is a forward
+                                       and a backward pass (colours reversed)   */
 } dndsmi_vr_setting;
 
 /* The reference's defaults, DNDS_FV_VR.hpp:172-228. */

@@ -737,6 +737,127 @@ extern "C" int oracle_jacobi_step(const oracle_mesh *m, const oracle_setup *s, c
     return ERR_OK;
 }
 
+// ReconstructionJacobiStep with SOR_Instead = true, DNDS_FV_VR.hpp:1495-1662:
+// one in-place Gauss-Seidel sweep over the cells in the order `scan`
+// (SOR_iScan2iCell; the identity, or a colour-major order).  Every term is
+// added to uRec[iCell] itself, so the far-field term M uRec[iCell] and the
+// Wall_Euler / Wall_NoSlip corrections see the cell's partial sum after
+// (1 - relax) scaling and the faces before them in cell2face order; the
+// product M uRec[iCell] is formed before it is added (Eigen evaluates a
+// matrix product into a temporary).  Neighbours are read in place: those
+// earlier in the scan carry this sweep's values.
+extern "C" int oracle_sor_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
+                               const double *u, double *urec, const int64_t *scan, double t) {
+    const double relax = cfg->jacobi_relax;
+    for (int64_t is = 0; is < m->n_cell; is++) {
+        const int64_t i = scan[is];
+        double *xi = urec + 36 * i;
+        for (int k = 0; k < 36; k++)
+            xi[k] = (1 - relax) * xi[k];
+        const int64_t b0 = m->cell_ptr[i];
+        const int nf = int(m->cell_ptr[i + 1] - b0);
+        for (int lf = 0; lf < nf; lf++) {
+            const int64_t f = m->cell2face[b0 + lf];
+            const int64_t j = m->face2cell[2 * f] == i ? m->face2cell[2 * f + 1] : m->face2cell[2 * f];
+            const double *M = s->ainv_b + 81 * (b0 + lf);
+            const double *bv = s->ainv_bvec + 9 * (b0 + lf);
+            const int32_t bc = m->face_bc[f];
+            if (j != kNoCell) {
+                const double *xj = urec + 36 * j;
+                double du[4];
+                for (int v = 0; v < 4; v++)
+                    du[v] = u[4 * j + v] - u[4 * i + v];
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double tt = 0;
+                        for (int c = 0; c < 9; c++)
+                            tt += M[9 * r + c] * xj[c + 9 * v];
+                        xi[r + 9 * v] += relax * (tt + du[v] * bv[r]);
+                    }
+            } else if (bc == BC_WALL) {
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++)
+                        xi[r + 9 * v] += relax * ((0.0 - u[4 * i + v]) * bv[r]);
+            } else if (is_far(bc)) {
+                double tt[36];
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += M[9 * r + c] * xi[c + 9 * v];
+                        tt[r + 9 * v] = acc;
+                    }
+                for (int k = 0; k < 36; k++)
+                    xi[k] += relax * tt[k];
+            } else if (bc == BC_WALL_EULER || bc == BC_WALL_NOSLIP) {
+                double BC[36] = {0};
+                const double w0 = s->face_weight[10 * f];
+                for (int g = 0; g < 3; g++) {
+                    const double *d = s->face_d0 + 54 * f + 18 * g;
+                    const double nx = s->face_norm[6 * f + 2 * g], ny = s->face_norm[6 * f + 2 * g + 1];
+                    double uBL[4];
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += d[c] * xi[c + 9 * v];
+                        uBL[v] = acc;
+                    }
+                    for (int v = 0; v < 4; v++)
+                        uBL[v] += u[4 * i + v];
+                    double uBV[4];
+                    double n0, n1;
+                    stable_normalized(nx, ny, n0, n1);
+                    if (bc == BC_WALL_EULER) {
+                        for (int v = 0; v < 4; v++)
+                            uBV[v] = uBL[v];
+                        double dd = n0 * uBV[1] + n1 * uBV[2];
+                        uBV[1] = uBL[1] - n0 * dd;
+                        uBV[2] = uBL[2] - n1 * dd;
+                    } else {
+                        double nrm[2] = {n0, n1}, pp[2] = {0, 0};
+                        int rc = boundary_value(*cfg, uBL, nrm, pp, t, bc, uBV);
+                        if (rc)
+                            return rc;
+                    }
+                    const double nl = std::sqrt(nx * nx + ny * ny);
+                    for (int r = 0; r < 9; r++)
+                        for (int v = 0; v < 4; v++) {
+                            double inc = d[r] * w0 * w0 * (uBV[v] - u[4 * i + v]);
+                            inc *= nl;
+                            BC[r + 9 * v] += inc * kLine3W[g];
+                        }
+                }
+                const double *Ai = s->ainv + 81 * i;
+                for (int r = 0; r < 9; r++)
+                    for (int v = 0; v < 4; v++) {
+                        double acc = 0;
+                        for (int c = 0; c < 9; c++)
+                            acc += Ai[9 * r + c] * BC[c + 9 * v];
+                        xi[r + 9 * v] += relax * acc;
+                    }
+            } else {
+                return ERR_UNSUPPORTED;
+            }
+        }
+    }
+    return ERR_OK;
+}
+
+// One reconstruction sweep of the steps: Jacobi, or with a scan order the
+// SOR_Instead sweep (forward, then backward with SOR_InverseScanning,
+// DNDS_FV_VR.hpp:1493-1502)
+static int rec_sweep(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                     double *urec, double *urec_new, double t) {
+    if (!cfg->sor_scan)
+        return oracle_jacobi_step(m, s, cfg, u, urec, urec_new, t);
+    int rc = oracle_sor_step(m, s, cfg, u, urec, cfg->sor_scan, t);
+    if (rc || !cfg->sor_inverse)
+        return rc;
+    std::vector<int64_t> rev(cfg->sor_scan, cfg->sor_scan + m->n_cell);
+    std::reverse(rev.begin(), rev.end());
+    return oracle_sor_step(m, s, cfg, u, urec, rev.data(), t);
+}
+
 // EvaluateDt, DNDS_FV_EulerEvaluator.cpp:14-183 (nTimeFilterPass = 0, muGas = 0)
 extern "C" int oracle_evaluate_dt(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg,
                                   const double *u, double cfl, double max_dt, int32_t use_local_dt, double *dt,
@@ -1426,7 +1547,7 @@ extern "C" int oracle_ssprk3_steps(const oracle_mesh *m, const oracle_setup *s, 
         for (int stage = 0; stage < 3; stage++) {
             const double ts = *t + ct[stage] * dtMin;
             for (int k = 0; k < n_sweeps; k++) {
-                rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), ts);
+                rc = rec_sweep(m, s, cfg, u, urec, urecNew.data(), ts);
                 if (rc)
                     return rc;
             }
@@ -1743,7 +1864,7 @@ extern "C" int oracle_implicit_euler(const oracle_mesh *m, const oracle_setup *s
         for (int64_t i = 0; i < nc; i++)
             dTau[i] /= alphaDiag;
         for (int k = 0; k < n_sweeps; k++)
-            if ((rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), t + dt)))
+            if ((rc = rec_sweep(m, s, cfg, u, urec, urecNew.data(), t + dt)))
                 return rc;
         if ((rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, t + dt, rhs.data())))
             return rc;
@@ -1811,7 +1932,7 @@ extern "C" int oracle_bdf_steps(const oracle_mesh *m, const oracle_setup *s, con
             for (int64_t i = 0; i < nc; i++)
                 dTau[i] /= alphaDiag;
             for (int q = 0; q < n_sweeps; q++)
-                if ((rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), t + dt)))
+                if ((rc = rec_sweep(m, s, cfg, u, urec, urecNew.data(), t + dt)))
                     return rc;
             if ((rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, t + dt, rhs.data())))
                 return rc;
@@ -1882,7 +2003,7 @@ extern "C" int oracle_sdirk4_step(const oracle_mesh *m, const oracle_setup *s, c
                 dTau[i] /= alphaDiag;
             const double ts = t + Cc[iB] * dt;
             for (int q = 0; q < n_sweeps; q++)
-                if ((rc = oracle_jacobi_step(m, s, cfg, u, urec, urecNew.data(), ts)))
+                if ((rc = rec_sweep(m, s, cfg, u, urec, urecNew.data(), ts)))
                     return rc;
             if ((rc = oracle_evaluate_rhs(m, s, cfg, dlf.data(), u, urec, ts, rhsbuf[size_t(iB)].data())))
                 return rc;

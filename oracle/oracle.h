@@ -61,6 +61,9 @@ typedef struct oracle_setting {
     int32_t limiter_procedure;     /* 0 WBAP V2, 1 WBAP V3 (limiterProcedure) */
     double mu_gas;                 /* idealGasProperty: 0 = Euler            */
     double pr_gas, r_gas, t_ref, c_sutherland;
+    const int64_t *sor_scan;       /* non-null: the steps' reconstruction sweeps are
+                                      SOR_Instead sweeps in this scan order      */
+    int32_t sor_inverse;           /* SOR_InverseScanning: then the reverse order */
 } oracle_setting;
 
 /* VRFiniteVolume2D::ReconstructionJacobiStep, SOR_Instead = false
@@ -68,6 +71,9 @@ typedef struct oracle_setting {
  * overwritten with it.  Returns 0 or a nonzero error code. */
 int oracle_jacobi_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
                        double *urec, double *urec_new, double t);
+/* SOR_Instead: one in-place sweep in the scan order `scan` (n_cell entries) */
+int oracle_sor_step(const oracle_mesh *m, const oracle_setup *s, const oracle_setting *cfg, const double *u,
+                    double *urec, const int64_t *scan, double t);
 
 /* EulerEvaluator::EvaluateDt (DNDS_FV_EulerEvaluator.cpp:14-183) for the
  * Euler model (muGas = 0): fills dt, *dt_min, lambda_face, delta_lambda_face. */

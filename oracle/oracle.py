@@ -43,7 +43,8 @@ class OSetting(C.Structure):
                 ("wbap_smooth_indicator_scale", C.c_double), ("wbap_n_std", C.c_double),
                 ("gmres_code", C.c_int32), ("n_gmres_space", C.c_int32), ("n_gmres_iter", C.c_int32),
                 ("limiter_procedure", C.c_int32), ("mu_gas", C.c_double), ("pr_gas", C.c_double),
-                ("r_gas", C.c_double), ("t_ref", C.c_double), ("c_sutherland", C.c_double)]
+                ("r_gas", C.c_double), ("t_ref", C.c_double), ("c_sutherland", C.c_double),
+                ("sor_scan", C.POINTER(C.c_int64)), ("sor_inverse", C.c_int32)]
 
 
 def build(quiet: bool = True) -> str:
@@ -88,6 +89,7 @@ def lib(variant: str = "parity") -> C.CDLL:
         L = C.CDLL(path)
         sig = {
             "oracle_jacobi_step": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, C.c_double],
+            "oracle_sor_step": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PI64, C.c_double],
             "oracle_evaluate_dt": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, C.c_double,
                                    C.c_double, C.c_int32, PD, PD, PD, PD],
             "oracle_evaluate_rhs": [C.POINTER(OMesh), C.POINTER(OSetup), C.POINTER(OSetting), PD, PD, PD, C.c_double,
@@ -151,7 +153,11 @@ class Oracle:
                  tang_weight_direction: int = 0, wbap_smooth_indicator_scale: float = 1e-10,
                  wbap_n_std: float = 10.0, gmres_code: int = 0, n_gmres_space: int = 10, n_gmres_iter: int = 2,
                  limiter_procedure: int = 0, mu_gas: float = 0.0, pr_gas: float = 0.72, r_gas: float = 1.0,
-                 t_ref: float = 273.15, c_sutherland: float = 110.4, variant: str = "parity"):
+                 t_ref: float = 273.15, c_sutherland: float = 110.4, variant: str = "parity", sor_scan=None,
+                 sor_inverse: bool = False):
+        """sor_scan: a scan order (iScan -> iCell) makes the steps'
+        reconstruction sweeps SOR_Instead sweeps in that order, forward then
+        backward with sor_inverse (SOR_InverseScanning)."""
         self._keep = {}
 
         def keep(name, a, dtype):
@@ -174,7 +180,11 @@ class Oracle:
         ff = (C.c_double * 4)(*far_field)
         self.cfg = OSetting(riemann_solver, gamma, ff, jacobi_relax, tang_weight, tang_weight_direction,
                             wbap_smooth_indicator_scale, wbap_n_std, gmres_code, n_gmres_space, n_gmres_iter,
-                            limiter_procedure, mu_gas, pr_gas, r_gas, t_ref, c_sutherland)
+                            limiter_procedure, mu_gas, pr_gas, r_gas, t_ref, c_sutherland,
+                            PI64() if sor_scan is None else keep("sor_scan", sor_scan, np.int64).ctypes.data_as(PI64),
+                            int(sor_inverse))
+        if sor_scan is not None:
+            assert np.array_equal(np.sort(self._keep["sor_scan"]), np.arange(self.n_cell))
         self.lib = lib(variant)
 
     def _check(self, rc, what):
@@ -187,6 +197,17 @@ class Oracle:
         new = np.empty_like(urec)
         self._check(self.lib.oracle_jacobi_step(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u),
                                                 _d(urec), _d(new), t), "jacobi_step")
+        return urec
+
+    def sor_step(self, u, urec, scan=None, t=0.0):
+        """one in-place SOR sweep (SOR_Instead) in the cell order `scan`
+        (default: natural order); returns the new reconstruction"""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        urec = np.array(urec, dtype=np.float64, order="C", copy=True)
+        scan = np.ascontiguousarray(np.arange(self.n_cell) if scan is None else scan, dtype=np.int64)
+        assert scan.shape == (self.n_cell,)
+        self._check(self.lib.oracle_sor_step(C.byref(self.m), C.byref(self.s), C.byref(self.cfg), _d(u), _d(urec),
+                                             scan.ctypes.data_as(PI64), t), "sor_step")
         return urec
 
     def evaluate_dt(self, u, cfl, max_dt=1e100, use_local_dt=False):
