@@ -2045,14 +2045,26 @@ int launch_sor(dndsmi_ctx *c, const double *u, double *x, double t) {
                     continue;
                 auto kern = c->nfmax == 3 ? (w ? k_sor<3, true> : k_sor<3, false>)
                                           : (w ? k_sor<4, true> : k_sor<4, false>);
+                // the boundary cells are a few thousand: one wave per
+                // workgroup spreads them over as many CUs as there are waves
+                const int block = w ? 64 : kBlock;
                 prof_begin(c, DNDSMI_K_JACOBI);
-                hipLaunchKernelGGL(kern, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, int32_t(n), int32_t(b),
+                hipLaunchKernelGGL(kern, dim3(unsigned((n + block - 1) / block)), dim3(block), 0, c->stream,
+                                   int32_t(n), int32_t(b),
                                    c->ldc, c->nbr, c->sbc, c->wall_idx, c->M, c->bv, u, x, x, c->relax,
                                    c->wall_ainv, c->wall_d0, c->wall_nrm, c->wall_w0, c->n_wall, c->gas, t, c->errw);
                 prof_end(c);
                 DNDSMI_TRY(c->launch_check("k_sor"));
             }
         }
+    return DNDSMI_OK;
+}
+
+int step_sweep(dndsmi_ctx *c, const double *u, double *&x, double *&xn, double t) {
+    if (c->sor)
+        return launch_sor(c, u, x, t);
+    DNDSMI_TRY(launch_jacobi(c, u, x, xn, t));
+    std::swap(x, xn);
     return DNDSMI_OK;
 }
 
@@ -2417,8 +2429,7 @@ extern "C" int dndsmi_ssprk3_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
     for (int stage = 0; stage < 3; stage++) {
         const double ts = t + ct[stage] * dtmin;
         for (int k = 0; k < n_sweeps; k++) {
-            DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, ts));
-            std::swap(uRec->d, uRecNew->d);
+            DNDSMI_TRY(step_sweep(c, u->d, uRec->d, uRecNew->d, ts));
         }
         const double cA = stage ? Coef[stage] / (1 - Coef[stage]) : 0.0;
         const double cB = stage ? 1 - Coef[stage] : 0.0;

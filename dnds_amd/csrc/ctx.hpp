@@ -242,6 +242,9 @@ bool same_ctx(const dndsmi_ctx *c, const dndsmi_array *a, int32_t kind);
 int launch_jacobi(dndsmi_ctx *c, const double *u, const double *x, double *xn, double t);
 // one SOR sweep (SOR_Instead) in place on x, colour by colour
 int launch_sor(dndsmi_ctx *c, const double *u, double *x, double t);
+// one reconstruction sweep of a step: Jacobi into xn, then the two swap; with
+// SOR_Instead in place on x (xn untouched)
+int step_sweep(dndsmi_ctx *c, const double *u, double *&x, double *&xn, double t);
 int launch_dt(dndsmi_ctx *c, const double *u, double cfl, double max_dt, double *dt_out);
 // xalt/limf (may be null): cells with limf set are read from xalt instead of x
 int launch_rhs(dndsmi_ctx *c, const double *u, const double *x, double t, double *rhs, int stage, double *xs,

@@ -836,8 +836,7 @@ extern "C" int dndsmi_implicit_euler_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi
         }
         // frhs at t + dt: sweeps then EvaluateRHS
         for (int k = 0; k < n_sweeps; k++) {
-            DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, t + dt));
-            std::swap(uRec->d, uRecNew->d);
+            DNDSMI_TRY(step_sweep(c, u->d, uRec->d, uRecNew->d, t + dt));
         }
         DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, t + dt, c->imp_rhs, 0, nullptr, nullptr, 0, 0));
         hipLaunchKernelGGL(k_dual_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->xlast, u->d, 1.0 / dt,
@@ -954,8 +953,7 @@ extern "C" int dndsmi_sdirk4_step(dndsmi_ctx *c, dndsmi_array *u, dndsmi_array *
             DNDSMI_TRY(c->launch_check("fdtau"));
             const double ts = t + kSC[ib] * dt;
             for (int q = 0; q < n_sweeps; q++) {
-                DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, ts));
-                std::swap(uRec->d, uRecNew->d);
+                DNDSMI_TRY(step_sweep(c, u->d, uRec->d, uRecNew->d, ts));
             }
             DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, ts, stage_rhs, 0, nullptr, nullptr, 0, 0));
             hipLaunchKernelGGL(k_sdirk_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, c->xlast, u->d,
@@ -1081,8 +1079,7 @@ extern "C" int dndsmi_bdf_step(dndsmi_bdf *b, dndsmi_array *u, dndsmi_array *uRe
         DNDSMI_TRY(c->launch_check("fdtau"));
         // frhs at t + dt
         for (int q = 0; q < n_sweeps; q++) {
-            DNDSMI_TRY(launch_jacobi(c, u->d, uRec->d, uRecNew->d, t + dt));
-            std::swap(uRec->d, uRecNew->d);
+            DNDSMI_TRY(step_sweep(c, u->d, uRec->d, uRecNew->d, t + dt));
         }
         DNDSMI_TRY(launch_rhs(c, u->d, uRec->d, t + dt, c->imp_rhs, 0, nullptr, nullptr, 0, 0));
         hipLaunchKernelGGL(k_bdf_rhs, dim3(grid_for(nc)), dim3(kBlock), 0, c->stream, nc, alpha0, u->d, -1. / dt,
