@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_sor(int32_t n, int32_t base, int64_t
                                                 const double *__restrict__ wall_nrm,
                                                 const double *__restrict__ wall_w0, int64_t ldw, GasParams gp,
                                                 double t, int32_t *errw) {
-    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t tid = xcd_block() * blockDim.x + threadIdx.x;
     if (tid >= n)
         return;
     const int32_t i = base + tid;
