@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_sor(int32_t n, int32_t base, int64_t
             for (int v = 0; v < 4; v++)
                 tt[v] += m * xj[c + 9 * v];
             if (c == 8) {
-                const double b = bmul * bs[r * ldc];
+                const double b = bmul * __builtin_nontemporal_load(bs + r * ldc);
 #pragma unroll
                 for (int v = 0; v < 4; v++)
                     nw[r + 9 * v] += relax * (tt[v] + du[v] * b);
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_sor(int32_t n, int32_t base, int64_t
     }
 #pragma unroll
     for (int k = 0; k < 36; k++)
-        xo[k * ldc + i] = nw[k];
+        __builtin_nontemporal_store(nw[k], &xo[k * ldc + i]);
 }
 
 // One block-Jacobi VR sweep, SOR off (ReconstructionJacobiStep,

@@ -64,6 +64,41 @@ def test_sor_sweeps_match_oracle_in_colour_order(kind, bcname, inverse):
     assert rel_err(ref, orc.jacobi_step(u, urec)) > 1e-6
 
 
+# p = 1 (the coefficients padded into the p = 3 slots) and the local inertia
+# coordinates of the reference's isentropic-vortex vfvSetting
+# (data/euler_config_IV.json:60-80), with far-field and no-slip faces
+VARIANTS = {
+    "p1": dict(rec_order=1, far_weight=1.0),
+    "local": dict(anisotropic_lengths=1, tang_weight=0.0, tang_weight_direction=0, wall_weight=0.5, far_weight=0.0,
+                  weight_len_scale=1.0, weight_scheme_geom=3, weight_scheme_dir=1, use_local_coord=1),
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("kind", [0, 2])
+def test_sor_sweeps_on_p1_and_local_coordinate_setups(variant, kind):
+    import oracle
+    from dnds_amd.host import EulerEvaluator
+    mesh, vfv = mesh_and_setup(kind=kind, nx=8, ny=6, jitter=0.2, seed=8, bc=(2, 4, 2, 3), limiter=False,
+                               sor_instead=1, **VARIANTS[variant])
+    far = (1.0, 0.3, -0.2, 2.6)
+    ev = EulerEvaluator(mesh, vfv, riemannSolverType="Roe", gamma=1.4, farFieldStaticValue=far)
+    st = vfv.setting
+    scan = ev.LUSGSScanOrder()
+    orc = oracle.Oracle(mesh.arrays(), vfv.arrays(), riemann_solver=1, gamma=1.4, far_field=far,
+                        jacobi_relax=st.jacobi_relax, tang_weight=st.tang_weight,
+                        tang_weight_direction=st.tang_weight_direction)
+    u, urec = random_state(mesh.n_cell, 36)
+    if variant == "p1":
+        urec.reshape(-1, 4, 9)[:, :, 2:] = 0.0  # p = 1 holds the two first-order rows
+    du, dr, dn = ev.BuildMean().upload(u), vfv.BuildRec().upload(urec), vfv.BuildRec()
+    ref = urec
+    for sweep in range(3):
+        vfv.ReconstructionJacobiStep(du, dr, dn)
+        ref = orc.sor_step(u, ref, scan)
+        assert rel_err(dr.download(), ref) <= TOL, (sweep, rel_err(dr.download(), ref))
+
+
 def test_sor_converges_to_the_jacobi_fixed_point_on_the_device():
     """Same equations, other iteration: the SOR and Jacobi devices meet."""
     from dnds_amd.host import EulerEvaluator
