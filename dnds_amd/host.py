@@ -28,7 +28,9 @@ from . import _capi as A
 # vfvSetting of the reference's 2D Riemann configuration
 # (data/euler_config_2DRiemann.json: anisotropicLengths, tangWeight 1 along
 # the face normal, S geometric weights, weightLenScale 0.5, farWeight 0,
-# wallWeight 0.5, WBAP_SmoothIndicatorScale 0.1, global coordinates).
+# wallWeight 0.5, WBAP_SmoothIndicatorScale 0.1, global coordinates).  That
+# file also sets SOR_Instead; the preset keeps the Jacobi sweep that the
+# workloads here are defined with, and sor_instead=1 adds the in-place sweep.
 RIEMANN_VR_SETTING = dict(anisotropic_lengths=1, tang_weight=1.0, tang_weight_direction=0, wall_weight=0.5,
                           far_weight=0.0, weight_len_scale=0.5, weight_scheme_geom=2, weight_scheme_dir=0,
                           wbap_smooth_indicator_scale=0.1, jacobi_relax=1.0)
