@@ -299,10 +299,12 @@ def main():
                     help="also run the learned warm start workload: harvest converged reconstructions of the "
                          "40x40 vortex at 8 mesh seeds (160 snapshots each, 4.1M pairs), train on the device, "
                          "report the sweeps to a 1e-10 rec increment at an unseen seed")
-    ap.add_argument("--sor", type=int, default=0,
+    ap.add_argument("--sor", type=int, default=2000,
                     help="also reconstruct the workload's initial state from zero to a 1e-10 rec increment (at most "
-                         "N sweeps) with the Jacobi sweep, the SOR_Instead sweep and SOR_Instead with "
-                         "SOR_InverseScanning, and report sweeps and times")
+                         "N sweeps) with the Jacobi sweep and the SOR_Instead sweep, and report sweeps and times "
+                         "(0 = skip)")
+    ap.add_argument("--sor-inverse", type=int, default=0,
+                    help="with --sor, also SOR_Instead with SOR_InverseScanning")
     ap.add_argument("--limiter-procedure", type=int, default=0, choices=[0, 1],
                     help="WBAP of the limited cases: 0 V2 (the reference solver's default), 1 V3")
     ap.add_argument("--mu-gas", type=float, default=0.0,
@@ -540,7 +542,8 @@ def main():
                "note": "the SOR sweeps run colour by colour in place; each sweep of this loop first copies uRec "
                        "(the loop measures the increment against the previous iterate)"}
         kj = A.KERNEL_NAMES.index("k_jacobi")
-        for name, (on, inverse) in (("jacobi", (0, 0)), ("sor", (1, 0)), ("sor_inverse", (1, 1))):
+        variants = [("jacobi", (0, 0)), ("sor", (1, 0))] + ([("sor_inverse", (1, 1))] if args.sor_inverse else [])
+        for name, (on, inverse) in variants:
             if on:
                 st = A.VRSetting.from_buffer_copy(case.vfv.setting)
                 st.sor_instead, st.sor_inverse_scanning = on, inverse
