@@ -157,3 +157,30 @@ def test_mixed_precision_step_refuses_sor():
     with pytest.raises(DNDSError, match="SOR_Instead"):
         ode.Step(du, dr, dn, 0.0)
     assert rel_err(du.download(), u) == 0.0
+
+
+def test_bdf_and_sdirk4_steps_with_sor_match_oracle():
+    """The dual-time BDF2 and SDIRK4 steps with SOR sweeps inside, a fixed
+    number of inner iterations, against the oracle in the same scan order."""
+    from dnds_amd.host import ImplicitBDFDualTimeStep, ImplicitSDIRK4DualTimeStep
+    mesh, vfv, ev, orc, scan = build_sor(2, "farfield", inverse=False)
+    u0, _ = random_state(mesh.n_cell, 37, amp=0.1)
+    bdf = ImplicitBDFDualTimeStep(ev, nInternalRecStep=2, CFL=10.0, nTimeStepInternal=3, rhsThresholdInternal=0.0,
+                                  k=2)
+    u, r, rn = ev.BuildMean().upload(u0), vfv.BuildRec().setConstant(0.0), vfv.BuildRec()
+    dt = 0.02
+    for step in range(3):
+        bdf.Step(u, r, rn, dt=dt, t=step * dt)
+    ou, orr, its = orc.bdf_steps(u0, np.zeros((mesh.n_cell, 36)), 2, 3, 3, 2, 10.0, dt, threshold=0.0,
+                                 scan2cell=scan)
+    assert list(its) == [3, 3, 3]
+    assert rel_err(u.download(), ou) <= TOL
+    assert rel_err(r.download(), orr) <= TOL
+    ode = ImplicitSDIRK4DualTimeStep(ev, nInternalRecStep=2, CFL=10.0, nTimeStepInternal=3, rhsThresholdInternal=0.0)
+    u, r = ev.BuildMean().upload(u0), vfv.BuildRec().setConstant(0.0)
+    its, _ = ode.Step(u, r, rn, dt=dt, t=0.0)
+    ou, orr, oits = orc.sdirk4_step(u0, np.zeros((mesh.n_cell, 36)), 3, 2, 10.0, dt, threshold=0.0, t=0.0,
+                                    scan2cell=scan)
+    assert list(its) == list(oits) == [3, 3, 3]
+    assert rel_err(u.download(), ou) <= TOL
+    assert rel_err(r.download(), orr) <= TOL
